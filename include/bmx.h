@@ -1,0 +1,202 @@
+/*
+ * bmx.h -- C-ABI of the MI355X (gfx950) bmad_standard multi-particle tracking library.
+ *
+ * The library replaces the inner loops of Bmad's beam tracking:
+ *   bmad/multiparticle/beam_mod.f90:40   track_beam   (lattice loop over elements, per bunch)
+ *   bmad/multiparticle/beam_mod.f90:103  track_bunch
+ *   bmad/multiparticle/beam_mod.f90:238  track1_bunch (one element, whole bunch)
+ *   bmad/multiparticle/beam_utils.f90:32 track1_bunch_hom (OpenMP loop over particles, :81-86)
+ *   bmad/code/track1.f90:44              track1       (one particle, one element)
+ *   bmad/code/track_all.f90:25           track_all    (one particle, all elements, stop on loss)
+ *   bsim/modules/lt_tracking_mod.f90:1571 ltt_run_beam_mode (turn loop)
+ * for the element kinds drift / quadrupole / sextupole / sbend (Hwang edge) / rfcavity / marker
+ * and elliptical or rectangular apertures, all arithmetic in FP64.
+ *
+ * Conventions (mirroring the reference):
+ *  - Phase-space coordinates are Bmad's (x, px, y, py, z, pz) with px, py, pz normalised by the
+ *    reference momentum p0c (bmad/modules/bmad_struct.f90:551-582, coord_struct%vec).
+ *  - Particle loss is data, not an error (bmad/code/track1.f90:39-41): it is recorded in state[]
+ *    with Bmad's codes (bmad_struct.f90:527-532), the element index in loss_ele[] (ix_ele,
+ *    track1.f90:88-94, 1-based like lat%branch%ele(1:n)) and the 0-based turn in loss_turn[]
+ *    (lt_tracking_mod.f90:1395).
+ *  - Every entry point returns BMX_OK (0) or a positive error code; bmx_last_error() gives the
+ *    message of the last failure on the calling thread. Nothing aborts or prints.
+ *  - All arrays passed to the upload, download and submit calls are caller-owned host memory; arrays
+ *    passed to bmx_beam_wrap are caller-owned device memory that must outlive the handle.
+ *  - A handle is not reentrant: one context per host thread (the server owns its own thread).
+ */
+#ifndef BMX_H
+#define BMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BMX_VERSION 1
+
+/* ---- status codes ---------------------------------------------------------------------- */
+#define BMX_OK              0
+#define BMX_ERR_ARG         1  /* bad argument (null handle, size mismatch, ...) */
+#define BMX_ERR_UNSUPPORTED 2  /* element kind or attribute outside this path (track1_bmad.f90:263-270) */
+#define BMX_ERR_HIP         3  /* HIP runtime failure */
+#define BMX_ERR_P0C         4  /* reference momentum mismatch (track1.f90:151-169) */
+#define BMX_ERR_NOMEM       5  /* device allocation failed */
+#define BMX_ERR_BUDGET      6  /* request larger than the server's HBM budget */
+#define BMX_ERR_STATE       7  /* call not valid in the handle's current state */
+#define BMX_ERR_NOT_FOUND   8  /* unknown request id */
+
+/* ---- coord_struct%state codes (bmad_struct.f90:527-532) -------------------------------- */
+#define BMX_STATE_ALIVE      1
+#define BMX_STATE_LOST       2
+#define BMX_STATE_LOST_NEG_X 3
+#define BMX_STATE_LOST_POS_X 4
+#define BMX_STATE_LOST_NEG_Y 5
+#define BMX_STATE_LOST_POS_Y 6
+#define BMX_STATE_LOST_Z     7
+#define BMX_STATE_LOST_PZ    8
+
+/* ---- element keys (bmad_struct.f90:1696-1710) ------------------------------------------ */
+#define BMX_DRIFT            1
+#define BMX_SBEND            2
+#define BMX_QUADRUPOLE       3
+#define BMX_SEXTUPOLE        5
+#define BMX_RFCAVITY         9
+#define BMX_MARKER          14
+#define BMX_KICKER          15
+#define BMX_OCTUPOLE        17
+#define BMX_MONITOR         31
+#define BMX_INSTRUMENT      32
+#define BMX_HKICKER         33
+#define BMX_VKICKER         34
+#define BMX_RCOLLIMATOR     35
+#define BMX_ECOLLIMATOR     36
+#define BMX_PIPE            44
+#define BMX_THICK_MULTIPOLE 67
+
+/* fringe_type (bmad_struct.f90:27,168-169) */
+#define BMX_FRINGE_NONE           1
+#define BMX_FRINGE_SOFT_EDGE_ONLY 2
+#define BMX_FRINGE_HARD_EDGE_ONLY 3
+#define BMX_FRINGE_FULL           4
+#define BMX_FRINGE_BASIC_BEND     7
+/* fringe_at / aperture_at (bmad_struct.f90:373-374) */
+#define BMX_ENTRANCE_END 1
+#define BMX_EXIT_END     2
+#define BMX_BOTH_ENDS    3
+#define BMX_NO_END       4
+#define BMX_CONTINUOUS   5
+/* aperture_type (bmad_struct.f90:158) */
+#define BMX_APERTURE_RECTANGULAR 2
+#define BMX_APERTURE_ELLIPTICAL  3
+
+#define BMX_N_POLE 22  /* n_pole_maxx + 1: multipole orders 0..21 */
+
+/*
+ * bmx_ele -- one lattice element, as the raw attribute values of ele_struct
+ * (bmad_struct.f90:1422-1530, ele%value(:) at :1469). A Fortran host fills it straight from
+ * lat%branch(ix)%ele(i)%value(...) after Bmad's own bookkeeping; ds_step is the bookkept value
+ * (attribute_bookkeeper.f90:292-398) and p0c / e_tot the element's reference momentum / energy.
+ * Zero means "not set" for every optional attribute, as in ele%value.
+ */
+typedef struct bmx_ele {
+    int32_t key;                   /* BMX_DRIFT ... */
+    int32_t is_on;                 /* ele%is_on */
+    int32_t fringe_type;           /* value(fringe_type$) */
+    int32_t fringe_at;             /* value(fringe_at$) */
+    int32_t aperture_type;         /* ele%aperture_type */
+    int32_t aperture_at;           /* ele%aperture_at */
+    int32_t offset_moves_aperture; /* ele%offset_moves_aperture */
+    int32_t n_multipole;           /* 0: a_pole/b_pole not associated; else number of orders used */
+    double l;
+    double k1, k2, k3;
+    double g, dg, e1, e2, fint, fintx, hgap, hgapx;
+    double hkick, vkick, kick;
+    double tilt, ref_tilt;
+    double x_offset, y_offset, z_offset, x_pitch, y_pitch;
+    double voltage, rf_frequency, phi0, phi0_multipass;
+    double x1_limit, x2_limit, y1_limit, y2_limit;
+    double px_aperture_width2, px_aperture_center, py_aperture_width2, py_aperture_center;
+    double z_aperture_width2, z_aperture_center, pz_aperture_width2, pz_aperture_center;
+    double ds_step;
+    double p0c, e_tot;
+    double r0_mag;
+    double a_pole[BMX_N_POLE];
+    double b_pole[BMX_N_POLE];
+} bmx_ele;
+
+/*
+ * bmx_beam_view -- structure-of-arrays particle state: coord_struct%vec(1:6), %beta, %state
+ * (bmad_struct.f90:551-582) plus the loss bookkeeping of track1.f90:88-94 / lt_tracking_mod.f90:1395.
+ * Depending on the call the pointers are host or device addresses.
+ */
+typedef struct bmx_beam_view {
+    double *x, *px, *y, *py, *z, *pz, *beta;
+    int32_t *state, *loss_ele, *loss_turn;
+} bmx_beam_view;
+
+typedef struct bmx_ctx bmx_ctx;
+typedef struct bmx_lattice bmx_lattice;
+typedef struct bmx_beam bmx_beam;
+typedef struct bmx_batch bmx_batch;
+typedef struct bmx_server bmx_server;
+
+/* ---- library ----------------------------------------------------------------------------- */
+/* Library version (BMX_VERSION). */
+int32_t bmx_version(void);
+/* Message of the last failed call on this thread (empty if none). Replaces out_io error
+ * reporting (sim_utils/io/output_mod.f90:198). */
+const char *bmx_last_error(void);
+
+/* ---- context: one device + one HIP stream ------------------------------------------------ */
+int32_t bmx_ctx_create(int32_t device, bmx_ctx **out);
+int32_t bmx_ctx_destroy(bmx_ctx *ctx);
+int32_t bmx_ctx_synchronize(bmx_ctx *ctx);
+/* The context's HIP stream (hipStream_t), for callers that order their own work against it. */
+void *bmx_ctx_stream(bmx_ctx *ctx);
+
+/* ---- lattice: flattened element table (lat%branch(ix)%ele(1:n_ele_track)) ----------------- *
+ * Replaces the per-particle re-derivation of element constants in track_a_* (e.g.
+ * multipole_ele_to_ab calls in track_a_quadrupole.f90:46-47, track_a_bend.f90:58-77) by one
+ * compilation of the table, uploaded once. mc2 [eV] and charge are the tracked species
+ * (particle_species_mod.f90:1498,1550); p0c of every element must equal the beam's p0c.      */
+int32_t bmx_lattice_create(bmx_ctx *ctx, const bmx_ele *eles, int32_t n_ele, double mc2,
+                           int32_t charge, bmx_lattice **out);
+int32_t bmx_lattice_destroy(bmx_lattice *lat);
+/* Number of elements and bit set of element kinds present (bit = key). */
+int32_t bmx_lattice_info(const bmx_lattice *lat, int32_t *n_ele, uint64_t *kind_mask);
+
+/* ---- beam: particle state resident in HBM --------------------------------------------------- */
+/* Library-owned device storage for n particles. */
+int32_t bmx_beam_create(bmx_ctx *ctx, int64_t n, bmx_beam **out);
+/* Caller-owned device storage (e.g. torch tensors); no copy is made. */
+int32_t bmx_beam_wrap(bmx_ctx *ctx, const bmx_beam_view *device, int64_t n, bmx_beam **out);
+/* Host -> device and device -> host copies of the first n particles (blocking). */
+int32_t bmx_beam_upload(bmx_beam *beam, const bmx_beam_view *host, int64_t n);
+int32_t bmx_beam_download(bmx_beam *beam, const bmx_beam_view *host, int64_t n);
+int32_t bmx_beam_destroy(bmx_beam *beam);
+/* Count of particles with state == alive (lt_tracking_mod.f90:1672, bunch%n_live). */
+int32_t bmx_beam_count_alive(bmx_beam *beam, int64_t *n_alive);
+
+/* ---- tracking ---------------------------------------------------------------------------- */
+/* Persistent tracking: n_turns full turns starting at element 1, turn index turn0 .. turn0+n_turns-1.
+ * Equivalent of ltt_run_beam_mode's turn loop (lt_tracking_mod.f90:1631-1716) calling
+ * track_bunch (beam_mod.f90:103) once per turn. State stays in registers across elements and
+ * turns; HBM is read once and written once per call. Asynchronous on the context's stream. */
+int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam, int32_t n_turns,
+                        int32_t turn0);
+/* One element for the whole bunch: track1_bunch_hom (beam_utils.f90:32-90) on the GPU, state
+ * read from and written back to HBM. ix_ele is 1-based. Asynchronous. */
+int32_t bmx_track_element(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam, int32_t ix_ele,
+                          int32_t turn);
+/* Plain path: n_turns x n_ele launches of the per-element kernel (the reference's loop nest:
+ * turn -> element -> particles, beam_mod.f90:159-166 + beam_utils.f90:81-86). Asynchronous. */
+int32_t bmx_track_turns_plain(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam,
+                              int32_t n_turns, int32_t turn0);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BMX_H */

@@ -1,0 +1,1292 @@
+/*
+ * bmad_oracle.c -- CPU restatement of Bmad's bmad_standard charged-particle tracking path,
+ * plain C99 + GNU __float128, FP64, scalar, compiled with -ffp-contract=off.
+ *
+ * TEST INFRASTRUCTURE: this is the checker the MI355X library is compared against. Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it. It follows the
+ * reference's routines statement by statement (same branches, same small-argument thresholds,
+ * same left-to-right evaluation order of each Fortran expression); each function cites the
+ * reference file:line it restates (paths relative to the bmad-ecosystem tree).
+ *
+ * Coverage: drift, quadrupole, sextupole / octupole / thick_multipole / kickers / collimators /
+ * instrument / monitor / pipe (thick multipole kick-drift-kick), sbend without body multipoles
+ * (Hwang edge kicks, exact and semi-linear body), rfcavity (standing wave, no coupler kick),
+ * marker; offsets, pitches and tilt of non-bend elements (offset_particle non-bend branch);
+ * elliptical and rectangular apertures incl. ecollimator/rcollimator momentum windows.
+ * Electric multipoles, bend misalignments, bend body multipoles and k1 in bends return
+ * BMX_ERR_UNSUPPORTED. Time (coord_struct%t, qp) is not carried: no result on this path
+ * depends on it under relative time tracking (bmad_struct.f90:2350).
+ */
+#include <math.h>
+#include <quadmath.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "bmad_oracle.h"
+
+/* sim_utils/interfaces/physical_constants.f90:11-33 */
+static const double PI = 3.141592653589793238462643383279;
+#define TWOPI (2 * PI)
+static const double C_LIGHT = 2.99792458e8;
+/* bmad_struct.f90:2314 bmad_com%max_aperture_limit */
+static const double MAX_APERTURE_LIMIT = 1e3;
+
+#define FIRST_TRACK_EDGE 11  /* bmad_struct.f90:379 */
+#define SECOND_TRACK_EDGE 12
+
+typedef struct {
+    double vec[6];
+    double beta, p0c, mc2;
+    int charge, state;
+} coord_t;
+
+static double sq(double x) { return x * x; }
+
+/* ------------------------------------------------------------------------------------------
+ * special functions
+ * ------------------------------------------------------------------------------------------ */
+
+/* sim_utils/special_functions/sqrt_one.f90:18-41 (nd = 0): sqrt(1+x) - 1 without cancellation. */
+double bmo_sqrt_one(double x)
+{
+    double s = sqrt(1 + x);
+    double rad = s + 1;
+    return x / rad;
+}
+
+/* sim_utils/special_functions/sinc.f90:15-40 (nd = 0). */
+double bmo_sinc(double x)
+{
+    if (fabs(x) < 1e-8) return 1.0;
+    return sin(x) / x;
+}
+
+/* sim_utils/special_functions/cosc.f90:15-40 (nd = 0): (1 - cos x) / x^2. */
+double bmo_cosc(double x)
+{
+    if (fabs(x) < 1e-8) return 0.5;
+    double s = bmo_sinc(x / 2);
+    return 0.5 * (s * s);
+}
+
+/* sim_utils/special_functions/sincc.f90:15-45 (nd = 0): (x - sin x) / x^3. */
+double bmo_sincc(double x)
+{
+    const double c0 = 1.0 / 6.0, c1 = -1.0 / 120.0, c2 = 1.0 / 5040.0, c3 = -1.0 / 362880.0;
+    double x2 = x * x;
+    if (fabs(x) < 0.1) return c0 + x2 * (c1 + x2 * (c2 + x2 * c3));
+    return (x - sin(x)) / (x * x2);
+}
+
+/* Fortran MODULO for reals: fmod, then shifted to the sign of p. */
+static double f_modulo(double a, double p)
+{
+    double r = fmod(a, p);
+    if (r != 0 && ((r < 0) != (p < 0))) r += p;
+    return r;
+}
+
+static __float128 f_moduloq(__float128 a, __float128 p)
+{
+    __float128 r = fmodq(a, p);
+    if (r != 0 && ((r < 0) != (p < 0))) r += p;
+    return r;
+}
+
+/* sim_utils/special_functions/modulo2_mod.f90:85-99 (modulo2_dp). */
+double bmo_modulo2(double x, double amp)
+{
+    double m = f_modulo(x, 2 * amp);
+    if (m >= amp) m = m - 2 * amp;
+    return m;
+}
+
+/* modulo2_mod.f90:121-135 (modulo2_qp). */
+static __float128 modulo2_q(__float128 x, __float128 amp)
+{
+    __float128 m = f_moduloq(x, 2 * amp);
+    if (m >= amp) m = m - 2 * amp;
+    return m;
+}
+
+/* bmad/low_level/c_multi.f90:19-63 with no_n_fact = .true.: the same table build (binomials,
+ * division by n!, sign pattern on m mod 4) and the same re-multiplication by n!. */
+double bmo_c_multi(int32_t n, int32_t m)
+{
+    static double c[BMX_N_POLE][BMX_N_POLE];
+    static double nfact[BMX_N_POLE];
+    static int init = 0;
+    if (!init) {
+        memset(c, 0, sizeof c);
+        c[0][0] = 1;
+        for (int in = 1; in < BMX_N_POLE; in++) {
+            c[in][0] = 1;
+            c[in][in] = 1;
+            for (int im = 1; im < in; im++) c[in][im] = c[in - 1][im - 1] + c[in - 1][im];
+        }
+        nfact[0] = 1;
+        for (int in = 0; in < BMX_N_POLE; in++) {
+            if (in > 0) nfact[in] = in * nfact[in - 1];
+            for (int im = 0; im <= in; im++) {
+                c[in][im] = c[in][im] / nfact[in];
+                if (im % 4 == 0) c[in][im] = -c[in][im];
+                if (im % 4 == 3) c[in][im] = -c[in][im];
+            }
+        }
+        init = 1;
+    }
+    return c[n][m] * nfact[n];
+}
+
+/* Fortran x**m for an integer variable m (libgcc __powidf2 repeated squaring). */
+static double powi(double x, int m)
+{
+    unsigned n = m < 0 ? -(unsigned)m : (unsigned)m;
+    double y = (n % 2) ? x : 1;
+    while (n >>= 1) {
+        x = x * x;
+        if (n % 2) y = y * x;
+    }
+    return m < 0 ? 1 / y : y;
+}
+
+/* bmad/low_level/mexp.f90:14: x^m with 0^0 = 1. */
+static double mexp(double x, int m)
+{
+    if (m < 0) return 0;
+    if (m == 0) return 1;
+    return powi(x, m);
+}
+
+/* Fortran NINT: round half away from zero. */
+static long f_nint(double x) { return lround(x); }
+
+/* ------------------------------------------------------------------------------------------
+ * element attribute helpers
+ * ------------------------------------------------------------------------------------------ */
+
+static int is_thick_multipole_key(int key)
+{
+    /* track1_bmad.f90:87-88, 172-173, 215-216 */
+    switch (key) {
+    case BMX_RCOLLIMATOR: case BMX_ECOLLIMATOR: case BMX_MONITOR: case BMX_INSTRUMENT:
+    case BMX_PIPE: case BMX_KICKER: case BMX_HKICKER: case BMX_VKICKER: case BMX_OCTUPOLE:
+    case BMX_THICK_MULTIPOLE: case BMX_SEXTUPOLE:
+        return 1;
+    }
+    return 0;
+}
+
+static int has_orientation_attributes(int key) { return key != BMX_MARKER; }
+
+static int has_hkick_attributes(int key)
+{
+    switch (key) {
+    case BMX_DRIFT: case BMX_MARKER: case BMX_KICKER: case BMX_HKICKER: case BMX_VKICKER:
+        return 0;
+    }
+    return 1;
+}
+
+static int is_misaligned(const bmx_ele *e)
+{
+    return e->x_offset != 0 || e->y_offset != 0 || e->z_offset != 0 || e->x_pitch != 0 ||
+           e->y_pitch != 0;
+}
+
+static int max_nonzero(const double *a, const double *b, int ubound)
+{
+    for (int i = ubound; i >= 0; i--) {
+        if (a[i] != 0) return i;
+        if (b[i] != 0) return i;
+    }
+    return -1;
+}
+
+/* bmad/code/multipole_ele_to_ab.f90:472-600 convert_this_ab, magnetic, use_ele_tilt = F
+ * (so the tilt difference is zero), field_master = F, scale_multipoles = T. */
+static int convert_this_ab(const bmx_ele *e, double *a, double *b)
+{
+    for (int n = 0; n < BMX_N_POLE; n++) {
+        a[n] = e->a_pole[n];
+        b[n] = e->b_pole[n];
+    }
+    if (max_nonzero(a, b, BMX_N_POLE - 1) < 0) return 0;
+    double cnst;
+    int ref_exp;
+    switch (e->key) {
+    case BMX_SBEND:
+        cnst = e->l * (e->g + e->dg); ref_exp = 0; break;
+    case BMX_KICKER:
+        if (e->hkick == 0) cnst = e->vkick;
+        else if (e->vkick == 0) cnst = e->hkick;
+        else cnst = sqrt(sq(e->hkick) + sq(e->vkick));
+        ref_exp = 0;
+        break;
+    case BMX_HKICKER: case BMX_VKICKER:
+        cnst = e->kick; ref_exp = 0; break;
+    case BMX_QUADRUPOLE:
+        cnst = e->k1 * e->l; ref_exp = 1; break;
+    case BMX_SEXTUPOLE:
+        cnst = e->k2 * e->l; ref_exp = 2; break;
+    case BMX_OCTUPOLE:
+        cnst = e->k3 * e->l; ref_exp = 3; break;
+    case BMX_THICK_MULTIPOLE: case BMX_RFCAVITY:
+        return 0; /* multipoles do not scale */
+    default:
+        return BMX_ERR_UNSUPPORTED;
+    }
+    double radius = e->r0_mag;
+    if (radius == 0) {
+        for (int n = 0; n < BMX_N_POLE; n++) {
+            a[n] = cnst * a[n];
+            b[n] = cnst * b[n];
+        }
+    } else {
+        double factor = cnst * powi(radius, ref_exp + 1);
+        for (int n = 0; n < BMX_N_POLE; n++) {
+            factor = factor / radius;
+            a[n] = factor * a[n];
+            b[n] = factor * b[n];
+        }
+    }
+    return 0;
+}
+
+/* multipole_ele_to_ab.f90:326-405 this_ele_non_multipoles, magnetic. */
+static void this_ele_non_multipoles(const bmx_ele *e, double *a_kick, double *b_kick)
+{
+    for (int i = 0; i < 4; i++) a_kick[i] = b_kick[i] = 0;
+    switch (e->key) {
+    case BMX_HKICKER: b_kick[0] = -e->kick; break;
+    case BMX_VKICKER: a_kick[0] = e->kick; break;
+    case BMX_KICKER:
+        b_kick[0] = -e->hkick;
+        a_kick[0] = e->vkick;
+        break;
+    default: {
+        double hk = e->hkick, vk = e->vkick;
+        if (has_hkick_attributes(e->key) && (hk != 0 || vk != 0)) {
+            double tilt = (e->key == BMX_SBEND) ? e->ref_tilt : e->tilt;
+            if (tilt != 0) {
+                double cos_t = cos(tilt), sin_t = sin(tilt);
+                b_kick[0] = -hk * cos_t - vk * sin_t;
+                a_kick[0] = -hk * sin_t + vk * cos_t;
+            } else {
+                b_kick[0] = -hk;
+                a_kick[0] = vk;
+            }
+        }
+    }
+    }
+    switch (e->key) {
+    case BMX_QUADRUPOLE: b_kick[1] = e->k1 * e->l; break;
+    case BMX_SBEND:
+        b_kick[0] = b_kick[0] + e->dg * e->l;
+        b_kick[1] = e->k1 * e->l;
+        b_kick[2] = e->k2 * e->l / 2.0;
+        break;
+    case BMX_SEXTUPOLE: b_kick[2] = e->k2 * e->l / 2.0; break;
+    case BMX_OCTUPOLE: b_kick[3] = e->k3 * e->l / 6.0; break;
+    }
+}
+
+/* multipole_ele_to_ab.f90:31-236 (non-slave element, magnetic, use_ele_tilt = F), with the
+ * optional b1 pull-out of pull_this_b1 (:664-676). Returns ix_pole_max; *err set on failure. */
+static int multipole_ele_to_ab(const bmx_ele *e, int include_kicks, double *a, double *b,
+                               double *b1, int *err)
+{
+    *err = 0;
+    for (int n = 0; n < BMX_N_POLE; n++) a[n] = b[n] = 0;
+    if (b1) *b1 = 0;
+    if (!e->is_on || e->key == BMX_PIPE) return -1;
+    if (e->n_multipole > 0) {
+        *err = convert_this_ab(e, a, b);
+        if (*err) return -1;
+    }
+    double ak[4], bk[4];
+    this_ele_non_multipoles(e, ak, bk);
+    if (include_kicks) {
+        for (int i = 0; i < 4; i++) {
+            a[i] = a[i] + ak[i];
+            b[i] = b[i] + bk[i];
+        }
+    }
+    int ix = max_nonzero(a, b, BMX_N_POLE - 1);
+    if (b1) {
+        *b1 = b[1];
+        b[1] = 0;
+        if (ix <= 1) ix = max_nonzero(a, b, 1);
+    }
+    return ix;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * coordinate transforms
+ * ------------------------------------------------------------------------------------------ */
+
+/* bmad/code/tilt_coords.f90:18-52 */
+static void tilt_coords(double tilt_val, double *v)
+{
+    if (tilt_val == 0) return;
+    double s = sin(tilt_val), c = cos(tilt_val);
+    double x = v[0], y = v[2], px = v[1], py = v[3];
+    v[0] = c * x + s * y;
+    v[2] = -s * x + c * y;
+    v[1] = c * px + s * py;
+    v[3] = -s * px + c * py;
+}
+
+/* bmad/geometry/floor_angles_to_w_mat.f90:17-57 with psi = 0 kept symbolic. */
+static void floor_angles_to_w_mat(double theta, double phi, double psi, double w[3][3], int inv)
+{
+    double s_the = sin(theta), c_the = cos(theta);
+    double s_phi = sin(phi), c_phi = cos(phi);
+    double s_psi = sin(psi), c_psi = cos(psi);
+    if (!inv) {
+        w[0][0] = c_the * c_psi - s_the * s_phi * s_psi;
+        w[0][1] = -c_the * s_psi - s_the * s_phi * c_psi;
+        w[0][2] = s_the * c_phi;
+        w[1][0] = c_phi * s_psi;
+        w[1][1] = c_phi * c_psi;
+        w[1][2] = s_phi;
+        w[2][0] = -s_the * c_psi - c_the * s_phi * s_psi;
+        w[2][1] = s_the * s_psi - c_the * s_phi * c_psi;
+        w[2][2] = c_the * c_phi;
+    } else {
+        w[0][0] = c_the * c_psi - s_the * s_phi * s_psi;
+        w[0][1] = c_phi * s_psi;
+        w[0][2] = -s_the * c_psi - c_the * s_phi * s_psi;
+        w[1][0] = -c_the * s_psi - s_the * s_phi * c_psi;
+        w[1][1] = c_phi * c_psi;
+        w[1][2] = s_the * s_psi - c_the * s_phi * c_psi;
+        w[2][0] = s_the * c_phi;
+        w[2][1] = s_phi;
+        w[2][2] = c_the * c_phi;
+    }
+}
+
+static void matvec3(double w[3][3], const double *r, double *out)
+{
+    double t[3];
+    for (int i = 0; i < 3; i++) t[i] = w[i][0] * r[0] + w[i][1] * r[1] + w[i][2] * r[2];
+    for (int i = 0; i < 3; i++) out[i] = t[i];
+}
+
+static void track_a_drift_ex(coord_t *o, double length, int include_ref_motion);
+
+/* bmad/code/offset_particle.f90:66-492, non-bend branch (lines 183-335 / 339-492), for an
+ * element with orientation = +1 and a particle with direction = time_dir = +1.
+ * has_s_pos/s_pos: optional s_pos argument (then drift_to_edge defaults to no$). */
+static void offset_particle(const bmx_ele *e, int set, coord_t *o, int set_tilt,
+                            int set_hvkicks, int has_s_pos, double s_pos)
+{
+    const int sign_z_vel = 1;
+    int set_hv = set_hvkicks && e->is_on && has_hkick_attributes(e->key);
+    if (e->key == BMX_KICKER || e->key == BMX_HKICKER || e->key == BMX_VKICKER)
+        set_hv = set_hvkicks && e->is_on;
+    int set_t = set_tilt && has_orientation_attributes(e->key);
+    int drift_to; /* 0 = no, 1 = upstream, 2 = downstream */
+    if (!has_orientation_attributes(e->key)) drift_to = 0;
+    else if (has_s_pos) drift_to = 0;
+    else if (set) drift_to = 1;
+    else drift_to = 2;
+
+    double length = e->l;
+    double L_half = 0.5 * length;
+    double s_pos0;
+    if (has_s_pos) s_pos0 = s_pos;
+    else if (set) s_pos0 = 0;
+    else s_pos0 = length;
+    double ds_center = L_half - s_pos0;
+    double s_target = (drift_to == 1) ? (1 - sign_z_vel) * L_half : (1 + 1) * L_half;
+    double rel_p = 1 + o->vec[5];
+    const double charge_dir = 1.0 * sign_z_vel;
+    int set_hv1 = 0, set_hv2 = 0;
+    if (set_hv) {
+        if (e->key == BMX_KICKER || e->key == BMX_HKICKER || e->key == BMX_VKICKER) set_hv2 = 1;
+        else set_hv1 = 1;
+    }
+    int misaligned = has_orientation_attributes(e->key) && is_misaligned(e);
+    double w[3][3], r[3], ds;
+
+    if (set) {
+        double s_body;
+        if (misaligned) {
+            r[0] = o->vec[0] - e->x_offset;
+            r[1] = o->vec[2] - e->y_offset;
+            r[2] = 0.0 - (e->z_offset + 1 * ds_center);
+            floor_angles_to_w_mat(e->x_pitch, e->y_pitch, 0.0, w, 1);
+            matvec3(w, r, r);
+            r[2] = r[2] + L_half;
+            s_body = r[2];
+            double pz = sq(rel_p) - sq(o->vec[1]) - sq(o->vec[3]);
+            if (pz <= 0) {
+                o->state = BMX_STATE_LOST_PZ;
+            } else {
+                double p0[3] = {o->vec[1], o->vec[3], sign_z_vel * sqrt(pz)}, p[3];
+                matvec3(w, p0, p);
+                o->vec[1] = p[0];
+                o->vec[3] = p[1];
+                o->vec[0] = r[0];
+                o->vec[2] = r[1];
+            }
+        } else {
+            s_body = s_pos0;
+        }
+        ds = s_target - s_body;
+        if (fabs(ds) > 1e-14 && drift_to != 0 && o->state == BMX_STATE_ALIVE)
+            track_a_drift_ex(o, ds, 0);
+        if (set_hv1) {
+            o->vec[1] = o->vec[1] + 0.5 * charge_dir * e->hkick;
+            o->vec[3] = o->vec[3] + 0.5 * charge_dir * e->vkick;
+        }
+        if (set_t && e->key != BMX_SBEND && e->tilt != 0) tilt_coords(e->tilt, o->vec);
+        if (set_hv2) {
+            if (e->key == BMX_HKICKER) o->vec[1] = o->vec[1] + 0.5 * charge_dir * e->kick;
+            else if (e->key == BMX_VKICKER) o->vec[3] = o->vec[3] + 0.5 * charge_dir * e->kick;
+            else {
+                o->vec[1] = o->vec[1] + 0.5 * charge_dir * e->hkick;
+                o->vec[3] = o->vec[3] + 0.5 * charge_dir * e->vkick;
+            }
+        }
+    } else {
+        if (set_hv2) {
+            if (e->key == BMX_HKICKER) o->vec[1] = o->vec[1] + 0.5 * charge_dir * e->kick;
+            else if (e->key == BMX_VKICKER) o->vec[3] = o->vec[3] + 0.5 * charge_dir * e->kick;
+            else {
+                o->vec[1] = o->vec[1] + 0.5 * charge_dir * e->hkick;
+                o->vec[3] = o->vec[3] + 0.5 * charge_dir * e->vkick;
+            }
+        }
+        if (set_t && e->key != BMX_SBEND) tilt_coords(-e->tilt, o->vec);
+        if (set_hv1) {
+            o->vec[1] = o->vec[1] + 0.5 * charge_dir * e->hkick;
+            o->vec[3] = o->vec[3] + 0.5 * charge_dir * e->vkick;
+        }
+        double s_lab;
+        if (misaligned) {
+            r[0] = o->vec[0];
+            r[1] = o->vec[2];
+            r[2] = 0.0 - ds_center;
+            floor_angles_to_w_mat(e->x_pitch, e->y_pitch, 0.0, w, 0);
+            matvec3(w, r, r);
+            r[0] = r[0] + e->x_offset;
+            r[1] = r[1] + e->y_offset;
+            r[2] = r[2] + (e->z_offset + L_half);
+            s_lab = r[2];
+            double pz = sq(rel_p) - sq(o->vec[1]) - sq(o->vec[3]);
+            if (pz <= 0) {
+                o->state = BMX_STATE_LOST_PZ;
+            } else {
+                double p0[3] = {o->vec[1], o->vec[3], sign_z_vel * sqrt(pz)}, p[3];
+                matvec3(w, p0, p);
+                o->vec[1] = p[0];
+                o->vec[3] = p[1];
+                o->vec[0] = r[0];
+                o->vec[2] = r[1];
+            }
+        } else {
+            s_lab = s_pos0;
+        }
+        ds = s_target - s_lab;
+        if (fabs(ds) > 1e-14 && drift_to != 0 && o->state == BMX_STATE_ALIVE)
+            track_a_drift_ex(o, ds, 0);
+    }
+}
+
+/* ------------------------------------------------------------------------------------------
+ * drift
+ * ------------------------------------------------------------------------------------------ */
+
+/* bmad/low_level/track_a_drift.f90:25-116 for orb%direction = ele_orientation = +1, beta > 0. */
+static void track_a_drift_ex(coord_t *o, double length, int include_ref_motion)
+{
+    if (length == 0) return;
+    double delta = o->vec[5];
+    double rel_pc = 1 + delta;
+    double px_rel = 1 * o->vec[1] / rel_pc;
+    double py_rel = 1 * o->vec[3] / rel_pc;
+    double pxy2 = sq(px_rel) + sq(py_rel);
+    if (pxy2 >= 1) {
+        o->state = BMX_STATE_LOST_PZ;
+        return;
+    }
+    double ps_rel = sqrt(1 - pxy2);
+    o->vec[0] = o->vec[0] + length * px_rel / ps_rel;
+    o->vec[2] = o->vec[2] + length * py_rel / ps_rel;
+    double dz;
+    if (include_ref_motion) {
+        double mc2 = o->mc2;
+        dz = length * (bmo_sqrt_one((sq(mc2) * (2 * delta + sq(delta))) /
+                                    (sq(o->p0c * rel_pc) + sq(mc2))) +
+                       bmo_sqrt_one(-pxy2) / ps_rel);
+    } else {
+        dz = -length / ps_rel;
+    }
+    o->vec[4] = o->vec[4] + 1 * dz;
+}
+
+static void track_a_drift(coord_t *o, double length) { track_a_drift_ex(o, length, 1); }
+
+/* ------------------------------------------------------------------------------------------
+ * multipole kicks
+ * ------------------------------------------------------------------------------------------ */
+
+/* bmad/modules/multipole_mod.f90:314-401 ab_multipole_kick, magnetic, ele_orientation = 1,
+ * same species as the reference (f = 1). */
+static void ab_multipole_kick(double a, double b, int n, const coord_t *o, double scale,
+                              double *kx, double *ky)
+{
+    *kx = 0;
+    *ky = 0;
+    if (a == 0 && b == 0) return;
+    double a2 = a * 1.0, b2 = b * 1.0;
+    a2 = scale * a2;
+    b2 = scale * b2;
+    double x = o->vec[0], y = o->vec[2];
+    for (int m = 0; m <= n; m += 2) {
+        double f = bmo_c_multi(n, m) * mexp(x, n - m) * mexp(y, m);
+        *kx = *kx + b2 * f;
+        *ky = *ky - a2 * f;
+    }
+    for (int m = 1; m <= n; m += 2) {
+        double f = bmo_c_multi(n, m) * mexp(x, n - m) * mexp(y, m);
+        *kx = *kx + a2 * f;
+        *ky = *ky + b2 * f;
+    }
+}
+
+/* multipole_mod.f90:82-174 ab_multipole_kicks, magnetic branch. */
+static void ab_multipole_kicks(const double *an, const double *bn, int ix_pole_max,
+                               const bmx_ele *e, coord_t *o, double scale)
+{
+    double kx_tot = 0, ky_tot = 0, kx, ky;
+    double r = e->p0c / o->p0c;
+    for (int n = 0; n <= ix_pole_max; n++) {
+        ab_multipole_kick(r * an[n], r * bn[n], n, o, scale, &kx, &ky);
+        kx_tot = kx_tot + kx;
+        ky_tot = ky_tot + ky;
+    }
+    o->vec[1] = o->vec[1] + kx_tot;
+    o->vec[3] = o->vec[3] + ky_tot;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * quadrupole
+ * ------------------------------------------------------------------------------------------ */
+
+/* bmad/low_level/quad_mat2_calc.f90:22-81 (mat2 and z_coef). */
+static void quad_mat2_calc(double k1, double length, double rel_p, double mat2[2][2],
+                           double zc[3])
+{
+    double sqrt_k = sqrt(fabs(k1));
+    double sk_l = sqrt_k * length;
+    double cx, sx;
+    if (fabs(sk_l) < 1e-10) {
+        double k_l2 = k1 * sq(length);
+        cx = 1 + k_l2 / 2;
+        sx = (1 + k_l2 / 6) * length;
+    } else if (k1 < 0) {
+        cx = cos(sk_l);
+        sx = sin(sk_l) / sqrt_k;
+    } else {
+        cx = cosh(sk_l);
+        sx = sinh(sk_l) / sqrt_k;
+    }
+    mat2[0][0] = cx;
+    mat2[0][1] = sx / rel_p;
+    mat2[1][0] = k1 * sx * rel_p;
+    mat2[1][1] = cx;
+    zc[0] = k1 * (-cx * sx + length) / 4;
+    zc[1] = -k1 * sq(sx) / (2 * rel_p);
+    zc[2] = -(cx * sx + length) / (4 * sq(rel_p));
+}
+
+/* bmad/low_level/low_energy_z_correction.f90:22-58 (no matrix). */
+static double low_energy_z_correction(const coord_t *o, const bmx_ele *e, double ds)
+{
+    double mass = o->mc2;
+    double p0c = o->p0c;
+    double e_tot = e->e_tot;
+    double beta0 = p0c / e_tot;
+    double pz = o->vec[5];
+    if (mass * sq(beta0 * pz) < 3e-7 * e_tot) {
+        double f = sq(beta0) * (2 * sq(beta0) - sq(mass / e_tot) / 2);
+        return ds * pz * (1 - 3 * (pz * sq(beta0)) / 2 + sq(pz) * f) * sq(mass / e_tot);
+    }
+    return ds * (o->beta - beta0) / beta0;
+}
+
+/* bmad/low_level/track_a_quadrupole.f90:17-148 (magnetic multipoles allowed, no fringe, no
+ * electric multipoles). */
+static int track_a_quadrupole(coord_t *o, const bmx_ele *e)
+{
+    const double charge_dir = 1.0;
+    double length = 1 * e->l;
+    double an[BMX_N_POLE], bn[BMX_N_POLE], b1;
+    int err;
+    int ix_mag_max = multipole_ele_to_ab(e, 1, an, bn, &b1, &err);
+    if (err) return err;
+    if (e->fringe_type != BMX_FRINGE_NONE && e->fringe_type != 0) return BMX_ERR_UNSUPPORTED;
+    int n_step = 1;
+    if (ix_mag_max > -1) {
+        long ns = f_nint(fabs(length) / e->ds_step);
+        n_step = ns > 1 ? (int)ns : 1;
+    }
+    double r_step = 1.0 / n_step;
+    double step_len = e->l * r_step;
+    if (length == 0) n_step = 0;
+
+    if (is_misaligned(e) || e->tilt != 0) offset_particle(e, 1, o, 1, 0, 0, 0);
+    if (ix_mag_max > -1) ab_multipole_kicks(an, bn, ix_mag_max, e, o, r_step / 2);
+
+    for (int i = 1; i <= n_step; i++) {
+        double rel_p = 1 + o->vec[5];
+        double k1 = charge_dir * b1 / (length * rel_p);
+        double mx[2][2], my[2][2], dzx[3], dzy[3];
+        quad_mat2_calc(-k1, step_len, rel_p, mx, dzx);
+        quad_mat2_calc(k1, step_len, rel_p, my, dzy);
+        double *v = o->vec;
+        v[4] = v[4] + (dzx[0] * sq(v[0]) + dzx[1] * v[0] * v[1] + dzx[2] * sq(v[1]) +
+                       dzy[0] * sq(v[2]) + dzy[1] * v[2] * v[3] + dzy[2] * sq(v[3]));
+        double x = v[0], px = v[1], y = v[2], py = v[3];
+        v[0] = mx[0][0] * x + mx[0][1] * px;
+        v[1] = mx[1][0] * x + mx[1][1] * px;
+        v[2] = my[0][0] * y + my[0][1] * py;
+        v[3] = my[1][0] * y + my[1][1] * py;
+        v[4] = v[4] + low_energy_z_correction(o, e, step_len);
+        if (ix_mag_max > -1)
+            ab_multipole_kicks(an, bn, ix_mag_max, e, o, i == n_step ? r_step / 2 : r_step);
+    }
+    if (is_misaligned(e) || e->tilt != 0) offset_particle(e, 0, o, 1, 0, 0, 0);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * thick multipole (sextupole, octupole, kickers, collimators, instrument, monitor, pipe)
+ * ------------------------------------------------------------------------------------------ */
+
+/* bmad/low_level/track_a_thick_multipole.f90:18-133 (no elseparator, no electric
+ * multipoles, ac_kicker amplitude = 1). */
+static int track_a_thick_multipole(coord_t *o, const bmx_ele *e)
+{
+    double length = e->l * 1;
+    double an[BMX_N_POLE], bn[BMX_N_POLE];
+    int err;
+    int ix_mag_max = multipole_ele_to_ab(e, 1, an, bn, NULL, &err);
+    if (err) return err;
+    int n_step;
+    if (ix_mag_max == -1) {
+        n_step = 1;
+    } else {
+        long ns = f_nint(fabs(length / e->ds_step));
+        n_step = ns > 1 ? (int)ns : 1;
+    }
+    double r_step = 1.0 / n_step;
+    double step_len = length / n_step;
+
+    if (is_misaligned(e) || e->tilt != 0) offset_particle(e, 1, o, 1, 0, 0, 0);
+    /* init_fringe_info: only sextupole/quad hard-edge fringes exist here (fringe_type none by
+     * default, set_ele_defaults.f90:37); anything else is outside the restatement. */
+    if (e->fringe_type != BMX_FRINGE_NONE && e->fringe_type != 0 &&
+        e->fringe_at != BMX_NO_END)
+        return BMX_ERR_UNSUPPORTED;
+
+    if (ix_mag_max > -1) ab_multipole_kicks(an, bn, ix_mag_max, e, o, 1.0 * r_step / 2);
+    for (int i = 1; i <= n_step; i++) {
+        track_a_drift(o, step_len);
+        if (ix_mag_max > -1)
+            ab_multipole_kicks(an, bn, ix_mag_max, e, o,
+                               i == n_step ? 1.0 * r_step / 2 : 1.0 * r_step);
+    }
+    if (is_misaligned(e) || e->tilt != 0) offset_particle(e, 0, o, 1, 0, 0, 0);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * sbend
+ * ------------------------------------------------------------------------------------------ */
+
+/* bmad/modules/fringe_mod.f90:241-419 hwang_bend_edge_kick, time_dir = +1 (orbit only). */
+void bmo_hwang_bend_edge_kick(const bmx_ele *e, int32_t charge, int32_t first_edge, double *vec)
+{
+    (void)charge;
+    const double c_dir = 1.0; /* rel_tracking_charge * orientation * direction */
+    double e_factor = 1 / (1 + vec[5]);
+    double g_tot, k1;
+    if (e->is_on) {
+        g_tot = (e->g + e->dg) * c_dir;
+        k1 = e->k1;
+    } else {
+        g_tot = 0;
+        k1 = 0;
+    }
+    double ang, fint_gap;
+    if (first_edge) {
+        ang = e->e1;
+        fint_gap = e->fint * e->hgap;
+    } else {
+        ang = e->e2;
+        fint_gap = e->fintx * e->hgapx;
+    }
+    if (e->fringe_type == BMX_FRINGE_HARD_EDGE_ONLY) fint_gap = 0;
+    double cos_e = cos(ang), sin_e = sin(ang);
+    double tan_e = sin_e / cos_e, sec_e = 1 / cos_e;
+    double gt = g_tot * tan_e;
+    double gt2 = g_tot * sq(tan_e);
+    double gs2 = g_tot * sq(sec_e);
+    double k1_tane = k1 * tan_e * c_dir;
+    double fg_factor = 2 * fint_gap * gs2 * g_tot * sec_e * (1 + sq(sin_e));
+    double v[6];
+    for (int i = 0; i < 6; i++) v[i] = vec[i];
+    double dx, dpx, dy, dpy, dz;
+    if (first_edge) {
+        dx = (-gt2 * sq(v[0]) + gs2 * sq(v[2])) * e_factor / 2;
+        dpx = (gt * g_tot * (1 + 2 * sq(tan_e)) * sq(v[2]) / 2 +
+               gt2 * (v[0] * v[1] - v[2] * v[3]) + k1_tane * (sq(v[0]) - sq(v[2]))) *
+              e_factor;
+        dy = gt2 * v[0] * v[2] * e_factor;
+        dpy = (fg_factor * v[2] - gt2 * v[0] * v[3] - (g_tot + gt2) * v[1] * v[2] -
+               2 * k1_tane * v[0] * v[2]) *
+              e_factor;
+        dz = sq(e_factor) * 0.5 *
+             (sq(v[2]) * fg_factor + v[0] * v[0] * v[0] * (4.0 * k1_tane - gt * gt2) / 6.0 +
+              0.5 * v[0] * sq(v[2]) * (-4.0 * k1_tane + gt * gs2) +
+              (sq(v[0]) * v[1] - 2.0 * v[0] * v[2] * v[3]) * gt2 - v[1] * sq(v[2]) * gs2);
+    } else {
+        dx = (gt2 * sq(v[0]) - gs2 * sq(v[2])) * e_factor / 2;
+        dpx = (gt2 * (v[2] * v[3] - v[0] * v[1]) + k1_tane * (sq(v[0]) - sq(v[2])) -
+               gt * gt2 * (sq(v[0]) + sq(v[2])) / 2) *
+              e_factor;
+        dy = -gt2 * v[0] * v[2] * e_factor;
+        dpy = (fg_factor * v[2] + gt2 * v[0] * v[3] + (g_tot + gt2) * v[1] * v[2] +
+               (gt * gs2 - 2 * k1_tane) * v[0] * v[2]) *
+              e_factor;
+        dz = sq(e_factor) * 0.5 *
+             (sq(v[2]) * fg_factor + v[0] * v[0] * v[0] * (4.0 * k1_tane - gt * gt2) / 6.0 +
+              0.5 * v[0] * sq(v[2]) * (-4.0 * k1_tane + gt * gs2) -
+              (sq(v[0]) * v[1] - 2.0 * v[0] * v[2] * v[3]) * gt2 + v[1] * sq(v[2]) * gs2);
+    }
+    vec[0] = v[0] + 1 * dx;
+    vec[1] = v[1] + 1 * (dpx + gt * v[0]);
+    vec[2] = v[2] + 1 * dy;
+    vec[3] = v[3] + 1 * (dpy - gt * v[2]);
+    vec[4] = v[4] + 1 * dz;
+}
+
+/* fringe_here (bmad/code/fringe_here.f90:15-35) and the sbend branch of
+ * apply_element_edge_kick (bmad/low_level/apply_element_edge_kick.f90:107-110,188-189) ->
+ * bend_edge_kick (fringe_mod.f90:40-128). hard_multipole_edge_kick is a no-op unless
+ * fringe_type = full (fringe_mod.f90:715-722). */
+static int bend_edge_kick(coord_t *o, const bmx_ele *e, int particle_at)
+{
+    int fringe_at = e->fringe_at;
+    int physical_end = particle_at == FIRST_TRACK_EDGE ? BMX_ENTRANCE_END : BMX_EXIT_END;
+    int here = (fringe_at == BMX_BOTH_ENDS) || (fringe_at == physical_end);
+    if (!here) return 0;
+    switch (e->fringe_type) {
+    case BMX_FRINGE_NONE: return 0;
+    case BMX_FRINGE_BASIC_BEND: case BMX_FRINGE_HARD_EDGE_ONLY:
+        bmo_hwang_bend_edge_kick(e, o->charge, particle_at == FIRST_TRACK_EDGE, o->vec);
+        return 0;
+    default: return BMX_ERR_UNSUPPORTED;
+    }
+}
+
+static int orbit_too_large(coord_t *o, double *unstable);
+
+/* bmad/low_level/track_a_bend.f90:18-333 for exact_multipoles = off, no body multipoles
+ * (ix_mag_max = ix_elec_max = -1, so n_step = 1), b1 = 0, no misalignment. */
+static int track_a_bend(coord_t *o, const bmx_ele *e)
+{
+    if (is_misaligned(e) || e->ref_tilt != 0) return BMX_ERR_UNSUPPORTED;
+    const double rel_charge_dir = 1.0;
+    int err = bend_edge_kick(o, e, FIRST_TRACK_EDGE);
+    if (err) return err;
+    if (orbit_too_large(o, NULL)) return 0;
+
+    double an[BMX_N_POLE], bn[BMX_N_POLE], b1;
+    int ix_mag_max = multipole_ele_to_ab(e, 1, an, bn, &b1, &err);
+    if (err) return err;
+    b1 = b1 * rel_charge_dir;
+    if (fabs(b1) < 1e-10) {
+        bn[1] = b1;
+        b1 = 0;
+    }
+    double length = 1 * e->l;
+    double g = e->g, dg;
+    if (length == 0) {
+        dg = 0;
+    } else {
+        dg = bn[0] / e->l;
+        bn[0] = 0;
+    }
+    double g_tot = (g + dg) * rel_charge_dir;
+    if (!e->is_on) {
+        dg = -g;
+        g_tot = 0;
+        b1 = 0;
+    }
+    /* Body multipoles (anything left in an/bn) and a k1 map are outside this restatement. */
+    if (b1 != 0) return BMX_ERR_UNSUPPORTED;
+    if (max_nonzero(an, bn, BMX_N_POLE - 1) > -1 && ix_mag_max > -1) return BMX_ERR_UNSUPPORTED;
+
+    double step_len = length / 1;
+    double angle = g * step_len;
+    double *v = o->vec;
+
+    if ((g == 0 && dg == 0) || step_len == 0) {
+        track_a_drift(o, step_len);
+    } else {
+        double x = v[0], px = v[1], y = v[2], py = v[3], z = v[4], pz = v[5];
+        double rel_p = 1 + pz;
+        double rel_p2 = sq(rel_p);
+        if (dg == 0 && fabs(x * g) < 1e-9 && fabs(px) < 1e-9 && fabs(py) < 1e-9 &&
+            fabs(pz) < 1e-9) {
+            /* semi-linear map near the zero orbit, track_a_bend.f90:136-159 */
+            double ll = step_len;
+            double cos_a = cos(angle), sin_a = sin(angle);
+            double sinc_a = bmo_sinc(angle), cosc_a = bmo_cosc(angle);
+            double gam2 = sq(o->mc2) / (sq(rel_p * o->p0c) + sq(o->mc2));
+            double m56 = ll * (gam2 - sq(g * ll) * bmo_sincc(angle));
+            v[0] = cos_a * x + ll * sinc_a * px + g * sq(ll) * cosc_a * pz;
+            v[1] = -g * sin_a * x + cos_a * px + g * ll * sinc_a * pz;
+            v[2] = y + ll * py;
+            v[4] = -g * ll * sinc_a * x - g * sq(ll) * cosc_a * px + z + m56 * pz;
+        } else {
+            /* exact body, track_a_bend.f90:163-209 */
+            double beta_ref = e->p0c / e->e_tot;
+            double sinc_a = bmo_sinc(angle);
+            double pt = sqrt(rel_p2 - sq(py));
+            if (fabs(px) > pt) {
+                o->state = BMX_STATE_LOST_PZ;
+                return 0;
+            }
+            double g_p = g_tot / pt;
+            double phi_1 = asin(px / pt);
+            double cos_a = cos(angle), sin_a = sin(angle);
+            double cosc_a = bmo_cosc(angle);
+            double cos_plus = cos(angle + phi_1);
+            double sin_plus = sin(angle + phi_1);
+            double alpha = 2 * (1 + g * x) * sin_plus * step_len * sinc_a -
+                           g_p * sq((1 + g * x) * step_len * sinc_a);
+            double r = sq(cos_plus) + g_p * alpha;
+            if (r < 0 || (fabs(g_p) < 1e-5 && fabs(cos_plus) < 1e-5)) {
+                o->state = BMX_STATE_LOST;
+                v[0] = 2 * MAX_APERTURE_LIMIT;
+                return 0;
+            }
+            double rad = sqrt(r), denom, xi;
+            if (cos_plus > 0) {
+                denom = rad + cos_plus;
+                xi = alpha / denom;
+            } else {
+                denom = rad - cos_plus;
+                xi = denom / g_p;
+            }
+            v[0] = x * cos_a - sq(step_len) * g * cosc_a + xi;
+            if (fabs(v[0]) > MAX_APERTURE_LIMIT) {
+                o->state = BMX_STATE_LOST;
+                return 0;
+            }
+            double L_u = xi;
+            double L_v = -1 * (step_len * sinc_a + x * sin_a);
+            double L_c = sqrt(sq(L_v) + sq(L_u));
+            double angle_p = 2 * (angle + phi_1 - 1 * atan2(L_u, -L_v));
+            double L_p = 1 * L_c / bmo_sinc(angle_p / 2);
+            v[1] = pt * sin(phi_1 + angle - angle_p);
+            v[2] = y + py * L_p / pt;
+            v[4] = z + o->beta * step_len / beta_ref - rel_p * L_p / pt;
+        }
+    }
+    if (orbit_too_large(o, NULL)) return 0;
+    return bend_edge_kick(o, e, SECOND_TRACK_EDGE);
+}
+
+/* ------------------------------------------------------------------------------------------
+ * rfcavity
+ * ------------------------------------------------------------------------------------------ */
+
+/* bmad/low_level/apply_energy_kick.f90:23-75 (ddE_dr = 0, no matrix). */
+static void apply_energy_kick(double dE, coord_t *o)
+{
+    double mc2 = o->mc2;
+    double p0c = o->p0c;
+    double pc = (1 + o->vec[5]) * p0c;
+    double beta_old = o->beta;
+    double E_old = pc / o->beta;
+    double E_new = E_old + dE;
+    if (E_new < mc2) {
+        o->vec[5] = -1;
+        o->beta = 0;
+        o->state = BMX_STATE_LOST_PZ;
+        return;
+    }
+    o->vec[5] = o->vec[5] + (1 + o->vec[5]) * bmo_sqrt_one((2 * E_old * dE + sq(dE)) / sq(pc));
+    double pc_new = p0c * (1 + o->vec[5]);
+    double beta_new = pc_new / E_new;
+    o->vec[4] = o->vec[4] * beta_new / beta_old;
+    o->beta = beta_new;
+}
+
+/* bmad/low_level/track_a_rfcavity.f90:17-146 with relative time tracking
+ * (bmad_struct.f90:2350), coupler_strength = 0 (rf_coupler_kick.f90:41), orientation = +1.
+ * The RF phase follows particle_rf_time.f90:32-120 (time = -z/(beta c) wrapped by
+ * modulo2 in quad precision) and rf_ref_time_offset.f90:19-45 (= 0 here); e_accel_field.f90:15-58
+ * gives voltage with field_autoscale ignored. */
+static int track_a_rfcavity(coord_t *o, const bmx_ele *e)
+{
+    if (e->rf_frequency == 0 && e->voltage != 0) {
+        o->state = BMX_STATE_LOST;
+        return 0;
+    }
+    double length = e->l * 1;
+    double r_step = 1.0 / 1;
+    double step_len = length / 1;
+    (void)step_len;
+    long ns = f_nint(fabs(length) / e->ds_step);
+    int n_slice = ns > 1 ? (int)ns : 1;
+    const double charge_dir = 1.0;
+    double an[BMX_N_POLE], bn[BMX_N_POLE];
+    int err;
+    int ix_mag_max = multipole_ele_to_ab(e, 1, an, bn, NULL, &err);
+    if (err) return err;
+
+    offset_particle(e, 1, o, 1, 1, 0, 0);
+    if (ix_mag_max > -1) ab_multipole_kicks(an, bn, ix_mag_max, e, o, 0.5 * r_step);
+
+    double voltage = (e->is_on ? e->voltage : 0.0) * charge_dir;
+    /* particle_rf_time: relative time, real(qp) result */
+    __float128 time = (__float128)(-o->vec[4] / (o->beta * C_LIGHT));
+    if (e->rf_frequency != 0) time = modulo2_q(time, (__float128)0.5 / (__float128)e->rf_frequency);
+    __float128 ph = (__float128)TWOPI *
+                    ((__float128)e->phi0 - (time - (__float128)0.0) * (__float128)e->rf_frequency);
+    double phase0 = (double)ph;
+    phase0 = phase0 + TWOPI * e->phi0_multipass;
+    double phase = bmo_modulo2(phase0, PI);
+
+    for (int i = 0; i <= n_slice; i++) {
+        double dE = 1 * voltage * sin(phase) / n_slice;
+        if (i == 0 || i == n_slice) dE = dE / 2;
+        apply_energy_kick(dE, o);
+        if (o->vec[5] == -1) {
+            o->state = BMX_STATE_LOST_PZ;
+            return 0;
+        }
+        if (i != n_slice) {
+            double z = o->vec[4];
+            track_a_drift(o, length / n_slice);
+            phase = phase + TWOPI * e->rf_frequency * (o->vec[4] - z) / (C_LIGHT * o->beta);
+        }
+    }
+    if (ix_mag_max > -1) ab_multipole_kicks(an, bn, ix_mag_max, e, o, 0.5 * r_step);
+    offset_particle(e, 0, o, 1, 1, 0, 0);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * apertures and validity
+ * ------------------------------------------------------------------------------------------ */
+
+/* bmad/low_level/orbit_too_large.f90:26-120 (charged particle, check_momentum = T). */
+static int orbit_too_large(coord_t *o, double *unstable)
+{
+    double *v = o->vec;
+    double u = 0;
+    int too = 1;
+    if (v[0] > MAX_APERTURE_LIMIT) {
+        o->state = BMX_STATE_LOST_POS_X;
+        u = (fabs(v[0]) - MAX_APERTURE_LIMIT) / MAX_APERTURE_LIMIT;
+    } else if (-v[0] > MAX_APERTURE_LIMIT) {
+        o->state = BMX_STATE_LOST_NEG_X;
+        u = (fabs(v[0]) - MAX_APERTURE_LIMIT) / MAX_APERTURE_LIMIT;
+    } else if (v[2] > MAX_APERTURE_LIMIT) {
+        o->state = BMX_STATE_LOST_POS_Y;
+        u = (fabs(v[2]) - MAX_APERTURE_LIMIT) / MAX_APERTURE_LIMIT;
+    } else if (-v[2] > MAX_APERTURE_LIMIT) {
+        o->state = BMX_STATE_LOST_NEG_Y;
+        u = (fabs(v[2]) - MAX_APERTURE_LIMIT) / MAX_APERTURE_LIMIT;
+    } else {
+        double rel_p = 1 + v[5];
+        if (rel_p < 0) {
+            o->state = BMX_STATE_LOST_PZ;
+            u = fabs(rel_p) + 1e-6;
+        } else {
+            double f_unstable = sq(v[1]) + sq(v[3]) - sq(rel_p);
+            if (f_unstable > 0) {
+                u = sqrt(f_unstable);
+                if (fabs(v[1]) > fabs(v[3])) o->state = v[1] > 0 ? BMX_STATE_LOST_POS_X : BMX_STATE_LOST_NEG_X;
+                else o->state = v[3] > 0 ? BMX_STATE_LOST_POS_Y : BMX_STATE_LOST_NEG_Y;
+            } else {
+                too = 0;
+            }
+        }
+    }
+    if (too && unstable) *unstable = u;
+    return too;
+}
+
+/* check_aperture_limit.f90:343-353 check_this_limit */
+static void check_this_limit(coord_t *o, double coord, double width2, double center, double *u)
+{
+    if (width2 == 0) return;
+    if (coord > center - width2 && coord < center + width2) return;
+    o->state = BMX_STATE_LOST;
+    double a = (center - width2) - coord, b = coord - (center + width2);
+    *u = a > b ? a : b;
+}
+
+/* check_aperture_limit.f90:358-385 elliptical_params_setup */
+static int elliptical_params_setup(double lim1, double lim2, double *pos, double *width2)
+{
+    if ((lim1 == 0 && lim2 != 0) || (lim1 != 0 && lim2 == 0)) return 1;
+    *width2 = (lim2 - lim1) / 2;
+    double center = (lim2 + lim1) / 2;
+    *pos = *pos - center;
+    if (*width2 == 0) *width2 = MAX_APERTURE_LIMIT;
+    if (*width2 < 0) *width2 = 0;
+    return 0;
+}
+
+/* bmad/low_level/check_aperture_limit.f90:30-337, non-wall-transition apertures
+ * (rectangular, elliptical), element without lords, aperture_limit_on = T. */
+static void check_aperture_limit(coord_t *o, const bmx_ele *e, int particle_at, double *uf)
+{
+    double u = 0;
+    *uf = 0;
+    if (orbit_too_large(o, uf)) return;
+    int physical_end = particle_at == FIRST_TRACK_EDGE ? BMX_ENTRANCE_END : BMX_EXIT_END;
+    int at = e->aperture_at;
+    int here;
+    if (at == BMX_NO_END || at == 0) here = 0;
+    else if (at == BMX_CONTINUOUS) here = 1;
+    else here = (at == BMX_BOTH_ENDS) || (at == physical_end);
+    if (!here) return;
+    double *v = o->vec;
+    if (e->key == BMX_ECOLLIMATOR) {
+        double xw = e->px_aperture_width2, yw = e->py_aperture_width2;
+        double x0 = e->px_aperture_center, y0 = e->py_aperture_center;
+        if (xw == 0 || yw == 0) {
+            check_this_limit(o, v[1], xw, x0, &u);
+            check_this_limit(o, v[3], yw, y0, &u);
+        } else {
+            double r = sq((v[1] - x0) / xw) + sq((v[3] - y0) / yw);
+            if (r > 1) {
+                o->state = BMX_STATE_LOST;
+                u = sqrt(r) - 1;
+            }
+        }
+        xw = e->z_aperture_width2; yw = e->pz_aperture_width2;
+        x0 = e->z_aperture_center; y0 = e->pz_aperture_center;
+        if (xw == 0 || yw == 0) {
+            check_this_limit(o, v[4], xw, x0, &u);
+            check_this_limit(o, v[5], yw, y0, &u);
+        } else {
+            double r = sq((v[4] - x0) / xw) + sq((v[5] - y0) / yw);
+            if (r > 1) {
+                o->state = BMX_STATE_LOST;
+                u = sqrt(r) - 1;
+            }
+        }
+    } else if (e->key == BMX_RCOLLIMATOR) {
+        check_this_limit(o, v[1], e->px_aperture_width2, e->px_aperture_center, &u);
+        check_this_limit(o, v[3], e->py_aperture_width2, e->py_aperture_center, &u);
+        check_this_limit(o, v[4], e->z_aperture_width2, e->z_aperture_center, &u);
+        check_this_limit(o, v[5], e->pz_aperture_width2, e->pz_aperture_center, &u);
+    }
+    double xp = v[0], yp = v[2];
+    if (e->offset_moves_aperture && (is_misaligned(e) || e->tilt != 0)) {
+        coord_t o2 = *o;
+        int do_tilt = (e->key == BMX_ECOLLIMATOR || e->key == BMX_RCOLLIMATOR);
+        double s_here = particle_at == FIRST_TRACK_EDGE ? 0.0 : e->l;
+        offset_particle(e, 1, &o2, do_tilt, 0, 1, s_here);
+        xp = o2.vec[0];
+        yp = o2.vec[2];
+    }
+    double x1 = -e->x1_limit, x2 = e->x2_limit, y1 = -e->y1_limit, y2 = e->y2_limit;
+    int type = e->aperture_type;
+    if (type == BMX_APERTURE_ELLIPTICAL) {
+        double xw, yw;
+        if (elliptical_params_setup(x1, x2, &xp, &xw)) return;
+        if (elliptical_params_setup(y1, y2, &yp, &yw)) return;
+        double r = sq(xp / xw) + sq(yp / yw);
+        if (r < 1) {
+            *uf = u;
+            return;
+        }
+        if (fabs(xp / xw) > fabs(yp / yw)) o->state = xp > 0 ? BMX_STATE_LOST_POS_X : BMX_STATE_LOST_NEG_X;
+        else o->state = yp > 0 ? BMX_STATE_LOST_POS_Y : BMX_STATE_LOST_NEG_Y;
+        u = sqrt(r) - 1;
+    } else {
+        double f;
+        if (x1 != 0 && xp < x1) {
+            f = fabs((xp - x1) / (x2 - x1));
+            if (f > u) { o->state = BMX_STATE_LOST_NEG_X; u = f; }
+        }
+        if (x2 != 0 && xp > x2) {
+            f = fabs((xp - x2) / (x2 - x1));
+            if (f > u) { o->state = BMX_STATE_LOST_POS_X; u = f; }
+        }
+        if (y1 != 0 && yp < y1) {
+            f = fabs((yp - y1) / (y2 - y1));
+            if (f > u) { o->state = BMX_STATE_LOST_NEG_Y; u = f; }
+        }
+        if (y2 != 0 && yp > y2) {
+            f = fabs((yp - y2) / (y2 - y1));
+            if (f > u) { o->state = BMX_STATE_LOST_POS_Y; u = f; }
+        }
+    }
+    *uf = u;
+}
+
+void bmo_check_aperture_limit(const bmx_ele *ele, int32_t second_edge, double *vec,
+                              int32_t *state, double *unstable_factor)
+{
+    coord_t o;
+    memset(&o, 0, sizeof o);
+    for (int i = 0; i < 6; i++) o.vec[i] = vec[i];
+    o.state = *state;
+    o.p0c = ele->p0c;
+    check_aperture_limit(&o, ele, second_edge ? SECOND_TRACK_EDGE : FIRST_TRACK_EDGE,
+                         unstable_factor);
+    for (int i = 0; i < 6; i++) vec[i] = o.vec[i];
+    *state = o.state;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * track1
+ * ------------------------------------------------------------------------------------------ */
+
+/* bmad/low_level/track1_bmad.f90:22-283 (charged particles). */
+static int track1_bmad(coord_t *o, const bmx_ele *e)
+{
+    int key = e->key;
+    if (!e->is_on && key != BMX_SBEND) key = BMX_DRIFT;
+    if (is_thick_multipole_key(key)) return track_a_thick_multipole(o, e);
+    switch (key) {
+    case BMX_DRIFT:
+        /* track1_bmad.f90:93-94: no offset_particle call for drifts (a misaligned drift
+         * tracks exactly like an aligned one). */
+        track_a_drift(o, e->l * 1);
+        return 0;
+    case BMX_MARKER: return 0;
+    case BMX_QUADRUPOLE: return track_a_quadrupole(o, e);
+    case BMX_RFCAVITY: return track_a_rfcavity(o, e);
+    case BMX_SBEND: return track_a_bend(o, e);
+    default: return BMX_ERR_UNSUPPORTED;
+    }
+}
+
+/* bmad/code/track1.f90:44-343, bmad_standard, no radiation/spin/space charge/hooks. */
+static int track1(coord_t *o, const bmx_ele *e)
+{
+    if (o->state != BMX_STATE_ALIVE) return 0;
+    if (e->p0c != 0 && fabs(e->p0c - o->p0c) > 1e-12 * fabs(e->p0c)) return BMX_ERR_P0C;
+    double u;
+    check_aperture_limit(o, e, FIRST_TRACK_EDGE, &u);
+    if (o->state != BMX_STATE_ALIVE) return 0;
+    int err = track1_bmad(o, e);
+    if (err) return err;
+    if (orbit_too_large(o, NULL)) return 0;
+    if (o->state != BMX_STATE_ALIVE) return 0;
+    check_aperture_limit(o, e, SECOND_TRACK_EDGE, &u);
+    return 0;
+}
+
+int32_t bmo_track1(const bmx_ele *ele, double mc2, int32_t charge, double p0c, double *vec,
+                   double *beta, int32_t *state)
+{
+    coord_t o;
+    for (int i = 0; i < 6; i++) o.vec[i] = vec[i];
+    o.beta = *beta;
+    o.p0c = p0c;
+    o.mc2 = mc2;
+    o.charge = charge;
+    o.state = *state;
+    int err = track1(&o, ele);
+    for (int i = 0; i < 6; i++) vec[i] = o.vec[i];
+    *beta = o.beta;
+    *state = o.state;
+    return err;
+}
+
+/* Bunch loop: ltt_run_beam_mode (lt_tracking_mod.f90:1631) -> track_bunch (beam_mod.f90:159-166)
+ * -> track1_bunch_hom (beam_utils.f90:81-86). */
+int32_t bmo_track_bunch(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge,
+                        double p0c, int64_t n, double *x, double *px, double *y, double *py,
+                        double *z, double *pz, double *beta, int32_t *state, int32_t *loss_ele,
+                        int32_t *loss_turn, int32_t n_turns, int32_t turn0, int32_t n_threads,
+                        int64_t *n_tracked)
+{
+    int32_t err_all = 0;
+    int64_t tracked = 0;
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+    (void)n_threads;
+#endif
+    for (int32_t it = 0; it < n_turns; it++) {
+        for (int32_t ie = 0; ie < n_ele; ie++) {
+            const bmx_ele *e = &eles[ie];
+            int64_t cnt = 0;
+            int32_t err_e = 0;
+#pragma omp parallel for schedule(static) reduction(+ : cnt) reduction(| : err_e)
+            for (int64_t ip = 0; ip < n; ip++) {
+                if (state[ip] != BMX_STATE_ALIVE) continue;
+                coord_t o;
+                o.vec[0] = x[ip]; o.vec[1] = px[ip]; o.vec[2] = y[ip];
+                o.vec[3] = py[ip]; o.vec[4] = z[ip]; o.vec[5] = pz[ip];
+                o.beta = beta[ip];
+                o.p0c = p0c;
+                o.mc2 = mc2;
+                o.charge = charge;
+                o.state = BMX_STATE_ALIVE;
+                int err = track1(&o, e);
+                err_e |= err;
+                cnt++;
+                x[ip] = o.vec[0]; px[ip] = o.vec[1]; y[ip] = o.vec[2];
+                py[ip] = o.vec[3]; z[ip] = o.vec[4]; pz[ip] = o.vec[5];
+                beta[ip] = o.beta;
+                if (o.state != BMX_STATE_ALIVE) {
+                    state[ip] = o.state;
+                    loss_ele[ip] = ie + 1;
+                    loss_turn[ip] = turn0 + it;
+                }
+            }
+            tracked += cnt;
+            if (err_e) err_all = err_e;
+        }
+        if (err_all) break;
+    }
+    if (n_tracked) *n_tracked = tracked;
+    return err_all;
+}
+
+int32_t bmo_track_trace(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge,
+                        double p0c, double *vec, double *beta, int32_t *state, int32_t n_turns,
+                        double *out)
+{
+    coord_t o;
+    for (int i = 0; i < 6; i++) o.vec[i] = vec[i];
+    o.beta = *beta;
+    o.p0c = p0c;
+    o.mc2 = mc2;
+    o.charge = charge;
+    o.state = *state;
+    int64_t k = 0;
+    for (int32_t it = 0; it < n_turns; it++) {
+        for (int32_t ie = 0; ie < n_ele; ie++) {
+            int err = track1(&o, &eles[ie]);
+            if (err) return err;
+            for (int i = 0; i < 6; i++) out[k++] = o.vec[i];
+        }
+    }
+    for (int i = 0; i < 6; i++) vec[i] = o.vec[i];
+    *beta = o.beta;
+    *state = o.state;
+    return 0;
+}

@@ -1,0 +1,139 @@
+"""ctypes access to the CPU restatement (libbmad_oracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, always as the checker.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbmad_oracle.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int32)
+
+
+def build(force: bool = False) -> str:
+    """Compile the restatement with its Makefile (gcc)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.bmo_track1.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_double,
+                                 _D, _D, _I]
+        L.bmo_track1.restype = ctypes.c_int32
+        L.bmo_track_bunch.argtypes = ([ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
+                                       ctypes.c_int32, ctypes.c_double, ctypes.c_int64]
+                                      + [_D] * 7 + [_I] * 3
+                                      + [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_int64)])
+        L.bmo_track_bunch.restype = ctypes.c_int32
+        L.bmo_track_trace.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
+                                      ctypes.c_int32, ctypes.c_double, _D, _D, _I, ctypes.c_int32,
+                                      _D]
+        L.bmo_track_trace.restype = ctypes.c_int32
+        L.bmo_hwang_bend_edge_kick.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _D]
+        L.bmo_check_aperture_limit.argtypes = [ctypes.c_void_p, ctypes.c_int32, _D, _I, _D]
+        for f in ("bmo_sqrt_one", "bmo_sinc", "bmo_cosc", "bmo_sincc"):
+            getattr(L, f).argtypes = [ctypes.c_double]
+            getattr(L, f).restype = ctypes.c_double
+        L.bmo_modulo2.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.bmo_modulo2.restype = ctypes.c_double
+        L.bmo_c_multi.argtypes = [ctypes.c_int32, ctypes.c_int32]
+        L.bmo_c_multi.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _ele_ptr(ele):
+    """Address of a BmxEle or of an ELE_DTYPE array."""
+    if isinstance(ele, np.ndarray):
+        return ele.ctypes.data
+    return ctypes.addressof(ele)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def track1(ele, vec, beta, state=1, mc2=0.51099895069e6, charge=-1, p0c=None):
+    """One particle through one element. Returns (vec, beta, state)."""
+    v = np.array(vec, dtype=np.float64)
+    b = ctypes.c_double(beta)
+    s = ctypes.c_int32(state)
+    if p0c is None:
+        p0c = ele.p0c
+    err = lib().bmo_track1(_ele_ptr(ele), mc2, charge, p0c, _ptr(v, _D), ctypes.byref(b),
+                           ctypes.byref(s))
+    if err:
+        raise OracleError(f"bmo_track1 error {err}")
+    return v, b.value, s.value
+
+
+def track_bunch(ele_arr: np.ndarray, beam: dict, n_turns: int, turn0: int = 0, p0c=None,
+                mc2=0.51099895069e6, charge=-1, n_threads: int = 0) -> int:
+    """Track a SoA beam dict in place; returns the number of particle-element passes."""
+    n = beam["x"].shape[0]
+    if p0c is None:
+        p0c = float(ele_arr[0]["p0c"])
+    for k in ("x", "px", "y", "py", "z", "pz", "beta"):
+        assert beam[k].dtype == np.float64 and beam[k].flags.c_contiguous
+    for k in ("state", "loss_ele", "loss_turn"):
+        assert beam[k].dtype == np.int32 and beam[k].flags.c_contiguous
+    tracked = ctypes.c_int64(0)
+    err = lib().bmo_track_bunch(
+        ele_arr.ctypes.data, len(ele_arr), mc2, charge, p0c, n,
+        *[_ptr(beam[k], _D) for k in ("x", "px", "y", "py", "z", "pz", "beta")],
+        *[_ptr(beam[k], _I) for k in ("state", "loss_ele", "loss_turn")],
+        n_turns, turn0, n_threads, ctypes.byref(tracked))
+    if err:
+        raise OracleError(f"bmo_track_bunch error {err}")
+    return tracked.value
+
+
+def track_trace(ele_arr: np.ndarray, vec, beta, n_turns: int, state=1, p0c=None,
+                mc2=0.51099895069e6, charge=-1):
+    """Coordinates after every element: array (n_turns * n_ele, 6)."""
+    v = np.array(vec, dtype=np.float64)
+    b = ctypes.c_double(beta)
+    s = ctypes.c_int32(state)
+    if p0c is None:
+        p0c = float(ele_arr[0]["p0c"])
+    out = np.zeros((n_turns * len(ele_arr), 6))
+    err = lib().bmo_track_trace(ele_arr.ctypes.data, len(ele_arr), mc2, charge, p0c, _ptr(v, _D),
+                                ctypes.byref(b), ctypes.byref(s), n_turns, _ptr(out, _D))
+    if err:
+        raise OracleError(f"bmo_track_trace error {err}")
+    return out, v, b.value, s.value
+
+
+def hwang_edge_kick(ele, vec, first_edge: bool, charge=-1):
+    v = np.array(vec, dtype=np.float64)
+    lib().bmo_hwang_bend_edge_kick(_ele_ptr(ele), charge, 1 if first_edge else 0, _ptr(v, _D))
+    return v
+
+
+def check_aperture(ele, vec, second_edge=True, state=1):
+    v = np.array(vec, dtype=np.float64)
+    s = ctypes.c_int32(state)
+    u = ctypes.c_double(0.0)
+    lib().bmo_check_aperture_limit(_ele_ptr(ele), 1 if second_edge else 0, _ptr(v, _D),
+                                   ctypes.byref(s), ctypes.byref(u))
+    return s.value, u.value

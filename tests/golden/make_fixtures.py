@@ -1,0 +1,45 @@
+"""Copy the reference's known-answer lines for this path into small fixture files.
+
+Run in a checkout that has the bmad-ecosystem tree (default /root/reference); the fixtures are
+committed, so tests never read the reference at run time. Each fixture line is the verbatim
+line of the reference's ``output.correct`` file ("name" TYPE tol values...), i.e. data.
+
+Sources:
+  regression_tests/tracking_method_test/output.correct  -- "<ELE>: Bmad_Standard" lines
+  regression_tests/fringe_test/output.correct           -- Up-Orb / Dn-Orb (Hwang edge kick)
+  regression_tests/aperture_test/output.correct         -- EC_*/RC_* state and unstable lines
+"""
+import os
+import sys
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+TRACKING_ELEMENTS = ["DRIFT1", "ECOLLIMATOR1", "HKICKER1", "INSTRUMENT1", "KICKER1", "MONITOR1",
+                     "OCTUPOLE1", "RCOLLIMATOR1", "RFCAVITY1", "RFCAVITY3", "SEXTUPOLE1",
+                     "THICK_MULTIPOLE1", "VKICKER1"]
+
+
+def grab(rel, keep):
+    path = os.path.join(REF, rel)
+    with open(path) as f:
+        return [ln.rstrip("\n") for ln in f if keep(ln)]
+
+
+def main():
+    lines = grab("regression_tests/tracking_method_test/output.correct",
+                 lambda ln: any(ln.startswith(f'"{e}: Bmad_Standard"') for e in TRACKING_ELEMENTS))
+    with open(os.path.join(HERE, "tracking_method_test.correct"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    lines = grab("regression_tests/fringe_test/output.correct",
+                 lambda ln: ln.startswith('"Up-Orb"') or ln.startswith('"Dn-Orb"'))
+    with open(os.path.join(HERE, "fringe_test.correct"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    lines = grab("regression_tests/aperture_test/output.correct",
+                 lambda ln: ln.startswith('"EC_') or ln.startswith('"RC_'))
+    with open(os.path.join(HERE, "aperture_test.correct"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+if __name__ == "__main__":
+    main()
