@@ -2,9 +2,9 @@
 
 Host side of the drop-in for Bmad's beam tracking path (track_beam / track_bunch /
 track1_bunch / track1, bmad/multiparticle/beam_mod.f90 and bmad/code/track1.f90): element
-records and bookkeeping (``elements``), the benchmark rings and beams (``lattices``) and the
-ctypes binding of the native library libbmx.so (``native``, ``tracking``).
+records and bookkeeping (``elements``), the benchmark rings and beams (``lattices``), the
+ctypes binding of the native library libbmx.so (``native``) and the host API (``tracking``).
 """
-from . import elements, lattices  # noqa: F401
+from . import elements, flops, lattices, native, tracking  # noqa: F401
 
-__all__ = ["elements", "lattices"]
+__all__ = ["elements", "flops", "lattices", "native", "tracking"]

@@ -1,0 +1,326 @@
+// C-ABI of the tracking library (declared in include/bmx.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "bmx_internal.hpp"
+
+namespace bmx {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+static int hip_fail(hipError_t e, const char *what)
+{
+    return set_error(BMX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(call, what)                                  \
+    do {                                                     \
+        hipError_t _e = (call);                              \
+        if (_e != hipSuccess) return bmx::hip_fail(_e, what); \
+    } while (0)
+
+}  // namespace bmx
+
+using namespace bmx;
+
+extern "C" {
+
+int32_t bmx_version(void) { return BMX_VERSION; }
+
+const char *bmx_last_error(void) { return g_last_error.c_str(); }
+
+int32_t bmx_ctx_create(int32_t device, bmx_ctx **out)
+{
+    if (!out) return set_error(BMX_ERR_ARG, "bmx_ctx_create: null out");
+    *out = nullptr;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) return set_error(BMX_ERR_ARG, "bmx_ctx_create: no such device");
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    bmx_ctx *c = new (std::nothrow) bmx_ctx();
+    if (!c) return set_error(BMX_ERR_NOMEM, "bmx_ctx_create: host allocation");
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    e = hipMalloc(&c->d_counter, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) {
+        c->d_work = c->d_counter + 1;
+        e = hipMemset(c->d_counter, 0, 2 * sizeof(unsigned long long));
+    }
+    if (e != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return hip_fail(e, "hipMalloc(counter)");
+    }
+    *out = c;
+    return BMX_OK;
+}
+
+int32_t bmx_ctx_destroy(bmx_ctx *ctx)
+{
+    if (!ctx) return BMX_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto &ev : ctx->events)
+        if (ev) hipEventDestroy(ev);
+    hipFree(ctx->d_counter);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return BMX_OK;
+}
+
+int32_t bmx_ctx_synchronize(bmx_ctx *ctx)
+{
+    if (!ctx) return set_error(BMX_ERR_ARG, "bmx_ctx_synchronize: null ctx");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return BMX_OK;
+}
+
+void *bmx_ctx_stream(bmx_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int32_t bmx_ctx_record(bmx_ctx *ctx, int32_t slot)
+{
+    if (!ctx || slot < 0 || slot >= BMX_N_EVENTS) return set_error(BMX_ERR_ARG, "bmx_ctx_record: bad slot");
+    hipSetDevice(ctx->device);
+    if (!ctx->events[slot]) HIP_TRY(hipEventCreate(&ctx->events[slot]), "hipEventCreate");
+    HIP_TRY(hipEventRecord(ctx->events[slot], ctx->stream), "hipEventRecord");
+    return BMX_OK;
+}
+
+int32_t bmx_ctx_elapsed_ms(bmx_ctx *ctx, int32_t slot0, int32_t slot1, float *ms)
+{
+    if (!ctx || !ms || slot0 < 0 || slot1 < 0 || slot0 >= BMX_N_EVENTS || slot1 >= BMX_N_EVENTS ||
+        !ctx->events[slot0] || !ctx->events[slot1])
+        return set_error(BMX_ERR_ARG, "bmx_ctx_elapsed_ms: slot not recorded");
+    hipSetDevice(ctx->device);
+    HIP_TRY(hipEventSynchronize(ctx->events[slot1]), "hipEventSynchronize");
+    HIP_TRY(hipEventElapsedTime(ms, ctx->events[slot0], ctx->events[slot1]), "hipEventElapsedTime");
+    return BMX_OK;
+}
+
+int32_t bmx_ctx_take_work(bmx_ctx *ctx, int64_t *passes)
+{
+    if (!ctx || !passes) return set_error(BMX_ERR_ARG, "bmx_ctx_take_work: null argument");
+    hipSetDevice(ctx->device);
+    unsigned long long v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, ctx->d_work, sizeof v, hipMemcpyDeviceToHost, ctx->stream), "copy");
+    HIP_TRY(hipMemsetAsync(ctx->d_work, 0, sizeof v, ctx->stream), "memset");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    *passes = (int64_t)v;
+    return BMX_OK;
+}
+
+int32_t bmx_lattice_create(bmx_ctx *ctx, const bmx_ele *eles, int32_t n_ele, double mc2,
+                           int32_t charge, bmx_lattice **out)
+{
+    if (!ctx || !out) return set_error(BMX_ERR_ARG, "bmx_lattice_create: null argument");
+    *out = nullptr;
+    bmx_lattice *L = new (std::nothrow) bmx_lattice();
+    if (!L) return set_error(BMX_ERR_NOMEM, "bmx_lattice_create: host allocation");
+    std::string err;
+    int rc = compile_table(eles, n_ele, mc2, charge, L->host, err);
+    if (rc != BMX_OK) {
+        delete L;
+        return set_error(rc, "bmx_lattice_create: " + err);
+    }
+    L->ctx = ctx;
+    L->device = ctx->device;
+    hipSetDevice(ctx->device);
+    const size_t nh = L->host.hdr.size(), np = L->host.prm.size();
+    hipError_t e = hipMalloc(&L->d_hdr, nh * sizeof(EleHdr));
+    if (e == hipSuccess) e = hipMalloc(&L->d_prm, (np ? np : 1) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&L->d_desc, sizeof(LatDesc));
+    if (e == hipSuccess) e = hipMemcpy(L->d_hdr, L->host.hdr.data(), nh * sizeof(EleHdr), hipMemcpyHostToDevice);
+    if (e == hipSuccess && np) e = hipMemcpy(L->d_prm, L->host.prm.data(), np * sizeof(double), hipMemcpyHostToDevice);
+    L->desc.hdr = L->d_hdr;
+    L->desc.prm = L->d_prm;
+    L->desc.n_ele = (int32_t)nh;
+    L->desc.sp = L->host.sp;
+    if (e == hipSuccess) e = hipMemcpy(L->d_desc, &L->desc, sizeof(LatDesc), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(L->d_hdr);
+        hipFree(L->d_prm);
+        hipFree(L->d_desc);
+        delete L;
+        return hip_fail(e, "bmx_lattice_create: device table");
+    }
+    *out = L;
+    return BMX_OK;
+}
+
+int32_t bmx_lattice_destroy(bmx_lattice *L)
+{
+    if (!L) return BMX_OK;
+    // Independent of the context's lifetime: synchronise the device, then free.
+    hipSetDevice(L->device);
+    hipDeviceSynchronize();
+    hipFree(L->d_hdr);
+    hipFree(L->d_prm);
+    hipFree(L->d_desc);
+    delete L;
+    return BMX_OK;
+}
+
+int32_t bmx_lattice_validate(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge,
+                             int32_t *n_params)
+{
+    HostTable t;
+    std::string err;
+    int rc = compile_table(eles, n_ele, mc2, charge, t, err);
+    if (rc != BMX_OK) return set_error(rc, "bmx_lattice_validate: " + err);
+    if (n_params) *n_params = (int32_t)t.prm.size();
+    return BMX_OK;
+}
+
+int32_t bmx_lattice_info(const bmx_lattice *L, int32_t *n_ele, uint64_t *kind_mask)
+{
+    if (!L) return set_error(BMX_ERR_ARG, "bmx_lattice_info: null lattice");
+    if (n_ele) *n_ele = (int32_t)L->host.hdr.size();
+    if (kind_mask) *kind_mask = L->host.kind_mask;
+    return BMX_OK;
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+int32_t bmx_beam_create(bmx_ctx *ctx, int64_t n, bmx_beam **out)
+{
+    if (!ctx || !out || n < 0) return set_error(BMX_ERR_ARG, "bmx_beam_create: bad argument");
+    *out = nullptr;
+    hipSetDevice(ctx->device);
+    const size_t nd = align256((size_t)(n ? n : 1) * 8), ni = align256((size_t)(n ? n : 1) * 4);
+    char *blk = nullptr;
+    hipError_t e = hipMalloc(&blk, 7 * nd + 3 * ni);
+    if (e != hipSuccess) return set_error(BMX_ERR_NOMEM, std::string("bmx_beam_create: ") + hipGetErrorString(e));
+    bmx_beam *B = new (std::nothrow) bmx_beam();
+    if (!B) {
+        hipFree(blk);
+        return set_error(BMX_ERR_NOMEM, "bmx_beam_create: host allocation");
+    }
+    B->ctx = ctx;
+    B->device = ctx->device;
+    B->n = n;
+    B->owned = true;
+    B->block = blk;
+    double **dp[7] = {&B->d.x, &B->d.px, &B->d.y, &B->d.py, &B->d.z, &B->d.pz, &B->d.beta};
+    for (int k = 0; k < 7; k++) *dp[k] = (double *)(blk + k * nd);
+    B->d.state = (int32_t *)(blk + 7 * nd);
+    B->d.loss_ele = (int32_t *)(blk + 7 * nd + ni);
+    B->d.loss_turn = (int32_t *)(blk + 7 * nd + 2 * ni);
+    *out = B;
+    return BMX_OK;
+}
+
+int32_t bmx_beam_wrap(bmx_ctx *ctx, const bmx_beam_view *dev, int64_t n, bmx_beam **out)
+{
+    if (!ctx || !dev || !out || n < 0) return set_error(BMX_ERR_ARG, "bmx_beam_wrap: bad argument");
+    if (n > 0 && (!dev->x || !dev->px || !dev->y || !dev->py || !dev->z || !dev->pz || !dev->beta ||
+                  !dev->state || !dev->loss_ele || !dev->loss_turn))
+        return set_error(BMX_ERR_ARG, "bmx_beam_wrap: null array");
+    bmx_beam *B = new (std::nothrow) bmx_beam();
+    if (!B) return set_error(BMX_ERR_NOMEM, "bmx_beam_wrap: host allocation");
+    B->ctx = ctx;
+    B->device = ctx->device;
+    B->n = n;
+    B->owned = false;
+    B->d = BeamPtrs{dev->x, dev->px, dev->y, dev->py, dev->z, dev->pz, dev->beta,
+                    dev->state, dev->loss_ele, dev->loss_turn};
+    *out = B;
+    return BMX_OK;
+}
+
+static int beam_copy(bmx_beam *B, const bmx_beam_view *h, int64_t n, bool up)
+{
+    if (!B || !h) return set_error(BMX_ERR_ARG, "beam copy: null argument");
+    if (n < 0 || n > B->n) return set_error(BMX_ERR_ARG, "beam copy: n exceeds the beam size");
+    if (n == 0) return BMX_OK;
+    hipSetDevice(B->ctx->device);
+    void *hp[10] = {h->x, h->px, h->y, h->py, h->z, h->pz, h->beta, h->state, h->loss_ele, h->loss_turn};
+    void *dp[10] = {B->d.x, B->d.px, B->d.y, B->d.py, B->d.z, B->d.pz, B->d.beta, B->d.state,
+                    B->d.loss_ele, B->d.loss_turn};
+    for (int k = 0; k < 10; k++) {
+        if (!hp[k]) return set_error(BMX_ERR_ARG, "beam copy: null host array");
+        const size_t bytes = (size_t)n * (k < 7 ? 8 : 4);
+        hipError_t e = up ? hipMemcpyAsync(dp[k], hp[k], bytes, hipMemcpyHostToDevice, B->ctx->stream)
+                          : hipMemcpyAsync(hp[k], dp[k], bytes, hipMemcpyDeviceToHost, B->ctx->stream);
+        if (e != hipSuccess) return hip_fail(e, "beam copy");
+    }
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream), "beam copy sync");
+    return BMX_OK;
+}
+
+int32_t bmx_beam_upload(bmx_beam *B, const bmx_beam_view *host, int64_t n) { return beam_copy(B, host, n, true); }
+int32_t bmx_beam_download(bmx_beam *B, const bmx_beam_view *host, int64_t n) { return beam_copy(B, host, n, false); }
+
+int32_t bmx_beam_destroy(bmx_beam *B)
+{
+    if (!B) return BMX_OK;
+    if (B->owned) {
+        hipSetDevice(B->device);
+        hipDeviceSynchronize();
+        hipFree(B->block);
+    }
+    delete B;
+    return BMX_OK;
+}
+
+int32_t bmx_beam_count_alive(bmx_beam *B, int64_t *n_alive)
+{
+    if (!B || !n_alive) return set_error(BMX_ERR_ARG, "bmx_beam_count_alive: null argument");
+    bmx_ctx *c = B->ctx;
+    hipSetDevice(c->device);
+    HIP_TRY(hipMemsetAsync(c->d_counter, 0, sizeof(unsigned long long), c->stream), "memset");
+    HIP_TRY(launch_count_alive(c->stream, B->d.state, B->n, c->d_counter), "k_count_alive");
+    unsigned long long v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, c->d_counter, sizeof v, hipMemcpyDeviceToHost, c->stream), "copy");
+    HIP_TRY(hipStreamSynchronize(c->stream), "sync");
+    *n_alive = (int64_t)v;
+    return BMX_OK;
+}
+
+int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t n_turns, int32_t turn0)
+{
+    if (!ctx || !L || !B) return set_error(BMX_ERR_ARG, "bmx_track_turns: null argument");
+    if (n_turns < 0) return set_error(BMX_ERR_ARG, "bmx_track_turns: negative turn count");
+    if (L->ctx->device != ctx->device || B->ctx->device != ctx->device)
+        return set_error(BMX_ERR_ARG, "bmx_track_turns: handles on different devices");
+    hipSetDevice(ctx->device);
+    HIP_TRY(launch_track_turns(ctx->stream, L->d_desc, nullptr, 0, B->d, B->n, n_turns, turn0,
+                               ctx->d_work),
+            "k_track_turns");
+    return BMX_OK;
+}
+
+int32_t bmx_track_element(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t ix_ele, int32_t turn)
+{
+    if (!ctx || !L || !B) return set_error(BMX_ERR_ARG, "bmx_track_element: null argument");
+    if (ix_ele < 1 || ix_ele > L->desc.n_ele) return set_error(BMX_ERR_ARG, "bmx_track_element: ix_ele out of range");
+    hipSetDevice(ctx->device);
+    HIP_TRY(launch_track_element(ctx->stream, L->desc, ix_ele - 1, turn, B->d, B->n, ctx->d_work), "k_track_element");
+    return BMX_OK;
+}
+
+int32_t bmx_track_turns_plain(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t n_turns, int32_t turn0)
+{
+    if (!ctx || !L || !B) return set_error(BMX_ERR_ARG, "bmx_track_turns_plain: null argument");
+    if (n_turns < 0) return set_error(BMX_ERR_ARG, "bmx_track_turns_plain: negative turn count");
+    hipSetDevice(ctx->device);
+    for (int32_t t = 0; t < n_turns; t++)
+        for (int32_t e = 0; e < L->desc.n_ele; e++)
+            HIP_TRY(launch_track_element(ctx->stream, L->desc, e, turn0 + t, B->d, B->n, ctx->d_work), "k_track_element");
+    return BMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,362 @@
+// Lattice table compiler: raw bmx_ele records -> compact device table.
+//
+// Per element this does, once, the element-constant work that Bmad's element routines redo per
+// particle per element: multipole normalisation and kick folding (multipole_ele_to_ab.f90:
+// 31-236, 326-405, 472-600), step counts (track_a_thick_multipole.f90:73-79,
+// track_a_rfcavity.f90:54), Hwang edge constants (fringe_mod.f90:264-288), bend-angle
+// functions (track_a_bend.f90:139-175) and aperture geometry (check_aperture_limit.f90:358-385).
+// Anything the GPU maps do not implement is rejected with BMX_ERR_UNSUPPORTED, mirroring
+// track1_bmad's "not implemented" error (track1_bmad.f90:263-270).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "bmx_internal.hpp"
+
+namespace bmx {
+namespace {
+
+constexpr double kMaxAp = 1e3;
+
+// sim_utils/special_functions/sinc.f90, cosc.f90, sincc.f90 (nd = 0)
+double h_sinc(double x) { return std::fabs(x) < 1e-8 ? 1.0 : std::sin(x) / x; }
+double h_cosc(double x)
+{
+    if (std::fabs(x) < 1e-8) return 0.5;
+    double s = h_sinc(x / 2);
+    return 0.5 * (s * s);
+}
+double h_sincc(double x)
+{
+    const double c0 = 1.0 / 6.0, c1 = -1.0 / 120.0, c2 = 1.0 / 5040.0, c3 = -1.0 / 362880.0;
+    double x2 = x * x;
+    if (std::fabs(x) < 0.1) return c0 + x2 * (c1 + x2 * (c2 + x2 * c3));
+    return (x - std::sin(x)) / (x * x2);
+}
+
+long f_nint(double x) { return std::lround(x); }
+
+bool misaligned(const bmx_ele &e)
+{
+    return e.x_offset != 0 || e.y_offset != 0 || e.z_offset != 0 || e.x_pitch != 0 ||
+           e.y_pitch != 0 || e.tilt != 0 || e.ref_tilt != 0;
+}
+
+std::string where(int i, const bmx_ele &e)
+{
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "element %d (key %d): ", i + 1, e.key);
+    return buf;
+}
+
+// Magnetic a/b strengths in the body frame incl. kicks (multipole_ele_to_ab with
+// include_kicks$, use_ele_tilt = F). Returns ix_pole_max or -2 on unsupported scaling.
+int ele_to_ab(const bmx_ele &e, double *a, double *b)
+{
+    for (int n = 0; n < BMX_N_POLE; n++) a[n] = b[n] = 0;
+    if (!e.is_on || e.key == BMX_PIPE) return -1;
+    if (e.n_multipole > 0) {
+        for (int n = 0; n < BMX_N_POLE; n++) {
+            a[n] = e.a_pole[n];
+            b[n] = e.b_pole[n];
+        }
+        bool any = false;
+        for (int n = 0; n < BMX_N_POLE; n++) any |= (a[n] != 0 || b[n] != 0);
+        if (any) {
+            double cnst = 0;
+            int ref_exp = 0;
+            bool scale = true;
+            switch (e.key) {
+            case BMX_SBEND: cnst = e.l * (e.g + e.dg); break;
+            case BMX_KICKER:
+                cnst = e.hkick == 0 ? e.vkick : (e.vkick == 0 ? e.hkick : std::sqrt(e.hkick * e.hkick + e.vkick * e.vkick));
+                break;
+            case BMX_HKICKER: case BMX_VKICKER: cnst = e.kick; break;
+            case BMX_QUADRUPOLE: cnst = e.k1 * e.l; ref_exp = 1; break;
+            case BMX_SEXTUPOLE: cnst = e.k2 * e.l; ref_exp = 2; break;
+            case BMX_OCTUPOLE: cnst = e.k3 * e.l; ref_exp = 3; break;
+            case BMX_THICK_MULTIPOLE: case BMX_RFCAVITY: scale = false; break;
+            default: return -2;
+            }
+            if (scale) {
+                if (e.r0_mag == 0) {
+                    for (int n = 0; n < BMX_N_POLE; n++) { a[n] = cnst * a[n]; b[n] = cnst * b[n]; }
+                } else {
+                    double factor = cnst * std::pow(e.r0_mag, ref_exp + 1);
+                    for (int n = 0; n < BMX_N_POLE; n++) {
+                        factor = factor / e.r0_mag;
+                        a[n] = factor * a[n];
+                        b[n] = factor * b[n];
+                    }
+                }
+            }
+        }
+    }
+    double ak[4] = {0, 0, 0, 0}, bk[4] = {0, 0, 0, 0};
+    switch (e.key) {
+    case BMX_HKICKER: bk[0] = -e.kick; break;
+    case BMX_VKICKER: ak[0] = e.kick; break;
+    case BMX_KICKER: bk[0] = -e.hkick; ak[0] = e.vkick; break;
+    case BMX_DRIFT: case BMX_MARKER: break;
+    default:
+        if (e.hkick != 0 || e.vkick != 0) {
+            double t = e.key == BMX_SBEND ? e.ref_tilt : e.tilt;
+            if (t != 0) {
+                double c = std::cos(t), s = std::sin(t);
+                bk[0] = -e.hkick * c - e.vkick * s;
+                ak[0] = -e.hkick * s + e.vkick * c;
+            } else {
+                bk[0] = -e.hkick;
+                ak[0] = e.vkick;
+            }
+        }
+    }
+    switch (e.key) {
+    case BMX_QUADRUPOLE: bk[1] = e.k1 * e.l; break;
+    case BMX_SBEND:
+        bk[0] = bk[0] + e.dg * e.l;
+        bk[1] = e.k1 * e.l;
+        bk[2] = e.k2 * e.l / 2.0;
+        break;
+    case BMX_SEXTUPOLE: bk[2] = e.k2 * e.l / 2.0; break;
+    case BMX_OCTUPOLE: bk[3] = e.k3 * e.l / 6.0; break;
+    default: break;
+    }
+    for (int i = 0; i < 4; i++) {
+        a[i] = a[i] + ak[i];
+        b[i] = b[i] + bk[i];
+    }
+    for (int n = BMX_N_POLE - 1; n >= 0; n--)
+        if (a[n] != 0 || b[n] != 0) return n;
+    return -1;
+}
+
+// Aperture flags and the 4 aperture values; returns false if unsupported.
+bool aperture_setup(const bmx_ele &e, int32_t &flags, double ap[4], std::string &why)
+{
+    int at = e.aperture_at;
+    if (at == BMX_CONTINUOUS) { why = "continuous apertures"; return false; }
+    if (at != BMX_ENTRANCE_END && at != BMX_EXIT_END && at != BMX_BOTH_ENDS) return true;
+    if (e.key == BMX_ECOLLIMATOR || e.key == BMX_RCOLLIMATOR) {
+        if (e.px_aperture_width2 != 0 || e.py_aperture_width2 != 0 || e.z_aperture_width2 != 0 ||
+            e.pz_aperture_width2 != 0) {
+            why = "collimator momentum apertures";
+            return false;
+        }
+    }
+    const double x1 = -e.x1_limit, x2 = e.x2_limit, y1 = -e.y1_limit, y2 = e.y2_limit;
+    int32_t f = 0;
+    if (e.aperture_type == BMX_APERTURE_ELLIPTICAL) {
+        // elliptical_params_setup: a half-set pair makes the reference skip the test.
+        if ((x1 == 0) != (x2 == 0) || (y1 == 0) != (y2 == 0)) return true;
+        double xw = (x2 - x1) / 2, xc = (x2 + x1) / 2;
+        double yw = (y2 - y1) / 2, yc = (y2 + y1) / 2;
+        if (xw == 0) xw = kMaxAp;
+        if (yw == 0) yw = kMaxAp;
+        xw = xw > 0 ? xw : 0;
+        yw = yw > 0 ? yw : 0;
+        ap[0] = xc;
+        ap[1] = 1 / xw;
+        ap[2] = yc;
+        ap[3] = 1 / yw;
+        f |= F_APE_ELLIP;
+    } else if (e.aperture_type == BMX_APERTURE_RECTANGULAR || e.aperture_type == 1 /* auto */ ||
+               e.aperture_type == 0) {
+        if (x1 == 0 && x2 == 0 && y1 == 0 && y2 == 0) return true;
+        ap[0] = x1;
+        ap[1] = x2;
+        ap[2] = y1;
+        ap[3] = y2;
+    } else {
+        why = "aperture type";
+        return false;
+    }
+    if (at == BMX_ENTRANCE_END || at == BMX_BOTH_ENDS) f |= F_APE_ENT;
+    if (at == BMX_EXIT_END || at == BMX_BOTH_ENDS) f |= F_APE_EXIT;
+    flags |= f;
+    return true;
+}
+
+void edge_constants(const bmx_ele &e, bool entrance, double *out)
+{
+    const double c_dir = 1.0;
+    double g_tot = (e.g + e.dg) * c_dir;
+    double k1 = e.k1;
+    double ang = entrance ? e.e1 : e.e2;
+    double fint_gap = entrance ? e.fint * e.hgap : e.fintx * e.hgapx;
+    if (e.fringe_type == BMX_FRINGE_HARD_EDGE_ONLY) fint_gap = 0;
+    double cos_e = std::cos(ang), sin_e = std::sin(ang);
+    double tan_e = sin_e / cos_e, sec_e = 1 / cos_e;
+    out[0] = g_tot * tan_e;                                             // gt
+    out[1] = g_tot * (tan_e * tan_e);                                   // gt2
+    out[2] = g_tot * (sec_e * sec_e);                                   // gs2
+    out[3] = k1 * tan_e * c_dir;                                        // k1_tane
+    out[4] = 2 * fint_gap * out[2] * g_tot * sec_e * (1 + sin_e * sin_e);  // fg_factor
+    out[5] = tan_e * tan_e;
+}
+
+}  // namespace
+
+int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge, HostTable &out,
+                  std::string &err)
+{
+    (void)charge;  // the tracked species is the reference species: charge ratios are 1
+    out.hdr.clear();
+    out.prm.clear();
+    out.kind_mask = 0;
+    if (n_ele <= 0 || !eles) { err = "empty lattice"; return BMX_ERR_ARG; }
+    if (!(mc2 > 0)) { err = "mass must be positive"; return BMX_ERR_ARG; }
+    const double p0c = eles[0].p0c;
+    if (!(p0c > 0)) { err = "element 1 has no reference momentum"; return BMX_ERR_ARG; }
+    out.sp.mc2 = mc2;
+    out.sp.p0c = p0c;
+    out.sp.mc2_sq = mc2 * mc2;
+
+    for (int i = 0; i < n_ele; i++) {
+        const bmx_ele &e = eles[i];
+        if (e.key > 0 && e.key < 64) out.kind_mask |= (uint64_t)1 << e.key;
+        if (e.p0c != p0c) {
+            err = where(i, e) + "reference momentum differs from element 1 (energy changing lattices are outside this path)";
+            return BMX_ERR_P0C;
+        }
+        EleHdr h{};
+        std::vector<double> q;
+        std::string why;
+        int key = e.key;
+        if (!e.is_on) {
+            if (key == BMX_SBEND) { err = where(i, e) + "switched-off bends are not supported"; return BMX_ERR_UNSUPPORTED; }
+            key = BMX_DRIFT;  // track1_bmad.f90:51-60
+        }
+        // Misalignments: only drifts ignore them in the reference (track1_bmad.f90:93-94).
+        if (key != BMX_DRIFT && key != BMX_MARKER && misaligned(e)) {
+            err = where(i, e) + "offsets, pitches and tilts are not supported on the GPU path";
+            return BMX_ERR_UNSUPPORTED;
+        }
+        double a[BMX_N_POLE], b[BMX_N_POLE];
+        switch (key) {
+        case BMX_DRIFT:
+        case BMX_MARKER:
+        case BMX_PIPE:
+        case BMX_MONITOR:
+        case BMX_INSTRUMENT:
+        case BMX_ECOLLIMATOR:
+        case BMX_RCOLLIMATOR:
+        case BMX_SEXTUPOLE:
+        case BMX_OCTUPOLE:
+        case BMX_THICK_MULTIPOLE:
+        case BMX_KICKER:
+        case BMX_HKICKER:
+        case BMX_VKICKER: {
+            int ix = (key == BMX_DRIFT || key == BMX_MARKER) ? -1 : ele_to_ab(e, a, b);
+            if (ix == -2) { err = where(i, e) + "multipole scaling"; return BMX_ERR_UNSUPPORTED; }
+            if (ix > 3) { err = where(i, e) + "multipole orders above 3"; return BMX_ERR_UNSUPPORTED; }
+            if (key == BMX_MARKER || (ix == -1 && e.l == 0)) {
+                h.kind = K_MARKER;
+            } else if (ix == -1) {
+                h.kind = K_DRIFT;
+                q.push_back(e.l);
+            } else {
+                h.kind = K_MULT;
+                long ns = f_nint(std::fabs(e.l / e.ds_step));
+                int n_step = ns > 1 ? (int)ns : 1;
+                double r_step = 1.0 / n_step;
+                double half = 1.0 * r_step / 2;
+                q.push_back(e.l / n_step);
+                int mask = 0;
+                for (int n = 0; n < 4; n++) {
+                    q.push_back(half * a[n]);
+                    if (a[n] != 0 || b[n] != 0) mask |= 1 << n;
+                }
+                for (int n = 0; n < 4; n++) q.push_back(half * b[n]);
+                h.n = n_step;
+                h.flags |= mask << F_ORDER_SHIFT;
+            }
+            break;
+        }
+        case BMX_QUADRUPOLE: {
+            int ix = ele_to_ab(e, a, b);
+            if (ix == -2) { err = where(i, e) + "multipole scaling"; return BMX_ERR_UNSUPPORTED; }
+            double b1 = b[1];
+            b[1] = 0;
+            for (int n = 0; n < BMX_N_POLE; n++)
+                if (a[n] != 0 || b[n] != 0) { err = where(i, e) + "quadrupole multipoles/kicks are not supported"; return BMX_ERR_UNSUPPORTED; }
+            if (e.fringe_type != BMX_FRINGE_NONE && e.fringe_type != 0) { err = where(i, e) + "quadrupole fringe"; return BMX_ERR_UNSUPPORTED; }
+            if (e.l == 0) {
+                h.kind = K_MARKER;
+                break;
+            }
+            h.kind = K_QUAD;
+            const double beta0 = p0c / e.e_tot;
+            const double me = mc2 / e.e_tot;
+            q = {e.l, 1.0 * b1, beta0, me * me, beta0 * beta0 * (2 * (beta0 * beta0) - (me * me) / 2),
+                 3e-7 * e.e_tot, mc2};
+            break;
+        }
+        case BMX_SBEND: {
+            int ix = ele_to_ab(e, a, b);
+            if (ix == -2) { err = where(i, e) + "multipole scaling"; return BMX_ERR_UNSUPPORTED; }
+            for (int n = 0; n < BMX_N_POLE; n++)
+                if (a[n] != 0 || b[n] != 0) { err = where(i, e) + "bend k1/k2/dg/multipoles/kicks are not supported"; return BMX_ERR_UNSUPPORTED; }
+            int ft = e.fringe_type;
+            bool kick_type = ft == BMX_FRINGE_BASIC_BEND || ft == BMX_FRINGE_HARD_EDGE_ONLY;
+            if (!kick_type && ft != BMX_FRINGE_NONE) { err = where(i, e) + "bend fringe_type"; return BMX_ERR_UNSUPPORTED; }
+            h.kind = K_BEND;
+            if (kick_type && (e.fringe_at == BMX_ENTRANCE_END || e.fringe_at == BMX_BOTH_ENDS)) h.flags |= F_EDGE_ENT;
+            if (kick_type && (e.fringe_at == BMX_EXIT_END || e.fringe_at == BMX_BOTH_ENDS)) h.flags |= F_EDGE_EXIT;
+            double fg_ent = e.fint * e.hgap, fg_exit = e.fintx * e.hgapx;
+            if (ft == BMX_FRINGE_HARD_EDGE_ONLY) fg_ent = fg_exit = 0;
+            if (e.e1 == 0 && e.e2 == 0 && fg_ent == 0 && fg_exit == 0 && e.k1 == 0) h.flags |= F_EDGE_SIMPLE;
+            const double g = e.g, dg = 0.0, L = e.l;
+            if ((g == 0 && dg == 0) || L == 0) h.flags |= F_BEND_DRIFT;
+            const double angle = g * L;
+            q.resize(kParams[K_BEND]);
+            q[0] = L;
+            q[1] = g;
+            q[2] = (g + dg) * 1.0;
+            q[3] = angle;
+            q[4] = std::cos(angle);
+            q[5] = std::sin(angle);
+            q[6] = h_sinc(angle);
+            q[7] = h_cosc(angle);
+            q[8] = e.p0c / e.e_tot;
+            q[9] = h_sincc(angle);
+            edge_constants(e, true, &q[10]);
+            edge_constants(e, false, &q[16]);
+            break;
+        }
+        case BMX_RFCAVITY: {
+            int ix = ele_to_ab(e, a, b);
+            if (ix != -1) { err = where(i, e) + "rf cavity multipoles/kicks are not supported"; return BMX_ERR_UNSUPPORTED; }
+            if (e.rf_frequency == 0) {
+                if (e.voltage != 0) { err = where(i, e) + "rf cavity with zero frequency"; return BMX_ERR_UNSUPPORTED; }
+            }
+            h.kind = K_RF;
+            const double length = e.l * 1;
+            long ns = f_nint(std::fabs(length) / e.ds_step);
+            const int n_slice = ns > 1 ? (int)ns : 1;
+            h.n = n_slice;
+            const double twopi = 2 * 3.141592653589793238462643383279;
+            const double f = e.rf_frequency;
+            q = {length / n_slice, e.voltage * 1.0, e.phi0, f, f != 0 ? 0.5 / f : 1e300, twopi * f,
+                 twopi * e.phi0_multipass};
+            break;
+        }
+        default:
+            err = where(i, e) + "element kind is not implemented on the GPU path";
+            return BMX_ERR_UNSUPPORTED;
+        }
+        double ap[4] = {0, 0, 0, 0};
+        if (!aperture_setup(e, h.flags, ap, why)) {
+            err = where(i, e) + why + " are not supported";
+            return BMX_ERR_UNSUPPORTED;
+        }
+        if ((int)q.size() != kParams[h.kind]) { err = where(i, e) + "internal parameter count"; return BMX_ERR_ARG; }
+        h.off = (int32_t)out.prm.size();
+        out.prm.insert(out.prm.end(), q.begin(), q.end());
+        if (h.flags & (F_APE_ENT | F_APE_EXIT)) out.prm.insert(out.prm.end(), ap, ap + 4);
+        out.hdr.push_back(h);
+    }
+    return BMX_OK;
+}
+
+}  // namespace bmx

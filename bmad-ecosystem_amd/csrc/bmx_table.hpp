@@ -1,0 +1,87 @@
+// Device element table shared by the host table compiler and the kernels.
+//
+// A lattice is compiled once from raw bmx_ele records into a compact table: one 16-byte header
+// per element plus a flat array of FP64 parameters holding only what the per-particle map of
+// that element kind needs (element constants that Bmad re-derives per particle per element,
+// e.g. multipole_ele_to_ab in track_a_quadrupole.f90:46-47 or the bend's cos/sin/sinc of the
+// bend angle in track_a_bend.f90:165-175, are computed here once).
+#pragma once
+#include <stdint.h>
+
+namespace bmx {
+
+enum Kind : int32_t {
+    K_DRIFT = 0,
+    K_QUAD = 1,
+    K_MULT = 2,   // thick multipole: kick - (drift - kick) * n_step (track_a_thick_multipole.f90)
+    K_BEND = 3,
+    K_RF = 4,
+    K_MARKER = 5,
+    K_NKIND = 6,
+};
+
+enum Flags : int32_t {
+    F_APE_ENT = 1 << 0,     // aperture checked at the entrance end
+    F_APE_EXIT = 1 << 1,    // aperture checked at the exit end
+    F_APE_ELLIP = 1 << 2,   // elliptical (else rectangular)
+    F_EDGE_ENT = 1 << 3,    // Hwang edge kick at the bend entrance
+    F_EDGE_EXIT = 1 << 4,   // Hwang edge kick at the bend exit
+    F_EDGE_SIMPLE = 1 << 5, // both edges have e = 0, fint*hgap = 0, k1 = 0 (only the g y^2 terms)
+    F_BEND_DRIFT = 1 << 6,  // g = dg = 0: the bend body is a drift (track_a_bend.f90:117-118)
+    F_ORDER_SHIFT = 8,      // K_MULT: bits 8..11 = mask of multipole orders 0..3 present
+};
+
+struct EleHdr {
+    int32_t kind;
+    int32_t flags;
+    int32_t n;    // K_MULT: n_step; K_RF: n_slice
+    int32_t off;  // offset of the element's parameters in the parameter array (doubles)
+};
+
+// Parameter layouts (index into prm + off):
+//  K_DRIFT : [0] L
+//  K_QUAD  : [0] L, [1] b1 = k1 L, [2] beta0 = p0c/E_tot, [3] (mc2/E_tot)^2,
+//            [4] f = beta0^2 (2 beta0^2 - (mc2/E)^2/2), [5] 3e-7 E_tot, [6] mc2 (series test)
+//  K_MULT  : [0] step_len, [1..4] a_n * r_step/2 (n = 0..3), [5..8] b_n * r_step/2
+//  K_BEND  : [0] L, [1] g, [2] g_tot, [3] angle = g L, [4] cos_a, [5] sin_a, [6] sinc_a,
+//            [7] cosc_a, [8] beta_ref, [9] sincc_a,
+//            [10..15] entrance gt, gt2, gs2, k1_tane, fg_factor, tan_e^2, [16..21] exit
+//  K_RF    : [0] L/n_slice, [1] voltage * charge_dir, [2] phi0, [3] rf_frequency,
+//            [4] 0.5/rf_frequency, [5] twopi * rf_frequency, [6] twopi * phi0_multipass
+//  K_MARKER: (none)
+// followed, when F_APE_ENT|F_APE_EXIT, by 4 aperture values:
+//  elliptical : x_center, 1/x_width2, y_center, 1/y_width2
+//  rectangular: x1_lim (= -x1_limit), x2_lim, y1_lim, y2_lim (0 = no limit)
+constexpr int kParams[K_NKIND] = {1, 7, 9, 22, 7, 0};
+
+struct Species {
+    double mc2;     // [eV]
+    double p0c;     // reference momentum [eV]
+    double mc2_sq;  // mc2^2
+};
+
+// One unit of persistent work: a run of particles of one lattice.
+struct Tile {
+    int32_t lat;      // lattice index (batched launches), 0 otherwise
+    int32_t count;    // particles in the tile
+    int64_t start;    // first particle index in the beam arrays
+    int32_t n_turns;  // turns to track in this launch
+    int32_t turn0;    // turn index of the first turn (loss_turn bookkeeping)
+};
+
+// Per-lattice device descriptor.
+struct LatDesc {
+    const EleHdr *hdr;
+    const double *prm;
+    int32_t n_ele;
+    int32_t pad;
+    Species sp;
+};
+
+// Device pointers of one beam (SoA).
+struct BeamPtrs {
+    double *x, *px, *y, *py, *z, *pz, *beta;
+    int32_t *state, *loss_ele, *loss_turn;
+};
+
+}  // namespace bmx

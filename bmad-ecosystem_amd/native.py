@@ -1,0 +1,99 @@
+"""ctypes binding of libbmx.so (include/bmx.h).
+
+The library is the product path: there is no CPU fallback. If libbmx.so is missing or a call
+fails, this module raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbmx.so")
+
+_lib = None
+
+
+class BmxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bmx error {code}: {msg}")
+        self.code = code
+
+
+class BeamView(ctypes.Structure):
+    """bmx_beam_view"""
+    _fields_ = [(n, ctypes.c_void_p) for n in ("x", "px", "y", "py", "z", "pz", "beta",
+                                               "state", "loss_ele", "loss_turn")]
+
+
+class ServerConfig(ctypes.Structure):
+    """bmx_server_config"""
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("chunk_turns", ctypes.c_int32),
+                ("max_active", ctypes.c_int32), ("tile", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+_P = ctypes.c_void_p
+_PP = ctypes.POINTER(ctypes.c_void_p)
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+
+# name: (restype, argtypes)
+_SIGS = {
+    "bmx_version": (_I32, []),
+    "bmx_last_error": (ctypes.c_char_p, []),
+    "bmx_ctx_create": (_I32, [_I32, _PP]),
+    "bmx_ctx_destroy": (_I32, [_P]),
+    "bmx_ctx_synchronize": (_I32, [_P]),
+    "bmx_ctx_stream": (_P, [_P]),
+    "bmx_ctx_take_work": (_I32, [_P, ctypes.POINTER(_I64)]),
+    "bmx_ctx_record": (_I32, [_P, _I32]),
+    "bmx_ctx_elapsed_ms": (_I32, [_P, _I32, _I32, ctypes.POINTER(ctypes.c_float)]),
+    "bmx_lattice_create": (_I32, [_P, _P, _I32, ctypes.c_double, _I32, _PP]),
+    "bmx_lattice_destroy": (_I32, [_P]),
+    "bmx_lattice_validate": (_I32, [_P, _I32, ctypes.c_double, _I32, ctypes.POINTER(_I32)]),
+    "bmx_lattice_info": (_I32, [_P, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_uint64)]),
+    "bmx_beam_create": (_I32, [_P, _I64, _PP]),
+    "bmx_beam_wrap": (_I32, [_P, ctypes.POINTER(BeamView), _I64, _PP]),
+    "bmx_beam_upload": (_I32, [_P, ctypes.POINTER(BeamView), _I64]),
+    "bmx_beam_download": (_I32, [_P, ctypes.POINTER(BeamView), _I64]),
+    "bmx_beam_destroy": (_I32, [_P]),
+    "bmx_beam_count_alive": (_I32, [_P, ctypes.POINTER(_I64)]),
+    "bmx_track_turns": (_I32, [_P, _P, _P, _I32, _I32]),
+    "bmx_track_element": (_I32, [_P, _P, _P, _I32, _I32]),
+    "bmx_track_turns_plain": (_I32, [_P, _P, _P, _I32, _I32]),
+}
+
+
+def load(reload: bool = False):
+    """Load libbmx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and not reload:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BmxError(-1, f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names in _SIGS resolvable in the loaded library."""
+    L = load()
+    return [n for n in _SIGS if hasattr(L, n)]
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().bmx_last_error()
+        raise BmxError(rc, msg.decode() if msg else "")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args))

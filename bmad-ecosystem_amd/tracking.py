@@ -1,0 +1,204 @@
+"""Host API over libbmx.so, named after the Bmad calls it replaces.
+
+  Context          one GPU + one HIP stream (bmx_ctx)
+  Lattice          flattened element table of lat%branch(ix)%ele(1:n) (bmx_lattice)
+  Beam             SoA particle state resident in HBM (bmx_beam)
+  track_beam       n_turns full turns, persistent kernel
+                   (bmad/multiparticle/beam_mod.f90:40 track_beam, per turn as in
+                   bsim/modules/lt_tracking_mod.f90:1631-1716)
+  track1_bunch     one element for the whole bunch (beam_mod.f90:238 -> beam_utils.f90:32)
+  track_beam_plain n_turns as n_turns x n_ele element launches (the reference loop nest)
+  track_all        single particles through the lattice, stopping at loss
+                   (bmad/code/track_all.f90:25-145)
+
+Loss is data: ``state`` holds Bmad's coord_struct%state codes, ``loss_ele`` the 1-based
+element index and ``loss_turn`` the 0-based turn (lt_tracking_mod.f90:1395).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import elements as E
+from . import native
+
+FLOAT_KEYS = ("x", "px", "y", "py", "z", "pz", "beta")
+INT_KEYS = ("state", "loss_ele", "loss_turn")
+KEYS = FLOAT_KEYS + INT_KEYS
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        native.call("bmx_ctx_create", device, ctypes.byref(h))
+        self.h = h
+        self.device = device
+
+    def synchronize(self):
+        native.call("bmx_ctx_synchronize", self.h)
+
+    @property
+    def stream(self) -> int:
+        return native.load().bmx_ctx_stream(self.h) or 0
+
+    def record(self, slot: int) -> None:
+        """Record HIP event `slot` on this context's stream."""
+        native.call("bmx_ctx_record", self.h, int(slot))
+
+    def elapsed_ms(self, slot0: int, slot1: int) -> float:
+        ms = ctypes.c_float()
+        native.call("bmx_ctx_elapsed_ms", self.h, int(slot0), int(slot1), ctypes.byref(ms))
+        return ms.value
+
+    def take_work(self) -> int:
+        """Particle-element passes tracked since the last call (synchronises)."""
+        v = ctypes.c_int64()
+        native.call("bmx_ctx_take_work", self.h, ctypes.byref(v))
+        return v.value
+
+    def close(self):
+        if self.h:
+            native.call("bmx_ctx_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Lattice:
+    def __init__(self, ctx: Context, eles, mc2: float = E.M_ELECTRON, charge: int = -1):
+        arr = eles if isinstance(eles, np.ndarray) else E.to_array(eles)
+        self.arr = np.ascontiguousarray(arr)
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        native.call("bmx_lattice_create", ctx.h, self.arr.ctypes.data, len(self.arr), mc2, charge,
+                    ctypes.byref(h))
+        self.h = h
+        n = ctypes.c_int32()
+        mask = ctypes.c_uint64()
+        native.call("bmx_lattice_info", self.h, ctypes.byref(n), ctypes.byref(mask))
+        self.n_ele = n.value
+        self.kind_mask = mask.value
+
+    def close(self):
+        if self.h:
+            native.call("bmx_lattice_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _host_view(beam: dict) -> native.BeamView:
+    for k in FLOAT_KEYS:
+        assert beam[k].dtype == np.float64 and beam[k].flags.c_contiguous, k
+    for k in INT_KEYS:
+        assert beam[k].dtype == np.int32 and beam[k].flags.c_contiguous, k
+    return native.BeamView(*[beam[k].ctypes.data for k in KEYS])
+
+
+class Beam:
+    """Particle state in HBM. Either library-owned (n particles) or wrapping device tensors."""
+
+    def __init__(self, ctx: Context, n: int | None = None, tensors: dict | None = None):
+        self.ctx = ctx
+        self.tensors = tensors
+        h = ctypes.c_void_p()
+        if tensors is not None:
+            n = int(tensors["x"].shape[0])
+            view = native.BeamView(*[int(tensors[k].data_ptr()) for k in KEYS])
+            native.call("bmx_beam_wrap", ctx.h, ctypes.byref(view), n, ctypes.byref(h))
+        else:
+            native.call("bmx_beam_create", ctx.h, int(n), ctypes.byref(h))
+        self.h = h
+        self.n = int(n)
+
+    @classmethod
+    def from_host(cls, ctx: Context, beam: dict) -> "Beam":
+        b = cls(ctx, int(beam["x"].shape[0]))
+        b.upload(beam)
+        return b
+
+    def upload(self, beam: dict):
+        v = _host_view(beam)
+        native.call("bmx_beam_upload", self.h, ctypes.byref(v), int(beam["x"].shape[0]))
+
+    def download(self, n: int | None = None) -> dict:
+        n = self.n if n is None else n
+        out = {k: np.empty(n, dtype=np.float64) for k in FLOAT_KEYS}
+        out.update({k: np.empty(n, dtype=np.int32) for k in INT_KEYS})
+        v = _host_view(out)
+        native.call("bmx_beam_download", self.h, ctypes.byref(v), n)
+        return out
+
+    def count_alive(self) -> int:
+        c = ctypes.c_int64()
+        native.call("bmx_beam_count_alive", self.h, ctypes.byref(c))
+        return c.value
+
+    def close(self):
+        if self.h:
+            native.call("bmx_beam_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def track_beam(ctx: Context, lat: Lattice, beam: Beam, n_turns: int, turn0: int = 0) -> None:
+    """Persistent tracking of n_turns full turns (asynchronous on ctx's stream)."""
+    native.call("bmx_track_turns", ctx.h, lat.h, beam.h, int(n_turns), int(turn0))
+
+
+def track1_bunch(ctx: Context, lat: Lattice, beam: Beam, ix_ele: int, turn: int = 0) -> None:
+    """One element (1-based ix_ele) for the whole bunch (asynchronous)."""
+    native.call("bmx_track_element", ctx.h, lat.h, beam.h, int(ix_ele), int(turn))
+
+
+def track_beam_plain(ctx: Context, lat: Lattice, beam: Beam, n_turns: int, turn0: int = 0) -> None:
+    """n_turns as element-by-element launches, state through HBM (asynchronous)."""
+    native.call("bmx_track_turns_plain", ctx.h, lat.h, beam.h, int(n_turns), int(turn0))
+
+
+def track_beam_host(eles, beam: dict, n_turns: int, device: int = 0, turn0: int = 0,
+                    mc2: float = E.M_ELECTRON, charge: int = -1) -> dict:
+    """Convenience: host SoA dict in, host SoA dict out (tracks a copy)."""
+    ctx = Context(device)
+    lat = Lattice(ctx, eles, mc2, charge)
+    b = Beam.from_host(ctx, beam)
+    track_beam(ctx, lat, b, n_turns, turn0)
+    ctx.synchronize()
+    out = b.download()
+    b.close()
+    lat.close()
+    ctx.close()
+    return out
+
+
+def track_all(ctx: Context, lat: Lattice, vec, beta, n_turns: int = 1):
+    """Single particles (rows of vec) through n_turns, stopping at loss like track_all.
+    Returns (vec_out, state, loss_ele, loss_turn)."""
+    vec = np.atleast_2d(np.asarray(vec, dtype=np.float64))
+    n = vec.shape[0]
+    beam = {k: np.ascontiguousarray(vec[:, i]) for i, k in enumerate(FLOAT_KEYS[:6])}
+    beam["beta"] = np.ascontiguousarray(np.broadcast_to(np.asarray(beta, dtype=np.float64), (n,)))
+    beam["state"] = np.full(n, E.STATE_ALIVE, dtype=np.int32)
+    beam["loss_ele"] = np.zeros(n, dtype=np.int32)
+    beam["loss_turn"] = np.full(n, -1, dtype=np.int32)
+    b = Beam.from_host(ctx, beam)
+    track_beam(ctx, lat, b, n_turns)
+    ctx.synchronize()
+    out = b.download()
+    b.close()
+    v = np.stack([out[k] for k in FLOAT_KEYS[:6]], axis=1)
+    return v, out["state"], out["loss_ele"], out["loss_turn"]

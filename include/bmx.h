@@ -155,6 +155,16 @@ int32_t bmx_ctx_destroy(bmx_ctx *ctx);
 int32_t bmx_ctx_synchronize(bmx_ctx *ctx);
 /* The context's HIP stream (hipStream_t), for callers that order their own work against it. */
 void *bmx_ctx_stream(bmx_ctx *ctx);
+/* Particle-element passes tracked by this context's tracking calls since the last take
+ * (element passes of particles alive at the element entrance, the work track1_bunch_hom
+ * does, beam_utils.f90:81-86); synchronises the stream and resets the count. */
+int32_t bmx_ctx_take_work(bmx_ctx *ctx, int64_t *passes);
+/* Timing on the context's stream: record HIP event `slot` (0..BMX_N_EVENTS-1) now, and the
+ * elapsed milliseconds between two recorded slots (synchronises on the later one). Replaces
+ * the wall-clock run_timer of the reference drivers (sim_utils/misc/run_timer.f90:32). */
+#define BMX_N_EVENTS 64
+int32_t bmx_ctx_record(bmx_ctx *ctx, int32_t slot);
+int32_t bmx_ctx_elapsed_ms(bmx_ctx *ctx, int32_t slot0, int32_t slot1, float *ms);
 
 /* ---- lattice: flattened element table (lat%branch(ix)%ele(1:n_ele_track)) ----------------- *
  * Replaces the per-particle re-derivation of element constants in track_a_* (e.g.
@@ -164,6 +174,11 @@ void *bmx_ctx_stream(bmx_ctx *ctx);
 int32_t bmx_lattice_create(bmx_ctx *ctx, const bmx_ele *eles, int32_t n_ele, double mc2,
                            int32_t charge, bmx_lattice **out);
 int32_t bmx_lattice_destroy(bmx_lattice *lat);
+/* Compile the element table on the host only (no device needed) and report whether every
+ * element is inside this path: BMX_OK, or BMX_ERR_UNSUPPORTED / BMX_ERR_P0C / BMX_ERR_ARG with
+ * the offending element in bmx_last_error(). n_params receives the table size in doubles. */
+int32_t bmx_lattice_validate(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge,
+                             int32_t *n_params);
 /* Number of elements and bit set of element kinds present (bit = key). */
 int32_t bmx_lattice_info(const bmx_lattice *lat, int32_t *n_ele, uint64_t *kind_mask);
 

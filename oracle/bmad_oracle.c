@@ -113,33 +113,35 @@ static __float128 modulo2_q(__float128 x, __float128 amp)
 }
 
 /* bmad/low_level/c_multi.f90:19-63 with no_n_fact = .true.: the same table build (binomials,
- * division by n!, sign pattern on m mod 4) and the same re-multiplication by n!. */
-double bmo_c_multi(int32_t n, int32_t m)
+ * division by n!, sign pattern on m mod 4) and the same re-multiplication by n!.
+ * The reference builds the table lazily behind a SAVEd flag on first call, which races when
+ * the first call happens inside an OpenMP region; here it is built once when the library is
+ * loaded. */
+static double c_multi_tab[BMX_N_POLE][BMX_N_POLE];
+static double c_multi_nfact[BMX_N_POLE];
+
+__attribute__((constructor)) static void c_multi_init(void)
 {
-    static double c[BMX_N_POLE][BMX_N_POLE];
-    static double nfact[BMX_N_POLE];
-    static int init = 0;
-    if (!init) {
-        memset(c, 0, sizeof c);
-        c[0][0] = 1;
-        for (int in = 1; in < BMX_N_POLE; in++) {
-            c[in][0] = 1;
-            c[in][in] = 1;
-            for (int im = 1; im < in; im++) c[in][im] = c[in - 1][im - 1] + c[in - 1][im];
-        }
-        nfact[0] = 1;
-        for (int in = 0; in < BMX_N_POLE; in++) {
-            if (in > 0) nfact[in] = in * nfact[in - 1];
-            for (int im = 0; im <= in; im++) {
-                c[in][im] = c[in][im] / nfact[in];
-                if (im % 4 == 0) c[in][im] = -c[in][im];
-                if (im % 4 == 3) c[in][im] = -c[in][im];
-            }
-        }
-        init = 1;
+    double (*c)[BMX_N_POLE] = c_multi_tab;
+    memset(c_multi_tab, 0, sizeof c_multi_tab);
+    c[0][0] = 1;
+    for (int in = 1; in < BMX_N_POLE; in++) {
+        c[in][0] = 1;
+        c[in][in] = 1;
+        for (int im = 1; im < in; im++) c[in][im] = c[in - 1][im - 1] + c[in - 1][im];
     }
-    return c[n][m] * nfact[n];
+    c_multi_nfact[0] = 1;
+    for (int in = 0; in < BMX_N_POLE; in++) {
+        if (in > 0) c_multi_nfact[in] = in * c_multi_nfact[in - 1];
+        for (int im = 0; im <= in; im++) {
+            c[in][im] = c[in][im] / c_multi_nfact[in];
+            if (im % 4 == 0) c[in][im] = -c[in][im];
+            if (im % 4 == 3) c[in][im] = -c[in][im];
+        }
+    }
 }
+
+double bmo_c_multi(int32_t n, int32_t m) { return c_multi_tab[n][m] * c_multi_nfact[n]; }
 
 /* Fortran x**m for an integer variable m (libgcc __powidf2 repeated squaring). */
 static double powi(double x, int m)

@@ -1,0 +1,92 @@
+"""Measure algorithmic FP64 operation counts per element kind (writes flop_counts.json).
+
+Runs the counting build of the CPU restatement (libbmad_oracle_count.so) on particles of the
+benchmark beam (3x sigmas, 5 GeV electrons) through each element kind of the 901-element
+ring, with and without its elliptical aperture, and averages per particle-element pass.
+
+    python oracle/count_flops.py [n_particles]
+"""
+import ctypes
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, ROOT)
+
+from __graft_entry__ import load_package  # noqa: E402
+
+
+def main(n=4000):
+    subprocess_make()
+    lib = ctypes.CDLL(os.path.join(HERE, "libbmad_oracle_count.so"))
+    D = ctypes.POINTER(ctypes.c_double)
+    lib.bmo_count_track1.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32,
+                                     ctypes.c_double, D, D, ctypes.POINTER(ctypes.c_int32)]
+    lib.bmo_count_track1.restype = ctypes.c_double
+    bmx = load_package()
+    E, L = bmx.elements, bmx.lattices
+    p0c, et = L.reference()
+    beam = L.gaussian_beam(n, seed=1, sigma=3 * L.SIGMA0)
+    ring = L.ring1()
+
+    def avg_count(ele):
+        tot, cnt = 0.0, 0
+        for i in range(n):
+            v = np.array([beam[k][i] for k in ("x", "px", "y", "py", "z", "pz")])
+            b = ctypes.c_double(beam["beta"][i])
+            s = ctypes.c_int32(1)
+            c = lib.bmo_count_track1(ctypes.addressof(ele), E.M_ELECTRON, -1, p0c,
+                                     v.ctypes.data_as(D), ctypes.byref(b), ctypes.byref(s))
+            if c < 0:
+                raise RuntimeError("count failed")
+            tot += c
+            cnt += 1
+        return tot / cnt
+
+    def plain(ele):
+        e = E.BmxEle.from_buffer_copy(bytes(ele))
+        e.x1_limit = e.x2_limit = e.y1_limit = e.y2_limit = 0.0
+        e.aperture_type = E.APERTURE_RECTANGULAR
+        return e
+
+    kinds = {}
+    samples = {"drift": [e for e in ring if e.key == E.DRIFT][:1],
+               "quadrupole": [e for e in ring if e.key == E.QUADRUPOLE][:2],
+               "sextupole": [e for e in ring if e.key == E.SEXTUPOLE][:2],
+               "sbend": [e for e in ring if e.key == E.SBEND][:1],
+               "rfcavity": [e for e in ring if e.key == E.RFCAVITY][:1]}
+    ap_diff = []
+    for name, eles in samples.items():
+        with_ap = np.mean([avg_count(e) for e in eles])
+        without = np.mean([avg_count(plain(e)) for e in eles])
+        kinds[name] = {"flops": round(float(without), 2)}
+        ap_diff.append(with_ap - without)
+    marker = avg_count(E.make_element(E.MARKER, p0c, et))
+    kinds["marker"] = {"flops": round(float(marker), 2)}
+    kinds["aperture_elliptical"] = {"flops": round(float(np.mean(ap_diff)), 2)}
+    rect = E.make_element(E.MARKER, p0c, et, x_limit=0.02, y_limit=0.01)
+    kinds["aperture_rectangular"] = {"flops": round(float(avg_count(rect) - marker), 2)}
+    out = {
+        "convention": "add/sub/mul/div = 1, sqrt = 1, libm call = 1; per-particle work only "
+                      "(operations with a particle-dependent operand); counted by running the "
+                      "CPU restatement with a counting FP64 type (oracle/count_flops.cpp)",
+        "beam": f"{n} particles, 5 GeV e-, seeded Gaussian with 3x base sigmas",
+        "per_kind": kinds,
+    }
+    path = os.path.join(ROOT, "bmad-ecosystem_amd", "flop_counts.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
+
+
+def subprocess_make():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+if __name__ == "__main__":
+    main(int(sys.argv[1]) if len(sys.argv) > 1 else 4000)

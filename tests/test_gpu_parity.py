@@ -1,0 +1,187 @@
+"""GPU tracking against the CPU restatement, through the C-ABI (needs an MI355X)."""
+import numpy as np
+import pytest
+
+import parity as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(bmx):
+    return bmx.tracking.Context(0)
+
+
+def _single_element_cases(bmx):
+    E = bmx.elements
+    L = bmx.lattices
+    p0c, et = L.reference()
+    ap = dict(x_limit=0.02, y_limit=0.01, aperture_type=E.APERTURE_ELLIPTICAL)
+    g = L.BEND_ANGLE / 2.0
+    return {
+        "drift": E.make_element(E.DRIFT, p0c, et, l=1.0, **ap),
+        "quad_f": E.make_element(E.QUADRUPOLE, p0c, et, l=0.3, k1=0.5, **ap),
+        "quad_d": E.make_element(E.QUADRUPOLE, p0c, et, l=0.3, k1=-0.5, **ap),
+        "quad_weak": E.make_element(E.QUADRUPOLE, p0c, et, l=0.3, k1=1e-21),
+        "sext_f": E.make_element(E.SEXTUPOLE, p0c, et, l=0.2, k2=20.0, **ap),
+        "sext_d": E.make_element(E.SEXTUPOLE, p0c, et, l=0.2, k2=-20.0, **ap),
+        "sext_steps": E.make_element(E.SEXTUPOLE, p0c, et, l=0.6, k2=35.0, num_steps=7),
+        "octupole": E.make_element(E.OCTUPOLE, p0c, et, l=0.4, k3=500.0, num_steps=3),
+        "sbend": E.make_element(E.SBEND, p0c, et, l=2.0, g=g, **ap),
+        "sbend_edges": E.make_element(E.SBEND, p0c, et, l=1.5, g=0.05, e1=0.1, e2=-0.07,
+                                      fint=0.5, hgap=0.03),
+        "sbend_hard": E.make_element(E.SBEND, p0c, et, l=1.5, g=0.05, e1=0.05, fint=0.4, hgap=0.02,
+                                     fringe_type=E.FRINGE_HARD_EDGE_ONLY),
+        "rfcavity": E.make_element(E.RFCAVITY, p0c, et, l=0.5, voltage=10e6, rf_frequency=500e6,
+                                   phi0=0.1 / (2 * E.PI), **ap),
+        "rect_marker": E.make_element(E.MARKER, p0c, et, x_limit=0.004, y_limit=0.002,
+                                      aperture_at=E.BOTH_ENDS),
+        "hkicker": E.make_element(E.HKICKER, p0c, et, l=0.5, kick=1e-4),
+    }
+
+
+@pytest.mark.parametrize("name", ["drift", "quad_f", "quad_d", "quad_weak", "sext_f", "sext_d",
+                                  "sext_steps", "octupole", "sbend", "sbend_edges", "sbend_hard",
+                                  "rfcavity", "rect_marker", "hkicker"])
+def test_single_element_maps(bmx, oracle, ctx, name):
+    """Each element map on 4096 particles (3 sigma beam, wide pz): <= 1e-12 relative."""
+    ele = _single_element_cases(bmx)[name]
+    arr = bmx.elements.to_array([ele])
+    beam = P.random_beam(bmx, 4096, seed=11, scale=3.0, pz_sigma=5e-3)
+    ref = P.copy_beam(beam)
+    oracle.track_bunch(arr, ref, n_turns=1)
+    lat = bmx.tracking.Lattice(ctx, arr)
+    b = bmx.tracking.Beam.from_host(ctx, beam)
+    bmx.tracking.track_beam(ctx, lat, b, 1)
+    ctx.synchronize()
+    out = b.download()
+    assert P.loss_mismatch(out, ref).size == 0
+    alive = ref["state"] == 1
+    assert P.rel_err(out, ref, alive) <= 1e-12
+    if name in ("rfcavity",):
+        assert np.max(np.abs(out["beta"] - ref["beta"]) / ref["beta"]) <= 1e-14
+
+
+def test_semilinear_bend_branch(bmx, oracle, ctx):
+    """Particles within 1e-9 of the zero orbit take the semi-linear bend map (track_a_bend.f90:136)."""
+    L = bmx.lattices
+    E = bmx.elements
+    p0c, et = L.reference()
+    arr = E.to_array([E.make_element(E.SBEND, p0c, et, l=2.0, g=L.BEND_ANGLE / 2)])
+    n = 512
+    beam = L.gaussian_beam(n, seed=3, sigma=[1e-10, 3e-11, 1e-10, 3e-11, 1e-3, 3e-11])
+    beam["pz"][: n // 2] = 0.0
+    # every particle inside the semi-linear domain (|x g|, |px|, |py|, |pz| < 1e-9); outside it
+    # the exact body computes px as a difference of O(angle) terms (absolute rounding ~1e-17)
+    for k in ("px", "py", "pz"):
+        assert np.all(np.abs(beam[k]) < 1e-9)
+    beam["beta"] = L.init_beta(beam["pz"], p0c, et)
+    ref = P.copy_beam(beam)
+    oracle.track_bunch(arr, ref, n_turns=1)
+    out = bmx.tracking.track_beam_host(arr, beam, 1)
+    assert P.loss_mismatch(out, ref).size == 0
+    assert P.rel_err(out, ref) <= 1e-12
+
+
+def test_ring0_turns(bmx, oracle, ctx):
+    """700-element ring, 2000 particles x 20 turns: <= 1e-10 relative, persistent == plain."""
+    L = bmx.lattices
+    arr = bmx.elements.to_array(L.ring0())
+    beam = L.gaussian_beam(2000, seed=0)
+    ref = P.copy_beam(beam)
+    oracle.track_bunch(arr, ref, n_turns=20)
+    lat = bmx.tracking.Lattice(ctx, arr)
+    b = bmx.tracking.Beam.from_host(ctx, beam)
+    bmx.tracking.track_beam(ctx, lat, b, 20)
+    ctx.synchronize()
+    out = b.download()
+    assert np.all(out["state"] == 1)
+    assert P.rel_err(out, ref) <= 1e-10
+    b2 = bmx.tracking.Beam.from_host(ctx, beam)
+    bmx.tracking.track_beam_plain(ctx, lat, b2, 20)
+    ctx.synchronize()
+    out2 = b2.download()
+    for k in P.KEYS:
+        assert np.array_equal(out2[k], out[k]), k
+
+
+def test_ring1_losses_bit_exact(bmx, oracle, ctx):
+    """901-element ring with apertures and RF, a beam wide enough to lose about half of its
+    particles in 30 turns: state code, loss element and loss turn identical for every particle.
+    (Coordinates of these large-amplitude survivors are chaotic and are not compared.)"""
+    L = bmx.lattices
+    arr = bmx.elements.to_array(L.ring1())
+    beam = P.random_beam(bmx, 3000, seed=21, scale=25.0, pz_sigma=4e-3)
+    ref = P.copy_beam(beam)
+    oracle.track_bunch(arr, ref, n_turns=30)
+    out = bmx.tracking.track_beam_host(arr, beam, 30)
+    lost = ref["state"] != 1
+    assert lost.sum() > 500, "test beam should lose particles"
+    assert len(np.unique(ref["loss_ele"][lost])) > 50
+    bad = P.loss_mismatch(out, ref)
+    assert bad.size == 0, (bad[:10], out["loss_ele"][bad[:10]], ref["loss_ele"][bad[:10]])
+
+
+def test_ring1_regular_orbits_1000_turns(bmx, oracle, ctx):
+    """The benchmark beam (3x sigmas) on the 901-element ring for 1000 turns: identical losses
+    (about 60% of this beam leaves the ring's ~1e-3 momentum acceptance) and <= 1e-9 relative
+    for every survivor (regular orbits)."""
+    L = bmx.lattices
+    arr = bmx.elements.to_array(L.ring1())
+    beam = L.gaussian_beam(1024, seed=1, sigma=3 * L.SIGMA0)
+    ref = P.copy_beam(beam)
+    oracle.track_bunch(arr, ref, n_turns=1000)
+    out = bmx.tracking.track_beam_host(arr, beam, 1000)
+    assert P.loss_mismatch(out, ref).size == 0
+    alive = ref["state"] == 1
+    assert alive.sum() > 200
+    err = P.rel_err(out, ref, alive)
+    assert err <= 1e-9, err
+
+
+def test_chunked_equals_single_call(bmx, ctx):
+    """Tracking 3 + 4 turns (turn0 = 3 on the second call) equals 7 turns in one call."""
+    L = bmx.lattices
+    arr = bmx.elements.to_array(L.ring1(n_cells=10))
+    beam = P.random_beam(bmx, 1500, seed=2, scale=20.0)
+    lat = bmx.tracking.Lattice(ctx, arr)
+    b1 = bmx.tracking.Beam.from_host(ctx, beam)
+    bmx.tracking.track_beam(ctx, lat, b1, 7)
+    b2 = bmx.tracking.Beam.from_host(ctx, beam)
+    bmx.tracking.track_beam(ctx, lat, b2, 3, 0)
+    bmx.tracking.track_beam(ctx, lat, b2, 4, 3)
+    ctx.synchronize()
+    o1, o2 = b1.download(), b2.download()
+    for k in o1:
+        assert np.array_equal(o1[k], o2[k]), k
+
+
+def test_edge_sizes(bmx, oracle, ctx):
+    """Empty beam, a single particle, ragged sizes and an all-lost beam."""
+    L = bmx.lattices
+    arr = bmx.elements.to_array(L.ring1(n_cells=3))
+    lat = bmx.tracking.Lattice(ctx, arr)
+    empty = bmx.tracking.Beam(ctx, 0)
+    bmx.tracking.track_beam(ctx, lat, empty, 5)
+    ctx.synchronize()
+    assert empty.count_alive() == 0
+    for n in (1, 63, 65, 257, 1000):
+        beam = P.random_beam(bmx, n, seed=n, scale=20.0)
+        ref = P.copy_beam(beam)
+        oracle.track_bunch(arr, ref, 5)
+        out = bmx.tracking.track_beam_host(arr, beam, 5)
+        assert P.loss_mismatch(out, ref).size == 0
+        assert P.rel_err(out, ref, ref["state"] == 1) <= 1e-12
+    dead = P.random_beam(bmx, 300, seed=1)
+    dead["state"][:] = 5
+    out = bmx.tracking.track_beam_host(arr, dead, 5)
+    assert np.array_equal(out["x"], dead["x"]) and np.all(out["state"] == 5)
+
+
+def test_unsupported_element_raises(bmx, ctx):
+    E = bmx.elements
+    p0c, et = bmx.lattices.reference()
+    bad = [E.make_element(E.QUADRUPOLE, p0c, et, l=0.3, k1=0.5, x_offset=1e-3)]
+    with pytest.raises(bmx.native.BmxError) as ei:
+        bmx.tracking.Lattice(ctx, bad)
+    assert ei.value.code == 2

@@ -1,0 +1,128 @@
+"""The C-ABI library on the host: it loads, exports what include/bmx.h declares, and its table
+compiler accepts the benchmark rings and rejects what the GPU path does not implement."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "bmx.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\**\s*(bmx_\w+)\s*\(", src, flags=re.M)))
+
+
+def test_header_symbols_exported(bmx):
+    lib = bmx.native.load()
+    names = declared_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert lib.bmx_version() == 1
+
+
+def test_ele_struct_layout_matches_header(bmx, tmp_path):
+    """sizeof/offsetof of bmx_ele seen by a C compiler equal the ctypes mirror."""
+    E = bmx.elements
+    c = tmp_path / "lay.c"
+    fields = ["l", "ds_step", "p0c", "a_pole", "b_pole", "pz_aperture_center", "r0_mag"]
+    body = "\n".join(f'printf("%zu\\n", offsetof(bmx_ele, {f}));' for f in fields)
+    c.write_text(f'#include <stdio.h>\n#include <stddef.h>\n#include "{ROOT}/include/bmx.h"\n'
+                 f'int main(void){{printf("%zu\\n", sizeof(bmx_ele));{body}return 0;}}\n')
+    exe = tmp_path / "lay"
+    subprocess.run(["gcc", str(c), "-o", str(exe)], check=True)
+    vals = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    assert vals[0] == ctypes.sizeof(E.BmxEle)
+    for f, off in zip(fields, vals[1:]):
+        assert getattr(E.BmxEle, f).offset == off, f
+
+
+def _validate(bmx, eles):
+    arr = bmx.elements.to_array(eles)
+    n = ctypes.c_int32()
+    rc = bmx.native.load().bmx_lattice_validate(arr.ctypes.data, len(arr), bmx.elements.M_ELECTRON,
+                                                -1, ctypes.byref(n))
+    return rc, n.value, bmx.native.load().bmx_last_error().decode()
+
+
+def test_benchmark_rings_compile(bmx):
+    L = bmx.lattices
+    for eles in (L.ring0(), L.ring1(), L.ring1_variants(seed=2, n_variants=2)[1]):
+        rc, nprm, msg = _validate(bmx, eles)
+        assert rc == 0, msg
+        assert 0 < nprm * 8 < 160 * 1024  # a whole ring table is far below one CU's LDS
+
+
+@pytest.mark.parametrize("attrs,code", [
+    (dict(key=3, l=0.3, k1=0.5, x_offset=1e-3), 2),       # misaligned quad
+    (dict(key=3, l=0.3, k1=0.5, tilt=0.1), 2),            # tilted quad
+    (dict(key=2, l=1.0, g=0.01, k1=0.1), 2),              # bend with k1
+    (dict(key=2, l=1.0, g=0.01, fringe_type=4), 2),       # full bend fringe
+    (dict(key=9, l=0.5, voltage=1e6, rf_frequency=0.0), 2),
+    (dict(key=23, l=1.0), 2),                             # solenoid: not on this path
+    (dict(key=1, l=1.0, aperture_at=5, x_limit=0.01), 2),  # continuous aperture
+])
+def test_outside_path_rejected(bmx, attrs, code):
+    E = bmx.elements
+    p0c, et = bmx.lattices.reference()
+    key = attrs.pop("key")
+    e = E.make_element(key, p0c, et, **attrs)
+    rc, _, msg = _validate(bmx, [e])
+    assert rc == code, msg
+    assert "element 1" in msg
+
+
+def test_p0c_mismatch_rejected(bmx):
+    E = bmx.elements
+    p0c, et = bmx.lattices.reference()
+    a = E.make_element(E.DRIFT, p0c, et, l=1.0)
+    b = E.make_element(E.DRIFT, p0c * 1.01, et, l=1.0)
+    rc, _, msg = _validate(bmx, [a, b])
+    assert rc == 4 and "element 2" in msg
+
+
+def test_misaligned_drift_accepted(bmx):
+    """track1_bmad tracks drifts without offset_particle (track1_bmad.f90:93-94)."""
+    E = bmx.elements
+    p0c, et = bmx.lattices.reference()
+    rc, _, msg = _validate(bmx, [E.make_element(E.DRIFT, p0c, et, l=4.5, x_offset=0.05, tilt=0.1)])
+    assert rc == 0, msg
+
+
+def test_bookkeeping_steps(bmx):
+    """ds_step bookkeeping (attribute_bookkeeper.f90:292-398 + check_bend) on the rings."""
+    L = bmx.lattices
+    eles = L.ring1()
+    sext = [e for e in eles if e.key == 5][0]
+    rf = [e for e in eles if e.key == 9][0]
+    sb = [e for e in eles if e.key == 2][0]
+    assert sext.ds_step == pytest.approx(0.2)          # num_steps = 1
+    assert rf.ds_step == pytest.approx(0.05)           # num_steps = 10 default
+    assert sb.ds_step == pytest.approx(1.0)            # check_bend met = 4 -> 2 steps
+    assert sb.fringe_type == 7 and sb.fintx == sb.fint
+
+
+def test_no_device_calls_fail_cleanly(bmx):
+    """Without a GPU, creating a context fails with an error code, not a crash."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    h = ctypes.c_void_p()
+    rc = bmx.native.load().bmx_ctx_create(0, ctypes.byref(h))
+    assert rc != 0
+    assert bmx.native.load().bmx_last_error()
+
+
+def test_gaussian_beam_is_seeded(bmx):
+    L = bmx.lattices
+    a = L.gaussian_beam(1000, seed=4)
+    b = L.gaussian_beam(1000, seed=4)
+    for k in a:
+        assert np.array_equal(a[k], b[k])
+    assert abs(np.std(a["x"]) - 100e-6) < 10e-6
+    assert a["beta"].dtype == np.float64 and np.all(a["beta"] < 1)
