@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU time of the cpu_baseline sample (0 = skip)")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--maps", choices=["fast", "reference"], default="fast",
+                    help="formulation of the element maps")
     return ap.parse_args()
 
 
@@ -68,8 +70,9 @@ def cpu_baseline(bmx, arr, beam, target_s):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X and has no "
-              "multi-GPU path (see DESIGN.md for how tracking would shard).", file=sys.stderr)
+        print("bench.py measures one GPU: tracking here runs on a single MI355X and has no "
+              "multi-GPU path (particles are independent; a beam would shard by particle "
+              "ranges, one process per GPU).", file=sys.stderr)
         sys.exit(2)
     bmx = load_package()
     L, E, T = bmx.lattices, bmx.elements, bmx.tracking
@@ -79,6 +82,7 @@ def main():
     flops = bmx.flops.per_element_pass(arr)
 
     ctx = T.Context(0)
+    ctx.set_maps(args.maps)
     lat = T.Lattice(ctx, arr)
     tps = args.turns_per_step
     if args.warmup > 0:
@@ -126,6 +130,7 @@ def main():
                         f"{args.steps * tps} turns in steps of {tps}",
             "particles": args.particles, "elements": len(arr), "turns": args.steps * tps,
             "turns_per_step": tps, "alive_at_end": alive, "wall_s": t_wall,
+            "maps": args.maps,
         },
         "roofline": {
             "bound": "mfma", "compute": "fp64 vector ALU", "achieved": achieved,

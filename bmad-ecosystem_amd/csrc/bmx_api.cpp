@@ -91,6 +91,14 @@ int32_t bmx_ctx_synchronize(bmx_ctx *ctx)
 
 void *bmx_ctx_stream(bmx_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
+int32_t bmx_ctx_set_maps(bmx_ctx *ctx, int32_t form)
+{
+    if (!ctx || (form != BMX_MAPS_FAST && form != BMX_MAPS_REFERENCE))
+        return set_error(BMX_ERR_ARG, "bmx_ctx_set_maps: bad argument");
+    ctx->fast_maps = form == BMX_MAPS_FAST;
+    return BMX_OK;
+}
+
 int32_t bmx_ctx_record(bmx_ctx *ctx, int32_t slot)
 {
     if (!ctx || slot < 0 || slot >= BMX_N_EVENTS) return set_error(BMX_ERR_ARG, "bmx_ctx_record: bad slot");
@@ -298,7 +306,7 @@ int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t
         return set_error(BMX_ERR_ARG, "bmx_track_turns: handles on different devices");
     hipSetDevice(ctx->device);
     HIP_TRY(launch_track_turns(ctx->stream, L->d_desc, nullptr, 0, B->d, B->n, n_turns, turn0,
-                               ctx->d_work),
+                               ctx->d_work, ctx->fast_maps),
             "k_track_turns");
     return BMX_OK;
 }
@@ -308,7 +316,8 @@ int32_t bmx_track_element(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32
     if (!ctx || !L || !B) return set_error(BMX_ERR_ARG, "bmx_track_element: null argument");
     if (ix_ele < 1 || ix_ele > L->desc.n_ele) return set_error(BMX_ERR_ARG, "bmx_track_element: ix_ele out of range");
     hipSetDevice(ctx->device);
-    HIP_TRY(launch_track_element(ctx->stream, L->desc, ix_ele - 1, turn, B->d, B->n, ctx->d_work), "k_track_element");
+    HIP_TRY(launch_track_element(ctx->stream, L->desc, ix_ele - 1, turn, B->d, B->n, ctx->d_work,
+                                 ctx->fast_maps), "k_track_element");
     return BMX_OK;
 }
 
@@ -319,7 +328,8 @@ int32_t bmx_track_turns_plain(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, i
     hipSetDevice(ctx->device);
     for (int32_t t = 0; t < n_turns; t++)
         for (int32_t e = 0; e < L->desc.n_ele; e++)
-            HIP_TRY(launch_track_element(ctx->stream, L->desc, e, turn0 + t, B->d, B->n, ctx->d_work), "k_track_element");
+            HIP_TRY(launch_track_element(ctx->stream, L->desc, e, turn0 + t, B->d, B->n, ctx->d_work,
+                                         ctx->fast_maps), "k_track_element");
     return BMX_OK;
 }
 

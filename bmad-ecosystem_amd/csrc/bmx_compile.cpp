@@ -211,11 +211,20 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
     out.sp.mc2 = mc2;
     out.sp.p0c = p0c;
     out.sp.mc2_sq = mc2 * mc2;
+    const double e_tot = eles[0].e_tot;
+    if (!(e_tot > 0)) { err = "element 1 has no reference energy"; return BMX_ERR_ARG; }
+    out.sp.beta0 = p0c / e_tot;
+    {
+        const double me = mc2 / e_tot;
+        out.sp.me2 = me * me;
+        out.sp.lez_f = out.sp.beta0 * out.sp.beta0 * (2 * (out.sp.beta0 * out.sp.beta0) - (me * me) / 2);
+    }
+    out.sp.lez_thr = 3e-7 * e_tot;
 
     for (int i = 0; i < n_ele; i++) {
         const bmx_ele &e = eles[i];
         if (e.key > 0 && e.key < 64) out.kind_mask |= (uint64_t)1 << e.key;
-        if (e.p0c != p0c) {
+        if (e.p0c != p0c || e.e_tot != e_tot) {
             err = where(i, e) + "reference momentum differs from element 1 (energy changing lattices are outside this path)";
             return BMX_ERR_P0C;
         }
@@ -289,7 +298,23 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             const double beta0 = p0c / e.e_tot;
             const double me = mc2 / e.e_tot;
             q = {e.l, 1.0 * b1, beta0, me * me, beta0 * beta0 * (2 * (beta0 * beta0) - (me * me) / 2),
-                 3e-7 * e.e_tot, mc2};
+                 3e-7 * e.e_tot, mc2, b1 * e.l, b1 / e.l};
+            // Series terms for the fast form: W = b1 L/(1+pz), sized for 1+pz >= 0.5 (further
+            // off momentum the kernel halves the argument); take the fewest terms whose first
+            // omitted term t^M/(4M)! is below 1e-18, and record the t range they cover.
+            const double wmax = std::fabs(b1 * e.l) / 0.5;
+            const double tmax = wmax * wmax;
+            double fact[40];
+            fact[0] = 1;
+            for (int k = 1; k < 40; k++) fact[k] = fact[k - 1] * k;
+            int nterms = 8;
+            for (int m = 2; m <= 8; m++) {
+                if (std::pow(tmax, m) / fact[4 * m] <= 1e-18) { nterms = m; break; }
+            }
+            h.n = nterms;
+            for (int m = 0; m < 8; m++)
+                for (int j = 0; j < 4; j++) q.push_back(1.0 / fact[4 * m + j]);
+            q.push_back(std::pow(1e-18 * fact[4 * nterms], 1.0 / nterms));  // t_lim
             break;
         }
         case BMX_SBEND: {
@@ -322,6 +347,9 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             q[9] = h_sincc(angle);
             edge_constants(e, true, &q[10]);
             edge_constants(e, false, &q[16]);
+            q[22] = L * L * g * q[7];
+            q[23] = L / q[8];
+            if (std::fabs(angle) <= 0.1) h.flags |= F_BEND_SERIES;
             break;
         }
         case BMX_RFCAVITY: {
@@ -352,8 +380,8 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
         }
         if ((int)q.size() != kParams[h.kind]) { err = where(i, e) + "internal parameter count"; return BMX_ERR_ARG; }
         h.off = (int32_t)out.prm.size();
-        out.prm.insert(out.prm.end(), q.begin(), q.end());
         if (h.flags & (F_APE_ENT | F_APE_EXIT)) out.prm.insert(out.prm.end(), ap, ap + 4);
+        out.prm.insert(out.prm.end(), q.begin(), q.end());
         out.hdr.push_back(h);
     }
     return BMX_OK;

@@ -13,9 +13,10 @@ namespace bmx {
 // ---- kernel launchers (bmx_kernels.hip) ----
 hipError_t launch_track_turns(hipStream_t s, const LatDesc *d_lats, const Tile *d_tiles,
                               int64_t n_tiles, const BeamPtrs &b, int64_t n, int32_t n_turns,
-                              int32_t turn0, unsigned long long *d_work);
+                              int32_t turn0, unsigned long long *d_work, bool fast);
 hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, int32_t turn,
-                                const BeamPtrs &b, int64_t n, unsigned long long *d_work);
+                                const BeamPtrs &b, int64_t n, unsigned long long *d_work,
+                                bool fast);
 hipError_t launch_count_alive(hipStream_t s, const int32_t *state, int64_t n,
                               unsigned long long *d_out);
 
@@ -41,6 +42,7 @@ struct bmx_ctx {
     unsigned long long *d_counter = nullptr;  // scratch for reductions
     unsigned long long *d_work = nullptr;     // particle-element passes since the last take
     hipEvent_t events[BMX_N_EVENTS] = {};
+    bool fast_maps = true;  // BMX_MAPS_FAST (default) or BMX_MAPS_REFERENCE
 };
 
 struct bmx_lattice {

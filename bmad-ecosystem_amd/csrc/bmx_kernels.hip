@@ -52,6 +52,7 @@ __device__ __forceinline__ void add_work(unsigned long long *work, unsigned long
 
 // Persistent tracking over tiles (tiles == nullptr: implicit tiles of kBlock particles of
 // lattice 0 covering [0, n) with the given n_turns / turn0).
+template <bool F>
 __global__ void __launch_bounds__(kBlock) k_track_turns(const LatDesc *__restrict__ lats,
                                                         const Tile *__restrict__ tiles, BeamPtrs b,
                                                         int64_t n, int32_t n_turns, int32_t turn0,
@@ -87,10 +88,11 @@ __global__ void __launch_bounds__(kBlock) k_track_turns(const LatDesc *__restric
             lost_e = 1;
             lost_t = t.turn0;
         } else {
+            const CHdr hdr = (CHdr)L.hdr;
+            const CP prm = (CP)L.prm;
             for (int32_t turn = 0; turn < t.n_turns; ++turn) {
                 for (int32_t e = 0; e < L.n_ele; ++e) {
-                    const EleHdr h = L.hdr[e];
-                    track_element(p, h, L.prm + h.off, L.sp);
+                    track_element<F>(p, hdr + 4 * e, prm, L.sp);
                     if (p.state != ST_ALIVE) {
                         lost_e = e + 1;
                         lost_t = t.turn0 + turn;
@@ -114,6 +116,7 @@ __global__ void __launch_bounds__(kBlock) k_track_turns(const LatDesc *__restric
 }
 
 // One element, whole bunch: track1 (track1.f90:44-343) per live particle.
+template <bool F>
 __global__ void __launch_bounds__(kBlock) k_track_element(LatDesc L, int32_t ie, int32_t turn,
                                                           BeamPtrs b, int64_t n,
                                                           unsigned long long *work)
@@ -124,10 +127,7 @@ __global__ void __launch_bounds__(kBlock) k_track_element(LatDesc L, int32_t ie,
         Particle p;
         load_particle(p, b, i);
         refresh_energy(p, L.sp);
-        if (!orbit_too_large(p)) {
-            const EleHdr h = L.hdr[ie];
-            track_element(p, h, L.prm + h.off, L.sp);
-        }
+        if (!orbit_too_large(p)) track_element<F>(p, (CHdr)L.hdr + 4 * ie, (CP)L.prm, L.sp);
         store_particle(p, b, i);
         if (p.state != ST_ALIVE) {
             b.state[i] = p.state;
@@ -156,22 +156,31 @@ __global__ void __launch_bounds__(kBlock) k_count_alive(const int32_t *__restric
 
 hipError_t launch_track_turns(hipStream_t s, const LatDesc *d_lats, const Tile *d_tiles,
                               int64_t n_tiles, const BeamPtrs &b, int64_t n, int32_t n_turns,
-                              int32_t turn0, unsigned long long *d_work)
+                              int32_t turn0, unsigned long long *d_work, bool fast)
 {
     int64_t grid = d_tiles ? n_tiles : (n + kBlock - 1) / kBlock;
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_track_turns, dim3((unsigned)grid), dim3(kBlock), 0, s, d_lats, d_tiles, b,
-                       n, n_turns, turn0, d_work);
+    if (fast)
+        hipLaunchKernelGGL(k_track_turns<true>, dim3((unsigned)grid), dim3(kBlock), 0, s, d_lats,
+                           d_tiles, b, n, n_turns, turn0, d_work);
+    else
+        hipLaunchKernelGGL(k_track_turns<false>, dim3((unsigned)grid), dim3(kBlock), 0, s, d_lats,
+                           d_tiles, b, n, n_turns, turn0, d_work);
     return hipGetLastError();
 }
 
 hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, int32_t turn,
-                                const BeamPtrs &b, int64_t n, unsigned long long *d_work)
+                                const BeamPtrs &b, int64_t n, unsigned long long *d_work,
+                                bool fast)
 {
     int64_t grid = (n + kBlock - 1) / kBlock;
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_track_element, dim3((unsigned)grid), dim3(kBlock), 0, s, lat, ie, turn, b, n,
-                       d_work);
+    if (fast)
+        hipLaunchKernelGGL(k_track_element<true>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat, ie,
+                           turn, b, n, d_work);
+    else
+        hipLaunchKernelGGL(k_track_element<false>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat, ie,
+                           turn, b, n, d_work);
     return hipGetLastError();
 }
 

@@ -2,16 +2,32 @@
 //
 // One lane tracks one particle; the particle lives in VGPRs for the whole call. Element
 // parameters are wave-uniform and are read through scalar loads (SGPR operands), so every
-// branch on the element kind is uniform across the wavefront. The maps restate Bmad's
-// bmad_standard tracking (file:line citations below); operation order may differ from the
-// reference at the rounding level (FMA contraction, reciprocal reuse), which the parity
-// tests bound at 1e-12 relative per element.
+// branch on the element kind is uniform across the wavefront.
+//
+// Two formulations of the drift, quadrupole and bend bodies, selected by the template flag F:
+//   F = false ("reference form"): Bmad's formulas as written (sqrt, divisions, sin/cos/asin/
+//             atan2/sinh/cosh of the particle's coordinates per element), file:line cited.
+//   F = true  ("fast form", the default): the same maps re-expressed without per-particle
+//             transcendentals and with most divisions replaced by reciprocals cached per
+//             particle -- convergent series with bounded truncation (< 1e-17 relative) where
+//             the arguments are small, angle-addition identities in the bend, rsqrt-based
+//             square roots. Outside the series domains each map falls back to the reference
+//             form for that particle.
+// Both agree with the CPU restatement to the rounding level (<= 1e-12 relative per element,
+// checked by the GPU parity tests); element-loss decisions are identical away from the
+// aperture edges.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include "bmx_table.hpp"
 
 namespace bmx {
+
+// Element parameters are read through the constant address space: wave-uniform indices into
+// it compile to scalar loads (s_load) whose results sit in SGPRs.
+#define BMX_CONST __attribute__((address_space(4)))
+typedef const double BMX_CONST *CP;
+typedef const int32_t BMX_CONST *CHdr;
 
 constexpr double kPi = 3.141592653589793238462643383279;
 constexpr double kTwoPi = 2 * kPi;
@@ -26,9 +42,11 @@ enum : int32_t {
 struct Particle {
     double x, px, y, py, z, pz, beta;
     int32_t state;
-    // Invariants of the energy, refreshed whenever pz changes (only in rf cavities here):
-    double rel_p;  // 1 + pz
-    double ez;     // sqrt_one(mc2^2 (2 pz + pz^2) / ((p0c rel_p)^2 + mc2^2)), track_a_drift.f90:69
+    // Functions of the energy, refreshed whenever pz or beta changes (only in rf cavities):
+    double rel_p;   // 1 + pz
+    double inv_rp;  // 1 / (1 + pz)
+    double ez;      // sqrt_one(mc2^2 (2 pz + pz^2) / ((p0c rel_p)^2 + mc2^2)), track_a_drift.f90:69
+    double lez;     // low_energy_z_correction per metre (low_energy_z_correction.f90:44-57)
 };
 
 __device__ __forceinline__ double sqr(double v) { return v * v; }
@@ -42,7 +60,13 @@ __device__ __forceinline__ double sinc(double v) { return fabs(v) < 1e-8 ? 1.0 :
 __device__ __forceinline__ void refresh_energy(Particle &p, const Species &sp)
 {
     p.rel_p = 1 + p.pz;
+    p.inv_rp = 1 / p.rel_p;
     p.ez = sqrt_one((sp.mc2_sq * (2 * p.pz + p.pz * p.pz)) / (sqr(sp.p0c * p.rel_p) + sp.mc2_sq));
+    const double pz = p.pz;
+    if (sp.mc2 * sqr(sp.beta0 * pz) < sp.lez_thr)
+        p.lez = pz * (1 - 3 * (pz * sp.beta0 * sp.beta0) / 2 + pz * pz * sp.lez_f) * sp.me2;
+    else
+        p.lez = (p.beta - sp.beta0) / sp.beta0;
 }
 
 // bmad/low_level/orbit_too_large.f90:26-120 (charged particle, momentum check on).
@@ -64,7 +88,7 @@ __device__ __forceinline__ bool orbit_too_large(Particle &p)
 
 // Aperture test of check_aperture_limit.f90:256-324 (elliptical with strict r < 1, or
 // rectangular). ap[] layout in bmx_table.hpp.
-__device__ __forceinline__ void aperture(Particle &p, int32_t flags, const double *__restrict__ ap)
+__device__ __forceinline__ void aperture(Particle &p, int32_t flags, CP ap)
 {
     if (flags & F_APE_ELLIP) {
         const double xp = p.x - ap[0], yp = p.y - ap[2];
@@ -86,11 +110,42 @@ __device__ __forceinline__ void aperture(Particle &p, int32_t flags, const doubl
     }
 }
 
-// bmad/low_level/track_a_drift.f90:25-116 (direction = orientation = +1, beta > 0).
-// sqrt_one(-pxy2) reuses ps_rel = sqrt(1 - pxy2), the same rounded square root.
+// ---- drift -------------------------------------------------------------------------------
+
+// q(p) = 1/sqrt(1 - p) - 1 = sum_k (2k)!/(4^k k!^2) p^k, p = (px^2 + py^2)/(1 + pz)^2.
+// For p < 2^-10 the terms after p^6 are below 1e-18 relative.
+constexpr double kDriftSeriesMax = 1.0 / 1024;
+__device__ __forceinline__ double inv_ps_minus_1(double p)
+{
+    double q = 0.2255859375;              // 231/1024
+    q = fma(q, p, 0.24609375);            // 63/256
+    q = fma(q, p, 0.2734375);             // 35/128
+    q = fma(q, p, 0.3125);                // 5/16
+    q = fma(q, p, 0.375);                 // 3/8
+    q = fma(q, p, 0.5);                   // 1/2
+    return q * p;
+}
+
+// bmad/low_level/track_a_drift.f90:25-116 (direction = orientation = +1, beta > 0):
+//   x += L px_rel/ps, y += L py_rel/ps, z += L (ez + sqrt_one(-pxy2)/ps), lost_pz if pxy2 >= 1.
+// Fast form: 1/ps = 1 + q(pxy2) and sqrt_one(-pxy2)/ps = 1 - 1/ps = -q(pxy2).
+template <bool F>
 __device__ __forceinline__ void drift(Particle &p, double L)
 {
     if (L == 0) return;
+    if (F) {
+        const double px_rel = p.px * p.inv_rp;
+        const double py_rel = p.py * p.inv_rp;
+        const double pxy2 = px_rel * px_rel + py_rel * py_rel;
+        if (pxy2 < kDriftSeriesMax) {
+            const double q = inv_ps_minus_1(pxy2);
+            const double lx = L * px_rel, ly = L * py_rel;
+            p.x = p.x + fma(lx, q, lx);
+            p.y = p.y + fma(ly, q, ly);
+            p.z = p.z + L * (p.ez - q);
+            return;
+        }
+    }
     const double px_rel = p.px / p.rel_p;
     const double py_rel = p.py / p.rel_p;
     const double pxy2 = px_rel * px_rel + py_rel * py_rel;
@@ -104,53 +159,108 @@ __device__ __forceinline__ void drift(Particle &p, double L)
     p.z = p.z + L * (p.ez + (-pxy2 / (ps + 1)) / ps);
 }
 
+// ---- quadrupole --------------------------------------------------------------------------
+
 // bmad/low_level/quad_mat2_calc.f90:22-81: 2x2 map and z coefficients of one plane with
-// strength k (k > 0 defocusing). cx/sx are precomputed by the caller for |k|.
+// strength k (k > 0 defocusing), given cx = cos/cosh(sqrt|k| L) and sx = sin/sinh(..)/sqrt|k|.
 struct Plane {
     double m11, m12, m21, m22, z1, z2, z3;
 };
 
-__device__ __forceinline__ Plane quad_plane(double k, double L, double rel_p, double cx, double sx)
+__device__ __forceinline__ Plane quad_plane(double k, double L, double rel_p, double inv_rp,
+                                            double cx, double sx, bool F)
 {
     Plane q;
     q.m11 = cx;
-    q.m12 = sx / rel_p;
-    q.m21 = k * sx * rel_p;
     q.m22 = cx;
-    q.z1 = k * (-cx * sx + L) / 4;
-    q.z2 = -k * sx * sx / (2 * rel_p);
-    q.z3 = -(cx * sx + L) / (4 * rel_p * rel_p);
+    if (F) {
+        q.m12 = sx * inv_rp;
+        q.m21 = k * sx * rel_p;
+        q.z1 = k * (L - cx * sx) * 0.25;
+        q.z2 = -k * sx * sx * inv_rp * 0.5;
+        q.z3 = -(cx * sx + L) * (inv_rp * inv_rp) * 0.25;
+    } else {
+        q.m12 = sx / rel_p;
+        q.m21 = k * sx * rel_p;
+        q.z1 = k * (-cx * sx + L) / 4;
+        q.z2 = -k * sx * sx / (2 * rel_p);
+        q.z3 = -(cx * sx + L) / (4 * rel_p * rel_p);
+    }
     return q;
 }
 
 // bmad/low_level/track_a_quadrupole.f90:17-148 for n_step = 1 (no multipoles other than k1),
 // no fringe, no misalignment; low_energy_z_correction.f90:22-58.
-__device__ __forceinline__ void quadrupole(Particle &p, const double *__restrict__ q)
+// Fast form: with W = k1 L^2/(1+pz) the plane functions are C(-+W), L S(-+W) where
+// C(w) = sum w^n/(2n)! (= cos sqrt(-w) or cosh sqrt(w)) and S(w) = sum w^n/(2n+1)!; both
+// planes come from four polynomials in t = W^2 (even / odd parts), n_terms each, valid for
+// t <= t_lim (first omitted term below 1e-18); larger |W| is reduced by halving.
+template <bool F>
+__device__ __forceinline__ void quadrupole(Particle &p, CP q,
+                                           int32_t n_terms)
 {
     const double L = q[0], b1 = q[1];
-    const double k1 = b1 / (L * p.rel_p);
-    const double sqrt_k = sqrt(fabs(k1));
-    const double sk_l = sqrt_k * L;
     double cx_x, sx_x, cx_y, sx_y;  // planes: x uses k = -k1, y uses k = +k1
-    if (fabs(sk_l) < 1e-10) {
-        const double kl2x = -k1 * L * L, kl2y = k1 * L * L;
-        cx_x = 1 + kl2x / 2;
-        sx_x = (1 + kl2x / 6) * L;
-        cx_y = 1 + kl2y / 2;
-        sx_y = (1 + kl2y / 6) * L;
+    double k1;
+    if constexpr (F) {
+        k1 = q[8] * p.inv_rp;                 // b1/L / (1 + pz)
+        double W = q[7] * p.inv_rp;           // b1 L / (1 + pz)
+        double t = W * W;
+        // Argument halving for large |W| (far off-momentum particles): C(4w) = 2 C(w)^2 - 1,
+        // S(4w) = S(w) C(w), the double-angle identities of cos/cosh and sin/sinh.
+        int halvings = 0;
+        const double t_lim = q[41];
+        while (t > t_lim && halvings < 64) {
+            W *= 0.25;
+            t *= 0.0625;
+            halvings++;
+        }
+        // A = sum t^m/(4m)!, B = sum t^m/(4m+2)!, A1 = sum t^m/(4m+1)!, B1 = sum t^m/(4m+3)!
+        CP c = q + 9;  // per term: 1/(4m)!, 1/(4m+1)!, 1/(4m+2)!, 1/(4m+3)!
+        double A = c[4 * (n_terms - 1)], A1 = c[4 * (n_terms - 1) + 1];
+        double B = c[4 * (n_terms - 1) + 2], B1 = c[4 * (n_terms - 1) + 3];
+        for (int m = n_terms - 2; m >= 0; m--) {
+            A = fma(A, t, c[4 * m]);
+            A1 = fma(A1, t, c[4 * m + 1]);
+            B = fma(B, t, c[4 * m + 2]);
+            B1 = fma(B1, t, c[4 * m + 3]);
+        }
+        double cxx = fma(-W, B, A), cxy = fma(W, B, A);
+        double sxx = fma(-W, B1, A1), sxy = fma(W, B1, A1);
+        for (int h = 0; h < halvings; h++) {
+            sxx = sxx * cxx;
+            sxy = sxy * cxy;
+            cxx = fma(2 * cxx, cxx, -1.0);
+            cxy = fma(2 * cxy, cxy, -1.0);
+        }
+        cx_x = cxx;
+        cx_y = cxy;
+        sx_x = L * sxx;
+        sx_y = L * sxy;
     } else {
-        double s, c;
-        sincos(sk_l, &s, &c);
-        const double ch = cosh(sk_l), sh = sinh(sk_l);
-        const double sf = s / sqrt_k, sd = sh / sqrt_k;
-        if (k1 > 0) {  // x: k = -k1 < 0 focusing; y: defocusing
-            cx_x = c; sx_x = sf; cx_y = ch; sx_y = sd;
+        k1 = b1 / (L * p.rel_p);
+        const double sqrt_k = sqrt(fabs(k1));
+        const double sk_l = sqrt_k * L;
+        if (fabs(sk_l) < 1e-10) {
+            const double kl2x = -k1 * L * L, kl2y = k1 * L * L;
+            cx_x = 1 + kl2x / 2;
+            sx_x = (1 + kl2x / 6) * L;
+            cx_y = 1 + kl2y / 2;
+            sx_y = (1 + kl2y / 6) * L;
         } else {
-            cx_x = ch; sx_x = sd; cx_y = c; sx_y = sf;
+            double s, c;
+            sincos(sk_l, &s, &c);
+            const double ch = cosh(sk_l), sh = sinh(sk_l);
+            const double sf = s / sqrt_k, sd = sh / sqrt_k;
+            if (k1 > 0) {  // x: k = -k1 < 0 focusing; y: defocusing
+                cx_x = c; sx_x = sf; cx_y = ch; sx_y = sd;
+            } else {
+                cx_x = ch; sx_x = sd; cx_y = c; sx_y = sf;
+            }
         }
     }
-    const Plane a = quad_plane(-k1, L, p.rel_p, cx_x, sx_x);
-    const Plane b = quad_plane(k1, L, p.rel_p, cx_y, sx_y);
+    const Plane a = quad_plane(-k1, L, p.rel_p, p.inv_rp, cx_x, sx_x, F);
+    const Plane b = quad_plane(k1, L, p.rel_p, p.inv_rp, cx_y, sx_y, F);
     const double x = p.x, px = p.px, y = p.y, py = p.py;
     p.z = p.z + (a.z1 * x * x + a.z2 * x * px + a.z3 * px * px + b.z1 * y * y + b.z2 * y * py +
                  b.z3 * py * py);
@@ -158,20 +268,25 @@ __device__ __forceinline__ void quadrupole(Particle &p, const double *__restrict
     p.px = a.m21 * x + a.m22 * px;
     p.y = b.m11 * y + b.m12 * py;
     p.py = b.m21 * y + b.m22 * py;
-    // low_energy_z_correction
-    const double beta0 = q[2], me2 = q[3], f = q[4], thr = q[5];
-    const double pz = p.pz;
-    if (q[6] * sqr(beta0 * pz) < thr)
-        p.z = p.z + L * pz * (1 - 3 * (pz * beta0 * beta0) / 2 + pz * pz * f) * me2;
-    else
-        p.z = p.z + L * (p.beta - beta0) / beta0;
+    if (F) {
+        p.z = p.z + L * p.lez;
+    } else {
+        const double beta0 = q[2], me2 = q[3], f = q[4], thr = q[5];
+        const double pz = p.pz;
+        if (q[6] * sqr(beta0 * pz) < thr)
+            p.z = p.z + L * pz * (1 - 3 * (pz * beta0 * beta0) / 2 + pz * pz * f) * me2;
+        else
+            p.z = p.z + L * (p.beta - beta0) / beta0;
+    }
 }
+
+// ---- thick multipole ---------------------------------------------------------------------
 
 // bmad/modules/multipole_mod.f90:82-174,314-401 (ab_multipole_kicks, magnetic): kick from
 // orders n in mask with strengths (a_n, b_n) * scale; c_multi's signed binomials summed as the
 // real and imaginary parts of (x + i y)^n: kx = -b Re + a Im, ky = a Re + b Im.
-__device__ __forceinline__ void multipole_kick(Particle &p, const double *__restrict__ ah,
-                                               const double *__restrict__ bh, int32_t mask,
+__device__ __forceinline__ void multipole_kick(Particle &p, CP ah,
+                                               CP bh, int32_t mask,
                                                double factor)
 {
     double kx = 0, ky = 0;
@@ -197,23 +312,26 @@ __device__ __forceinline__ void multipole_kick(Particle &p, const double *__rest
 // bmad/low_level/track_a_thick_multipole.f90:18-133 (magnetic, no misalignment):
 // half kick, then n_step x (drift, kick), the last kick halved. As in the reference, a drift
 // that loses the particle does not end the loop (track_a_drift does not test the state).
-__device__ __forceinline__ void thick_multipole(Particle &p, const double *__restrict__ m,
+template <bool F>
+__device__ __forceinline__ void thick_multipole(Particle &p, CP m,
                                                 int32_t n_step, int32_t mask)
 {
     const double step = m[0];
     multipole_kick(p, m + 1, m + 5, mask, 1.0);
     for (int i = 1; i <= n_step; i++) {
-        drift(p, step);
+        drift<F>(p, step);
         multipole_kick(p, m + 1, m + 5, mask, i == n_step ? 1.0 : 2.0);
     }
 }
 
+// ---- sbend -------------------------------------------------------------------------------
+
 // bmad/modules/fringe_mod.f90:241-419 hwang_bend_edge_kick (time_dir = +1), orbit only.
-// e[] = gt, gt2, gs2, k1_tane, fg_factor of that face; g_tot of the bend.
-__device__ __forceinline__ void hwang_edge(Particle &p, const double *__restrict__ e, double g_tot,
+// e[] = gt, gt2, gs2, k1_tane, fg_factor, tan_e^2 of that face; g_tot of the bend.
+__device__ __forceinline__ void hwang_edge(Particle &p, CP e, double g_tot,
                                            bool entering, bool simple)
 {
-    const double ef = 1 / p.rel_p;
+    const double ef = p.inv_rp;
     const double x = p.x, px = p.px, y = p.y, py = p.py;
     if (simple) {
         // e = 0, fint*hgap = 0, k1 = 0: gt = gt2 = k1_tane = fg = 0, gs2 = g_tot.
@@ -258,10 +376,98 @@ __device__ __forceinline__ void hwang_edge(Particle &p, const double *__restrict
     p.z = p.z + dz;
 }
 
+// asin(s)/s = sum_n (2n)!/(4^n n!^2 (2n+1)) s^(2n); for |s| <= 1/8 the terms after s^18 are
+// below 1e-18.
+constexpr double kArcSeriesMax = 0.125;
+__device__ __forceinline__ double asin_over(double s2)
+{
+    double r = 0.009761609529194078;
+    r = fma(r, s2, 0.011551800896139705);
+    r = fma(r, s2, 0.01396484375);
+    r = fma(r, s2, 0.017352764423076924);
+    r = fma(r, s2, 0.022372159090909092);
+    r = fma(r, s2, 0.030381944444444444);
+    r = fma(r, s2, 0.044642857142857144);
+    r = fma(r, s2, 0.075);
+    r = fma(r, s2, 0.16666666666666666);
+    return fma(r, s2, 1.0);
+}
+
+// h / sin(h) for the angle h with sine sh and cosine ch (|h| < pi): the asin(s)/s series
+// after k half-angle steps s' = s / sqrt(2 (1 + c)), c' = sqrt((1 + c)/2), which bring |s| to
+// the series domain; h / sin h = 2^k asin(s_k)/s_k * s_k / sh.
+__device__ __forceinline__ double h_over_sin(double sh, double ch)
+{
+    if (fabs(sh) <= kArcSeriesMax && ch > 0) return asin_over(sh * sh);
+    if (sh == 0) return 1.0;
+    double s = sh, c = ch, scale = 1.0;
+    for (int k = 0; k < 8 && !(fabs(s) <= kArcSeriesMax && c > 0); k++) {
+        const double t = sqrt(2 * (1 + c));
+        s = s / t;
+        c = 0.5 * t;
+        scale *= 2;
+    }
+    return scale * asin_over(s * s) * (s / sh);
+}
+
+// sin and cos of |phi| <= 4 (an rf phase wrapped to [-pi, pi)): quadrant reduction by pi/2
+// (two-part constant) and Taylor polynomials on |r| <= pi/4 (truncation below 1e-20).
+__device__ __forceinline__ void sincos_reduced(double phi, double *sp, double *cp)
+{
+    const double kPio2Hi = 1.5707963267948966;      // pi/2 rounded to double
+    const double kPio2Lo = 6.123233995736766e-17;   // pi/2 - kPio2Hi
+    const double kq = rint(phi * 0.6366197723675814);  // 2/pi
+    const double r = fma(-kq, kPio2Lo, fma(-kq, kPio2Hi, phi));
+    const double r2 = r * r;
+    // sin r = r (1 - r^2/3! + ... - r^18/19!), cos r = 1 - r^2/2! + ... + r^20/20!
+    double s = -8.220635246624329e-18;  // -1/19!
+    s = fma(s, r2, 2.8114572543455206e-15);   // 1/17!
+    s = fma(s, r2, -7.647163731819816e-13);   // -1/15!
+    s = fma(s, r2, 1.6059043836821613e-10);   // 1/13!
+    s = fma(s, r2, -2.505210838544172e-08);   // -1/11!
+    s = fma(s, r2, 2.7557319223985893e-06);   // 1/9!
+    s = fma(s, r2, -0.0001984126984126984);   // -1/7!
+    s = fma(s, r2, 0.008333333333333333);     // 1/5!
+    s = fma(s, r2, -0.16666666666666666);     // -1/3!
+    s = r * fma(s, r2, 1.0);
+    double c = 4.110317623312165e-19;   // 1/20!
+    c = fma(c, r2, -1.5619206968586225e-16);  // -1/18!
+    c = fma(c, r2, 4.779477332387385e-14);    // 1/16!
+    c = fma(c, r2, -1.1470745597729725e-11);  // -1/14!
+    c = fma(c, r2, 2.08767569878681e-09);     // 1/12!
+    c = fma(c, r2, -2.755731922398589e-07);   // -1/10!
+    c = fma(c, r2, 2.48015873015873e-05);     // 1/8!
+    c = fma(c, r2, -0.001388888888888889);    // -1/6!
+    c = fma(c, r2, 0.041666666666666664);     // 1/4!
+    c = fma(c, r2, -0.5);                     // -1/2!
+    c = fma(c, r2, 1.0);
+    const int q = ((int)kq) & 3;
+    *sp = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+    *cp = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
+}
+
+// sqrt(1 - u) for u < 2^-10 (terms after u^6 below 1e-19).
+__device__ __forceinline__ double sqrt_1m(double u)
+{
+    double r = -0.0205078125;
+    r = fma(r, u, -0.02734375);
+    r = fma(r, u, -0.0390625);
+    r = fma(r, u, -0.0625);
+    r = fma(r, u, -0.125);
+    r = fma(r, u, -0.5);
+    return fma(r, u, 1.0);
+}
+
 // bmad/low_level/track_a_bend.f90:18-333: exact_multipoles = off, no body multipoles, b1 = 0,
 // n_step = 1, no misalignment. Exact sector-bend body (:163-209), semi-linear body near the
 // zero orbit (:136-159), drift when g = dg = 0 (:117-118).
-__device__ __forceinline__ void sbend(Particle &p, int32_t flags, const double *__restrict__ b,
+// Fast form of the exact body: with s1 = px/pt = sin(phi_1) and the bend angle's cos/sin,
+// sin/cos(angle + phi_1) follow by angle addition; with psi = atan2(L_u, -L_v) known through
+// its cosine -L_v/L_c and sine L_u/L_c, the half exit angle h = angle_p/2 = angle + phi_1 - psi
+// has sin h and cos h by angle subtraction, L_p = L_c h / sin h from the asin(s)/s series, and
+// px = pt sin(angle + phi_1 - 2h) by double-angle identities. No transcendental calls.
+template <bool F>
+__device__ __forceinline__ void sbend(Particle &p, int32_t flags, CP b,
                                       const Species &sp)
 {
     const double L = b[0], g = b[1], g_tot = b[2], angle = b[3];
@@ -269,7 +475,7 @@ __device__ __forceinline__ void sbend(Particle &p, int32_t flags, const double *
     if (flags & F_EDGE_ENT) hwang_edge(p, b + 10, g_tot, true, simple);
     if (orbit_too_large(p)) return;
     if (flags & F_BEND_DRIFT) {
-        drift(p, L);
+        drift<F>(p, L);
     } else {
         const double x = p.x, px = p.px, y = p.y, py = p.py, z = p.z;
         const double rel_p = p.rel_p;
@@ -284,45 +490,98 @@ __device__ __forceinline__ void sbend(Particle &p, int32_t flags, const double *
             p.z = -g * ll * sinc_a * x - g * ll * ll * cosc_a * px + z + m56 * p.pz;
         } else {
             const double beta_ref = b[8];
-            const double pt = sqrt(rel_p * rel_p - py * py);
-            if (fabs(px) > pt) {
-                p.state = ST_LOST_PZ;
-                return;
+            const double pt2 = rel_p * rel_p - py * py;
+            if (F) {
+                // pt2 <= 0 gives NaN coordinates here as sqrt does in the reference form
+                const double inv_pt = rsqrt(pt2);
+                const double pt = pt2 * inv_pt;
+                if (fabs(px) > pt) {
+                    p.state = ST_LOST_PZ;
+                    return;
+                }
+                const double g_p = g_tot * inv_pt;
+                const double s1 = px * inv_pt;
+                const double s1s = s1 * s1;
+                const double c1 = s1s < kDriftSeriesMax ? sqrt_1m(s1s) : sqrt(1 - s1s);
+                const double sin_plus = fma(sin_a, c1, cos_a * s1);
+                const double cos_plus = fma(cos_a, c1, -sin_a * s1);
+                const double u = (1 + g * x) * (L * sinc_a);
+                const double alpha = 2 * sin_plus * u - g_p * (u * u);
+                const double r = fma(cos_plus, cos_plus, g_p * alpha);
+                if (r < 0 || (fabs(g_p) < 1e-5 && fabs(cos_plus) < 1e-5)) {
+                    p.state = ST_LOST;
+                    p.x = 2 * kMaxAperture;
+                    return;
+                }
+                const double rad = sqrt(r);
+                double xi;
+                if (cos_plus > 0) xi = alpha / (rad + cos_plus);
+                else xi = (rad - cos_plus) / g_p;
+                p.x = x * cos_a - b[22] + xi;
+                if (fabs(p.x) > kMaxAperture) {
+                    p.state = ST_LOST;
+                    return;
+                }
+                const double L_u = xi;
+                const double mL_v = L * sinc_a + x * sin_a;  // -L_v
+                const double lc2 = mL_v * mL_v + L_u * L_u;
+                const double inv_lc = rsqrt(lc2);
+                const double L_c = lc2 * inv_lc;
+                const double cpsi = mL_v * inv_lc, spsi = L_u * inv_lc;
+                const double sh = fma(sin_plus, cpsi, -cos_plus * spsi);  // sin(angle_p/2)
+                const double ch = fma(cos_plus, cpsi, sin_plus * spsi);   // cos(angle_p/2)
+                // L_p = L_c / sinc(h) = L_c h / sin h
+                const double L_p = L_c * h_over_sin(sh, ch);
+                // sin(angle + phi_1 - 2h) with sin/cos 2h from sh, ch
+                const double s2h = 2 * sh * ch;
+                const double c2h = fma(-2 * sh, sh, 1.0);
+                p.px = pt * fma(sin_plus, c2h, -cos_plus * s2h);
+                const double lp_pt = L_p * inv_pt;
+                p.y = fma(py, lp_pt, y);
+                p.z = z + (p.beta * b[23] - rel_p * lp_pt);
+            } else {
+                const double pt = sqrt(pt2);
+                if (fabs(px) > pt) {
+                    p.state = ST_LOST_PZ;
+                    return;
+                }
+                const double g_p = g_tot / pt;
+                const double phi_1 = asin(px / pt);
+                double sin_plus, cos_plus;
+                sincos(angle + phi_1, &sin_plus, &cos_plus);
+                const double u = (1 + g * x) * L * sinc_a;
+                const double alpha = 2 * (1 + g * x) * sin_plus * L * sinc_a - g_p * u * u;
+                const double r = cos_plus * cos_plus + g_p * alpha;
+                if (r < 0 || (fabs(g_p) < 1e-5 && fabs(cos_plus) < 1e-5)) {
+                    p.state = ST_LOST;
+                    p.x = 2 * kMaxAperture;
+                    return;
+                }
+                const double rad = sqrt(r);
+                double xi;
+                if (cos_plus > 0) xi = alpha / (rad + cos_plus);
+                else xi = (rad - cos_plus) / g_p;
+                p.x = x * cos_a - L * L * g * cosc_a + xi;
+                if (fabs(p.x) > kMaxAperture) {
+                    p.state = ST_LOST;
+                    return;
+                }
+                const double L_u = xi;
+                const double L_v = -(L * sinc_a + x * sin_a);
+                const double L_c = sqrt(L_v * L_v + L_u * L_u);
+                const double angle_p = 2 * (angle + phi_1 - atan2(L_u, -L_v));
+                const double L_p = L_c / sinc(angle_p / 2);
+                p.px = pt * sin(phi_1 + angle - angle_p);
+                p.y = y + py * L_p / pt;
+                p.z = z + p.beta * L / beta_ref - rel_p * L_p / pt;
             }
-            const double g_p = g_tot / pt;
-            const double phi_1 = asin(px / pt);
-            double sin_plus, cos_plus;
-            sincos(angle + phi_1, &sin_plus, &cos_plus);
-            const double u = (1 + g * x) * L * sinc_a;
-            const double alpha = 2 * (1 + g * x) * sin_plus * L * sinc_a - g_p * u * u;
-            const double r = cos_plus * cos_plus + g_p * alpha;
-            if (r < 0 || (fabs(g_p) < 1e-5 && fabs(cos_plus) < 1e-5)) {
-                p.state = ST_LOST;
-                p.x = 2 * kMaxAperture;
-                return;
-            }
-            const double rad = sqrt(r);
-            double xi;
-            if (cos_plus > 0) xi = alpha / (rad + cos_plus);
-            else xi = (rad - cos_plus) / g_p;
-            p.x = x * cos_a - L * L * g * cosc_a + xi;
-            if (fabs(p.x) > kMaxAperture) {
-                p.state = ST_LOST;
-                return;
-            }
-            const double L_u = xi;
-            const double L_v = -(L * sinc_a + x * sin_a);
-            const double L_c = sqrt(L_v * L_v + L_u * L_u);
-            const double angle_p = 2 * (angle + phi_1 - atan2(L_u, -L_v));
-            const double L_p = L_c / sinc(angle_p / 2);
-            p.px = pt * sin(phi_1 + angle - angle_p);
-            p.y = y + py * L_p / pt;
-            p.z = z + p.beta * L / beta_ref - rel_p * L_p / pt;
         }
     }
     if (orbit_too_large(p)) return;
     if (flags & F_EDGE_EXIT) hwang_edge(p, b + 16, g_tot, false, simple);
 }
+
+// ---- rf cavity ---------------------------------------------------------------------------
 
 // Fortran MODULO for reals followed by modulo2_mod.f90:85-99: result in [-amp, amp).
 __device__ __forceinline__ double modulo2(double x, double amp)
@@ -365,8 +624,9 @@ __device__ __forceinline__ bool energy_kick(Particle &p, double dE, const Specie
 // particle_rf_time.f90:86-90,116-118 (t = -z/(beta c), wrapped to [-1/2f, 1/2f)), computed in
 // double-double where the reference uses quad precision, then modulo2(phase0, pi); n_slice + 1
 // energy kicks (halved at the ends) between n_slice drifts (:96-133).
-__device__ __forceinline__ void rfcavity(Particle &p, const double *__restrict__ r, int32_t n_slice,
-                                         const Species &sp)
+template <bool F>
+__device__ __forceinline__ void rfcavity(Particle &p, CP r, int32_t n_slice,
+                                         const Species &sp_)
 {
     const double dl = r[0], voltage = r[1], phi0 = r[2], freq = r[3], half_period = r[4];
     const double twopi_f = r[5];
@@ -376,24 +636,59 @@ __device__ __forceinline__ void rfcavity(Particle &p, const double *__restrict__
     const double tf_hi = t * freq;
     const double tf_lo = fma(t, freq, -tf_hi);
     const double d_hi = phi0 - tf_hi;
-    const double d_lo = ((phi0 - d_hi) - tf_hi) - tf_lo;
+    const double d_bb = d_hi - phi0;
+    const double d_lo = ((phi0 - (d_hi - d_bb)) + (-tf_hi - d_bb)) - tf_lo;
     const double ph_hi = kTwoPi * d_hi;
     const double ph_lo = fma(kTwoPi, d_hi, -ph_hi) + kTwoPi * d_lo;
     double phase0 = ph_hi + ph_lo;
     phase0 = phase0 + r[6];
     double phase = modulo2(phase0, kPi);
+    if (F) {
+        // sin(phase) of each slice by rotating (sin, cos) through the per-slice phase step
+        // d = 2 pi f dz/(c beta); d is tiny (|d| << 1e-2), so cos d and sin d come from short
+        // series (truncation below 1e-20); a larger step re-evaluates sincos.
+        double sp, cp;
+        sincos_reduced(phase, &sp, &cp);
+        for (int i = 0; i <= n_slice; i++) {
+            double dE = voltage * sp / n_slice;
+            if (i == 0 || i == n_slice) dE = dE / 2;
+            if (!energy_kick(p, dE, sp_)) return;
+            refresh_energy(p, sp_);
+            if (p.pz == -1) {
+                p.state = ST_LOST_PZ;
+                return;
+            }
+            if (i != n_slice) {
+                const double z0 = p.z;
+                drift<F>(p, dl);
+                const double d = twopi_f * (p.z - z0) / (kCLight * p.beta);
+                phase = phase + d;
+                if (fabs(d) < 1e-3) {
+                    const double d2 = d * d;
+                    const double cd = fma(fma(d2, 1.0 / 24, -0.5), d2, 1.0);
+                    const double sd = d * fma(fma(d2, 1.0 / 120, -1.0 / 6), d2, 1.0);
+                    const double ns = fma(sp, cd, cp * sd);
+                    cp = fma(cp, cd, -sp * sd);
+                    sp = ns;
+                } else {
+                    sincos_reduced(modulo2(phase, kPi), &sp, &cp);
+                }
+            }
+        }
+        return;
+    }
     for (int i = 0; i <= n_slice; i++) {
         double dE = voltage * sin(phase) / n_slice;
         if (i == 0 || i == n_slice) dE = dE / 2;
-        if (!energy_kick(p, dE, sp)) return;
-        refresh_energy(p, sp);
+        if (!energy_kick(p, dE, sp_)) return;
+        refresh_energy(p, sp_);
         if (p.pz == -1) {
             p.state = ST_LOST_PZ;
             return;
         }
         if (i != n_slice) {
             const double z0 = p.z;
-            drift(p, dl);
+            drift<F>(p, dl);
             phase = phase + twopi_f * (p.z - z0) / (kCLight * p.beta);
         }
     }
@@ -401,25 +696,28 @@ __device__ __forceinline__ void rfcavity(Particle &p, const double *__restrict__
 
 // One element for one particle: the body of track1 (bmad/code/track1.f90:196-329) after the
 // entrance validity check, i.e. entrance aperture, element map, orbit_too_large (:291),
-// exit aperture (:329).
-__device__ __forceinline__ void track_element(Particle &p, const EleHdr h,
-                                              const double *__restrict__ prm, const Species &sp)
+// exit aperture (:329). hdr points at the element's 4-int header (bmx_table.hpp); prm at the
+// lattice's parameter array.
+template <bool F>
+__device__ __forceinline__ void track_element(Particle &p, CHdr hdr, CP prm, const Species &sp)
 {
-    const double *ap = prm + kParams[h.kind];
-    if (h.flags & F_APE_ENT) {
-        aperture(p, h.flags, ap);
+    const int32_t kind = hdr[0], flags = hdr[1], n = hdr[2], off = hdr[3];
+    CP ap = prm + off;
+    CP q = (flags & (F_APE_ENT | F_APE_EXIT)) ? ap + 4 : ap;
+    if (flags & F_APE_ENT) {
+        aperture(p, flags, ap);
         if (p.state != ST_ALIVE) return;
     }
-    switch (h.kind) {
-    case K_DRIFT: drift(p, prm[0]); break;
-    case K_QUAD: quadrupole(p, prm); break;
-    case K_MULT: thick_multipole(p, prm, h.n, (h.flags >> F_ORDER_SHIFT) & 0xF); break;
-    case K_BEND: sbend(p, h.flags, prm, sp); break;
-    case K_RF: rfcavity(p, prm, h.n, sp); break;
+    switch (kind) {
+    case K_DRIFT: drift<F>(p, q[0]); break;
+    case K_QUAD: quadrupole<F>(p, q, n); break;
+    case K_MULT: thick_multipole<F>(p, q, n, (flags >> F_ORDER_SHIFT) & 0xF); break;
+    case K_BEND: sbend<F>(p, flags, q, sp); break;
+    case K_RF: rfcavity<F>(p, q, n, sp); break;
     default: break;
     }
     orbit_too_large(p);
-    if (p.state == ST_ALIVE && (h.flags & F_APE_EXIT)) aperture(p, h.flags, ap);
+    if (p.state == ST_ALIVE && (flags & F_APE_EXIT)) aperture(p, flags, ap);
 }
 
 }  // namespace bmx

@@ -28,6 +28,7 @@ enum Flags : int32_t {
     F_EDGE_EXIT = 1 << 4,   // Hwang edge kick at the bend exit
     F_EDGE_SIMPLE = 1 << 5, // both edges have e = 0, fint*hgap = 0, k1 = 0 (only the g y^2 terms)
     F_BEND_DRIFT = 1 << 6,  // g = dg = 0: the bend body is a drift (track_a_bend.f90:117-118)
+    F_BEND_SERIES = 1 << 7, // |angle| small enough for the fast exact-body form
     F_ORDER_SHIFT = 8,      // K_MULT: bits 8..11 = mask of multipole orders 0..3 present
 };
 
@@ -41,23 +42,31 @@ struct EleHdr {
 // Parameter layouts (index into prm + off):
 //  K_DRIFT : [0] L
 //  K_QUAD  : [0] L, [1] b1 = k1 L, [2] beta0 = p0c/E_tot, [3] (mc2/E_tot)^2,
-//            [4] f = beta0^2 (2 beta0^2 - (mc2/E)^2/2), [5] 3e-7 E_tot, [6] mc2 (series test)
+//            [4] f = beta0^2 (2 beta0^2 - (mc2/E)^2/2), [5] 3e-7 E_tot, [6] mc2 (series test),
+//            [7] b1 L, [8] b1 / L, [9 + 4m + j] 1/(4m + j)! for m < 8, j < 4, [41] t_lim
+//            (fast form; hdr.n = number of series terms used)
 //  K_MULT  : [0] step_len, [1..4] a_n * r_step/2 (n = 0..3), [5..8] b_n * r_step/2
 //  K_BEND  : [0] L, [1] g, [2] g_tot, [3] angle = g L, [4] cos_a, [5] sin_a, [6] sinc_a,
 //            [7] cosc_a, [8] beta_ref, [9] sincc_a,
-//            [10..15] entrance gt, gt2, gs2, k1_tane, fg_factor, tan_e^2, [16..21] exit
+//            [10..15] entrance gt, gt2, gs2, k1_tane, fg_factor, tan_e^2, [16..21] exit,
+//            [22] L^2 g cosc_a, [23] L / beta_ref
 //  K_RF    : [0] L/n_slice, [1] voltage * charge_dir, [2] phi0, [3] rf_frequency,
 //            [4] 0.5/rf_frequency, [5] twopi * rf_frequency, [6] twopi * phi0_multipass
 //  K_MARKER: (none)
-// followed, when F_APE_ENT|F_APE_EXIT, by 4 aperture values:
+// preceded, when F_APE_ENT|F_APE_EXIT, by 4 aperture values (hdr.off points at them):
 //  elliptical : x_center, 1/x_width2, y_center, 1/y_width2
 //  rectangular: x1_lim (= -x1_limit), x2_lim, y1_lim, y2_lim (0 = no limit)
-constexpr int kParams[K_NKIND] = {1, 7, 9, 22, 7, 0};
+constexpr int kParams[K_NKIND] = {1, 42, 9, 24, 7, 0};
 
 struct Species {
-    double mc2;     // [eV]
-    double p0c;     // reference momentum [eV]
-    double mc2_sq;  // mc2^2
+    double mc2;      // [eV]
+    double p0c;      // reference momentum [eV]
+    double mc2_sq;   // mc2^2
+    // low_energy_z_correction.f90:22-58 constants of the (single) reference energy
+    double beta0;    // p0c / E_tot
+    double me2;      // (mc2 / E_tot)^2
+    double lez_f;    // beta0^2 (2 beta0^2 - (mc2/E_tot)^2 / 2)
+    double lez_thr;  // 3e-7 E_tot
 };
 
 // One unit of persistent work: a run of particles of one lattice.

@@ -57,11 +57,12 @@ def ring0(n_cells: int = 100, e_tot: float = E_TOT_5GEV):
 
 
 def ring1(n_cells: int = 100, e_tot: float = E_TOT_5GEV, k1_scale=None, k2_scale=None,
-          aperture: bool = True, rf: bool = True):
+          aperture: bool = True, rf: bool = True, rf_phi0: float = 0.1 / (2 * E.PI)):
     """The 901-element ring: FODO cells with sextupoles, one rfcavity, apertures everywhere.
 
     k1_scale / k2_scale: optional per-instance multipliers (len = 2 * n_cells) for the
-    quadrupole k1 and sextupole k2 of each cell, in ring order."""
+    quadrupole k1 and sextupole k2 of each cell, in ring order. rf_phi0 is the cavity's phi0
+    in units of 2 pi (0.1 rad off the zero crossing by default)."""
     p0c, et = reference(e_tot)
     g = BEND_ANGLE / 2.0
     ap = _ap(aperture)
@@ -84,7 +85,7 @@ def ring1(n_cells: int = 100, e_tot: float = E_TOT_5GEV, k1_scale=None, k2_scale
         ]
     if rf:
         out.append(E.make_element(E.RFCAVITY, p0c, et, l=0.5, voltage=10e6, rf_frequency=500e6,
-                                  phi0=0.1 / (2 * E.PI), **ap))
+                                  phi0=rf_phi0, **ap))
     return out
 
 

@@ -34,6 +34,7 @@ class Context:
         native.call("bmx_ctx_create", device, ctypes.byref(h))
         self.h = h
         self.device = device
+        self.maps = "fast"
 
     def synchronize(self):
         native.call("bmx_ctx_synchronize", self.h)
@@ -41,6 +42,11 @@ class Context:
     @property
     def stream(self) -> int:
         return native.load().bmx_ctx_stream(self.h) or 0
+
+    def set_maps(self, form: str) -> None:
+        """'fast' (default) or 'reference' formulation of the element maps."""
+        native.call("bmx_ctx_set_maps", self.h, {"fast": 1, "reference": 2}[form])
+        self.maps = form
 
     def record(self, slot: int) -> None:
         """Record HIP event `slot` on this context's stream."""
@@ -171,9 +177,10 @@ def track_beam_plain(ctx: Context, lat: Lattice, beam: Beam, n_turns: int, turn0
 
 
 def track_beam_host(eles, beam: dict, n_turns: int, device: int = 0, turn0: int = 0,
-                    mc2: float = E.M_ELECTRON, charge: int = -1) -> dict:
+                    mc2: float = E.M_ELECTRON, charge: int = -1, maps: str = "fast") -> dict:
     """Convenience: host SoA dict in, host SoA dict out (tracks a copy)."""
     ctx = Context(device)
+    ctx.set_maps(maps)
     lat = Lattice(ctx, eles, mc2, charge)
     b = Beam.from_host(ctx, beam)
     track_beam(ctx, lat, b, n_turns, turn0)

@@ -159,6 +159,14 @@ void *bmx_ctx_stream(bmx_ctx *ctx);
  * (element passes of particles alive at the element entrance, the work track1_bunch_hom
  * does, beam_utils.f90:81-86); synchronises the stream and resets the count. */
 int32_t bmx_ctx_take_work(bmx_ctx *ctx, int64_t *passes);
+/* Formulation of the per-particle maps used by this context's tracking calls:
+ * BMX_MAPS_FAST (default) re-expresses the drift, quadrupole and bend bodies without
+ * per-particle transcendental calls (convergent series, angle-addition identities, cached
+ * reciprocals); BMX_MAPS_REFERENCE evaluates the reference's formulas as written
+ * (track_a_drift.f90, quad_mat2_calc.f90, track_a_bend.f90:163-209). Both agree to rounding. */
+#define BMX_MAPS_FAST 1
+#define BMX_MAPS_REFERENCE 2
+int32_t bmx_ctx_set_maps(bmx_ctx *ctx, int32_t form);
 /* Timing on the context's stream: record HIP event `slot` (0..BMX_N_EVENTS-1) now, and the
  * elapsed milliseconds between two recorded slots (synchronises on the later one). Replaces
  * the wall-clock run_timer of the reference drivers (sim_utils/misc/run_timer.f90:32). */

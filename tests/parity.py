@@ -34,6 +34,20 @@ def rel_err(got, ref, mask=None):
     return worst
 
 
+def rel_dev(got, ref, mask):
+    """Per particle: max over coordinates of the rel_err measure (inf outside mask)."""
+    dev = np.full(mask.shape, np.inf)
+    if not mask.any():
+        return dev
+    d = np.zeros(int(mask.sum()))
+    for k in COORDS:
+        g, r = got[k][mask], ref[k][mask]
+        scale = max(FLOOR[k], float(np.sqrt(np.mean(r * r))))
+        d = np.maximum(d, np.abs(g - r) / np.maximum(np.abs(r), scale))
+    dev[mask] = d
+    return dev
+
+
 def loss_mismatch(got, ref):
     """Indices where state / loss element / loss turn differ."""
     bad = ((got["state"] != ref["state"]) | (got["loss_ele"] != ref["loss_ele"]) |
