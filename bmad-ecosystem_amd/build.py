@@ -52,16 +52,26 @@ def build_lib(extra_flags=None) -> str:
 
 FORTRAN_SRC = os.path.join(HERE, "fortran", "bmx_bmad_mod.f90")
 FORTRAN_OBJ_DIR = os.path.join(HERE, "fortran", "_build")
+FORTRAN_DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "ftrack.f90")
+FORTRAN_DRIVER = os.path.join(FORTRAN_OBJ_DIR, "ftrack")
+
+
+def fortran_compiler() -> str | None:
+    return os.environ.get("FC") or shutil.which("amdflang") or shutil.which("flang")
 
 
 def build_fortran() -> str | None:
-    """Compile the iso_c_binding module with the image's Fortran compiler (amdflang)."""
-    fc = os.environ.get("FC") or shutil.which("amdflang") or shutil.which("flang")
+    """Compile the iso_c_binding module with the image's Fortran compiler (amdflang) and link
+    the test suite's Fortran host program against libbmx.so (rpath to the package dir)."""
+    fc = fortran_compiler()
     if not fc or not os.path.exists(FORTRAN_SRC):
         return None
     os.makedirs(FORTRAN_OBJ_DIR, exist_ok=True)
     obj = os.path.join(FORTRAN_OBJ_DIR, "bmx_bmad_mod.o")
     _run([fc, "-c", "-O2", "-fPIC", f"-J{FORTRAN_OBJ_DIR}", FORTRAN_SRC, "-o", obj])
+    if os.path.exists(FORTRAN_DRIVER_SRC):
+        _run([fc, "-O2", f"-I{FORTRAN_OBJ_DIR}", f"-J{FORTRAN_OBJ_DIR}", FORTRAN_DRIVER_SRC, obj,
+              f"-L{HERE}", "-lbmx", "-Wl,-rpath,$ORIGIN/../..", "-o", FORTRAN_DRIVER])
     return obj
 
 

@@ -333,4 +333,30 @@ int32_t bmx_track_turns_plain(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, i
     return BMX_OK;
 }
 
+int32_t bmx_track_orbit(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t ix_start,
+                        int32_t ix_end, int32_t turn, double *orbit)
+{
+    if (!ctx || !L || !B || !orbit) return set_error(BMX_ERR_ARG, "bmx_track_orbit: null argument");
+    if (ix_start < 1 || ix_end > L->desc.n_ele || ix_start > ix_end)
+        return set_error(BMX_ERR_ARG, "bmx_track_orbit: element range out of bounds");
+    hipSetDevice(ctx->device);
+    const size_t bytes = (size_t)B->n * (size_t)(ix_end - ix_start + 1) * 8 * sizeof(double);
+    if (bytes == 0) return BMX_OK;
+    double *d_rec = nullptr;
+    if (hipMalloc(&d_rec, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(BMX_ERR_NOMEM, "bmx_track_orbit: cannot allocate the orbit record");
+    }
+    int32_t st = BMX_OK;
+    hipError_t e = hipMemsetAsync(d_rec, 0, bytes, ctx->stream);
+    if (e == hipSuccess)
+        e = launch_track_range(ctx->stream, L->desc, ix_start - 1, ix_end - 1, turn, B->d, B->n,
+                               d_rec, ctx->d_work, ctx->fast_maps);
+    if (e == hipSuccess) e = hipMemcpyAsync(orbit, d_rec, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) st = set_error(BMX_ERR_HIP, std::string("bmx_track_orbit: ") + hipGetErrorString(e));
+    hipFree(d_rec);
+    return st;
+}
+
 }  // extern "C"

@@ -17,6 +17,9 @@ hipError_t launch_track_turns(hipStream_t s, const LatDesc *d_lats, const Tile *
 hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, int32_t turn,
                                 const BeamPtrs &b, int64_t n, unsigned long long *d_work,
                                 bool fast);
+hipError_t launch_track_range(hipStream_t s, const LatDesc &lat, int32_t e0, int32_t e1,
+                              int32_t turn, const BeamPtrs &b, int64_t n, double *d_rec,
+                              unsigned long long *d_work, bool fast);
 hipError_t launch_count_alive(hipStream_t s, const int32_t *state, int64_t n,
                               unsigned long long *d_out);
 

@@ -138,6 +138,63 @@ __global__ void __launch_bounds__(kBlock) k_track_element(LatDesc L, int32_t ie,
     add_work(work, active ? 1ull : 0ull);
 }
 
+// Elements e0..e1 (0-based, inclusive) once for every live particle, recording the state at
+// every element exit: track_all / track_many (track_all.f90:25-145, track_many.f90:50-157)
+// with orbit(ix) kept per particle. rec[(i * (e1 - e0 + 1) + (e - e0)) * 8 + k] holds
+// x, px, y, py, z, pz, beta, state of particle i after element e. For the element a particle
+// is lost in, a loss at the entrance end records zero coordinates with the loss state
+// (track_all.f90:84); nothing is written for later elements.
+__device__ __forceinline__ void record_orbit(double *r, const Particle &p, bool at_entrance)
+{
+    r[0] = at_entrance ? 0.0 : p.x;
+    r[1] = at_entrance ? 0.0 : p.px;
+    r[2] = at_entrance ? 0.0 : p.y;
+    r[3] = at_entrance ? 0.0 : p.py;
+    r[4] = at_entrance ? 0.0 : p.z;
+    r[5] = at_entrance ? 0.0 : p.pz;
+    r[6] = p.beta;
+    r[7] = (double)p.state;
+}
+
+template <bool F>
+__global__ void __launch_bounds__(kBlock) k_track_range(LatDesc L, int32_t e0, int32_t e1,
+                                                        int32_t turn, BeamPtrs b, int64_t n,
+                                                        double *__restrict__ rec,
+                                                        unsigned long long *work)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool active = i < n && b.state[i] == ST_ALIVE;
+    unsigned long long passes = 0;
+    if (active) {
+        Particle p;
+        load_particle(p, b, i);
+        refresh_energy(p, L.sp);
+        const int64_t ne = (int64_t)(e1 - e0 + 1);
+        int32_t lost_e = 0;
+        if (orbit_too_large(p)) {
+            lost_e = e0 + 1;
+            record_orbit(rec + i * ne * 8, p, true);
+        } else {
+            for (int32_t e = e0; e <= e1; ++e) {
+                const bool at_entrance = track_element<F>(p, (CHdr)L.hdr + 4 * e, (CP)L.prm, L.sp);
+                passes++;
+                record_orbit(rec + (i * ne + (e - e0)) * 8, p, at_entrance);
+                if (p.state != ST_ALIVE) {
+                    lost_e = e + 1;
+                    break;
+                }
+            }
+        }
+        store_particle(p, b, i);
+        if (p.state != ST_ALIVE) {
+            b.state[i] = p.state;
+            b.loss_ele[i] = lost_e;
+            b.loss_turn[i] = turn;
+        }
+    }
+    add_work(work, passes);
+}
+
 __global__ void __launch_bounds__(kBlock) k_count_alive(const int32_t *__restrict__ state, int64_t n,
                                                         unsigned long long *out)
 {
@@ -181,6 +238,21 @@ hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, i
     else
         hipLaunchKernelGGL(k_track_element<false>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat, ie,
                            turn, b, n, d_work);
+    return hipGetLastError();
+}
+
+hipError_t launch_track_range(hipStream_t s, const LatDesc &lat, int32_t e0, int32_t e1,
+                              int32_t turn, const BeamPtrs &b, int64_t n, double *d_rec,
+                              unsigned long long *d_work, bool fast)
+{
+    int64_t grid = (n + kBlock - 1) / kBlock;
+    if (grid <= 0) return hipSuccess;
+    if (fast)
+        hipLaunchKernelGGL(k_track_range<true>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat, e0,
+                           e1, turn, b, n, d_rec, d_work);
+    else
+        hipLaunchKernelGGL(k_track_range<false>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat, e0,
+                           e1, turn, b, n, d_rec, d_work);
     return hipGetLastError();
 }
 

@@ -697,16 +697,17 @@ __device__ __forceinline__ void rfcavity(Particle &p, CP r, int32_t n_slice,
 // One element for one particle: the body of track1 (bmad/code/track1.f90:196-329) after the
 // entrance validity check, i.e. entrance aperture, element map, orbit_too_large (:291),
 // exit aperture (:329). hdr points at the element's 4-int header (bmx_table.hpp); prm at the
-// lattice's parameter array.
+// lattice's parameter array. Returns true when the particle was lost at the entrance
+// aperture (location = upstream_end$).
 template <bool F>
-__device__ __forceinline__ void track_element(Particle &p, CHdr hdr, CP prm, const Species &sp)
+__device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, const Species &sp)
 {
     const int32_t kind = hdr[0], flags = hdr[1], n = hdr[2], off = hdr[3];
     CP ap = prm + off;
     CP q = (flags & (F_APE_ENT | F_APE_EXIT)) ? ap + 4 : ap;
     if (flags & F_APE_ENT) {
         aperture(p, flags, ap);
-        if (p.state != ST_ALIVE) return;
+        if (p.state != ST_ALIVE) return true;
     }
     switch (kind) {
     case K_DRIFT: drift<F>(p, q[0]); break;
@@ -718,6 +719,7 @@ __device__ __forceinline__ void track_element(Particle &p, CHdr hdr, CP prm, con
     }
     orbit_too_large(p);
     if (p.state == ST_ALIVE && (flags & F_APE_EXIT)) aperture(p, flags, ap);
+    return false;
 }
 
 }  // namespace bmx

@@ -1,0 +1,412 @@
+!+
+! Module bmx_bmad_mod
+!
+! iso_c_binding layer between a Fortran tracking host and the MI355X tracking library
+! (include/bmx.h). It keeps the calling convention of Bmad's beam tracking:
+!
+!   track_beam / track_bunch  (bmad/multiparticle/beam_mod.f90:40,103): a bunch of particles,
+!       each a coord_struct (bmad/modules/bmad_struct.f90:551-582), tracked through the
+!       lattice branch; particle loss is recorded in %state, never reported as an error
+!       (bmad/code/track1.f90:39-41).
+!   track_all (bmad/code/track_all.f90:25): one particle through the elements, orbit(0:n)
+!       at every element end, track_state = index of the element where it was lost.
+!
+! The library works on structure-of-arrays FP64 buffers. This module converts the host's
+! particle records to SoA and back field by field: coord_struct%t is real(qp), so the record
+! layout depends on the compiler and is never copied as raw memory.
+!
+! bmx_coord_struct carries the coord_struct components the path reads and writes, with the
+! same names and meanings; a host built on Bmad assigns them from/to its coord_struct array
+! (the adapter is a loop of component assignments).
+!-
+
+module bmx_bmad_mod
+
+use, intrinsic :: iso_c_binding
+implicit none
+
+private
+
+integer, parameter, public :: bmx_rp = c_double
+
+! Status codes (bmx.h)
+integer(c_int32_t), parameter, public :: bmx_ok = 0, bmx_err_arg = 1, bmx_err_unsupported = 2, &
+    bmx_err_hip = 3, bmx_err_p0c = 4, bmx_err_nomem = 5, bmx_err_budget = 6, bmx_err_state = 7, &
+    bmx_err_not_found = 8
+
+! coord_struct%state codes (bmad_struct.f90:527-532)
+integer, parameter, public :: bmx_not_set = -999, bmx_alive = 1, bmx_lost = 2, &
+    bmx_lost_neg_x = 3, bmx_lost_pos_x = 4, bmx_lost_neg_y = 5, bmx_lost_pos_y = 6, &
+    bmx_lost_z = 7, bmx_lost_pz = 8
+
+! track_all's track_state when no particle was lost (bmad_struct.f90: moving_forward$)
+integer, parameter, public :: bmx_moving_forward = -1
+
+! Element keys (bmad_struct.f90:1696-1710)
+integer(c_int32_t), parameter, public :: bmx_drift = 1, bmx_sbend = 2, bmx_quadrupole = 3, &
+    bmx_sextupole = 5, bmx_rfcavity = 9, bmx_marker = 14, bmx_kicker = 15, bmx_octupole = 17, &
+    bmx_monitor = 31, bmx_instrument = 32, bmx_hkicker = 33, bmx_vkicker = 34, &
+    bmx_rcollimator = 35, bmx_ecollimator = 36, bmx_pipe = 44, bmx_thick_multipole = 67
+
+integer, parameter, public :: bmx_n_pole = 22
+
+! bmx_ele of bmx.h: the raw ele%value attributes of one element (bmad_struct.f90:1469).
+type, bind(c), public :: bmx_ele_struct
+  integer(c_int32_t) :: key = 0, is_on = 1, fringe_type = 0, fringe_at = 0
+  integer(c_int32_t) :: aperture_type = 0, aperture_at = 0, offset_moves_aperture = 0, n_multipole = 0
+  real(c_double) :: l = 0
+  real(c_double) :: k1 = 0, k2 = 0, k3 = 0
+  real(c_double) :: g = 0, dg = 0, e1 = 0, e2 = 0, fint = 0, fintx = 0, hgap = 0, hgapx = 0
+  real(c_double) :: hkick = 0, vkick = 0, kick = 0
+  real(c_double) :: tilt = 0, ref_tilt = 0
+  real(c_double) :: x_offset = 0, y_offset = 0, z_offset = 0, x_pitch = 0, y_pitch = 0
+  real(c_double) :: voltage = 0, rf_frequency = 0, phi0 = 0, phi0_multipass = 0
+  real(c_double) :: x1_limit = 0, x2_limit = 0, y1_limit = 0, y2_limit = 0
+  real(c_double) :: px_aperture_width2 = 0, px_aperture_center = 0
+  real(c_double) :: py_aperture_width2 = 0, py_aperture_center = 0
+  real(c_double) :: z_aperture_width2 = 0, z_aperture_center = 0
+  real(c_double) :: pz_aperture_width2 = 0, pz_aperture_center = 0
+  real(c_double) :: ds_step = 0
+  real(c_double) :: p0c = 0, e_tot = 0
+  real(c_double) :: r0_mag = 0
+  real(c_double) :: a_pole(0:bmx_n_pole-1) = 0
+  real(c_double) :: b_pole(0:bmx_n_pole-1) = 0
+end type
+
+! bmx_beam_view of bmx.h
+type, bind(c) :: bmx_beam_view
+  type(c_ptr) :: x = c_null_ptr, px = c_null_ptr, y = c_null_ptr, py = c_null_ptr
+  type(c_ptr) :: z = c_null_ptr, pz = c_null_ptr, beta = c_null_ptr
+  type(c_ptr) :: state = c_null_ptr, loss_ele = c_null_ptr, loss_turn = c_null_ptr
+end type
+
+! The coord_struct components on the path (bmad_struct.f90:551-582).
+type, public :: bmx_coord_struct
+  real(bmx_rp) :: vec(6) = 0
+  real(bmx_rp) :: p0c = 0
+  real(bmx_rp) :: beta = -1
+  integer :: ix_ele = -1
+  integer :: ix_turn = 0
+  integer :: state = bmx_not_set
+end type
+
+! One device context plus one compiled lattice branch.
+type, public :: bmx_lat_struct
+  type(c_ptr) :: ctx = c_null_ptr
+  type(c_ptr) :: lat = c_null_ptr
+  integer :: n_ele = 0
+  real(bmx_rp) :: p0c = 0
+end type
+
+interface
+  function c_ctx_create(device, ctx) bind(c, name = 'bmx_ctx_create') result(st)
+    import :: c_int32_t, c_ptr
+    integer(c_int32_t), value :: device
+    type(c_ptr) :: ctx
+    integer(c_int32_t) :: st
+  end function
+
+  function c_ctx_destroy(ctx) bind(c, name = 'bmx_ctx_destroy') result(st)
+    import :: c_int32_t, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int32_t) :: st
+  end function
+
+  function c_ctx_synchronize(ctx) bind(c, name = 'bmx_ctx_synchronize') result(st)
+    import :: c_int32_t, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int32_t) :: st
+  end function
+
+  function c_lattice_create(ctx, eles, n_ele, mc2, charge, lat) bind(c, name = 'bmx_lattice_create') result(st)
+    import :: c_int32_t, c_ptr, c_double, bmx_ele_struct
+    type(c_ptr), value :: ctx
+    type(bmx_ele_struct), intent(in) :: eles(*)
+    integer(c_int32_t), value :: n_ele
+    real(c_double), value :: mc2
+    integer(c_int32_t), value :: charge
+    type(c_ptr) :: lat
+    integer(c_int32_t) :: st
+  end function
+
+  function c_lattice_destroy(lat) bind(c, name = 'bmx_lattice_destroy') result(st)
+    import :: c_int32_t, c_ptr
+    type(c_ptr), value :: lat
+    integer(c_int32_t) :: st
+  end function
+
+  function c_beam_create(ctx, n, beam) bind(c, name = 'bmx_beam_create') result(st)
+    import :: c_int32_t, c_int64_t, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int64_t), value :: n
+    type(c_ptr) :: beam
+    integer(c_int32_t) :: st
+  end function
+
+  function c_beam_upload(beam, host, n) bind(c, name = 'bmx_beam_upload') result(st)
+    import :: c_int32_t, c_int64_t, c_ptr, bmx_beam_view
+    type(c_ptr), value :: beam
+    type(bmx_beam_view), intent(in) :: host
+    integer(c_int64_t), value :: n
+    integer(c_int32_t) :: st
+  end function
+
+  function c_beam_download(beam, host, n) bind(c, name = 'bmx_beam_download') result(st)
+    import :: c_int32_t, c_int64_t, c_ptr, bmx_beam_view
+    type(c_ptr), value :: beam
+    type(bmx_beam_view), intent(in) :: host
+    integer(c_int64_t), value :: n
+    integer(c_int32_t) :: st
+  end function
+
+  function c_beam_destroy(beam) bind(c, name = 'bmx_beam_destroy') result(st)
+    import :: c_int32_t, c_ptr
+    type(c_ptr), value :: beam
+    integer(c_int32_t) :: st
+  end function
+
+  function c_track_turns(ctx, lat, beam, n_turns, turn0) bind(c, name = 'bmx_track_turns') result(st)
+    import :: c_int32_t, c_ptr
+    type(c_ptr), value :: ctx, lat, beam
+    integer(c_int32_t), value :: n_turns, turn0
+    integer(c_int32_t) :: st
+  end function
+
+  function c_track_orbit(ctx, lat, beam, ix_start, ix_end, turn, orbit) bind(c, name = 'bmx_track_orbit') result(st)
+    import :: c_int32_t, c_ptr
+    type(c_ptr), value :: ctx, lat, beam
+    integer(c_int32_t), value :: ix_start, ix_end, turn
+    type(c_ptr), value :: orbit
+    integer(c_int32_t) :: st
+  end function
+
+  function c_last_error() bind(c, name = 'bmx_last_error') result(msg)
+    import :: c_ptr
+    type(c_ptr) :: msg
+  end function
+
+  function c_strlen(s) bind(c, name = 'strlen') result(n)
+    import :: c_ptr, c_size_t
+    type(c_ptr), value :: s
+    integer(c_size_t) :: n
+  end function
+end interface
+
+public :: bmx_lat_init, bmx_lat_free, bmx_track_bunch, bmx_track_all, bmx_last_error_message
+
+contains
+
+!+
+! Function bmx_last_error_message() result(msg)
+! Message of the last failed library call on this thread (bmx_last_error).
+!-
+function bmx_last_error_message() result(msg)
+  character(:), allocatable :: msg
+  type(c_ptr) :: p
+  character(kind=c_char), pointer :: chars(:)
+  integer :: i, n
+  p = c_last_error()
+  if (.not. c_associated(p)) then
+    msg = ''
+    return
+  endif
+  n = int(c_strlen(p))
+  call c_f_pointer(p, chars, [n])
+  allocate(character(n) :: msg)
+  do i = 1, n
+    msg(i:i) = chars(i)
+  enddo
+end function
+
+!+
+! Subroutine bmx_lat_init (lat, eles, mc2, charge, err_flag, device)
+!
+! Compiles the element table eles(1:n) (the tracking part of a lattice branch, in order) on
+! device `device` (default 0). mc2 [eV] and charge are the species' (particle_species_mod.f90).
+! err_flag is set, and the message is in bmx_last_error_message(), if an element is outside
+! the supported set or the reference momenta differ.
+!-
+subroutine bmx_lat_init (lat, eles, mc2, charge, err_flag, device)
+  type(bmx_lat_struct), intent(inout) :: lat
+  type(bmx_ele_struct), intent(in) :: eles(:)
+  real(bmx_rp), intent(in) :: mc2
+  integer, intent(in) :: charge
+  logical, intent(out) :: err_flag
+  integer, optional, intent(in) :: device
+  integer(c_int32_t) :: dev, st
+
+  err_flag = .true.
+  dev = 0
+  if (present(device)) dev = device
+  if (.not. c_associated(lat%ctx)) then
+    st = c_ctx_create(dev, lat%ctx)
+    if (st /= bmx_ok) return
+  endif
+  if (c_associated(lat%lat)) then
+    st = c_lattice_destroy(lat%lat)
+    lat%lat = c_null_ptr
+  endif
+  st = c_lattice_create(lat%ctx, eles, int(size(eles), c_int32_t), real(mc2, c_double), &
+                        int(charge, c_int32_t), lat%lat)
+  if (st /= bmx_ok) return
+  lat%n_ele = size(eles)
+  lat%p0c = 0
+  if (size(eles) > 0) lat%p0c = eles(1)%p0c
+  err_flag = .false.
+end subroutine
+
+subroutine bmx_lat_free (lat)
+  type(bmx_lat_struct), intent(inout) :: lat
+  integer(c_int32_t) :: st
+  if (c_associated(lat%lat)) st = c_lattice_destroy(lat%lat)
+  if (c_associated(lat%ctx)) st = c_ctx_destroy(lat%ctx)
+  lat%lat = c_null_ptr
+  lat%ctx = c_null_ptr
+  lat%n_ele = 0
+end subroutine
+
+!+
+! Subroutine bmx_track_bunch (lat, particle, n_turns, err_flag, ix_turn0)
+!
+! track_bunch (beam_mod.f90:103) from the start of the branch to its end, n_turns times
+! (the turn loop of ltt_run_beam_mode, lt_tracking_mod.f90:1631-1716), for particle(:).
+! Particles not alive on entry are left untouched. A particle lost on the way keeps the
+! coordinates it had when lost, gets Bmad's loss code in %state, the 1-based index of the
+! element it was lost in in %ix_ele and the 0-based turn in %ix_turn. Survivors end with
+! %ix_ele = n_ele and %ix_turn = ix_turn0 + n_turns - 1.
+!-
+subroutine bmx_track_bunch (lat, particle, n_turns, err_flag, ix_turn0)
+  type(bmx_lat_struct), intent(in) :: lat
+  type(bmx_coord_struct), intent(inout) :: particle(:)
+  integer, intent(in) :: n_turns
+  logical, intent(out) :: err_flag
+  integer, optional, intent(in) :: ix_turn0
+
+  real(c_double), allocatable, target :: x(:), px(:), y(:), py(:), z(:), pz(:), beta(:)
+  integer(c_int32_t), allocatable, target :: state(:), loss_ele(:), loss_turn(:)
+  type(bmx_beam_view) :: view
+  type(c_ptr) :: beam
+  integer(c_int32_t) :: st, st2, t0
+  integer(c_int64_t) :: n
+  integer :: i
+
+  err_flag = .true.
+  n = size(particle, kind = c_int64_t)
+  t0 = 0
+  if (present(ix_turn0)) t0 = ix_turn0
+  allocate(x(n), px(n), y(n), py(n), z(n), pz(n), beta(n), state(n), loss_ele(n), loss_turn(n))
+  do i = 1, int(n)
+    x(i) = particle(i)%vec(1); px(i) = particle(i)%vec(2)
+    y(i) = particle(i)%vec(3); py(i) = particle(i)%vec(4)
+    z(i) = particle(i)%vec(5); pz(i) = particle(i)%vec(6)
+    beta(i) = particle(i)%beta
+    state(i) = particle(i)%state
+    loss_ele(i) = 0
+    loss_turn(i) = -1
+  enddo
+  call fill_view(view, x, px, y, py, z, pz, beta, state, loss_ele, loss_turn)
+
+  beam = c_null_ptr
+  st = c_beam_create(lat%ctx, n, beam)
+  if (st /= bmx_ok) return
+  st = c_beam_upload(beam, view, n)
+  if (st == bmx_ok) st = c_track_turns(lat%ctx, lat%lat, beam, int(n_turns, c_int32_t), t0)
+  if (st == bmx_ok) st = c_beam_download(beam, view, n)
+  st2 = c_beam_destroy(beam)
+  if (st /= bmx_ok .or. st2 /= bmx_ok) return
+
+  do i = 1, int(n)
+    if (particle(i)%state /= bmx_alive) cycle
+    particle(i)%vec = [x(i), px(i), y(i), py(i), z(i), pz(i)]
+    particle(i)%beta = beta(i)
+    particle(i)%state = state(i)
+    if (state(i) == bmx_alive) then
+      particle(i)%ix_ele = lat%n_ele
+      particle(i)%ix_turn = t0 + n_turns - 1
+    else
+      particle(i)%ix_ele = loss_ele(i)
+      particle(i)%ix_turn = loss_turn(i)
+    endif
+  enddo
+  err_flag = .false.
+end subroutine
+
+!+
+! Subroutine bmx_track_all (lat, orbit, track_state, err_flag)
+!
+! track_all (track_all.f90:25-145) for one particle: orbit(0) is the start, on return
+! orbit(i) holds the particle at the exit of element i for every element it passed. If the
+! particle is lost in element i: track_state = i, orbit(i) holds the loss state (with zero
+! coordinates if it was lost at the element's entrance), orbit(i+1:) are reset to
+! bmx_coord_struct() and err_flag is set, as track_all does (:81-88). Otherwise
+! track_state = bmx_moving_forward. A library failure sets err_flag with track_state =
+! bmx_moving_forward and the message in bmx_last_error_message().
+!-
+subroutine bmx_track_all (lat, orbit, track_state, err_flag)
+  type(bmx_lat_struct), intent(in) :: lat
+  type(bmx_coord_struct), intent(inout) :: orbit(0:)
+  integer, intent(out) :: track_state
+  logical, intent(out) :: err_flag
+
+  real(c_double), target :: x(1), px(1), y(1), py(1), z(1), pz(1), beta(1)
+  integer(c_int32_t), target :: state(1), loss_ele(1), loss_turn(1)
+  real(c_double), allocatable, target :: rec(:, :)
+  type(bmx_beam_view) :: view
+  type(c_ptr) :: beam
+  integer(c_int32_t) :: st, st2
+  integer :: i, n_ele
+
+  err_flag = .true.
+  track_state = bmx_moving_forward
+  n_ele = lat%n_ele
+  if (ubound(orbit, 1) < n_ele) return
+  x = orbit(0)%vec(1); px = orbit(0)%vec(2); y = orbit(0)%vec(3)
+  py = orbit(0)%vec(4); z = orbit(0)%vec(5); pz = orbit(0)%vec(6)
+  beta = orbit(0)%beta
+  state = orbit(0)%state
+  loss_ele = 0
+  loss_turn = -1
+  call fill_view(view, x, px, y, py, z, pz, beta, state, loss_ele, loss_turn)
+  ! rec(1:8, i): x, px, y, py, z, pz, beta, state at the exit of element i (i = 1..n_ele)
+  allocate(rec(8, n_ele))
+  rec = 0
+
+  beam = c_null_ptr
+  st = c_beam_create(lat%ctx, 1_c_int64_t, beam)
+  if (st /= bmx_ok) return
+  st = c_beam_upload(beam, view, 1_c_int64_t)
+  if (st == bmx_ok) st = c_track_orbit(lat%ctx, lat%lat, beam, 1_c_int32_t, int(n_ele, c_int32_t), &
+                                         int(orbit(0)%ix_turn, c_int32_t), c_loc(rec))
+  if (st == bmx_ok) st = c_beam_download(beam, view, 1_c_int64_t)
+  st2 = c_beam_destroy(beam)
+  if (st /= bmx_ok .or. st2 /= bmx_ok) return
+
+  err_flag = .false.
+  do i = 1, n_ele
+    if (track_state /= bmx_moving_forward) then
+      orbit(i) = bmx_coord_struct()
+      cycle
+    endif
+    orbit(i)%vec = rec(1:6, i)
+    orbit(i)%beta = rec(7, i)
+    orbit(i)%state = nint(rec(8, i))
+    orbit(i)%p0c = orbit(0)%p0c
+    orbit(i)%ix_ele = i
+    orbit(i)%ix_turn = orbit(0)%ix_turn
+    if (orbit(i)%state /= bmx_alive) then
+      track_state = i
+      err_flag = .true.
+    endif
+  enddo
+end subroutine
+
+subroutine fill_view (view, x, px, y, py, z, pz, beta, state, loss_ele, loss_turn)
+  type(bmx_beam_view), intent(out) :: view
+  real(c_double), target, intent(in) :: x(:), px(:), y(:), py(:), z(:), pz(:), beta(:)
+  integer(c_int32_t), target, intent(in) :: state(:), loss_ele(:), loss_turn(:)
+  view%x = c_loc(x); view%px = c_loc(px); view%y = c_loc(y); view%py = c_loc(py)
+  view%z = c_loc(z); view%pz = c_loc(pz); view%beta = c_loc(beta)
+  view%state = c_loc(state); view%loss_ele = c_loc(loss_ele); view%loss_turn = c_loc(loss_turn)
+end subroutine
+
+end module

@@ -63,6 +63,7 @@ _SIGS = {
     "bmx_track_turns": (_I32, [_P, _P, _P, _I32, _I32]),
     "bmx_track_element": (_I32, [_P, _P, _P, _I32, _I32]),
     "bmx_track_turns_plain": (_I32, [_P, _P, _P, _I32, _I32]),
+    "bmx_track_orbit": (_I32, [_P, _P, _P, _I32, _I32, _I32, _P]),
 }
 
 

@@ -192,9 +192,25 @@ def track_beam_host(eles, beam: dict, n_turns: int, device: int = 0, turn0: int 
     return out
 
 
-def track_all(ctx: Context, lat: Lattice, vec, beta, n_turns: int = 1):
-    """Single particles (rows of vec) through n_turns, stopping at loss like track_all.
-    Returns (vec_out, state, loss_ele, loss_turn)."""
+def track_orbit(ctx: Context, lat: Lattice, beam: Beam, ix_start: int = 1,
+                ix_end: int | None = None, turn: int = 0) -> np.ndarray:
+    """Elements ix_start..ix_end (1-based, inclusive) once, recording every particle after every
+    element (track_many, track_many.f90:50-157). Returns an array (n, n_range, 8) of x, px, y,
+    py, z, pz, beta, state; rows after the element a particle is lost in are zero. Blocking."""
+    if ix_end is None:
+        ix_end = lat.n_ele
+    n_range = ix_end - ix_start + 1
+    rec = np.zeros((beam.n, max(n_range, 0), 8), dtype=np.float64)
+    native.call("bmx_track_orbit", ctx.h, lat.h, beam.h, int(ix_start), int(ix_end), int(turn),
+                rec.ctypes.data_as(ctypes.c_void_p))
+    return rec
+
+
+def track_all(ctx: Context, lat: Lattice, vec, beta):
+    """track_all (track_all.f90:25-145) for single particles (rows of vec): one pass over all
+    elements. Returns (orbit, track_state): orbit[i, j] = (x, px, y, py, z, pz, beta, state) of
+    particle i at the exit of element j + 1 (zero after its loss), track_state[i] = the 1-based
+    element index where particle i was lost, or -1 (moving_forward$) if it was not."""
     vec = np.atleast_2d(np.asarray(vec, dtype=np.float64))
     n = vec.shape[0]
     beam = {k: np.ascontiguousarray(vec[:, i]) for i, k in enumerate(FLOAT_KEYS[:6])}
@@ -203,9 +219,8 @@ def track_all(ctx: Context, lat: Lattice, vec, beta, n_turns: int = 1):
     beam["loss_ele"] = np.zeros(n, dtype=np.int32)
     beam["loss_turn"] = np.full(n, -1, dtype=np.int32)
     b = Beam.from_host(ctx, beam)
-    track_beam(ctx, lat, b, n_turns)
-    ctx.synchronize()
+    orbit = track_orbit(ctx, lat, b)
     out = b.download()
     b.close()
-    v = np.stack([out[k] for k in FLOAT_KEYS[:6]], axis=1)
-    return v, out["state"], out["loss_ele"], out["loss_turn"]
+    track_state = np.where(out["state"] == E.STATE_ALIVE, -1, out["loss_ele"]).astype(np.int32)
+    return orbit, track_state

@@ -218,6 +218,16 @@ int32_t bmx_track_element(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam, 
 int32_t bmx_track_turns_plain(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam,
                               int32_t n_turns, int32_t turn0);
 
+/* Elements ix_start..ix_end (1-based, inclusive) once for every live particle, with the state
+ * at every element exit written to orbit (caller-owned host memory of
+ * n * (ix_end - ix_start + 1) * 8 doubles; orbit[(i * n_range + j) * 8 + k] = x, px, y, py, z,
+ * pz, beta, state of particle i after element ix_start + j; zero after the element a
+ * particle is lost in). track_all / track_many with their orbit(0:n) arrays
+ * (track_all.f90:25-145, track_many.f90:50-157); loss_turn of a particle lost here = turn.
+ * Blocking. */
+int32_t bmx_track_orbit(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam, int32_t ix_start,
+                        int32_t ix_end, int32_t turn, double *orbit);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,122 @@
+"""The Fortran host (bmx_bmad_mod, iso_c_binding) against the Python host and the CPU
+restatement: the track_bunch and track_all conventions through the same C-ABI."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import parity as P
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "bmad-ecosystem_amd", "fortran", "_build", "ftrack")
+
+
+def _write_inputs(d, arr, beam, n_turns, mc2):
+    with open(os.path.join(d, "lattice.bin"), "wb") as f:
+        f.write(np.int32(len(arr)).tobytes())
+        f.write(np.float64(mc2).tobytes())
+        f.write(arr.tobytes())
+    n = len(beam["x"])
+    with open(os.path.join(d, "beam.bin"), "wb") as f:
+        f.write(np.int32(n).tobytes())
+        f.write(np.int32(n_turns).tobytes())
+        for k in P.COORDS + ("beta",):
+            f.write(np.ascontiguousarray(beam[k], dtype=np.float64).tobytes())
+        f.write(np.ascontiguousarray(beam["state"], dtype=np.int32).tobytes())
+
+
+def _read_outputs(d, n, n_ele):
+    raw = open(os.path.join(d, "out.bin"), "rb").read()
+    out = {}
+    off = 0
+    for k in P.COORDS + ("beta",):
+        out[k] = np.frombuffer(raw, np.float64, n, off)
+        off += 8 * n
+    for k in ("state", "ix_ele", "ix_turn"):
+        out[k] = np.frombuffer(raw, np.int32, n, off)
+        off += 4 * n
+    raw = open(os.path.join(d, "orbit.bin"), "rb").read()
+    track_state = int(np.frombuffer(raw, np.int32, 1, 0)[0])
+    orbit = np.frombuffer(raw, np.float64, (n_ele + 1) * 8, 4).reshape(n_ele + 1, 8)
+    return out, track_state, orbit
+
+
+def _traj_err(got, ref):
+    """Max relative error along a trajectory, each coordinate relative to max(|value|, its
+    RMS along the trajectory) as parity.rel_err does for a beam."""
+    scale = np.maximum(np.sqrt(np.mean(ref * ref, axis=0)), [1e-6, 1e-7, 1e-6, 1e-7, 1e-6, 1e-7])
+    return float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), scale)))
+
+
+def _run(tmp_path):
+    if not os.path.exists(DRIVER):
+        pytest.fail("Fortran host not built (build() compiles it with amdflang)")
+    r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_fortran_track_bunch_and_track_all(bmx, oracle, tmp_path):
+    """ring1 with 3 cells, 2000 particles of a wide beam, 20 turns: the Fortran host's results
+    equal the Python host's bit for bit (same kernels), its loss bookkeeping is Bmad's
+    (ix_ele / ix_turn of lost particles), and track_all's orbit(0:n) matches the CPU
+    restatement element by element."""
+    L, E = bmx.lattices, bmx.elements
+    arr = E.to_array(L.ring1(n_cells=3))
+    n_ele = len(arr)
+    beam = P.random_beam(bmx, 2000, seed=9, scale=20.0, pz_sigma=4e-3)
+    beam["x"][0], beam["y"][0] = 3e-3, -2e-3   # particle 1 (track_all) survives one pass
+    beam["pz"][0] = 1e-3
+    beam["beta"] = L.init_beta(beam["pz"], *L.reference())
+    n_turns = 20
+    _write_inputs(tmp_path, arr, beam, n_turns, E.M_ELECTRON)
+    _run(tmp_path)
+    out, track_state, orbit = _read_outputs(tmp_path, len(beam["x"]), n_ele)
+
+    py = bmx.tracking.track_beam_host(arr, beam, n_turns)
+    for k in P.COORDS + ("beta", "state"):
+        assert np.array_equal(out[k], py[k]), k
+    lost = py["state"] != 1
+    assert lost.sum() > 100
+    assert np.array_equal(out["ix_ele"][lost], py["loss_ele"][lost])
+    assert np.array_equal(out["ix_turn"][lost], py["loss_turn"][lost])
+    assert np.all(out["ix_ele"][~lost] == n_ele)
+    assert np.all(out["ix_turn"][~lost] == n_turns - 1)
+
+    ref = P.copy_beam(beam)
+    oracle.track_bunch(arr, ref, n_turns)
+    assert P.loss_mismatch(py, ref).size == 0
+
+    v0 = [beam[k][0] for k in P.COORDS]
+    trace, _, _, st = oracle.track_trace(arr, v0, beam["beta"][0], 1)
+    assert st == 1 and track_state == -1
+    assert np.array_equal(orbit[0, :6], v0)
+    assert _traj_err(orbit[1:, :6], trace) <= 1e-12
+    assert np.all(orbit[1:, 7] == 1)
+
+
+def test_fortran_track_all_loss(bmx, oracle, tmp_path):
+    """track_all on a particle that is lost part way: track_state is the element it was lost
+    in (track_all.f90:81-88), the orbit stops there, and the loss code matches the CPU."""
+    L, E = bmx.lattices, bmx.elements
+    arr = E.to_array(L.ring1(n_cells=3))
+    n_ele = len(arr)
+    beam = P.random_beam(bmx, 64, seed=3)
+    beam["x"][0], beam["px"][0] = 0.018, 0.0   # lost on the 18th element (-x aperture)
+    _write_inputs(tmp_path, arr, beam, 1, E.M_ELECTRON)
+    _run(tmp_path)
+    _, track_state, orbit = _read_outputs(tmp_path, 64, n_ele)
+    ref = P.copy_beam(beam)
+    oracle.track_bunch(arr, ref, 1)
+    assert ref["state"][0] != 1
+    assert track_state == ref["loss_ele"][0]
+    assert orbit[track_state, 7] == ref["state"][0]
+    trace, _, _, _ = oracle.track_trace(arr, [beam[k][0] for k in P.COORDS], beam["beta"][0], 1)
+    assert track_state > 5
+    assert _traj_err(orbit[1:track_state, :6], trace[: track_state - 1]) <= 1e-12
+    # orbit(track_state + 1:) = coord_struct(): zero vec, beta -1, state not_set
+    assert np.all(orbit[track_state + 1:, 7] == -999)
+    assert np.all(orbit[track_state + 1:, 6] == -1)
+    assert np.all(orbit[track_state + 1:, :6] == 0)

