@@ -74,6 +74,7 @@ int32_t bmx_ctx_destroy(bmx_ctx *ctx)
     if (!ctx) return BMX_OK;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    ctx->run.release();
     for (auto &ev : ctx->events)
         if (ev) hipEventDestroy(ev);
     hipFree(ctx->d_counter);
@@ -304,10 +305,36 @@ int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t
     if (n_turns < 0) return set_error(BMX_ERR_ARG, "bmx_track_turns: negative turn count");
     if (L->ctx->device != ctx->device || B->ctx->device != ctx->device)
         return set_error(BMX_ERR_ARG, "bmx_track_turns: handles on different devices");
+    if (B->n >= (int64_t)UINT32_MAX) return set_error(BMX_ERR_ARG, "bmx_track_turns: more than 2^32-1 particles");
+    if (n_turns == 0 || B->n == 0) return BMX_OK;
     hipSetDevice(ctx->device);
-    HIP_TRY(launch_track_turns(ctx->stream, L->d_desc, nullptr, 0, B->d, B->n, n_turns, turn0,
-                               ctx->d_work, ctx->fast_maps),
-            "k_track_turns");
+    // the previous call's group upload must have been consumed before the host copy changes
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    ctx->run.device = ctx->device;
+    HIP_TRY(ctx->run.reserve(1, (size_t)B->n, (size_t)(B->n + 63) / 64), "run buffers");
+    Group &g = ctx->group;
+    g.lat = L->desc;
+    g.b = B->d;
+    g.n = (uint32_t)B->n;
+    g.list_off = 0;
+    g.n_turns = n_turns;
+    g.turn0 = turn0;
+    RunConfig cfg;
+    cfg.chunk_turns = ctx->chunk_turns;
+    cfg.fast = ctx->fast_maps;
+    cfg.waves = ctx->waves;
+    cfg.grid = ctx->grid > 0 ? ctx->grid : default_grid(ctx->device, ctx->fast_maps, cfg.waves);
+    HIP_TRY(enqueue_run(ctx->stream, ctx->run, &g, 1, true, n_turns, cfg, ctx->d_work), "k_track_tiles");
+    return BMX_OK;
+}
+
+int32_t bmx_ctx_set_run(bmx_ctx *ctx, int32_t chunk_turns, int32_t grid, int32_t waves)
+{
+    if (!ctx || chunk_turns < 0 || grid < 0 || (waves != 0 && (waves < 2 || waves > 4)))
+        return set_error(BMX_ERR_ARG, "bmx_ctx_set_run: bad argument");
+    ctx->chunk_turns = chunk_turns;
+    ctx->grid = grid;
+    ctx->waves = waves ? waves : 3;
     return BMX_OK;
 }
 

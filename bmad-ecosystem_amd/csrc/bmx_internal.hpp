@@ -11,9 +11,15 @@
 namespace bmx {
 
 // ---- kernel launchers (bmx_kernels.hip) ----
-hipError_t launch_track_turns(hipStream_t s, const LatDesc *d_lats, const Tile *d_tiles,
-                              int64_t n_tiles, const BeamPtrs &b, int64_t n, int32_t n_turns,
-                              int32_t turn0, unsigned long long *d_work, bool fast);
+hipError_t launch_track_tiles(hipStream_t s, int grid, int waves, const Group *d_groups,
+                              const WaveTile *d_tiles, const uint32_t *d_n_tiles,
+                              const uint32_t *d_list_in, uint32_t *d_list_out, uint32_t *d_cnt_out,
+                              uint32_t *d_tile_ctr, int32_t chunk_off, int32_t chunk_turns,
+                              unsigned long long *d_work, bool fast);
+hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_groups,
+                              const uint32_t *d_cnt_in, int32_t next_off, WaveTile *d_tiles,
+                              uint32_t *d_n_tiles, uint32_t *d_cnt_zero, uint32_t *d_tile_ctr);
+hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu);
 hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, int32_t turn,
                                 const BeamPtrs &b, int64_t n, unsigned long long *d_work,
                                 bool fast);
@@ -37,6 +43,41 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
 // ---- error reporting (bmx_api.cpp) ----
 int set_error(int code, const std::string &msg);
 
+// ---- chunked tile runs (bmx_run.cpp) ----
+// Device buffers of a tile run over a set of groups: the groups, two alive lists, two tile
+// lists with their counts, per-group alive counters for the two lists and the tile counter.
+struct RunBuffers {
+    int device = -1;
+    size_t cap_groups = 0, cap_particles = 0, cap_tiles = 0;
+    Group *d_groups = nullptr;
+    uint32_t *d_list[2] = {nullptr, nullptr};
+    WaveTile *d_tiles[2] = {nullptr, nullptr};
+    uint32_t *d_cnt[2] = {nullptr, nullptr};
+    uint32_t *d_small = nullptr;  // [0..1] tile counts of the two tile lists, [2] tile counter
+    void release();
+    // Grows the buffers to hold the given sizes (allocates; not for use under graph capture).
+    hipError_t reserve(size_t n_groups, size_t n_particles, size_t n_tiles);
+};
+
+// Parameters of one run.
+struct RunConfig {
+    int32_t chunk_turns = 10;  // turns per chunk (alive lists are rebuilt between chunks)
+    int32_t grid = 0;          // workgroups of the tracking kernel (0: occupancy limit x CUs)
+    int32_t waves = 3;         // waves per SIMD the tracking kernel is compiled for (2, 3, 4)
+    bool fast = true;
+};
+
+// Uploads the groups (asynchronously, from host memory that must stay valid until the stream
+// reaches the copy) and enqueues the chunks that track every group for its n_turns: the tile
+// build of the first chunk, then per chunk the tracking kernel and the next chunk's tile
+// build. Enqueues nothing that allocates or synchronises, so a run can be captured in a graph
+// once upload is false and the groups are already on the device.
+hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int32_t n_groups,
+                       bool upload, int32_t max_turns, const RunConfig &cfg,
+                       unsigned long long *d_work);
+// Workgroups of the tracking kernel that fit on the device at once.
+int default_grid(int device, bool fast, int waves);
+
 }  // namespace bmx
 
 struct bmx_ctx {
@@ -46,6 +87,11 @@ struct bmx_ctx {
     unsigned long long *d_work = nullptr;     // particle-element passes since the last take
     hipEvent_t events[BMX_N_EVENTS] = {};
     bool fast_maps = true;  // BMX_MAPS_FAST (default) or BMX_MAPS_REFERENCE
+    int32_t chunk_turns = 10;
+    int32_t grid = 0;       // 0: default_grid
+    int32_t waves = 3;      // waves per SIMD of the tracking kernel variant
+    bmx::RunBuffers run;    // buffers of bmx_track_turns
+    bmx::Group group{};     // host copy of the single group uploaded by bmx_track_turns
 };
 
 struct bmx_lattice {

@@ -1,8 +1,9 @@
 // Tracking kernels for gfx950.
 //
-//  k_track_turns   persistent tracking: one lane per particle, state in registers for all
-//                  elements and turns of the call; work comes as tiles (one tile per
-//                  workgroup) so single-lattice, batched and served launches share the kernel.
+//  k_track_tiles   persistent tracking of one chunk of turns: one lane per particle, state in
+//                  registers for all elements and turns of the chunk; work comes as wave tiles
+//                  of one group each, so single-lattice, batched and served runs share it.
+//  k_build_tiles   next chunk's tiles from the alive counts (lost particles drop out).
 //  k_track_element one element for every particle, state round-trips HBM: the GPU form of
 //                  track1_bunch_hom (bmad/multiparticle/beam_utils.f90:32-90).
 //  k_count_alive   number of live particles (lt_tracking_mod.f90:1672 bunch%n_live).
@@ -50,69 +51,157 @@ __device__ __forceinline__ void add_work(unsigned long long *work, unsigned long
     if ((threadIdx.x & 63) == 0 && s) atomicAdd(work, s);
 }
 
-// Persistent tracking over tiles (tiles == nullptr: implicit tiles of kBlock particles of
-// lattice 0 covering [0, n) with the given n_turns / turn0).
+// Tracks one particle through up to n_turns full turns starting at turn index turn0; records
+// the loss element (1-based) and turn. Returns the number of element passes made.
 template <bool F>
-__global__ void __launch_bounds__(kBlock) k_track_turns(const LatDesc *__restrict__ lats,
-                                                        const Tile *__restrict__ tiles, BeamPtrs b,
-                                                        int64_t n, int32_t n_turns, int32_t turn0,
-                                                        unsigned long long *work)
+__device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, const BeamPtrs &b,
+                                                             uint32_t i, int32_t n_turns,
+                                                             int32_t turn0)
 {
-    Tile t;
-    if (tiles) {
-        t = tiles[blockIdx.x];
+    Particle p;
+    load_particle(p, b, i);
+    refresh_energy(p, L.sp);
+    int32_t lost_e = 0, lost_t = 0;
+    // Entrance validity check of the first element (track1.f90:196 -> orbit_too_large); later
+    // elements need none: a particle leaves element e with the coordinates it enters e+1 with,
+    // and element e already ended with the same test (track1.f90:291).
+    if (orbit_too_large(p)) {
+        lost_e = 1;
+        lost_t = turn0;
     } else {
-        t.lat = 0;
-        t.start = (int64_t)blockIdx.x * kBlock;
-        const int64_t rem = n - t.start;
-        t.count = rem < kBlock ? (int32_t)rem : kBlock;
-        t.n_turns = n_turns;
-        t.turn0 = turn0;
-    }
-    const int li = threadIdx.x;
-    const int64_t i = t.start + li;
-    // No early return: every lane of the wave takes part in the work reduction at the end.
-    const bool active = li < t.count && t.n_turns > 0 && b.state[i] == ST_ALIVE;
-    unsigned long long passes = 0;
-    if (active) {
-        const LatDesc L = lats[t.lat];
-        Particle p;
-        load_particle(p, b, i);
-        refresh_energy(p, L.sp);
-
-        int32_t lost_e = 0, lost_t = 0;
-        // Entrance validity check of the first element (track1.f90:196 -> orbit_too_large);
-        // later elements need none: a particle leaves element e with the coordinates it enters
-        // e+1 with, and element e already ended with the same test (track1.f90:291).
-        if (orbit_too_large(p)) {
-            lost_e = 1;
-            lost_t = t.turn0;
-        } else {
-            const CHdr hdr = (CHdr)L.hdr;
-            const CP prm = (CP)L.prm;
-            for (int32_t turn = 0; turn < t.n_turns; ++turn) {
-                for (int32_t e = 0; e < L.n_ele; ++e) {
-                    track_element<F>(p, hdr + 4 * e, prm, L.sp);
-                    if (p.state != ST_ALIVE) {
-                        lost_e = e + 1;
-                        lost_t = t.turn0 + turn;
-                        break;
-                    }
+        const CHdr hdr = (CHdr)L.hdr;
+        const CP prm = (CP)L.prm;
+        for (int32_t turn = 0; turn < n_turns; ++turn) {
+            for (int32_t e = 0; e < L.n_ele; ++e) {
+                track_element<F>(p, hdr + 4 * e, prm, L.sp);
+                if (p.state != ST_ALIVE) {
+                    lost_e = e + 1;
+                    lost_t = turn0 + turn;
+                    break;
                 }
-                if (p.state != ST_ALIVE) break;
+            }
+            if (p.state != ST_ALIVE) break;
+        }
+    }
+    store_particle(p, b, i);
+    if (p.state != ST_ALIVE) {
+        b.state[i] = p.state;
+        b.loss_ele[i] = lost_e;
+        b.loss_turn[i] = lost_t;
+        return (unsigned long long)(lost_t - turn0) * L.n_ele + lost_e;
+    }
+    return (unsigned long long)n_turns * L.n_ele;
+}
+
+// Persistent tracking of one chunk of turns over a list of wave tiles.
+//
+// Each wave repeatedly takes the next tile (atomic counter) and tracks its particles for the
+// chunk's turns with the state in registers. Particles still alive at the end are appended to
+// their group's segment of the next alive list (one atomic per wave), so the next chunk's
+// tiles hold live particles only and waves stay full as the beam is lost. The order inside a
+// list depends on scheduling; each particle's result does not.
+//
+// Only the first w_eff waves take tiles, w_eff = ceil(n_tiles / ceil(n_tiles / waves)), so the
+// tiles divide evenly over the waves that run (no last partial round).
+// chunk_off: turns already done by earlier chunks of this call; chunk_turns: turns per chunk.
+// W: waves per SIMD the register allocation targets (3: up to 168 VGPRs, no spills; 4: up to
+// 128, spilling in the rf and bend paths; 2: up to 256).
+template <bool F, int W>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) k_track_tiles(
+    const Group *__restrict__ groups, const WaveTile *__restrict__ tiles,
+    const uint32_t *__restrict__ n_tiles_p, const uint32_t *__restrict__ list_in,
+    uint32_t *__restrict__ list_out, uint32_t *__restrict__ cnt_out, uint32_t *tile_ctr,
+    int32_t chunk_off, int32_t chunk_turns, unsigned long long *work)
+{
+    const uint32_t nt = *n_tiles_p;
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    const uint32_t rounds = (nt + waves - 1) / waves;
+    const uint32_t w_eff = rounds ? (nt + rounds - 1) / rounds : 0;
+    if (wave >= w_eff) return;  // whole wave
+    unsigned long long passes = 0;
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(tile_ctr, 1u);
+        t = __builtin_amdgcn_readfirstlane(t);
+        if (t >= nt) break;
+        const WaveTile T = tiles[t];
+        const Group &G = groups[T.group];
+        const int32_t rem = G.n_turns - chunk_off;
+        const int32_t n_turns = rem < chunk_turns ? rem : chunk_turns;
+        const int32_t turn0 = G.turn0 + chunk_off;
+        bool alive_end = false;
+        uint32_t idx = 0;
+        if ((uint32_t)lane < T.count) {
+            idx = list_in ? list_in[G.list_off + T.start + lane] : T.start + lane;
+            if (G.b.state[idx] == ST_ALIVE) {
+                passes += track_particle<F>(G.lat, G.b, idx, n_turns, turn0);
+                alive_end = G.b.state[idx] == ST_ALIVE;
             }
         }
-        store_particle(p, b, i);
-        if (p.state != ST_ALIVE) {
-            b.state[i] = p.state;
-            b.loss_ele[i] = lost_e;
-            b.loss_turn[i] = lost_t;
-            passes = (unsigned long long)(lost_t - t.turn0) * L.n_ele + lost_e;
-        } else {
-            passes = (unsigned long long)t.n_turns * L.n_ele;
+        const unsigned long long m = __ballot(alive_end);
+        if (m) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&cnt_out[T.group], (uint32_t)__popcll(m));
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (alive_end) {
+                const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1));
+                list_out[G.list_off + base + rank] = idx;
+            }
         }
     }
     add_work(work, passes);
+}
+
+// Builds the wave tiles of the next chunk from the alive counts per group (cnt_in; nullptr
+// for the first chunk: every particle of each group, indexed directly) and resets the counters
+// the next chunk appends to. One workgroup of kBuildThreads threads; groups that have no turns
+// left after next_off get no tiles.
+constexpr int kBuildThreads = 1024;
+__global__ void __launch_bounds__(kBuildThreads) k_build_tiles(
+    const Group *__restrict__ groups, int32_t n_groups, const uint32_t *__restrict__ cnt_in,
+    int32_t next_off, WaveTile *__restrict__ tiles, uint32_t *__restrict__ n_tiles_p,
+    uint32_t *__restrict__ cnt_zero, uint32_t *__restrict__ tile_ctr)
+{
+    __shared__ uint32_t part[kBuildThreads];
+    const int tid = threadIdx.x;
+    // contiguous run of groups per thread, so tiles come out in group order
+    const int per = (n_groups + kBuildThreads - 1) / kBuildThreads;
+    const int g0 = tid * per, g1 = min(n_groups, g0 + per);
+    uint32_t mine = 0;
+    for (int g = g0; g < g1; g++) {
+        const uint32_t c = cnt_in ? cnt_in[g] : groups[g].n;
+        if (groups[g].n_turns > next_off) mine += (c + 63) / 64;
+    }
+    part[tid] = mine;
+    __syncthreads();
+    // inclusive scan (Hillis-Steele)
+    for (int o = 1; o < kBuildThreads; o <<= 1) {
+        const uint32_t v = tid >= o ? part[tid - o] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t pos = part[tid] - mine;
+    for (int g = g0; g < g1; g++) {
+        const uint32_t c = cnt_in ? cnt_in[g] : groups[g].n;
+        if (groups[g].n_turns > next_off) {
+            for (uint32_t s = 0; s < c; s += 64) {
+                WaveTile w;
+                w.group = g;
+                w.start = s;
+                w.count = c - s < 64 ? c - s : 64;
+                w.pad = 0;
+                tiles[pos++] = w;
+            }
+        }
+        cnt_zero[g] = 0;
+    }
+    if (tid == kBuildThreads - 1) {
+        *n_tiles_p = part[tid];
+        *tile_ctr = 0;
+    }
 }
 
 // One element, whole bunch: track1 (track1.f90:44-343) per live particle.
@@ -211,19 +300,53 @@ __global__ void __launch_bounds__(kBlock) k_count_alive(const int32_t *__restric
     }
 }
 
-hipError_t launch_track_turns(hipStream_t s, const LatDesc *d_lats, const Tile *d_tiles,
-                              int64_t n_tiles, const BeamPtrs &b, int64_t n, int32_t n_turns,
-                              int32_t turn0, unsigned long long *d_work, bool fast)
+template <bool F, int W>
+static void launch_tiles(hipStream_t s, int grid, const Group *g, const WaveTile *t, const uint32_t *nt,
+                         const uint32_t *li, uint32_t *lo, uint32_t *co, uint32_t *tc, int32_t off,
+                         int32_t K, unsigned long long *w)
 {
-    int64_t grid = d_tiles ? n_tiles : (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((k_track_tiles<F, W>), dim3((unsigned)grid), dim3(kBlock), 0, s, g, t, nt, li,
+                       lo, co, tc, off, K, w);
+}
+
+hipError_t launch_track_tiles(hipStream_t s, int grid, int waves, const Group *d_groups,
+                              const WaveTile *d_tiles, const uint32_t *d_n_tiles,
+                              const uint32_t *d_list_in, uint32_t *d_list_out, uint32_t *d_cnt_out,
+                              uint32_t *d_tile_ctr, int32_t chunk_off, int32_t chunk_turns,
+                              unsigned long long *d_work, bool fast)
+{
     if (grid <= 0) return hipSuccess;
-    if (fast)
-        hipLaunchKernelGGL(k_track_turns<true>, dim3((unsigned)grid), dim3(kBlock), 0, s, d_lats,
-                           d_tiles, b, n, n_turns, turn0, d_work);
-    else
-        hipLaunchKernelGGL(k_track_turns<false>, dim3((unsigned)grid), dim3(kBlock), 0, s, d_lats,
-                           d_tiles, b, n, n_turns, turn0, d_work);
+#define BMX_TILES(F, W)                                                                          \
+    launch_tiles<F, W>(s, grid, d_groups, d_tiles, d_n_tiles, d_list_in, d_list_out, d_cnt_out,  \
+                       d_tile_ctr, chunk_off, chunk_turns, d_work)
+    if (fast) {
+        if (waves == 2) BMX_TILES(true, 2);
+        else if (waves == 4) BMX_TILES(true, 4);
+        else BMX_TILES(true, 3);
+    } else {
+        if (waves == 2) BMX_TILES(false, 2);
+        else if (waves == 4) BMX_TILES(false, 4);
+        else BMX_TILES(false, 3);
+    }
+#undef BMX_TILES
     return hipGetLastError();
+}
+
+hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_groups,
+                              const uint32_t *d_cnt_in, int32_t next_off, WaveTile *d_tiles,
+                              uint32_t *d_n_tiles, uint32_t *d_cnt_zero, uint32_t *d_tile_ctr)
+{
+    hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(kBuildThreads), 0, s, d_groups, n_groups,
+                       d_cnt_in, next_off, d_tiles, d_n_tiles, d_cnt_zero, d_tile_ctr);
+    return hipGetLastError();
+}
+
+hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu)
+{
+    const void *k;
+    if (fast) k = waves == 2 ? (const void *)k_track_tiles<true, 2> : waves == 4 ? (const void *)k_track_tiles<true, 4> : (const void *)k_track_tiles<true, 3>;
+    else k = waves == 2 ? (const void *)k_track_tiles<false, 2> : waves == 4 ? (const void *)k_track_tiles<false, 4> : (const void *)k_track_tiles<false, 3>;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, kBlock, 0);
 }
 
 hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, int32_t turn,

@@ -1,0 +1,117 @@
+// Chunked tile runs: the host side of k_track_tiles / k_build_tiles.
+//
+// A run tracks a set of groups (lattice + particle range + turn range) chunk by chunk. Before
+// each chunk k_build_tiles turns the alive counts of the previous chunk into wave tiles, so
+// lost particles stop taking lanes; the tracking kernel is a persistent grid sized to the
+// device. This replaces the reference's per-element fork/join over all particles with
+// per-chunk compaction (beam_utils.f90:81-86 skips dead particles one by one).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "bmx_internal.hpp"
+
+namespace bmx {
+
+void RunBuffers::release()
+{
+    if (device >= 0) hipSetDevice(device);
+    hipFree(d_groups);
+    for (int k = 0; k < 2; k++) {
+        hipFree(d_list[k]);
+        hipFree(d_tiles[k]);
+        hipFree(d_cnt[k]);
+        d_list[k] = nullptr;
+        d_tiles[k] = nullptr;
+        d_cnt[k] = nullptr;
+    }
+    hipFree(d_small);
+    d_groups = nullptr;
+    d_small = nullptr;
+    cap_groups = cap_particles = cap_tiles = 0;
+}
+
+hipError_t RunBuffers::reserve(size_t n_groups, size_t n_particles, size_t n_tiles)
+{
+    hipError_t e = hipSuccess;
+    if (!d_small) {
+        e = hipMalloc(&d_small, 4 * sizeof(uint32_t));
+        if (e != hipSuccess) return e;
+    }
+    if (n_groups > cap_groups) {
+        const size_t c = std::max(n_groups, 2 * cap_groups);
+        hipFree(d_groups);
+        for (int k = 0; k < 2; k++) hipFree(d_cnt[k]);
+        d_groups = nullptr;
+        d_cnt[0] = d_cnt[1] = nullptr;
+        cap_groups = 0;
+        e = hipMalloc(&d_groups, c * sizeof(Group));
+        for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMalloc(&d_cnt[k], c * sizeof(uint32_t));
+        if (e != hipSuccess) return e;
+        cap_groups = c;
+    }
+    if (n_particles > cap_particles) {
+        const size_t c = std::max(n_particles, cap_particles + cap_particles / 2);
+        for (int k = 0; k < 2; k++) {
+            hipFree(d_list[k]);
+            d_list[k] = nullptr;
+        }
+        cap_particles = 0;
+        for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMalloc(&d_list[k], c * sizeof(uint32_t));
+        if (e != hipSuccess) return e;
+        cap_particles = c;
+    }
+    if (n_tiles > cap_tiles) {
+        const size_t c = std::max(n_tiles, cap_tiles + cap_tiles / 2);
+        for (int k = 0; k < 2; k++) {
+            hipFree(d_tiles[k]);
+            d_tiles[k] = nullptr;
+        }
+        cap_tiles = 0;
+        for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMalloc(&d_tiles[k], c * sizeof(WaveTile));
+        if (e != hipSuccess) return e;
+        cap_tiles = c;
+    }
+    return e;
+}
+
+int default_grid(int device, bool fast, int waves)
+{
+    int blocks = 0, cus = 0;
+    if (track_tiles_occupancy(fast, waves, &blocks) != hipSuccess || blocks <= 0) blocks = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+        cus = 1;
+    return blocks * cus;
+}
+
+hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int32_t n_groups,
+                       bool upload, int32_t max_turns, const RunConfig &cfg,
+                       unsigned long long *d_work)
+{
+    if (n_groups <= 0 || max_turns <= 0) return hipSuccess;
+    hipError_t e = hipSuccess;
+    if (upload) {
+        e = hipMemcpyAsync(rb.d_groups, h_groups, (size_t)n_groups * sizeof(Group),
+                           hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return e;
+    }
+    const int32_t K = cfg.chunk_turns > 0 ? cfg.chunk_turns : max_turns;
+    uint32_t *n_tiles[2] = {rb.d_small, rb.d_small + 1};
+    uint32_t *tile_ctr = rb.d_small + 2;
+    // chunk 0 tiles index the particles directly; the chunk appends to list/cnt 1
+    e = launch_build_tiles(s, rb.d_groups, n_groups, nullptr, 0, rb.d_tiles[0], n_tiles[0],
+                           rb.d_cnt[1], tile_ctr);
+    int c = 0;
+    for (int32_t off = 0; off < max_turns && e == hipSuccess; off += K, c++) {
+        const int cur = c & 1, nxt = cur ^ 1;
+        e = launch_track_tiles(s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur], n_tiles[cur],
+                               c == 0 ? nullptr : rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt],
+                               tile_ctr, off, K, d_work, cfg.fast);
+        if (e == hipSuccess && off + K < max_turns)
+            e = launch_build_tiles(s, rb.d_groups, n_groups, rb.d_cnt[nxt], off + K,
+                                   rb.d_tiles[nxt], n_tiles[nxt], rb.d_cnt[cur], tile_ctr);
+    }
+    return e;
+}
+
+}  // namespace bmx

@@ -69,15 +69,6 @@ struct Species {
     double lez_thr;  // 3e-7 E_tot
 };
 
-// One unit of persistent work: a run of particles of one lattice.
-struct Tile {
-    int32_t lat;      // lattice index (batched launches), 0 otherwise
-    int32_t count;    // particles in the tile
-    int64_t start;    // first particle index in the beam arrays
-    int32_t n_turns;  // turns to track in this launch
-    int32_t turn0;    // turn index of the first turn (loss_turn bookkeeping)
-};
-
 // Per-lattice device descriptor.
 struct LatDesc {
     const EleHdr *hdr;
@@ -91,6 +82,27 @@ struct LatDesc {
 struct BeamPtrs {
     double *x, *px, *y, *py, *z, *pz, *beta;
     int32_t *state, *loss_ele, *loss_turn;
+};
+
+// A group: particles [0, n) of one beam view tracked through one lattice for turns
+// turn0 .. turn0 + n_turns - 1. A single-lattice call is one group; batched variants and served
+// requests are one group each. list_off locates the group's segment in the alive lists.
+struct Group {
+    LatDesc lat;
+    BeamPtrs b;
+    uint32_t n;
+    uint32_t list_off;
+    int32_t n_turns;
+    int32_t turn0;
+};
+
+// One wave's unit of work: up to 64 particles of one group for one chunk of turns. start
+// indexes the group's alive-list segment (or the particles themselves in the first chunk).
+struct WaveTile {
+    int32_t group;
+    uint32_t start;
+    uint32_t count;
+    uint32_t pad;
 };
 
 }  // namespace bmx

@@ -49,6 +49,7 @@ _SIGS = {
     "bmx_ctx_take_work": (_I32, [_P, ctypes.POINTER(_I64)]),
     "bmx_ctx_record": (_I32, [_P, _I32]),
     "bmx_ctx_set_maps": (_I32, [_P, _I32]),
+    "bmx_ctx_set_run": (_I32, [_P, _I32, _I32, _I32]),
     "bmx_ctx_elapsed_ms": (_I32, [_P, _I32, _I32, ctypes.POINTER(ctypes.c_float)]),
     "bmx_lattice_create": (_I32, [_P, _P, _I32, ctypes.c_double, _I32, _PP]),
     "bmx_lattice_destroy": (_I32, [_P]),

@@ -48,6 +48,11 @@ class Context:
         native.call("bmx_ctx_set_maps", self.h, {"fast": 1, "reference": 2}[form])
         self.maps = form
 
+    def set_run(self, chunk_turns: int = 10, grid: int = 0, waves: int = 0) -> None:
+        """Turns per chunk of bmx_track_turns (0 = one chunk), persistent grid size (0 = device
+        occupancy limit) and waves per SIMD of the kernel variant (0 = default 3; 2 or 4)."""
+        native.call("bmx_ctx_set_run", self.h, int(chunk_turns), int(grid), int(waves))
+
     def record(self, slot: int) -> None:
         """Record HIP event `slot` on this context's stream."""
         native.call("bmx_ctx_record", self.h, int(slot))

@@ -167,6 +167,11 @@ int32_t bmx_ctx_take_work(bmx_ctx *ctx, int64_t *passes);
 #define BMX_MAPS_FAST 1
 #define BMX_MAPS_REFERENCE 2
 int32_t bmx_ctx_set_maps(bmx_ctx *ctx, int32_t form);
+/* Run shape of this context's bmx_track_turns calls: turns per chunk (alive particles are
+ * compacted between chunks; 0 = the whole call is one chunk; default 10), workgroups of the
+ * persistent tracking grid (0 = as many as fit on the device at once, the default) and the
+ * waves per SIMD the tracking kernel variant is compiled for (0 = default 3; 2 or 4). */
+int32_t bmx_ctx_set_run(bmx_ctx *ctx, int32_t chunk_turns, int32_t grid, int32_t waves);
 /* Timing on the context's stream: record HIP event `slot` (0..BMX_N_EVENTS-1) now, and the
  * elapsed milliseconds between two recorded slots (synchronises on the later one). Replaces
  * the wall-clock run_timer of the reference drivers (sim_utils/misc/run_timer.f90:32). */
@@ -206,7 +211,8 @@ int32_t bmx_beam_count_alive(bmx_beam *beam, int64_t *n_alive);
 /* Persistent tracking: n_turns full turns starting at element 1, turn index turn0 .. turn0+n_turns-1.
  * Equivalent of ltt_run_beam_mode's turn loop (lt_tracking_mod.f90:1631-1716) calling
  * track_bunch (beam_mod.f90:103) once per turn. State stays in registers across elements and
- * turns; HBM is read once and written once per call. Asynchronous on the context's stream. */
+ * the turns of a chunk; between chunks lost particles are dropped from the work list.
+ * At most 2^32-1 particles. Asynchronous on the context's stream. */
 int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam, int32_t n_turns,
                         int32_t turn0);
 /* One element for the whole bunch: track1_bunch_hom (beam_utils.f90:32-90) on the GPU, state
@@ -217,6 +223,32 @@ int32_t bmx_track_element(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam, 
  * turn -> element -> particles, beam_mod.f90:159-166 + beam_utils.f90:81-86). Asynchronous. */
 int32_t bmx_track_turns_plain(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam,
                               int32_t n_turns, int32_t turn0);
+
+/* ---- batches: many lattices and particle sets in one launch sequence ------------------------ *
+ * A group is a range of one beam tracked through one lattice. A batch tracks all of its groups
+ * together: their particles share the persistent grid wave by wave (every wave works on one
+ * group at a time), lost particles are compacted out between chunks, and the whole sequence
+ * of chunks is captured once as a hipGraph and replayed by every bmx_batch_track call with the
+ * same turn count. Groups may share beams (disjoint ranges) and lattices. The batch keeps the
+ * handles; they must outlive it. */
+typedef struct bmx_group {
+    const bmx_lattice *lat;
+    bmx_beam *beam;
+    int64_t first;  /* first particle of the range */
+    int64_t count;  /* particles in the range */
+} bmx_group;
+int32_t bmx_batch_create(bmx_ctx *ctx, const bmx_group *groups, int32_t n_groups, bmx_batch **out);
+int32_t bmx_batch_destroy(bmx_batch *batch);
+/* Every group n_turns turns, turn indices turn0 .. turn0+n_turns-1 (asynchronous). */
+int32_t bmx_batch_track(bmx_batch *batch, int32_t n_turns, int32_t turn0);
+/* Run shape of the batch: turns per chunk, persistent grid (0 = device limit) and the waves
+ * per SIMD the tracking kernel is compiled for (0 = compiler's choice, 3 or 4). */
+int32_t bmx_batch_set_run(bmx_batch *batch, int32_t chunk_turns, int32_t grid, int32_t waves);
+/* Autotuner: times sample_turns turns of every candidate run shape (chunk length x kernel
+ * occupancy x grid size) on a scratch copy of the batch's particles (the batch's beams are not
+ * modified), keeps the fastest for the batch and returns it. Blocking. */
+int32_t bmx_batch_tune(bmx_batch *batch, int32_t sample_turns, int32_t *chunk_turns,
+                       int32_t *grid, int32_t *waves, float *best_ms);
 
 /* Elements ix_start..ix_end (1-based, inclusive) once for every live particle, with the state
  * at every element exit written to orbit (caller-owned host memory of

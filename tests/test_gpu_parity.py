@@ -258,6 +258,31 @@ def test_chunked_equals_single_call(bmx, ctx):
         assert np.array_equal(o1[k], o2[k]), k
 
 
+def test_run_shape_does_not_change_results(bmx, ctx):
+    """Chunk length (compaction interval) and persistent grid size change only the schedule:
+    every particle's result is bit-identical for chunks of 1, 7, 10 turns or one chunk, and
+    for a full-device grid or 8 workgroups."""
+    L = bmx.lattices
+    arr = bmx.elements.to_array(L.ring1(n_cells=10))
+    beam = P.random_beam(bmx, 5000, seed=4, scale=15.0, pz_sigma=4e-3)
+    lat = bmx.tracking.Lattice(ctx, arr)
+    outs = []
+    try:
+        for chunk, grid in ((0, 0), (1, 0), (7, 0), (10, 0), (10, 8), (3, 8)):
+            ctx.set_run(chunk, grid)
+            b = bmx.tracking.Beam.from_host(ctx, beam)
+            bmx.tracking.track_beam(ctx, lat, b, 23, 5)
+            ctx.synchronize()
+            outs.append(b.download())
+    finally:
+        ctx.set_run(10, 0)
+    lost = outs[0]["state"] != 1
+    assert 500 < lost.sum() < 4900
+    for o in outs[1:]:
+        for k in o:
+            assert np.array_equal(o[k], outs[0][k]), k
+
+
 def test_edge_sizes(bmx, oracle, ctx):
     """Empty beam, a single particle, ragged sizes and an all-lost beam."""
     L = bmx.lattices
