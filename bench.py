@@ -43,6 +43,14 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--maps", choices=["fast", "reference"], default="fast",
                     help="formulation of the element maps")
+    ap.add_argument("--chunk-turns", type=int, default=10,
+                    help="turns between alive-list compactions (0 = none)")
+    ap.add_argument("--extras", default="batched,da,served",
+                    help="comma list of the other workloads to run after the main one "
+                         "(batched, da, served; empty = none)")
+    ap.add_argument("--served-requests", type=int, default=24)
+    ap.add_argument("--waves", type=int, default=0,
+                    help="waves per SIMD of the tracking kernel variant (0 = default 3; 2 or 4)")
     return ap.parse_args()
 
 
@@ -83,6 +91,7 @@ def main():
 
     ctx = T.Context(0)
     ctx.set_maps(args.maps)
+    ctx.set_run(args.chunk_turns, 0, args.waves)
     lat = T.Lattice(ctx, arr)
     tps = args.turns_per_step
     if args.warmup > 0:
@@ -93,6 +102,8 @@ def main():
         wb.close()
     b = T.Beam.from_host(ctx, beam)
     ctx.take_work()
+    ctx.profile(True)
+    ctx.kernel_time()
     ctx.synchronize()
     t_wall0 = time.perf_counter()
     ctx.record(0)
@@ -106,11 +117,14 @@ def main():
     total_ms = ctx.elapsed_ms(0, 63)
     launch_ms = [ctx.elapsed_ms(1 + 2 * s, 2 + 2 * s) for s in range(args.steps)]
     passes = ctx.take_work()
+    kernel_ms, launches = ctx.kernel_time()
+    ctx.profile(False)
     alive = b.count_alive()
 
     value = passes / (total_ms * 1e-3)
-    kernel_s = np.mean(launch_ms) * 1e-3
-    achieved = flops * (passes / args.steps) / kernel_s / 1e12
+    # roofline of the tracking kernel: algorithmic FLOPs per launch / average launch duration
+    kernel_s = kernel_ms * 1e-3 / max(launches, 1)
+    achieved = flops * (passes / max(launches, 1)) / kernel_s / 1e12
     out = {
         "metric": METRIC,
         "value": value,
@@ -136,12 +150,106 @@ def main():
             "bound": "mfma", "compute": "fp64 vector ALU", "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": None, "flop_per_particle_element": flops,
-            "kernel": "k_track_turns", "kernel_ms_avg": float(np.mean(launch_ms)),
+            "kernel": "k_track_tiles", "kernel_ms_avg": kernel_ms / max(launches, 1),
+            "launches": launches, "step_ms": [float(x) for x in launch_ms],
         },
     }
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(bmx, arr, beam, args.cpu_seconds)
+    extras = [e for e in args.extras.split(",") if e]
+    out["extras"] = {}
+    if "batched" in extras:
+        out["extras"]["batched"] = run_batched(bmx, ctx, flops)
+    if "da" in extras:
+        out["extras"]["dynamic_aperture"] = run_da(bmx, ctx)
+    if "served" in extras:
+        out["extras"]["served"] = run_served(bmx, ctx, args.served_requests)
     print(json.dumps(out))
+
+
+def run_batched(bmx, ctx, flops, n_var=512, turns=500, seed=2):
+    """512 error variants of the ring (sigma 1e-3 on every k1, k2), log-uniform particle
+    counts in [256, 65536], 500 turns: one batch, autotuned, replayed as one captured graph."""
+    L, T = bmx.lattices, bmx.tracking
+    variants = L.ring1_variants(seed, n_var)
+    counts = L.variant_particle_counts(seed, n_var)
+    total = int(counts.sum())
+    beam = L.gaussian_beam(total, seed=seed, sigma=3 * L.SIGMA0)
+    lats = [T.Lattice(ctx, bmx.elements.to_array(v)) for v in variants]
+    b = T.Beam.from_host(ctx, beam)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    batch = T.Batch(ctx, [(lats[i], b, int(starts[i]), int(counts[i])) for i in range(n_var)])
+    t0 = time.perf_counter()
+    shape = batch.tune(sample_turns=10)
+    t_tune = time.perf_counter() - t0
+    ctx.take_work()
+    ctx.record(10)
+    batch.track(turns, 0)
+    ctx.record(11)
+    ctx.synchronize()
+    ms = ctx.elapsed_ms(10, 11)
+    passes = ctx.take_work()
+    alive = b.count_alive()
+    batch.close()
+    b.close()
+    for lt in lats:
+        lt.close()
+    rate = passes / (ms * 1e-3)
+    return {"workload": f"{n_var} error variants x log-uniform[256, 65536] particles "
+                        f"({total} total), {turns} turns, one captured graph",
+            "value": rate, "unit": UNIT, "ms": ms, "alive_at_end": alive,
+            "tflops_fp64": rate * flops / 1e12, "frac_fp64_peak": rate * flops / 1e12 / FP64_PEAK_TFLOPS,
+            "tuned": shape, "tune_s": t_tune}
+
+
+def run_da(bmx, ctx, n_var=4, turns=10_000, seed=3):
+    """Dynamic-aperture grid (11 angles x 64 radii x 4 pz) on 4 error variants, 1e4 turns:
+    survival turn per particle."""
+    L, T = bmx.lattices, bmx.tracking
+    variants = L.ring1_variants(seed, n_var)
+    lats = [T.Lattice(ctx, bmx.elements.to_array(v)) for v in variants]
+    grid = L.da_grid()
+    ctx.take_work()
+    t0 = time.perf_counter()
+    surv, _, _ = T.dynamic_aperture(ctx, lats, grid, turns)
+    dt = time.perf_counter() - t0
+    passes = ctx.take_work()
+    for lt in lats:
+        lt.close()
+    return {"workload": f"{n_var} variants x {len(grid['x'])} grid points, {turns} turns",
+            "value": passes / dt, "unit": UNIT, "s": dt,
+            "survivors": int((surv == turns).sum()), "particles": int(surv.size),
+            "median_survival_turn": float(np.median(surv))}
+
+
+def run_served(bmx, ctx, n_requests, seed=4, load=0.8, budget=16 << 30):
+    """Served stream against one-job-at-a-time tracking: Poisson arrivals of requests on a pool
+    of 32 error variants, particles log-uniform [1e3, 1e6], turns log-uniform [10, 1e4],
+    100-turn chunks, HBM budget 16 GiB. The arrival rate is set to `load` times the served
+    capacity measured on the same requests submitted at once."""
+    L, T = bmx.lattices, bmx.tracking
+    S = bmx.serving
+    lats = [T.Lattice(ctx, bmx.elements.to_array(v)) for v in L.ring1_variants(seed, 32)]
+    reqs, gaps = S.make_stream(seed, n_requests)
+    # capacity: everything submitted at t = 0
+    cap = S.run_served(ctx, lats, reqs, np.zeros(len(reqs)), budget)
+    mean_gap = cap["makespan_s"] / len(reqs) / load
+    arrivals = np.cumsum(gaps) * mean_gap
+    arrivals -= arrivals[0]
+    served = S.run_served(ctx, lats, reqs, arrivals, budget)
+    plain = S.run_plain(ctx, lats, reqs, arrivals)
+    for lt in lats:
+        lt.close()
+    sv = served["passes"] / served["makespan_s"]
+    pl = plain["passes"] / plain["makespan_s"]
+    return {"workload": f"{n_requests} requests, Poisson arrivals at {load} of served capacity "
+                        f"(mean gap {mean_gap:.3f} s), 32 variants, N log-uniform[1e3,1e6], "
+                        f"T log-uniform[10,1e4], 100-turn chunks, budget {budget} B",
+            "served": {"value": sv, **S.percentiles(served["latency_s"]),
+                       "makespan_s": served["makespan_s"], "chunks": served["chunks"]},
+            "plain_one_job_per_launch": {"value": pl, **S.percentiles(plain["latency_s"]),
+                                         "makespan_s": plain["makespan_s"]},
+            "speedup": sv / pl, "unit": UNIT}
 
 
 if __name__ == "__main__":

@@ -5,6 +5,6 @@ track1_bunch / track1, bmad/multiparticle/beam_mod.f90 and bmad/code/track1.f90)
 records and bookkeeping (``elements``), the benchmark rings and beams (``lattices``), the
 ctypes binding of the native library libbmx.so (``native``) and the host API (``tracking``).
 """
-from . import elements, flops, lattices, native, tracking  # noqa: F401
+from . import elements, flops, lattices, native, serving, tracking  # noqa: F401
 
 __all__ = ["elements", "flops", "lattices", "native", "tracking"]

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libbmx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BMX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["bmx_kernels.hip", "bmx_compile.cpp", "bmx_run.cpp", "bmx_batch.cpp", "bmx_api.cpp"]
+SOURCES = ["bmx_kernels.hip", "bmx_compile.cpp", "bmx_run.cpp", "bmx_batch.cpp", "bmx_server.cpp", "bmx_api.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

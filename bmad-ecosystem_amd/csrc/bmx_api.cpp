@@ -75,6 +75,7 @@ int32_t bmx_ctx_destroy(bmx_ctx *ctx)
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     ctx->run.release();
+    ctx->timer.release();
     for (auto &ev : ctx->events)
         if (ev) hipEventDestroy(ev);
     hipFree(ctx->d_counter);
@@ -324,7 +325,34 @@ int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t
     cfg.fast = ctx->fast_maps;
     cfg.waves = ctx->waves;
     cfg.grid = ctx->grid > 0 ? ctx->grid : default_grid(ctx->device, ctx->fast_maps, cfg.waves);
+    cfg.timer = ctx->profile ? &ctx->timer : nullptr;
     HIP_TRY(enqueue_run(ctx->stream, ctx->run, &g, 1, true, n_turns, cfg, ctx->d_work), "k_track_tiles");
+    return BMX_OK;
+}
+
+int32_t bmx_ctx_profile(bmx_ctx *ctx, int32_t on)
+{
+    if (!ctx) return set_error(BMX_ERR_ARG, "bmx_ctx_profile: null ctx");
+    ctx->profile = on != 0;
+    return BMX_OK;
+}
+
+int32_t bmx_ctx_kernel_time(bmx_ctx *ctx, double *ms, int64_t *launches)
+{
+    if (!ctx || !ms || !launches) return set_error(BMX_ERR_ARG, "bmx_ctx_kernel_time: null argument");
+    hipSetDevice(ctx->device);
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    double total = 0;
+    for (auto &p : ctx->timer.pending) {
+        float t = 0;
+        HIP_TRY(hipEventElapsedTime(&t, p.first, p.second), "hipEventElapsedTime");
+        total += t;
+        ctx->timer.pool.push_back(p.first);
+        ctx->timer.pool.push_back(p.second);
+    }
+    *launches = (int64_t)ctx->timer.pending.size();
+    ctx->timer.pending.clear();
+    *ms = total;
     return BMX_OK;
 }
 

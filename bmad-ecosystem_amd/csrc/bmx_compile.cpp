@@ -295,10 +295,7 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
                 break;
             }
             h.kind = K_QUAD;
-            const double beta0 = p0c / e.e_tot;
-            const double me = mc2 / e.e_tot;
-            q = {e.l, 1.0 * b1, beta0, me * me, beta0 * beta0 * (2 * (beta0 * beta0) - (me * me) / 2),
-                 3e-7 * e.e_tot, mc2, b1 * e.l, b1 / e.l};
+            q = {e.l, 1.0 * b1, b1 * e.l, b1 / e.l};
             // Series terms for the fast form: W = b1 L/(1+pz), sized for 1+pz >= 0.5 (further
             // off momentum the kernel halves the argument); take the fewest terms whose first
             // omitted term t^M/(4M)! is below 1e-18, and record the t range they cover.
@@ -312,8 +309,6 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
                 if (std::pow(tmax, m) / fact[4 * m] <= 1e-18) { nterms = m; break; }
             }
             h.n = nterms;
-            for (int m = 0; m < 8; m++)
-                for (int j = 0; j < 4; j++) q.push_back(1.0 / fact[4 * m + j]);
             q.push_back(std::pow(1e-18 * fact[4 * nterms], 1.0 / nterms));  // t_lim
             break;
         }

@@ -59,12 +59,21 @@ struct RunBuffers {
     hipError_t reserve(size_t n_groups, size_t n_particles, size_t n_tiles);
 };
 
+// Event pairs around tracking-kernel launches (profiling of bmx_track_turns).
+struct KernelTimer {
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    hipError_t take(hipEvent_t *ev);
+    void release();
+};
+
 // Parameters of one run.
 struct RunConfig {
     int32_t chunk_turns = 10;  // turns per chunk (alive lists are rebuilt between chunks)
     int32_t grid = 0;          // workgroups of the tracking kernel (0: occupancy limit x CUs)
     int32_t waves = 3;         // waves per SIMD the tracking kernel is compiled for (2, 3, 4)
     bool fast = true;
+    KernelTimer *timer = nullptr;  // if set: an event pair around every tracking launch
 };
 
 // Uploads the groups (asynchronously, from host memory that must stay valid until the stream
@@ -92,6 +101,8 @@ struct bmx_ctx {
     int32_t waves = 3;      // waves per SIMD of the tracking kernel variant
     bmx::RunBuffers run;    // buffers of bmx_track_turns
     bmx::Group group{};     // host copy of the single group uploaded by bmx_track_turns
+    bool profile = false;   // time every tracking launch of bmx_track_turns
+    bmx::KernelTimer timer;
 };
 
 struct bmx_lattice {

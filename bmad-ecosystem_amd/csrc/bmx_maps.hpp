@@ -86,6 +86,27 @@ __device__ __forceinline__ bool orbit_too_large(Particle &p)
     return true;
 }
 
+// The tests of orbit_too_large without the state assignment: true if it would flag the
+// particle. |x| > limit is x > limit or -x > limit; px^2 + py^2 > rel_p^2 has the sign of
+// f_unstable.
+__device__ __forceinline__ bool orbit_bad(const Particle &p)
+{
+    const double f_unstable = p.px * p.px + p.py * p.py - p.rel_p * p.rel_p;
+    return (fabs(p.x) > kMaxAperture) | (fabs(p.y) > kMaxAperture) | (p.rel_p < 0) | (f_unstable > 0);
+}
+
+// True if the aperture test of check_aperture_limit.f90:256-324 loses the particle (no state
+// change): elliptical r >= 1 (alive is strictly r < 1), rectangular outside a set limit.
+__device__ __forceinline__ bool aperture_fails(const Particle &p, int32_t flags, CP ap)
+{
+    if (flags & F_APE_ELLIP) {
+        const double xr = (p.x - ap[0]) * ap[1], yr = (p.y - ap[2]) * ap[3];
+        return !(xr * xr + yr * yr < 1);
+    }
+    const double x1 = ap[0], x2 = ap[1], y1 = ap[2], y2 = ap[3];
+    return (x1 != 0 && p.x < x1) | (x2 != 0 && p.x > x2) | (y1 != 0 && p.y < y1) | (y2 != 0 && p.y > y2);
+}
+
 // Aperture test of check_aperture_limit.f90:256-324 (elliptical with strict r < 1, or
 // rectangular). ap[] layout in bmx_table.hpp.
 __device__ __forceinline__ void aperture(Particle &p, int32_t flags, CP ap)
@@ -189,6 +210,17 @@ __device__ __forceinline__ Plane quad_plane(double k, double L, double rel_p, do
     return q;
 }
 
+// 1/(4m + j)! for m < 8, j < 4: coefficients of the quadrupole's plane-function series.
+__constant__ const double kQuadSeries[32] = {
+    1.0, 1.0, 0.5, 0.16666666666666666,
+    0.041666666666666664, 0.008333333333333333, 0.001388888888888889, 0.0001984126984126984,
+    2.48015873015873e-05, 2.7557319223985893e-06, 2.755731922398589e-07, 2.505210838544172e-08,
+    2.08767569878681e-09, 1.6059043836821613e-10, 1.1470745597729725e-11, 7.647163731819816e-13,
+    4.779477332387385e-14, 2.8114572543455206e-15, 1.5619206968586225e-16, 8.22063524662433e-18,
+    4.110317623312165e-19, 1.9572941063391263e-20, 8.896791392450574e-22, 3.868170170630684e-23,
+    1.6117375710961184e-24, 6.446950284384474e-26, 2.4795962632247976e-27, 9.183689863795546e-29,
+    3.279889237069838e-30, 1.1309962886447716e-31, 3.7699876288159054e-33, 1.2161250415535181e-34};
+
 // bmad/low_level/track_a_quadrupole.f90:17-148 for n_step = 1 (no multipoles other than k1),
 // no fringe, no misalignment; low_energy_z_correction.f90:22-58.
 // Fast form: with W = k1 L^2/(1+pz) the plane functions are C(-+W), L S(-+W) where
@@ -196,27 +228,26 @@ __device__ __forceinline__ Plane quad_plane(double k, double L, double rel_p, do
 // planes come from four polynomials in t = W^2 (even / odd parts), n_terms each, valid for
 // t <= t_lim (first omitted term below 1e-18); larger |W| is reduced by halving.
 template <bool F>
-__device__ __forceinline__ void quadrupole(Particle &p, CP q,
-                                           int32_t n_terms)
+__device__ __forceinline__ void quadrupole(Particle &p, CP q, int32_t n_terms, const Species &sp)
 {
     const double L = q[0], b1 = q[1];
     double cx_x, sx_x, cx_y, sx_y;  // planes: x uses k = -k1, y uses k = +k1
     double k1;
     if constexpr (F) {
-        k1 = q[8] * p.inv_rp;                 // b1/L / (1 + pz)
-        double W = q[7] * p.inv_rp;           // b1 L / (1 + pz)
+        k1 = q[3] * p.inv_rp;                 // b1/L / (1 + pz)
+        double W = q[2] * p.inv_rp;           // b1 L / (1 + pz)
         double t = W * W;
         // Argument halving for large |W| (far off-momentum particles): C(4w) = 2 C(w)^2 - 1,
         // S(4w) = S(w) C(w), the double-angle identities of cos/cosh and sin/sinh.
         int halvings = 0;
-        const double t_lim = q[41];
+        const double t_lim = q[4];
         while (t > t_lim && halvings < 64) {
             W *= 0.25;
             t *= 0.0625;
             halvings++;
         }
         // A = sum t^m/(4m)!, B = sum t^m/(4m+2)!, A1 = sum t^m/(4m+1)!, B1 = sum t^m/(4m+3)!
-        CP c = q + 9;  // per term: 1/(4m)!, 1/(4m+1)!, 1/(4m+2)!, 1/(4m+3)!
+        CP c = (CP)kQuadSeries;  // per term: 1/(4m)!, 1/(4m+1)!, 1/(4m+2)!, 1/(4m+3)!
         double A = c[4 * (n_terms - 1)], A1 = c[4 * (n_terms - 1) + 1];
         double B = c[4 * (n_terms - 1) + 2], B1 = c[4 * (n_terms - 1) + 3];
         for (int m = n_terms - 2; m >= 0; m--) {
@@ -271,9 +302,9 @@ __device__ __forceinline__ void quadrupole(Particle &p, CP q,
     if (F) {
         p.z = p.z + L * p.lez;
     } else {
-        const double beta0 = q[2], me2 = q[3], f = q[4], thr = q[5];
+        const double beta0 = sp.beta0, me2 = sp.me2, f = sp.lez_f, thr = sp.lez_thr;
         const double pz = p.pz;
-        if (q[6] * sqr(beta0 * pz) < thr)
+        if (sp.mc2 * sqr(beta0 * pz) < thr)
             p.z = p.z + L * pz * (1 - 3 * (pz * beta0 * beta0) / 2 + pz * pz * f) * me2;
         else
             p.z = p.z + L * (p.beta - beta0) / beta0;
@@ -711,14 +742,20 @@ __device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, con
     }
     switch (kind) {
     case K_DRIFT: drift<F>(p, q[0]); break;
-    case K_QUAD: quadrupole<F>(p, q, n); break;
+    case K_QUAD: quadrupole<F>(p, q, n, sp); break;
     case K_MULT: thick_multipole<F>(p, q, n, (flags >> F_ORDER_SHIFT) & 0xF); break;
     case K_BEND: sbend<F>(p, flags, q, sp); break;
     case K_RF: rfcavity<F>(p, q, n, sp); break;
     default: break;
     }
-    orbit_too_large(p);
-    if (p.state == ST_ALIVE && (flags & F_APE_EXIT)) aperture(p, flags, ap);
+    // orbit_too_large (overwrites any state), else the exit aperture for live particles; both
+    // tests first without branches, the state codes only on the (rare) losing lanes
+    const bool bad = orbit_bad(p);
+    const bool ape = (flags & F_APE_EXIT) ? aperture_fails(p, flags, ap) : false;
+    if (bad | (ape & (p.state == ST_ALIVE))) {
+        if (bad) orbit_too_large(p);
+        else aperture(p, flags, ap);
+    }
     return false;
 }
 

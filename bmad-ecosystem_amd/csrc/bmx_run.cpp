@@ -75,6 +75,30 @@ hipError_t RunBuffers::reserve(size_t n_groups, size_t n_particles, size_t n_til
     return e;
 }
 
+hipError_t KernelTimer::take(hipEvent_t *ev)
+{
+    if (pool.empty()) {
+        hipEvent_t e = nullptr;
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+        pool.push_back(e);
+    }
+    *ev = pool.back();
+    pool.pop_back();
+    return hipSuccess;
+}
+
+void KernelTimer::release()
+{
+    for (auto &p : pending) {
+        pool.push_back(p.first);
+        pool.push_back(p.second);
+    }
+    pending.clear();
+    for (hipEvent_t e : pool) hipEventDestroy(e);
+    pool.clear();
+}
+
 int default_grid(int device, bool fast, int waves)
 {
     int blocks = 0, cus = 0;
@@ -104,9 +128,20 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
     int c = 0;
     for (int32_t off = 0; off < max_turns && e == hipSuccess; off += K, c++) {
         const int cur = c & 1, nxt = cur ^ 1;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        if (cfg.timer) {
+            e = cfg.timer->take(&ev0);
+            if (e == hipSuccess) e = cfg.timer->take(&ev1);
+            if (e == hipSuccess) e = hipEventRecord(ev0, s);
+            if (e != hipSuccess) break;
+        }
         e = launch_track_tiles(s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur], n_tiles[cur],
                                c == 0 ? nullptr : rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt],
                                tile_ctr, off, K, d_work, cfg.fast);
+        if (cfg.timer && e == hipSuccess) {
+            e = hipEventRecord(ev1, s);
+            cfg.timer->pending.emplace_back(ev0, ev1);
+        }
         if (e == hipSuccess && off + K < max_turns)
             e = launch_build_tiles(s, rb.d_groups, n_groups, rb.d_cnt[nxt], off + K,
                                    rb.d_tiles[nxt], n_tiles[nxt], rb.d_cnt[cur], tile_ctr);

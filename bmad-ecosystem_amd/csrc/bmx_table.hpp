@@ -41,10 +41,8 @@ struct EleHdr {
 
 // Parameter layouts (index into prm + off):
 //  K_DRIFT : [0] L
-//  K_QUAD  : [0] L, [1] b1 = k1 L, [2] beta0 = p0c/E_tot, [3] (mc2/E_tot)^2,
-//            [4] f = beta0^2 (2 beta0^2 - (mc2/E)^2/2), [5] 3e-7 E_tot, [6] mc2 (series test),
-//            [7] b1 L, [8] b1 / L, [9 + 4m + j] 1/(4m + j)! for m < 8, j < 4, [41] t_lim
-//            (fast form; hdr.n = number of series terms used)
+//  K_QUAD  : [0] L, [1] b1 = k1 L, [2] b1 L, [3] b1 / L, [4] t_lim (fast form's series
+//            domain; hdr.n = number of series terms used)
 //  K_MULT  : [0] step_len, [1..4] a_n * r_step/2 (n = 0..3), [5..8] b_n * r_step/2
 //  K_BEND  : [0] L, [1] g, [2] g_tot, [3] angle = g L, [4] cos_a, [5] sin_a, [6] sinc_a,
 //            [7] cosc_a, [8] beta_ref, [9] sincc_a,
@@ -56,7 +54,7 @@ struct EleHdr {
 // preceded, when F_APE_ENT|F_APE_EXIT, by 4 aperture values (hdr.off points at them):
 //  elliptical : x_center, 1/x_width2, y_center, 1/y_width2
 //  rectangular: x1_lim (= -x1_limit), x2_lim, y1_lim, y2_lim (0 = no limit)
-constexpr int kParams[K_NKIND] = {1, 42, 9, 24, 7, 0};
+constexpr int kParams[K_NKIND] = {1, 5, 9, 24, 7, 0};
 
 struct Species {
     double mc2;      // [eV]

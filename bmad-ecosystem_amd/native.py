@@ -26,17 +26,29 @@ class BeamView(ctypes.Structure):
                                                "state", "loss_ele", "loss_turn")]
 
 
-class ServerConfig(ctypes.Structure):
-    """bmx_server_config"""
-    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("chunk_turns", ctypes.c_int32),
-                ("max_active", ctypes.c_int32), ("tile", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
-
-
 _P = ctypes.c_void_p
 _PP = ctypes.POINTER(ctypes.c_void_p)
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
+
+
+class GroupSpec(ctypes.Structure):
+    """ctypes mirror of bmx_group (include/bmx.h)."""
+    _fields_ = [("lat", ctypes.c_void_p), ("beam", ctypes.c_void_p), ("first", ctypes.c_int64),
+                ("count", ctypes.c_int64)]
+
+
+class ServerConfig(ctypes.Structure):
+    """ctypes mirror of bmx_server_config."""
+    _fields_ = [("hbm_budget", ctypes.c_int64), ("chunk_turns", ctypes.c_int32),
+                ("compact_turns", ctypes.c_int32)]
+
+
+class RequestStats(ctypes.Structure):
+    """ctypes mirror of bmx_request_stats."""
+    _fields_ = [("submit_ns", ctypes.c_int64), ("admit_ns", ctypes.c_int64),
+                ("done_ns", ctypes.c_int64), ("status", ctypes.c_int32), ("chunks", ctypes.c_int32)]
+
 
 # name: (restype, argtypes)
 _SIGS = {
@@ -50,6 +62,8 @@ _SIGS = {
     "bmx_ctx_record": (_I32, [_P, _I32]),
     "bmx_ctx_set_maps": (_I32, [_P, _I32]),
     "bmx_ctx_set_run": (_I32, [_P, _I32, _I32, _I32]),
+    "bmx_ctx_profile": (_I32, [_P, _I32]),
+    "bmx_ctx_kernel_time": (_I32, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]),
     "bmx_ctx_elapsed_ms": (_I32, [_P, _I32, _I32, ctypes.POINTER(ctypes.c_float)]),
     "bmx_lattice_create": (_I32, [_P, _P, _I32, ctypes.c_double, _I32, _PP]),
     "bmx_lattice_destroy": (_I32, [_P]),
@@ -65,6 +79,19 @@ _SIGS = {
     "bmx_track_element": (_I32, [_P, _P, _P, _I32, _I32]),
     "bmx_track_turns_plain": (_I32, [_P, _P, _P, _I32, _I32]),
     "bmx_track_orbit": (_I32, [_P, _P, _P, _I32, _I32, _I32, _P]),
+    "bmx_server_create": (_I32, [_P, ctypes.POINTER(ServerConfig), _PP]),
+    "bmx_server_destroy": (_I32, [_P]),
+    "bmx_server_submit": (_I32, [_P, _P, ctypes.POINTER(BeamView), ctypes.POINTER(BeamView), _I64,
+                                 _I32, _I32, ctypes.POINTER(_I64)]),
+    "bmx_server_poll": (_I32, [_P, _I64, ctypes.POINTER(_I32)]),
+    "bmx_server_wait": (_I32, [_P, _I64, ctypes.POINTER(RequestStats)]),
+    "bmx_server_totals": (_I32, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "bmx_batch_create": (_I32, [_P, ctypes.POINTER(GroupSpec), _I32, _PP]),
+    "bmx_batch_destroy": (_I32, [_P]),
+    "bmx_batch_track": (_I32, [_P, _I32, _I32]),
+    "bmx_batch_set_run": (_I32, [_P, _I32, _I32, _I32]),
+    "bmx_batch_tune": (_I32, [_P, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32),
+                              ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_float)]),
 }
 
 

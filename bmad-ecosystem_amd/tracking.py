@@ -53,6 +53,16 @@ class Context:
         occupancy limit) and waves per SIMD of the kernel variant (0 = default 3; 2 or 4)."""
         native.call("bmx_ctx_set_run", self.h, int(chunk_turns), int(grid), int(waves))
 
+    def profile(self, on: bool = True) -> None:
+        """Bracket every tracking-kernel launch of track_beam with HIP events."""
+        native.call("bmx_ctx_profile", self.h, 1 if on else 0)
+
+    def kernel_time(self):
+        """(total ms, launches) of the tracking kernel since the last call (synchronises)."""
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        native.call("bmx_ctx_kernel_time", self.h, ctypes.byref(ms), ctypes.byref(n))
+        return ms.value, n.value
+
     def record(self, slot: int) -> None:
         """Record HIP event `slot` on this context's stream."""
         native.call("bmx_ctx_record", self.h, int(slot))
@@ -195,6 +205,126 @@ def track_beam_host(eles, beam: dict, n_turns: int, device: int = 0, turn0: int 
     lat.close()
     ctx.close()
     return out
+
+
+class Batch:
+    """Many (lattice, beam range) groups tracked together in one captured launch sequence
+    (bmx_batch_*). groups: list of (Lattice, Beam, first, count)."""
+
+    def __init__(self, ctx: Context, groups):
+        self.ctx = ctx
+        self._keep = [(g[0], g[1]) for g in groups]
+        specs = (native.GroupSpec * max(len(groups), 1))()
+        for i, (lat, beam, first, count) in enumerate(groups):
+            specs[i] = native.GroupSpec(lat.h.value, beam.h.value, int(first), int(count))
+        h = ctypes.c_void_p()
+        native.call("bmx_batch_create", ctx.h, specs, len(groups), ctypes.byref(h))
+        self.h = h
+
+    def track(self, n_turns: int, turn0: int = 0) -> None:
+        native.call("bmx_batch_track", self.h, int(n_turns), int(turn0))
+
+    def set_run(self, chunk_turns: int = 10, grid: int = 0, waves: int = 0) -> None:
+        native.call("bmx_batch_set_run", self.h, int(chunk_turns), int(grid), int(waves))
+
+    def tune(self, sample_turns: int = 20) -> dict:
+        """Autotune the run shape on a scratch copy; returns the chosen shape."""
+        c, g, w = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        ms = ctypes.c_float()
+        native.call("bmx_batch_tune", self.h, int(sample_turns), ctypes.byref(c), ctypes.byref(g),
+                    ctypes.byref(w), ctypes.byref(ms))
+        return {"chunk_turns": c.value, "grid": g.value, "waves": w.value, "ms": ms.value}
+
+    def close(self):
+        if self.h:
+            native.call("bmx_batch_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching tracking server (bmx_server_*): requests are admitted by device pool
+    space, stay resident between scheduling chunks and advance together."""
+
+    def __init__(self, ctx: Context, hbm_budget: int, chunk_turns: int = 100,
+                 compact_turns: int = 10):
+        cfg = native.ServerConfig(int(hbm_budget), int(chunk_turns), int(compact_turns))
+        h = ctypes.c_void_p()
+        native.call("bmx_server_create", ctx.h, ctypes.byref(cfg), ctypes.byref(h))
+        self.h = h
+        self.ctx = ctx
+        self._held = {}
+
+    def submit(self, lat: Lattice, beam: dict, n_turns: int, turn0: int = 0) -> int:
+        """Queue a request; its input and output arrays are kept alive until wait()."""
+        n = int(beam["x"].shape[0])
+        inp = {k: np.ascontiguousarray(v) for k, v in beam.items()}
+        out = {k: np.empty(n, dtype=np.float64) for k in FLOAT_KEYS}
+        out.update({k: np.empty(n, dtype=np.int32) for k in INT_KEYS})
+        vin, vout = _host_view(inp), _host_view(out)
+        rid = ctypes.c_int64()
+        native.call("bmx_server_submit", self.h, lat.h, ctypes.byref(vin), ctypes.byref(vout), n,
+                    int(n_turns), int(turn0), ctypes.byref(rid))
+        self._held[rid.value] = (inp, out, vin, vout, lat)
+        return rid.value
+
+    def poll(self, rid: int) -> bool:
+        d = ctypes.c_int32()
+        native.call("bmx_server_poll", self.h, int(rid), ctypes.byref(d))
+        return bool(d.value)
+
+    def wait(self, rid: int):
+        """Block until done; returns (result beam dict, stats dict with times in seconds)."""
+        st = native.RequestStats()
+        try:
+            native.call("bmx_server_wait", self.h, int(rid), ctypes.byref(st))
+        finally:
+            held = self._held.pop(rid, None)
+        stats = {"submit_s": st.submit_ns * 1e-9, "admit_s": st.admit_ns * 1e-9,
+                 "done_s": st.done_ns * 1e-9, "latency_s": (st.done_ns - st.submit_ns) * 1e-9,
+                 "chunks": st.chunks}
+        return held[1], stats
+
+    def totals(self) -> dict:
+        p, c, b = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        native.call("bmx_server_totals", self.h, ctypes.byref(p), ctypes.byref(c), ctypes.byref(b))
+        return {"passes": p.value, "chunks": c.value, "busy_s": b.value * 1e-9}
+
+    def close(self):
+        if self.h:
+            native.call("bmx_server_destroy", self.h)
+            self.h = None
+            self._held.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dynamic_aperture(ctx: Context, lattices, grid_beam: dict, n_turns: int):
+    """Survival turns of a dynamic-aperture grid on each lattice (one batch, one group per
+    lattice): the loss turn of each lost particle (= completed turns, the 0-based turn it was
+    lost in, lt_tracking_mod.f90:1395) or n_turns for survivors. Returns (survival_turn,
+    loss_ele, final beam dicts) with one row per lattice."""
+    n = int(grid_beam["x"].shape[0])
+    beams = [Beam.from_host(ctx, grid_beam) for _ in lattices]
+    batch = Batch(ctx, [(lat, b, 0, n) for lat, b in zip(lattices, beams)])
+    batch.track(n_turns, 0)
+    ctx.synchronize()
+    outs = [b.download() for b in beams]
+    batch.close()
+    for b in beams:
+        b.close()
+    surv = np.stack([np.where(o["state"] == E.STATE_ALIVE, n_turns, o["loss_turn"]) for o in outs])
+    loss_ele = np.stack([o["loss_ele"] for o in outs])
+    return surv.astype(np.int32), loss_ele, outs
 
 
 def track_orbit(ctx: Context, lat: Lattice, beam: Beam, ix_start: int = 1,

@@ -172,6 +172,11 @@ int32_t bmx_ctx_set_maps(bmx_ctx *ctx, int32_t form);
  * persistent tracking grid (0 = as many as fit on the device at once, the default) and the
  * waves per SIMD the tracking kernel variant is compiled for (0 = default 3; 2 or 4). */
 int32_t bmx_ctx_set_run(bmx_ctx *ctx, int32_t chunk_turns, int32_t grid, int32_t waves);
+/* Kernel timing of bmx_track_turns: while on, every launch of the tracking kernel is bracketed
+ * by HIP events on the context's stream; bmx_ctx_kernel_time synchronises and returns the sum
+ * of those launch durations and their number since the last call. */
+int32_t bmx_ctx_profile(bmx_ctx *ctx, int32_t on);
+int32_t bmx_ctx_kernel_time(bmx_ctx *ctx, double *ms, int64_t *launches);
 /* Timing on the context's stream: record HIP event `slot` (0..BMX_N_EVENTS-1) now, and the
  * elapsed milliseconds between two recorded slots (synchronises on the later one). Replaces
  * the wall-clock run_timer of the reference drivers (sim_utils/misc/run_timer.f90:32). */
@@ -249,6 +254,41 @@ int32_t bmx_batch_set_run(bmx_batch *batch, int32_t chunk_turns, int32_t grid, i
  * modified), keeps the fastest for the batch and returns it. Blocking. */
 int32_t bmx_batch_tune(bmx_batch *batch, int32_t sample_turns, int32_t *chunk_turns,
                        int32_t *grid, int32_t *waves, float *best_ms);
+
+/* ---- server: continuous batching of tracking requests ---------------------------------------- *
+ * A server owns a worker thread, a HIP stream and a device pool of hbm_budget bytes for
+ * particle state. Submitted requests (a lattice, n particles in caller-owned host arrays, a
+ * turn count) wait in FIFO order until their particles fit in the pool; admitted requests stay
+ * resident and advance together in scheduling chunks of chunk_turns turns (one tile run over
+ * all active requests per chunk, lost particles compacted every compact_turns turns); requests
+ * arriving while a chunk runs join at the next chunk boundary. A finished request's particles
+ * are copied to its host_out arrays and its pool space is reused. Each particle's result is
+ * the same as tracking its request alone with bmx_track_turns. */
+typedef struct bmx_server_config {
+    int64_t hbm_budget;     /* bytes of device memory for particle state (76 B per particle) */
+    int32_t chunk_turns;    /* turns per scheduling chunk (0 = 100) */
+    int32_t compact_turns;  /* turns between alive-list compactions inside a chunk (0 = 10) */
+} bmx_server_config;
+typedef struct bmx_request_stats {
+    int64_t submit_ns, admit_ns, done_ns;  /* steady clock, ns since the server was created */
+    int32_t status;                        /* BMX_OK, or the error that ended the request */
+    int32_t chunks;                        /* scheduling chunks the request took part in */
+} bmx_request_stats;
+int32_t bmx_server_create(bmx_ctx *ctx, const bmx_server_config *cfg, bmx_server **out);
+/* Waits for every submitted request, then stops the worker and frees the pool. */
+int32_t bmx_server_destroy(bmx_server *server);
+/* Queue a request. host_in / host_out (may be the same arrays) must stay valid until the
+ * request is waited for; BMX_ERR_BUDGET if n particles can never fit in the pool. */
+int32_t bmx_server_submit(bmx_server *server, const bmx_lattice *lat, const bmx_beam_view *host_in,
+                          const bmx_beam_view *host_out, int64_t n, int32_t n_turns, int32_t turn0,
+                          int64_t *request_id);
+/* done = 1 once the request's results are in host_out. */
+int32_t bmx_server_poll(bmx_server *server, int64_t request_id, int32_t *done);
+/* Block until the request is done, return its timing and forget it. */
+int32_t bmx_server_wait(bmx_server *server, int64_t request_id, bmx_request_stats *stats);
+/* Totals since creation: particle-element passes tracked, scheduling chunks run, and the
+ * time the worker spent with at least one request active. */
+int32_t bmx_server_totals(bmx_server *server, int64_t *passes, int64_t *chunks, int64_t *busy_ns);
 
 /* Elements ix_start..ix_end (1-based, inclusive) once for every live particle, with the state
  * at every element exit written to orbit (caller-owned host memory of

@@ -13,7 +13,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbmad_oracle.so")
-_lib = None
+LIB_FMA_PATH = os.path.join(_HERE, "libbmad_oracle_fma.so")
+_libs = {}
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int32)
@@ -26,12 +27,14 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(fma: bool = False):
+    """The restatement; fma=True: the build with FMA contraction (same algorithm, other
+    rounding)."""
+    path = LIB_FMA_PATH if fma else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         L.bmo_track1.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_double,
                                  _D, _D, _I]
         L.bmo_track1.restype = ctypes.c_int32
@@ -54,8 +57,8 @@ def lib():
         L.bmo_modulo2.restype = ctypes.c_double
         L.bmo_c_multi.argtypes = [ctypes.c_int32, ctypes.c_int32]
         L.bmo_c_multi.restype = ctypes.c_double
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _ptr(a, t):
@@ -88,8 +91,9 @@ def track1(ele, vec, beta, state=1, mc2=0.51099895069e6, charge=-1, p0c=None):
 
 
 def track_bunch(ele_arr: np.ndarray, beam: dict, n_turns: int, turn0: int = 0, p0c=None,
-                mc2=0.51099895069e6, charge=-1, n_threads: int = 0) -> int:
-    """Track a SoA beam dict in place; returns the number of particle-element passes."""
+                mc2=0.51099895069e6, charge=-1, n_threads: int = 0, fma: bool = False) -> int:
+    """Track a SoA beam dict in place; returns the number of particle-element passes.
+    fma=True runs the FMA-contracted build of the same source."""
     n = beam["x"].shape[0]
     if p0c is None:
         p0c = float(ele_arr[0]["p0c"])
@@ -98,7 +102,7 @@ def track_bunch(ele_arr: np.ndarray, beam: dict, n_turns: int, turn0: int = 0, p
     for k in ("state", "loss_ele", "loss_turn"):
         assert beam[k].dtype == np.int32 and beam[k].flags.c_contiguous
     tracked = ctypes.c_int64(0)
-    err = lib().bmo_track_bunch(
+    err = lib(fma).bmo_track_bunch(
         ele_arr.ctypes.data, len(ele_arr), mc2, charge, p0c, n,
         *[_ptr(beam[k], _D) for k in ("x", "px", "y", "py", "z", "pz", "beta")],
         *[_ptr(beam[k], _I) for k in ("state", "loss_ele", "loss_turn")],

@@ -48,6 +48,26 @@ def rel_dev(got, ref, mask):
     return dev
 
 
+def fma_noise(oracle, arr, beam, n_turns, turn0=0):
+    """Per-particle deviation between the two FP64 builds of the CPU restatement (without and
+    with FMA contraction) after n_turns: how much rounding alone moves each orbit. Returns
+    (plain-build result, deviation; inf where the builds disagree on survival)."""
+    ref = copy_beam(beam)
+    oracle.track_bunch(arr, ref, n_turns, turn0=turn0)
+    fma = copy_beam(beam)
+    oracle.track_bunch(arr, fma, n_turns, turn0=turn0, fma=True)
+    both = (ref["state"] == 1) & (fma["state"] == 1)
+    return ref, rel_dev(fma, ref, both)
+
+
+def within_noise(got, ref, noise, factor=10.0, floor=1e-10):
+    """Indices of surviving particles whose GPU deviation exceeds max(floor, factor x noise)."""
+    alive = ref["state"] == 1
+    dev = rel_dev(got, ref, alive)
+    bad = alive & (dev > np.maximum(floor, factor * noise))
+    return np.nonzero(bad)[0]
+
+
 def loss_mismatch(got, ref):
     """Indices where state / loss element / loss turn differ."""
     bad = ((got["state"] != ref["state"]) | (got["loss_ele"] != ref["loss_ele"]) |

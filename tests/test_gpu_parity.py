@@ -172,29 +172,26 @@ def test_ring0_1000_turns(bmx, oracle, ctx):
 _CPU_1000 = {}
 
 
-def _regular_orbits(oracle, arr, beam, n_turns, copies=4, eps=1e-12, bound=1e-10, seed=12345):
-    """Particles whose trajectory amplifies small perturbations by at most bound/eps (100 by
-    default) over n_turns: tracked on the CPU as `copies` copies, each initial coordinate
-    scaled by (1 + eps N(0,1)), every copy survives and ends within `bound` relative of the
-    unperturbed run. Returns (reference result, regular mask)."""
+def _regular_orbits(oracle, arr, beam, n_turns, bound=1e-10):
+    """Particles on which two FP64 builds of the CPU restatement that differ only in rounding
+    (without and with FMA contraction) end within `bound` relative of each other after n_turns:
+    orbits that do not amplify rounding differences beyond that. Returns (reference result,
+    regular mask)."""
     ref = P.copy_beam(beam)
     oracle.track_bunch(arr, ref, n_turns=n_turns)
-    rng = np.random.default_rng(seed)
-    rep = {k: np.repeat(v, copies) for k, v in beam.items()}
-    for k in P.COORDS:
-        rep[k] = rep[k] * (1 + eps * rng.standard_normal(rep[k].shape))
-    oracle.track_bunch(arr, rep, n_turns=n_turns)
-    ref_rep = {k: np.repeat(v, copies) for k, v in ref.items()}
-    ok = P.rel_dev(rep, ref_rep, (rep["state"] == 1) & (ref_rep["state"] == 1)) <= bound
-    return ref, ok.reshape(-1, copies).all(axis=1)
+    fma = P.copy_beam(beam)
+    oracle.track_bunch(arr, fma, n_turns=n_turns, fma=True)
+    both = (ref["state"] == 1) & (fma["state"] == 1)
+    return ref, P.rel_dev(fma, ref, both) <= bound
 
 
 def test_ring1_regular_orbits_1000_turns(bmx, oracle, ctx):
     """Sextupoles and apertures (the 901-element ring with its rf cavity switched off), a
-    1-sigma beam, 1000 turns without resynchronisation: every regular orbit (perturbations
-    grow by at most 100x in 1000 turns, see _regular_orbits) within 1e-9 relative. The
-    amplitude-dependent tunes of the other orbits turn the GPU's and the CPU's different
-    last-bit rounding into phase differences that grow with the turn count."""
+    1-sigma beam, 1000 turns without resynchronisation. The amplitude-dependent tunes turn
+    last-bit rounding differences into phase differences that grow with the turn count: the
+    CPU restatement built with FMA contraction already differs from the plain build by 3e-10
+    (median) .. 3e-8 (max) after 1000 turns. Regular orbits -- those on which the two CPU builds
+    agree to 1e-10 -- must agree with the GPU to 1e-9."""
     if "norf" not in _CPU_1000:
         L = bmx.lattices
         arr = bmx.elements.to_array(L.ring1(rf=False))
@@ -203,8 +200,8 @@ def test_ring1_regular_orbits_1000_turns(bmx, oracle, ctx):
         _CPU_1000["norf"] = (arr, beam, ref, regular)
     arr, beam, ref, regular = _CPU_1000["norf"]
     out = _host(bmx, ctx, arr, beam, 1000)
-    assert regular.sum() >= 40, regular.sum()
-    assert np.all(out["state"][regular] == 1)
+    assert regular.sum() >= 100, regular.sum()
+    assert np.all(out["state"] == 1)
     dev = P.rel_dev(out, ref, regular)[regular]
     assert dev.max() <= 1e-9, np.percentile(dev, [50, 90, 100])
 
