@@ -237,19 +237,28 @@ def run_served(bmx, ctx, n_requests, seed=4, load=0.8, budget=16 << 30):
     arrivals = np.cumsum(gaps) * mean_gap
     arrivals -= arrivals[0]
     served = S.run_served(ctx, lats, reqs, arrivals, budget)
-    plain = S.run_plain(ctx, lats, reqs, arrivals)
+    one = S.run_plain(ctx, lats, reqs, arrivals)
+    plain = S.run_plain_elementwise(ctx, lats, reqs, arrivals)
     for lt in lats:
         lt.close()
     sv = served["passes"] / served["makespan_s"]
-    pl = plain["passes"] / plain["makespan_s"]
+    on = one["passes"] / one["makespan_s"]
+    pl = served["passes"] / plain["makespan_s"]
     return {"workload": f"{n_requests} requests, Poisson arrivals at {load} of served capacity "
                         f"(mean gap {mean_gap:.3f} s), 32 variants, N log-uniform[1e3,1e6], "
                         f"T log-uniform[10,1e4], 100-turn chunks, budget {budget} B",
             "served": {"value": sv, **S.percentiles(served["latency_s"]),
                        "makespan_s": served["makespan_s"], "chunks": served["chunks"]},
-            "plain_one_job_per_launch": {"value": pl, **S.percentiles(plain["latency_s"]),
-                                         "makespan_s": plain["makespan_s"]},
-            "speedup": sv / pl, "unit": UNIT}
+            "plain": {"path": "bmx_track_turns_plain: one launch per element per turn, state in "
+                              "HBM, one request at a time",
+                      "value": pl, **S.percentiles(plain["latency_s"]),
+                      "makespan_s": plain["makespan_s"], "method": plain["method"]},
+            "one_job_at_a_time": {"path": "bmx_track_turns (persistent, compacted) per request, "
+                                          "no batching across requests",
+                                  "value": on, **S.percentiles(one["latency_s"]),
+                                  "makespan_s": one["makespan_s"]},
+            "speedup_vs_plain": sv / pl, "speedup_vs_one_job": sv / on,
+            "served_capacity": cap["passes"] / cap["makespan_s"], "unit": UNIT}
 
 
 if __name__ == "__main__":

@@ -13,6 +13,7 @@ using namespace bmx;
 
 struct bmx_batch {
     bmx_ctx *ctx = nullptr;
+    int device = 0;
     std::vector<Group> groups;        // host copy; n_turns / turn0 set per call
     std::vector<const bmx_lattice *> lats;
     size_t n_particles = 0;
@@ -58,6 +59,7 @@ int32_t bmx_batch_create(bmx_ctx *ctx, const bmx_group *groups, int32_t n_groups
     bmx_batch *B = new (std::nothrow) bmx_batch();
     if (!B) return set_error(BMX_ERR_NOMEM, "bmx_batch_create: host allocation");
     B->ctx = ctx;
+    B->device = ctx->device;
     size_t off = 0;
     for (int32_t i = 0; i < n_groups; i++) {
         const bmx_group &g = groups[i];
@@ -104,8 +106,9 @@ int32_t bmx_batch_create(bmx_ctx *ctx, const bmx_group *groups, int32_t n_groups
 int32_t bmx_batch_destroy(bmx_batch *B)
 {
     if (!B) return BMX_OK;
-    hipSetDevice(B->ctx->device);
-    hipStreamSynchronize(B->ctx->stream);
+    // Independent of the context's lifetime (like lattices and beams): device-wide sync.
+    hipSetDevice(B->device);
+    hipDeviceSynchronize();
     B->drop_graphs();
     B->rb.release();
     delete B;

@@ -51,6 +51,12 @@ struct Particle {
 
 __device__ __forceinline__ double sqr(double v) { return v * v; }
 
+// True on every lane if c holds on any active lane. Rare per-particle cases (a coordinate
+// outside a series domain, a loss) are gated by this wave-uniform test so the common path is
+// straight-line code; inside the gated block each lane still takes its own branch, so a
+// particle's result never depends on the other particles of its wave.
+__device__ __forceinline__ bool any_lane(bool c) { return __ballot(c) != 0; }
+
 // sim_utils/special_functions/sqrt_one.f90:18-41
 __device__ __forceinline__ double sqrt_one(double v) { return v / (sqrt(1 + v) + 1); }
 
@@ -150,23 +156,10 @@ __device__ __forceinline__ double inv_ps_minus_1(double p)
 // bmad/low_level/track_a_drift.f90:25-116 (direction = orientation = +1, beta > 0):
 //   x += L px_rel/ps, y += L py_rel/ps, z += L (ez + sqrt_one(-pxy2)/ps), lost_pz if pxy2 >= 1.
 // Fast form: 1/ps = 1 + q(pxy2) and sqrt_one(-pxy2)/ps = 1 - 1/ps = -q(pxy2).
-template <bool F>
-__device__ __forceinline__ void drift(Particle &p, double L)
+// Exact drift body (track_a_drift.f90:44-80); px_rel = px/(1+pz) computed by division as in
+// the reference.
+__device__ __forceinline__ void drift_exact(Particle &p, double L)
 {
-    if (L == 0) return;
-    if (F) {
-        const double px_rel = p.px * p.inv_rp;
-        const double py_rel = p.py * p.inv_rp;
-        const double pxy2 = px_rel * px_rel + py_rel * py_rel;
-        if (pxy2 < kDriftSeriesMax) {
-            const double q = inv_ps_minus_1(pxy2);
-            const double lx = L * px_rel, ly = L * py_rel;
-            p.x = p.x + fma(lx, q, lx);
-            p.y = p.y + fma(ly, q, ly);
-            p.z = p.z + L * (p.ez - q);
-            return;
-        }
-    }
     const double px_rel = p.px / p.rel_p;
     const double py_rel = p.py / p.rel_p;
     const double pxy2 = px_rel * px_rel + py_rel * py_rel;
@@ -178,6 +171,30 @@ __device__ __forceinline__ void drift(Particle &p, double L)
     p.x = p.x + L * px_rel / ps;
     p.y = p.y + L * py_rel / ps;
     p.z = p.z + L * (p.ez + (-pxy2 / (ps + 1)) / ps);
+}
+
+template <bool F>
+__device__ __forceinline__ void drift(Particle &p, double L)
+{
+    if (L == 0) return;
+    if (!F) {
+        drift_exact(p, L);
+        return;
+    }
+    const double px_rel = p.px * p.inv_rp;
+    const double py_rel = p.py * p.inv_rp;
+    const double pxy2 = px_rel * px_rel + py_rel * py_rel;
+    const bool series = pxy2 < kDriftSeriesMax;
+    if (any_lane(!series)) {
+        if (!series) drift_exact(p, L);
+    }
+    if (series) {
+        const double q = inv_ps_minus_1(pxy2);
+        const double lx = L * px_rel, ly = L * py_rel;
+        p.x = p.x + fma(lx, q, lx);
+        p.y = p.y + fma(ly, q, ly);
+        p.z = p.z + L * (p.ez - q);
+    }
 }
 
 // ---- quadrupole --------------------------------------------------------------------------
@@ -210,16 +227,8 @@ __device__ __forceinline__ Plane quad_plane(double k, double L, double rel_p, do
     return q;
 }
 
-// 1/(4m + j)! for m < 8, j < 4: coefficients of the quadrupole's plane-function series.
-__constant__ const double kQuadSeries[32] = {
-    1.0, 1.0, 0.5, 0.16666666666666666,
-    0.041666666666666664, 0.008333333333333333, 0.001388888888888889, 0.0001984126984126984,
-    2.48015873015873e-05, 2.7557319223985893e-06, 2.755731922398589e-07, 2.505210838544172e-08,
-    2.08767569878681e-09, 1.6059043836821613e-10, 1.1470745597729725e-11, 7.647163731819816e-13,
-    4.779477332387385e-14, 2.8114572543455206e-15, 1.5619206968586225e-16, 8.22063524662433e-18,
-    4.110317623312165e-19, 1.9572941063391263e-20, 8.896791392450574e-22, 3.868170170630684e-23,
-    1.6117375710961184e-24, 6.446950284384474e-26, 2.4795962632247976e-27, 9.183689863795546e-29,
-    3.279889237069838e-30, 1.1309962886447716e-31, 3.7699876288159054e-33, 1.2161250415535181e-34};
+// Domain of the quadrupole's 5-term plane-function series: t = (k1 L^2/(1+pz))^2 <= 1.
+constexpr double kQuadTLim = 1.0;
 
 // bmad/low_level/track_a_quadrupole.f90:17-148 for n_step = 1 (no multipoles other than k1),
 // no fringe, no misalignment; low_energy_z_correction.f90:22-58.
@@ -237,32 +246,41 @@ __device__ __forceinline__ void quadrupole(Particle &p, CP q, int32_t n_terms, c
         k1 = q[3] * p.inv_rp;                 // b1/L / (1 + pz)
         double W = q[2] * p.inv_rp;           // b1 L / (1 + pz)
         double t = W * W;
-        // Argument halving for large |W| (far off-momentum particles): C(4w) = 2 C(w)^2 - 1,
-        // S(4w) = S(w) C(w), the double-angle identities of cos/cosh and sin/sinh.
+        // Argument quartering for large |W| (far off-momentum particles or strong quads):
+        // C(16t) = 2 C(t)^2 - 1, S(16t) = S(t) C(t) in t = W^2, the double-angle identities of
+        // cos/cosh and sin/sinh.
         int halvings = 0;
-        const double t_lim = q[4];
-        while (t > t_lim && halvings < 64) {
-            W *= 0.25;
-            t *= 0.0625;
-            halvings++;
+        const bool big = t > kQuadTLim;
+        if (any_lane(big)) {
+            if (big) {
+                do {
+                    W *= 0.25;
+                    t *= 0.0625;
+                    halvings++;
+                } while (t > kQuadTLim && halvings < 64);
+            }
         }
-        // A = sum t^m/(4m)!, B = sum t^m/(4m+2)!, A1 = sum t^m/(4m+1)!, B1 = sum t^m/(4m+3)!
-        CP c = (CP)kQuadSeries;  // per term: 1/(4m)!, 1/(4m+1)!, 1/(4m+2)!, 1/(4m+3)!
-        double A = c[4 * (n_terms - 1)], A1 = c[4 * (n_terms - 1) + 1];
-        double B = c[4 * (n_terms - 1) + 2], B1 = c[4 * (n_terms - 1) + 3];
-        for (int m = n_terms - 2; m >= 0; m--) {
-            A = fma(A, t, c[4 * m]);
-            A1 = fma(A1, t, c[4 * m + 1]);
-            B = fma(B, t, c[4 * m + 2]);
-            B1 = fma(B1, t, c[4 * m + 3]);
-        }
+        // A = sum t^m/(4m)!, A1 = sum t^m/(4m+1)!, B = sum t^m/(4m+2)!, B1 = sum t^m/(4m+3)!,
+        // m = 0..4; the first omitted term t^5/20! is below 4.2e-19 for t <= 1.
+        double A = 4.779477332387385e-14, A1 = 2.8114572543455206e-15;   // 1/16!, 1/17!
+        double B = 1.5619206968586225e-16, B1 = 8.22063524662433e-18;    // 1/18!, 1/19!
+        A = fma(A, t, 2.08767569878681e-09);   A1 = fma(A1, t, 1.6059043836821613e-10);
+        B = fma(B, t, 1.1470745597729725e-11); B1 = fma(B1, t, 7.647163731819816e-13);
+        A = fma(A, t, 2.48015873015873e-05);   A1 = fma(A1, t, 2.7557319223985893e-06);
+        B = fma(B, t, 2.755731922398589e-07);  B1 = fma(B1, t, 2.505210838544172e-08);
+        A = fma(A, t, 0.041666666666666664);   A1 = fma(A1, t, 0.008333333333333333);
+        B = fma(B, t, 0.001388888888888889);   B1 = fma(B1, t, 0.0001984126984126984);
+        A = fma(A, t, 1.0);                    A1 = fma(A1, t, 1.0);
+        B = fma(B, t, 0.5);                    B1 = fma(B1, t, 0.16666666666666666);
         double cxx = fma(-W, B, A), cxy = fma(W, B, A);
         double sxx = fma(-W, B1, A1), sxy = fma(W, B1, A1);
-        for (int h = 0; h < halvings; h++) {
-            sxx = sxx * cxx;
-            sxy = sxy * cxy;
-            cxx = fma(2 * cxx, cxx, -1.0);
-            cxy = fma(2 * cxy, cxy, -1.0);
+        if (any_lane(halvings > 0)) {
+            for (int h = 0; h < halvings; h++) {
+                sxx = sxx * cxx;
+                sxy = sxy * cxy;
+                cxx = fma(2 * cxx, cxx, -1.0);
+                cxy = fma(2 * cxy, cxy, -1.0);
+            }
         }
         cx_x = cxx;
         cx_y = cxy;
@@ -424,23 +442,6 @@ __device__ __forceinline__ double asin_over(double s2)
     return fma(r, s2, 1.0);
 }
 
-// h / sin(h) for the angle h with sine sh and cosine ch (|h| < pi): the asin(s)/s series
-// after k half-angle steps s' = s / sqrt(2 (1 + c)), c' = sqrt((1 + c)/2), which bring |s| to
-// the series domain; h / sin h = 2^k asin(s_k)/s_k * s_k / sh.
-__device__ __forceinline__ double h_over_sin(double sh, double ch)
-{
-    if (fabs(sh) <= kArcSeriesMax && ch > 0) return asin_over(sh * sh);
-    if (sh == 0) return 1.0;
-    double s = sh, c = ch, scale = 1.0;
-    for (int k = 0; k < 8 && !(fabs(s) <= kArcSeriesMax && c > 0); k++) {
-        const double t = sqrt(2 * (1 + c));
-        s = s / t;
-        c = 0.5 * t;
-        scale *= 2;
-    }
-    return scale * asin_over(s * s) * (s / sh);
-}
-
 // sin and cos of |phi| <= 4 (an rf phase wrapped to [-pi, pi)): quadrant reduction by pi/2
 // (two-part constant) and Taylor polynomials on |r| <= pi/4 (truncation below 1e-20).
 __device__ __forceinline__ void sincos_reduced(double phi, double *sp, double *cp)
@@ -489,126 +490,199 @@ __device__ __forceinline__ double sqrt_1m(double u)
     return fma(r, u, 1.0);
 }
 
-// bmad/low_level/track_a_bend.f90:18-333: exact_multipoles = off, no body multipoles, b1 = 0,
-// n_step = 1, no misalignment. Exact sector-bend body (:163-209), semi-linear body near the
-// zero orbit (:136-159), drift when g = dg = 0 (:117-118).
+// h / sin(h) for the angle h with sine sh and cosine ch (|h| < pi) outside the direct series
+// domain: k half-angle steps s' = s / sqrt(2 (1 + c)), c' = sqrt((1 + c)/2) bring |s| below 1/8,
+// then h / sin h = 2^k asin(s_k)/s_k * s_k / sh.
+__device__ __forceinline__ double h_over_sin_far(double sh, double ch)
+{
+    if (sh == 0) return 1.0;
+    double s = sh, c = ch, scale = 1.0;
+    for (int k = 0; k < 8 && !(fabs(s) <= kArcSeriesMax && c > 0); k++) {
+        const double t = sqrt(2 * (1 + c));
+        s = s / t;
+        c = 0.5 * t;
+        scale *= 2;
+    }
+    return scale * asin_over(s * s) * (s / sh);
+}
+
+// Semi-linear body for particles within 1e-9 of the zero orbit (track_a_bend.f90:136-159).
+__device__ __forceinline__ void sbend_semilinear(Particle &p, CP b, const Species &sp)
+{
+    const double L = b[0], g = b[1];
+    const double cos_a = b[4], sin_a = b[5], sinc_a = b[6], cosc_a = b[7];
+    const double x = p.x, px = p.px, y = p.y, py = p.py, z = p.z;
+    const double ll = L;
+    const double gam2 = sp.mc2_sq / (sqr(p.rel_p * sp.p0c) + sp.mc2_sq);
+    const double m56 = ll * (gam2 - sqr(g * ll) * b[9]);
+    p.x = cos_a * x + ll * sinc_a * px + g * ll * ll * cosc_a * p.pz;
+    p.px = -g * sin_a * x + cos_a * px + g * ll * sinc_a * p.pz;
+    p.y = y + ll * py;
+    p.z = -g * ll * sinc_a * x - g * ll * ll * cosc_a * px + z + m56 * p.pz;
+}
+
+// Exact sector-bend body as written in the reference (track_a_bend.f90:163-209).
+__device__ __forceinline__ void sbend_exact_ref(Particle &p, CP b)
+{
+    const double L = b[0], g = b[1], g_tot = b[2], angle = b[3];
+    const double cos_a = b[4], sin_a = b[5], sinc_a = b[6], cosc_a = b[7], beta_ref = b[8];
+    const double x = p.x, px = p.px, y = p.y, py = p.py, z = p.z;
+    const double rel_p = p.rel_p;
+    const double pt2 = rel_p * rel_p - py * py;
+    const double pt = sqrt(pt2);
+    if (fabs(px) > pt) {
+        p.state = ST_LOST_PZ;
+        return;
+    }
+    const double g_p = g_tot / pt;
+    const double phi_1 = asin(px / pt);
+    double sin_plus, cos_plus;
+    sincos(angle + phi_1, &sin_plus, &cos_plus);
+    const double u = (1 + g * x) * L * sinc_a;
+    const double alpha = 2 * (1 + g * x) * sin_plus * L * sinc_a - g_p * u * u;
+    const double r = cos_plus * cos_plus + g_p * alpha;
+    if (r < 0 || (fabs(g_p) < 1e-5 && fabs(cos_plus) < 1e-5)) {
+        p.state = ST_LOST;
+        p.x = 2 * kMaxAperture;
+        return;
+    }
+    const double rad = sqrt(r);
+    double xi;
+    if (cos_plus > 0) xi = alpha / (rad + cos_plus);
+    else xi = (rad - cos_plus) / g_p;
+    p.x = x * cos_a - L * L * g * cosc_a + xi;
+    if (fabs(p.x) > kMaxAperture) {
+        p.state = ST_LOST;
+        return;
+    }
+    const double L_u = xi;
+    const double L_v = -(L * sinc_a + x * sin_a);
+    const double L_c = sqrt(L_v * L_v + L_u * L_u);
+    const double angle_p = 2 * (angle + phi_1 - atan2(L_u, -L_v));
+    const double L_p = L_c / sinc(angle_p / 2);
+    p.px = pt * sin(phi_1 + angle - angle_p);
+    p.y = y + py * L_p / pt;
+    p.z = z + p.beta * L / beta_ref - rel_p * L_p / pt;
+}
+
 // Fast form of the exact body: with s1 = px/pt = sin(phi_1) and the bend angle's cos/sin,
 // sin/cos(angle + phi_1) follow by angle addition; with psi = atan2(L_u, -L_v) known through
 // its cosine -L_v/L_c and sine L_u/L_c, the half exit angle h = angle_p/2 = angle + phi_1 - psi
 // has sin h and cos h by angle subtraction, L_p = L_c h / sin h from the asin(s)/s series, and
-// px = pt sin(angle + phi_1 - 2h) by double-angle identities. No transcendental calls.
-template <bool F>
-__device__ __forceinline__ void sbend(Particle &p, int32_t flags, CP b,
-                                      const Species &sp)
+// px = pt sin(angle + phi_1 - 2h) by double-angle identities. No transcendental calls; the
+// rare cases (a square root outside the series domain, a losing particle, a particle turned
+// by more than 90 degrees) are gated per wave and decided per lane.
+__device__ __forceinline__ void sbend_exact_fast(Particle &p, CP b)
 {
-    const double L = b[0], g = b[1], g_tot = b[2], angle = b[3];
+    const double L = b[0], g = b[1], g_tot = b[2];
+    const double cos_a = b[4], sin_a = b[5], sinc_a = b[6];
+    const double x = p.x, px = p.px, y = p.y, py = p.py, z = p.z;
+    const double rel_p = p.rel_p;
+    const double pt2 = rel_p * rel_p - py * py;
+    // pt2 <= 0 gives NaN coordinates here as sqrt does in the reference form
+    const double inv_pt = rsqrt(pt2);
+    const double pt = pt2 * inv_pt;
+    const double g_p = g_tot * inv_pt;
+    const double s1 = px * inv_pt;
+    const double s1s = s1 * s1;
+    double c1 = sqrt_1m(s1s);
+    const bool c1_far = !(s1s < kDriftSeriesMax);
+    if (any_lane(c1_far)) {
+        if (c1_far) c1 = sqrt(1 - s1s);
+    }
+    const double sin_plus = fma(sin_a, c1, cos_a * s1);
+    const double cos_plus = fma(cos_a, c1, -sin_a * s1);
+    const double u = (1 + g * x) * (L * sinc_a);
+    const double alpha = 2 * sin_plus * u - g_p * (u * u);
+    const double r = fma(cos_plus, cos_plus, g_p * alpha);
+    const double rad = sqrt(r);
+    double xi = alpha / (rad + cos_plus);
+    const bool turned = !(cos_plus > 0);
+    if (any_lane(turned)) {
+        if (turned) xi = (rad - cos_plus) / g_p;
+    }
+    const double x_new = x * cos_a - b[22] + xi;
+    const bool lost_pz = fabs(px) > pt;
+    const bool lost_geo = (r < 0) | ((fabs(g_p) < 1e-5) & (fabs(cos_plus) < 1e-5));
+    const bool lost_x = fabs(x_new) > kMaxAperture;
+    if (any_lane(lost_pz | lost_geo | lost_x)) {
+        if (lost_pz) {
+            p.state = ST_LOST_PZ;
+            return;
+        }
+        if (lost_geo) {
+            p.state = ST_LOST;
+            p.x = 2 * kMaxAperture;
+            return;
+        }
+        if (lost_x) {
+            p.state = ST_LOST;
+            p.x = x_new;
+            return;
+        }
+    }
+    const double L_u = xi;
+    const double mL_v = L * sinc_a + x * sin_a;  // -L_v
+    const double lc2 = mL_v * mL_v + L_u * L_u;
+    const double inv_lc = rsqrt(lc2);
+    const double L_c = lc2 * inv_lc;
+    const double cpsi = mL_v * inv_lc, spsi = L_u * inv_lc;
+    const double sh = fma(sin_plus, cpsi, -cos_plus * spsi);  // sin(angle_p/2)
+    const double ch = fma(cos_plus, cpsi, sin_plus * spsi);   // cos(angle_p/2)
+    // L_p = L_c / sinc(h) = L_c h / sin h
+    double hs = asin_over(sh * sh);
+    const bool h_far = !(fabs(sh) <= kArcSeriesMax && ch > 0);
+    if (any_lane(h_far)) {
+        if (h_far) hs = h_over_sin_far(sh, ch);
+    }
+    const double L_p = L_c * hs;
+    // sin(angle + phi_1 - 2h) with sin/cos 2h from sh, ch
+    const double s2h = 2 * sh * ch;
+    const double c2h = fma(-2 * sh, sh, 1.0);
+    p.x = x_new;
+    p.px = pt * fma(sin_plus, c2h, -cos_plus * s2h);
+    const double lp_pt = L_p * inv_pt;
+    p.y = fma(py, lp_pt, y);
+    p.z = z + (p.beta * b[23] - rel_p * lp_pt);
+}
+
+// orbit_too_large at a point inside an element: true (and the state set) if it flags the
+// particle; the test is branch-free, the state assignment gated.
+__device__ __forceinline__ bool orbit_check(Particle &p)
+{
+    const bool bad = orbit_bad(p);
+    if (any_lane(bad)) {
+        if (bad) orbit_too_large(p);
+    }
+    return bad;
+}
+
+// bmad/low_level/track_a_bend.f90:18-333: exact_multipoles = off, no body multipoles, b1 = 0,
+// n_step = 1, no misalignment. Hwang entrance kick, orbit_too_large (:54), body (exact
+// :163-209, semi-linear near the zero orbit :136-159, drift when g = dg = 0 :117-118),
+// orbit_too_large (:319), Hwang exit kick.
+template <bool F>
+__device__ __forceinline__ void sbend(Particle &p, int32_t flags, CP b, const Species &sp)
+{
+    const double L = b[0], g = b[1], g_tot = b[2];
     const bool simple = flags & F_EDGE_SIMPLE;
     if (flags & F_EDGE_ENT) hwang_edge(p, b + 10, g_tot, true, simple);
-    if (orbit_too_large(p)) return;
+    if (orbit_check(p)) return;
     if (flags & F_BEND_DRIFT) {
         drift<F>(p, L);
     } else {
-        const double x = p.x, px = p.px, y = p.y, py = p.py, z = p.z;
-        const double rel_p = p.rel_p;
-        const double cos_a = b[4], sin_a = b[5], sinc_a = b[6], cosc_a = b[7];
-        if (fabs(x * g) < 1e-9 && fabs(px) < 1e-9 && fabs(py) < 1e-9 && fabs(p.pz) < 1e-9) {
-            const double ll = L;
-            const double gam2 = sp.mc2_sq / (sqr(rel_p * sp.p0c) + sp.mc2_sq);
-            const double m56 = ll * (gam2 - sqr(g * ll) * b[9]);
-            p.x = cos_a * x + ll * sinc_a * px + g * ll * ll * cosc_a * p.pz;
-            p.px = -g * sin_a * x + cos_a * px + g * ll * sinc_a * p.pz;
-            p.y = y + ll * py;
-            p.z = -g * ll * sinc_a * x - g * ll * ll * cosc_a * px + z + m56 * p.pz;
-        } else {
-            const double beta_ref = b[8];
-            const double pt2 = rel_p * rel_p - py * py;
-            if (F) {
-                // pt2 <= 0 gives NaN coordinates here as sqrt does in the reference form
-                const double inv_pt = rsqrt(pt2);
-                const double pt = pt2 * inv_pt;
-                if (fabs(px) > pt) {
-                    p.state = ST_LOST_PZ;
-                    return;
-                }
-                const double g_p = g_tot * inv_pt;
-                const double s1 = px * inv_pt;
-                const double s1s = s1 * s1;
-                const double c1 = s1s < kDriftSeriesMax ? sqrt_1m(s1s) : sqrt(1 - s1s);
-                const double sin_plus = fma(sin_a, c1, cos_a * s1);
-                const double cos_plus = fma(cos_a, c1, -sin_a * s1);
-                const double u = (1 + g * x) * (L * sinc_a);
-                const double alpha = 2 * sin_plus * u - g_p * (u * u);
-                const double r = fma(cos_plus, cos_plus, g_p * alpha);
-                if (r < 0 || (fabs(g_p) < 1e-5 && fabs(cos_plus) < 1e-5)) {
-                    p.state = ST_LOST;
-                    p.x = 2 * kMaxAperture;
-                    return;
-                }
-                const double rad = sqrt(r);
-                double xi;
-                if (cos_plus > 0) xi = alpha / (rad + cos_plus);
-                else xi = (rad - cos_plus) / g_p;
-                p.x = x * cos_a - b[22] + xi;
-                if (fabs(p.x) > kMaxAperture) {
-                    p.state = ST_LOST;
-                    return;
-                }
-                const double L_u = xi;
-                const double mL_v = L * sinc_a + x * sin_a;  // -L_v
-                const double lc2 = mL_v * mL_v + L_u * L_u;
-                const double inv_lc = rsqrt(lc2);
-                const double L_c = lc2 * inv_lc;
-                const double cpsi = mL_v * inv_lc, spsi = L_u * inv_lc;
-                const double sh = fma(sin_plus, cpsi, -cos_plus * spsi);  // sin(angle_p/2)
-                const double ch = fma(cos_plus, cpsi, sin_plus * spsi);   // cos(angle_p/2)
-                // L_p = L_c / sinc(h) = L_c h / sin h
-                const double L_p = L_c * h_over_sin(sh, ch);
-                // sin(angle + phi_1 - 2h) with sin/cos 2h from sh, ch
-                const double s2h = 2 * sh * ch;
-                const double c2h = fma(-2 * sh, sh, 1.0);
-                p.px = pt * fma(sin_plus, c2h, -cos_plus * s2h);
-                const double lp_pt = L_p * inv_pt;
-                p.y = fma(py, lp_pt, y);
-                p.z = z + (p.beta * b[23] - rel_p * lp_pt);
-            } else {
-                const double pt = sqrt(pt2);
-                if (fabs(px) > pt) {
-                    p.state = ST_LOST_PZ;
-                    return;
-                }
-                const double g_p = g_tot / pt;
-                const double phi_1 = asin(px / pt);
-                double sin_plus, cos_plus;
-                sincos(angle + phi_1, &sin_plus, &cos_plus);
-                const double u = (1 + g * x) * L * sinc_a;
-                const double alpha = 2 * (1 + g * x) * sin_plus * L * sinc_a - g_p * u * u;
-                const double r = cos_plus * cos_plus + g_p * alpha;
-                if (r < 0 || (fabs(g_p) < 1e-5 && fabs(cos_plus) < 1e-5)) {
-                    p.state = ST_LOST;
-                    p.x = 2 * kMaxAperture;
-                    return;
-                }
-                const double rad = sqrt(r);
-                double xi;
-                if (cos_plus > 0) xi = alpha / (rad + cos_plus);
-                else xi = (rad - cos_plus) / g_p;
-                p.x = x * cos_a - L * L * g * cosc_a + xi;
-                if (fabs(p.x) > kMaxAperture) {
-                    p.state = ST_LOST;
-                    return;
-                }
-                const double L_u = xi;
-                const double L_v = -(L * sinc_a + x * sin_a);
-                const double L_c = sqrt(L_v * L_v + L_u * L_u);
-                const double angle_p = 2 * (angle + phi_1 - atan2(L_u, -L_v));
-                const double L_p = L_c / sinc(angle_p / 2);
-                p.px = pt * sin(phi_1 + angle - angle_p);
-                p.y = y + py * L_p / pt;
-                p.z = z + p.beta * L / beta_ref - rel_p * L_p / pt;
-            }
+        const bool semi = (fabs(p.x * g) < 1e-9) & (fabs(p.px) < 1e-9) & (fabs(p.py) < 1e-9) &
+                          (fabs(p.pz) < 1e-9);
+        if (any_lane(semi)) {
+            if (semi) sbend_semilinear(p, b, sp);
         }
+        if (!semi) {
+            if (F) sbend_exact_fast(p, b);
+            else sbend_exact_ref(p, b);
+        }
+        if (p.state != ST_ALIVE) return;
     }
-    if (orbit_too_large(p)) return;
+    if (orbit_check(p)) return;
     if (flags & F_EDGE_EXIT) hwang_edge(p, b + 16, g_tot, false, simple);
 }
 
@@ -737,8 +811,13 @@ __device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, con
     CP ap = prm + off;
     CP q = (flags & (F_APE_ENT | F_APE_EXIT)) ? ap + 4 : ap;
     if (flags & F_APE_ENT) {
-        aperture(p, flags, ap);
-        if (p.state != ST_ALIVE) return true;
+        const bool out = aperture_fails(p, flags, ap);
+        if (any_lane(out)) {
+            if (out) {
+                aperture(p, flags, ap);
+                return true;
+            }
+        }
     }
     switch (kind) {
     case K_DRIFT: drift<F>(p, q[0]); break;
@@ -752,9 +831,12 @@ __device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, con
     // tests first without branches, the state codes only on the (rare) losing lanes
     const bool bad = orbit_bad(p);
     const bool ape = (flags & F_APE_EXIT) ? aperture_fails(p, flags, ap) : false;
-    if (bad | (ape & (p.state == ST_ALIVE))) {
-        if (bad) orbit_too_large(p);
-        else aperture(p, flags, ap);
+    const bool hit = bad | (ape & (p.state == ST_ALIVE));
+    if (any_lane(hit)) {
+        if (hit) {
+            if (bad) orbit_too_large(p);
+            else aperture(p, flags, ap);
+        }
     }
     return false;
 }

@@ -172,38 +172,27 @@ def test_ring0_1000_turns(bmx, oracle, ctx):
 _CPU_1000 = {}
 
 
-def _regular_orbits(oracle, arr, beam, n_turns, bound=1e-10):
-    """Particles on which two FP64 builds of the CPU restatement that differ only in rounding
-    (without and with FMA contraction) end within `bound` relative of each other after n_turns:
-    orbits that do not amplify rounding differences beyond that. Returns (reference result,
-    regular mask)."""
-    ref = P.copy_beam(beam)
-    oracle.track_bunch(arr, ref, n_turns=n_turns)
-    fma = P.copy_beam(beam)
-    oracle.track_bunch(arr, fma, n_turns=n_turns, fma=True)
-    both = (ref["state"] == 1) & (fma["state"] == 1)
-    return ref, P.rel_dev(fma, ref, both) <= bound
-
-
-def test_ring1_regular_orbits_1000_turns(bmx, oracle, ctx):
+def test_ring1_no_rf_1000_turns(bmx, oracle, ctx):
     """Sextupoles and apertures (the 901-element ring with its rf cavity switched off), a
     1-sigma beam, 1000 turns without resynchronisation. The amplitude-dependent tunes turn
     last-bit rounding differences into phase differences that grow with the turn count: the
-    CPU restatement built with FMA contraction already differs from the plain build by 3e-10
-    (median) .. 3e-8 (max) after 1000 turns. Regular orbits -- those on which the two CPU builds
-    agree to 1e-10 -- must agree with the GPU to 1e-9."""
+    CPU restatement built with FMA contraction differs from the plain build by ~3e-10 (median)
+    to ~3e-8 (max) after 1000 turns, so that is the resolution at which any FP64
+    implementation can match it. The GPU must stay within 3x of that spread at the median,
+    90th and 99th percentile, and within 1e-9 at the median."""
     if "norf" not in _CPU_1000:
         L = bmx.lattices
         arr = bmx.elements.to_array(L.ring1(rf=False))
         beam = L.gaussian_beam(1024, seed=1, sigma=L.SIGMA0)
-        ref, regular = _regular_orbits(oracle, arr, beam, 1000)
-        _CPU_1000["norf"] = (arr, beam, ref, regular)
-    arr, beam, ref, regular = _CPU_1000["norf"]
+        ref, noise = P.fma_noise(oracle, arr, beam, 1000)
+        _CPU_1000["norf"] = (arr, beam, ref, noise)
+    arr, beam, ref, noise = _CPU_1000["norf"]
     out = _host(bmx, ctx, arr, beam, 1000)
-    assert regular.sum() >= 100, regular.sum()
-    assert np.all(out["state"] == 1)
-    dev = P.rel_dev(out, ref, regular)[regular]
-    assert dev.max() <= 1e-9, np.percentile(dev, [50, 90, 100])
+    assert np.all(out["state"] == 1) and np.all(ref["state"] == 1) and np.all(np.isfinite(noise))
+    dev = P.rel_dev(out, ref, ref["state"] == 1)
+    for q in (50, 90, 99):
+        assert np.percentile(dev, q) <= 3 * np.percentile(noise, q), (q, np.percentile(dev, q), np.percentile(noise, q))
+    assert np.median(dev) <= 1e-9
 
 
 def test_ring1_chaotic_1000_turns(bmx, oracle, ctx):
