@@ -50,7 +50,7 @@ def parse():
                          "(batched, da, served; empty = none)")
     ap.add_argument("--served-requests", type=int, default=24)
     ap.add_argument("--waves", type=int, default=0,
-                    help="waves per SIMD of the tracking kernel variant (0 = default 3; 2 or 4)")
+                    help="waves per SIMD of the tracking kernel variant (0 = default 3; 4 or 5)")
     return ap.parse_args()
 
 

@@ -358,7 +358,7 @@ int32_t bmx_ctx_kernel_time(bmx_ctx *ctx, double *ms, int64_t *launches)
 
 int32_t bmx_ctx_set_run(bmx_ctx *ctx, int32_t chunk_turns, int32_t grid, int32_t waves)
 {
-    if (!ctx || chunk_turns < 0 || grid < 0 || (waves != 0 && (waves < 2 || waves > 4)))
+    if (!ctx || chunk_turns < 0 || grid < 0 || (waves != 0 && (waves < 3 || waves > 5)))
         return set_error(BMX_ERR_ARG, "bmx_ctx_set_run: bad argument");
     ctx->chunk_turns = chunk_turns;
     ctx->grid = grid;

@@ -117,7 +117,7 @@ int32_t bmx_batch_destroy(bmx_batch *B)
 
 int32_t bmx_batch_set_run(bmx_batch *B, int32_t chunk_turns, int32_t grid, int32_t waves)
 {
-    if (!B || chunk_turns < 0 || grid < 0 || (waves != 0 && (waves < 2 || waves > 4)))
+    if (!B || chunk_turns < 0 || grid < 0 || (waves != 0 && (waves < 3 || waves > 5)))
         return set_error(BMX_ERR_ARG, "bmx_batch_set_run: bad argument");
     hipSetDevice(B->ctx->device);
     hipStreamSynchronize(B->ctx->stream);
@@ -215,7 +215,7 @@ int32_t bmx_batch_tune(bmx_batch *B, int32_t sample_turns, int32_t *chunk_out, i
     RunConfig best = B->cfg;
     float best_ms = -1;
     const int32_t chunks[] = {5, 10, 25, 50};
-    const int32_t waves[] = {2, 3, 4};
+    const int32_t waves[] = {3, 4, 5};
     for (int32_t w : waves) {
         const int full = default_grid(ctx->device, B->cfg.fast, w);
         for (int32_t K : chunks) {

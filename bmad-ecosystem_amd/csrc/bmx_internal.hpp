@@ -71,7 +71,7 @@ struct KernelTimer {
 struct RunConfig {
     int32_t chunk_turns = 10;  // turns per chunk (alive lists are rebuilt between chunks)
     int32_t grid = 0;          // workgroups of the tracking kernel (0: occupancy limit x CUs)
-    int32_t waves = 3;         // waves per SIMD the tracking kernel is compiled for (2, 3, 4)
+    int32_t waves = 3;         // waves per SIMD the tracking kernel is compiled for (3, 4, 5)
     bool fast = true;
     KernelTimer *timer = nullptr;  // if set: an event pair around every tracking launch
 };

@@ -71,16 +71,27 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
     } else {
         const CHdr hdr = (CHdr)L.hdr;
         const CP prm = (CP)L.prm;
-        for (int32_t turn = 0; turn < n_turns; ++turn) {
-            for (int32_t e = 0; e < L.n_ele; ++e) {
-                track_element<F>(p, hdr + 4 * e, prm, L.sp);
-                if (p.state != ST_ALIVE) {
-                    lost_e = e + 1;
-                    lost_t = turn0 + turn;
-                    break;
+        // The rf cavities sit outside the inner element loop: the loop over the other kinds
+        // runs until the next cavity, which is tracked on its own, so the cavity's code does not
+        // shape the register assignment of the common path.
+        for (int32_t turn = 0; turn < n_turns && p.state == ST_ALIVE; ++turn) {
+            int32_t e = 0;
+            for (;;) {
+                for (; e < L.n_ele; ++e) {
+                    const CHdr h = hdr + 4 * e;
+                    if (h[0] == K_RF) break;
+                    track_element<F, MAPS_NO_RF>(p, h, prm, L.sp);
+                    if (p.state != ST_ALIVE) break;
                 }
+                if (p.state != ST_ALIVE || e >= L.n_ele) break;
+                track_element<F, MAPS_RF_ONLY>(p, hdr + 4 * e, prm, L.sp);
+                if (p.state != ST_ALIVE) break;
+                ++e;
             }
-            if (p.state != ST_ALIVE) break;
+            if (p.state != ST_ALIVE) {
+                lost_e = e + 1;
+                lost_t = turn0 + turn;
+            }
         }
     }
     store_particle(p, b, i);
@@ -104,8 +115,8 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
 // Only the first w_eff waves take tiles, w_eff = ceil(n_tiles / ceil(n_tiles / waves)), so the
 // tiles divide evenly over the waves that run (no last partial round).
 // chunk_off: turns already done by earlier chunks of this call; chunk_turns: turns per chunk.
-// W: waves per SIMD the register allocation targets (3: up to 168 VGPRs, no spills; 4: up to
-// 128, spilling in the rf and bend paths; 2: up to 256).
+// W: waves per SIMD the register allocation targets at least (3: up to 168 VGPRs; 4: up to 128;
+// 5: up to 102, spilling a few values of the rf and bend paths to scratch).
 template <bool F, int W>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) k_track_tiles(
     const Group *__restrict__ groups, const WaveTile *__restrict__ tiles,
@@ -320,11 +331,11 @@ hipError_t launch_track_tiles(hipStream_t s, int grid, int waves, const Group *d
     launch_tiles<F, W>(s, grid, d_groups, d_tiles, d_n_tiles, d_list_in, d_list_out, d_cnt_out,  \
                        d_tile_ctr, chunk_off, chunk_turns, d_work)
     if (fast) {
-        if (waves == 2) BMX_TILES(true, 2);
+        if (waves == 5) BMX_TILES(true, 5);
         else if (waves == 4) BMX_TILES(true, 4);
         else BMX_TILES(true, 3);
     } else {
-        if (waves == 2) BMX_TILES(false, 2);
+        if (waves == 5) BMX_TILES(false, 5);
         else if (waves == 4) BMX_TILES(false, 4);
         else BMX_TILES(false, 3);
     }
@@ -344,8 +355,8 @@ hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_gr
 hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu)
 {
     const void *k;
-    if (fast) k = waves == 2 ? (const void *)k_track_tiles<true, 2> : waves == 4 ? (const void *)k_track_tiles<true, 4> : (const void *)k_track_tiles<true, 3>;
-    else k = waves == 2 ? (const void *)k_track_tiles<false, 2> : waves == 4 ? (const void *)k_track_tiles<false, 4> : (const void *)k_track_tiles<false, 3>;
+    if (fast) k = waves == 5 ? (const void *)k_track_tiles<true, 5> : waves == 4 ? (const void *)k_track_tiles<true, 4> : (const void *)k_track_tiles<true, 3>;
+    else k = waves == 5 ? (const void *)k_track_tiles<false, 5> : waves == 4 ? (const void *)k_track_tiles<false, 4> : (const void *)k_track_tiles<false, 3>;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, kBlock, 0);
 }
 

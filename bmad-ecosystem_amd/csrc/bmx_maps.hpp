@@ -51,6 +51,15 @@ struct Particle {
 
 __device__ __forceinline__ double sqr(double v) { return v * v; }
 
+// A constant table read through a pointer the compiler cannot follow: the loads of its entries
+// stay where they are used instead of being hoisted out of the loops into registers held for
+// the whole kernel. For coefficients of rarely taken paths (rf phase, far arguments).
+__device__ __forceinline__ CP launder(CP p)
+{
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 // True on every lane if c holds on any active lane. Rare per-particle cases (a coordinate
 // outside a series domain, a loss) are gated by this wave-uniform test so the common path is
 // straight-line code; inside the gated block each lane still takes its own branch, so a
@@ -261,17 +270,27 @@ __device__ __forceinline__ void quadrupole(Particle &p, CP q, int32_t n_terms, c
             }
         }
         // A = sum t^m/(4m)!, A1 = sum t^m/(4m+1)!, B = sum t^m/(4m+2)!, B1 = sum t^m/(4m+3)!,
-        // m = 0..4; the first omitted term t^5/20! is below 4.2e-19 for t <= 1.
-        double A = 4.779477332387385e-14, A1 = 2.8114572543455206e-15;   // 1/16!, 1/17!
-        double B = 1.5619206968586225e-16, B1 = 8.22063524662433e-18;    // 1/18!, 1/19!
-        A = fma(A, t, 2.08767569878681e-09);   A1 = fma(A1, t, 1.6059043836821613e-10);
-        B = fma(B, t, 1.1470745597729725e-11); B1 = fma(B1, t, 7.647163731819816e-13);
-        A = fma(A, t, 2.48015873015873e-05);   A1 = fma(A1, t, 2.7557319223985893e-06);
-        B = fma(B, t, 2.755731922398589e-07);  B1 = fma(B1, t, 2.505210838544172e-08);
-        A = fma(A, t, 0.041666666666666664);   A1 = fma(A1, t, 0.008333333333333333);
-        B = fma(B, t, 0.001388888888888889);   B1 = fma(B1, t, 0.0001984126984126984);
-        A = fma(A, t, 1.0);                    A1 = fma(A1, t, 1.0);
-        B = fma(B, t, 0.5);                    B1 = fma(B1, t, 0.16666666666666666);
+        // m = 0..4 (the first omitted term t^5/20! is below 4.2e-19 for t <= 1), in power form
+        // with the coefficients as multiplicands (scalar operands of the FMAs); B1 is summed as
+        // 6 B1 so that every sum starts from an inline constant (1 or 1/2).
+        const double t2 = t * t, t3 = t2 * t, t4 = t2 * t2;
+        double A = fma(t, 0.041666666666666664, 1.0);     // 1/4!
+        A = fma(2.48015873015873e-05, t2, A);              // 1/8!
+        A = fma(2.08767569878681e-09, t3, A);              // 1/12!
+        A = fma(4.779477332387385e-14, t4, A);             // 1/16!
+        double A1 = fma(t, 0.008333333333333333, 1.0);    // 1/5!
+        A1 = fma(2.7557319223985893e-06, t2, A1);          // 1/9!
+        A1 = fma(1.6059043836821613e-10, t3, A1);          // 1/13!
+        A1 = fma(2.8114572543455206e-15, t4, A1);          // 1/17!
+        double B = fma(t, 0.001388888888888889, 0.5);     // 1/6!, 1/2!
+        B = fma(2.755731922398589e-07, t2, B);             // 1/10!
+        B = fma(1.1470745597729725e-11, t3, B);            // 1/14!
+        B = fma(1.5619206968586225e-16, t4, B);            // 1/18!
+        double B1 = fma(t, 0.0011904761904761906, 1.0);   // 6/7!
+        B1 = fma(1.5031265031265032e-07, t2, B1);          // 6/11!
+        B1 = fma(4.58829823909189e-12, t3, B1);          // 6/15!
+        B1 = fma(4.932381147974598e-17, t4, B1);           // 6/19!
+        B1 = B1 * 0.16666666666666666;                     // 1/3!
         double cxx = fma(-W, B, A), cxy = fma(W, B, A);
         double sxx = fma(-W, B1, A1), sxy = fma(W, B1, A1);
         if (any_lane(halvings > 0)) {
@@ -428,22 +447,42 @@ __device__ __forceinline__ void hwang_edge(Particle &p, CP e, double g_tot,
 // asin(s)/s = sum_n (2n)!/(4^n n!^2 (2n+1)) s^(2n); for |s| <= 1/8 the terms after s^18 are
 // below 1e-18.
 constexpr double kArcSeriesMax = 0.125;
+__constant__ const double kAsinCoef[10] = {
+    0.009761609529194078, 0.011551800896139705, 0.01396484375, 0.017352764423076924,
+    0.022372159090909092, 0.030381944444444444, 0.044642857142857144, 0.075,
+    0.16666666666666666, 1.0};
 __device__ __forceinline__ double asin_over(double s2)
 {
-    double r = 0.009761609529194078;
-    r = fma(r, s2, 0.011551800896139705);
-    r = fma(r, s2, 0.01396484375);
-    r = fma(r, s2, 0.017352764423076924);
-    r = fma(r, s2, 0.022372159090909092);
-    r = fma(r, s2, 0.030381944444444444);
-    r = fma(r, s2, 0.044642857142857144);
-    r = fma(r, s2, 0.075);
-    r = fma(r, s2, 0.16666666666666666);
-    return fma(r, s2, 1.0);
+    CP c = launder((CP)kAsinCoef);
+    double r = c[0];
+#pragma unroll
+    for (int k = 1; k < 10; k++) r = fma(r, s2, c[k]);
+    return r;
+}
+
+// The same series for s^2 <= 2^-12 (|s| <= 1/64): terms up to s^8 (the next is below 2e-20),
+// in power form with the coefficients as scalar multiplicands.
+constexpr double kArcShortMax = 1.0 / 4096;
+__device__ __forceinline__ double asin_over_short(double s2)
+{
+    const double s4 = s2 * s2, s6 = s4 * s2, s8 = s4 * s4;
+    double r = fma(s2, 0.16666666666666666, 1.0);  // 1/6
+    r = fma(0.075, s4, r);                           // 3/40
+    r = fma(0.044642857142857144, s6, r);            // 5/112
+    r = fma(0.030381944444444444, s8, r);            // 35/1152
+    return r;
 }
 
 // sin and cos of |phi| <= 4 (an rf phase wrapped to [-pi, pi)): quadrant reduction by pi/2
 // (two-part constant) and Taylor polynomials on |r| <= pi/4 (truncation below 1e-20).
+// Coefficients (Horner order): sin -1/19!, 1/17!, ..., -1/3!; cos 1/20!, -1/18!, ..., 1/4!.
+__constant__ const double kSinCosCoef[18] = {
+    -8.22063524662433e-18, 2.8114572543455206e-15, -7.647163731819816e-13,
+    1.6059043836821613e-10, -2.505210838544172e-08, 2.7557319223985893e-06,
+    -0.0001984126984126984, 0.008333333333333333, -0.16666666666666666,
+    4.110317623312165e-19, -1.5619206968586225e-16, 4.779477332387385e-14,
+    -1.1470745597729725e-11, 2.08767569878681e-09, -2.755731922398589e-07,
+    2.48015873015873e-05, -0.001388888888888889, 0.041666666666666664};
 __device__ __forceinline__ void sincos_reduced(double phi, double *sp, double *cp)
 {
     const double kPio2Hi = 1.5707963267948966;      // pi/2 rounded to double
@@ -452,30 +491,19 @@ __device__ __forceinline__ void sincos_reduced(double phi, double *sp, double *c
     const double r = fma(-kq, kPio2Lo, fma(-kq, kPio2Hi, phi));
     const double r2 = r * r;
     // sin r = r (1 - r^2/3! + ... - r^18/19!), cos r = 1 - r^2/2! + ... + r^20/20!
-    double s = -8.220635246624329e-18;  // -1/19!
-    s = fma(s, r2, 2.8114572543455206e-15);   // 1/17!
-    s = fma(s, r2, -7.647163731819816e-13);   // -1/15!
-    s = fma(s, r2, 1.6059043836821613e-10);   // 1/13!
-    s = fma(s, r2, -2.505210838544172e-08);   // -1/11!
-    s = fma(s, r2, 2.7557319223985893e-06);   // 1/9!
-    s = fma(s, r2, -0.0001984126984126984);   // -1/7!
-    s = fma(s, r2, 0.008333333333333333);     // 1/5!
-    s = fma(s, r2, -0.16666666666666666);     // -1/3!
+    CP c = launder((CP)kSinCosCoef);
+    double s = c[0];
+#pragma unroll
+    for (int k = 1; k < 9; k++) s = fma(s, r2, c[k]);
     s = r * fma(s, r2, 1.0);
-    double c = 4.110317623312165e-19;   // 1/20!
-    c = fma(c, r2, -1.5619206968586225e-16);  // -1/18!
-    c = fma(c, r2, 4.779477332387385e-14);    // 1/16!
-    c = fma(c, r2, -1.1470745597729725e-11);  // -1/14!
-    c = fma(c, r2, 2.08767569878681e-09);     // 1/12!
-    c = fma(c, r2, -2.755731922398589e-07);   // -1/10!
-    c = fma(c, r2, 2.48015873015873e-05);     // 1/8!
-    c = fma(c, r2, -0.001388888888888889);    // -1/6!
-    c = fma(c, r2, 0.041666666666666664);     // 1/4!
-    c = fma(c, r2, -0.5);                     // -1/2!
-    c = fma(c, r2, 1.0);
+    double cc = c[9];
+#pragma unroll
+    for (int k = 10; k < 18; k++) cc = fma(cc, r2, c[k]);
+    cc = fma(cc, r2, -0.5);
+    const double co = fma(cc, r2, 1.0);
     const int q = ((int)kq) & 3;
-    *sp = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
-    *cp = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
+    *sp = q == 0 ? s : (q == 1 ? co : (q == 2 ? -s : -co));
+    *cp = q == 0 ? co : (q == 1 ? -s : (q == 2 ? -co : s));
 }
 
 // sqrt(1 - u) for u < 2^-10 (terms after u^6 below 1e-19).
@@ -630,10 +658,14 @@ __device__ __forceinline__ void sbend_exact_fast(Particle &p, CP b)
     const double sh = fma(sin_plus, cpsi, -cos_plus * spsi);  // sin(angle_p/2)
     const double ch = fma(cos_plus, cpsi, sin_plus * spsi);   // cos(angle_p/2)
     // L_p = L_c / sinc(h) = L_c h / sin h
-    double hs = asin_over(sh * sh);
-    const bool h_far = !(fabs(sh) <= kArcSeriesMax && ch > 0);
-    if (any_lane(h_far)) {
-        if (h_far) hs = h_over_sin_far(sh, ch);
+    const double sh2 = sh * sh;
+    double hs = asin_over_short(sh2);
+    const bool h_mid = !(sh2 <= kArcShortMax && ch > 0);
+    if (any_lane(h_mid)) {
+        if (h_mid) {
+            if (fabs(sh) <= kArcSeriesMax && ch > 0) hs = asin_over(sh2);
+            else hs = h_over_sin_far(sh, ch);
+        }
     }
     const double L_p = L_c * hs;
     // sin(angle + phi_1 - 2h) with sin/cos 2h from sh, ch
@@ -804,7 +836,11 @@ __device__ __forceinline__ void rfcavity(Particle &p, CP r, int32_t n_slice,
 // exit aperture (:329). hdr points at the element's 4-int header (bmx_table.hpp); prm at the
 // lattice's parameter array. Returns true when the particle was lost at the entrance
 // aperture (location = upstream_end$).
-template <bool F>
+// MAPS selects the element kinds compiled in: MAPS_ALL, or MAPS_NO_RF / MAPS_RF_ONLY for
+// loops that keep the (rare, register-hungry) rf cavity out of the common element path.
+enum MapSet : int { MAPS_ALL = 0, MAPS_NO_RF = 1, MAPS_RF_ONLY = 2 };
+
+template <bool F, int MAPS = MAPS_ALL>
 __device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, const Species &sp)
 {
     const int32_t kind = hdr[0], flags = hdr[1], n = hdr[2], off = hdr[3];
@@ -819,13 +855,19 @@ __device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, con
             }
         }
     }
-    switch (kind) {
-    case K_DRIFT: drift<F>(p, q[0]); break;
-    case K_QUAD: quadrupole<F>(p, q, n, sp); break;
-    case K_MULT: thick_multipole<F>(p, q, n, (flags >> F_ORDER_SHIFT) & 0xF); break;
-    case K_BEND: sbend<F>(p, flags, q, sp); break;
-    case K_RF: rfcavity<F>(p, q, n, sp); break;
-    default: break;
+    if constexpr (MAPS == MAPS_RF_ONLY) {
+        rfcavity<F>(p, q, n, sp);
+    } else {
+        switch (kind) {
+        case K_DRIFT: drift<F>(p, q[0]); break;
+        case K_QUAD: quadrupole<F>(p, q, n, sp); break;
+        case K_MULT: thick_multipole<F>(p, q, n, (flags >> F_ORDER_SHIFT) & 0xF); break;
+        case K_BEND: sbend<F>(p, flags, q, sp); break;
+        case K_RF:
+            if constexpr (MAPS == MAPS_ALL) rfcavity<F>(p, q, n, sp);
+            break;
+        default: break;
+        }
     }
     // orbit_too_large (overwrites any state), else the exit aperture for live particles; both
     // tests first without branches, the state codes only on the (rare) losing lanes

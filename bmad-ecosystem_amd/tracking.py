@@ -50,7 +50,7 @@ class Context:
 
     def set_run(self, chunk_turns: int = 10, grid: int = 0, waves: int = 0) -> None:
         """Turns per chunk of bmx_track_turns (0 = one chunk), persistent grid size (0 = device
-        occupancy limit) and waves per SIMD of the kernel variant (0 = default 3; 2 or 4)."""
+        occupancy limit) and waves per SIMD of the kernel variant (0 = default 3; 4 or 5)."""
         native.call("bmx_ctx_set_run", self.h, int(chunk_turns), int(grid), int(waves))
 
     def profile(self, on: bool = True) -> None:
