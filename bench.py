@@ -49,8 +49,8 @@ def parse():
                     help="comma list of the other workloads to run after the main one "
                          "(batched, da, served; empty = none)")
     ap.add_argument("--served-requests", type=int, default=24)
-    ap.add_argument("--waves", type=int, default=0,
-                    help="waves per SIMD of the tracking kernel variant (0 = default 3; 4 or 5)")
+    ap.add_argument("--waves", type=int, default=5,
+                    help="waves per SIMD of the tracking kernel variant (3, 4 or 5)")
     return ap.parse_args()
 
 
@@ -156,7 +156,7 @@ def main():
     }
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(bmx, arr, beam, args.cpu_seconds)
-    extras = [e for e in args.extras.split(",") if e]
+    extras = [e for e in args.extras.split(",") if e and e != "none"]
     out["extras"] = {}
     if "batched" in extras:
         out["extras"]["batched"] = run_batched(bmx, ctx, flops)

@@ -73,24 +73,34 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
         const CP prm = (CP)L.prm;
         // The rf cavities sit outside the inner element loop: the loop over the other kinds
         // runs until the next cavity, which is tracked on its own, so the cavity's code does not
-        // shape the register assignment of the common path.
-        for (int32_t turn = 0; turn < n_turns && p.state == ST_ALIVE; ++turn) {
+        // shape the register assignment of the common path. The loops are wave-uniform (the
+        // element index stays a scalar, so the table is read by scalar loads); a lost particle's
+        // lane is masked off for the rest of the turn, and a wave stops once all its
+        // particles are lost.
+        for (int32_t turn = 0; turn < n_turns; ++turn) {
+            if (!any_lane(p.state == ST_ALIVE)) break;
             int32_t e = 0;
             for (;;) {
                 for (; e < L.n_ele; ++e) {
                     const CHdr h = hdr + 4 * e;
                     if (h[0] == K_RF) break;
-                    track_element<F, MAPS_NO_RF>(p, h, prm, L.sp);
-                    if (p.state != ST_ALIVE) break;
+                    if (p.state == ST_ALIVE) {
+                        track_element<F, MAPS_NO_RF>(p, h, prm, L.sp);
+                        if (p.state != ST_ALIVE) {
+                            lost_e = e + 1;
+                            lost_t = turn0 + turn;
+                        }
+                    }
                 }
-                if (p.state != ST_ALIVE || e >= L.n_ele) break;
-                track_element<F, MAPS_RF_ONLY>(p, hdr + 4 * e, prm, L.sp);
-                if (p.state != ST_ALIVE) break;
+                if (e >= L.n_ele) break;
+                if (p.state == ST_ALIVE) {
+                    track_element<F, MAPS_RF_ONLY>(p, hdr + 4 * e, prm, L.sp);
+                    if (p.state != ST_ALIVE) {
+                        lost_e = e + 1;
+                        lost_t = turn0 + turn;
+                    }
+                }
                 ++e;
-            }
-            if (p.state != ST_ALIVE) {
-                lost_e = e + 1;
-                lost_t = turn0 + turn;
             }
         }
     }
