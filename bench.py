@@ -54,6 +54,18 @@ def parse():
     return ap.parse_args()
 
 
+def profiled_traffic():
+    """HBM bytes per tracking-kernel launch from the committed rocprofv3 summary (FETCH_SIZE and
+    WRITE_SIZE runs scaled by a copy-kernel calibration), or None if it is absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_track_tiles_ring1_1e6_traffic.md")
+    try:
+        m = re.search(r"\*\*([0-9.]+) MB per launch\*\*", open(path).read())
+    except OSError:
+        return None
+    return float(m.group(1)) * 1e6 if m else None
+
+
 def cpu_baseline(bmx, arr, beam, target_s):
     """The CPU restatement (OpenMP over particles per element, as track1_bunch_hom) on a
     bounded sample of the same workload: a particle subset over a few turns."""
@@ -149,7 +161,10 @@ def main():
         "roofline": {
             "bound": "mfma", "compute": "fp64 vector ALU", "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None, "flop_per_particle_element": flops,
+            "traffic": profiled_traffic(),
+            "traffic_note": "HBM bytes of the first 10-turn launch over 1e6 particles, "
+                            "profiles/k_track_tiles_ring1_1e6_traffic.md",
+            "flop_per_particle_element": flops,
             "kernel": "k_track_tiles", "kernel_ms_avg": kernel_ms / max(launches, 1),
             "launches": launches, "step_ms": [float(x) for x in launch_ms],
         },

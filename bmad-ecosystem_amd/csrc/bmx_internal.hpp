@@ -18,7 +18,8 @@ hipError_t launch_track_tiles(hipStream_t s, int grid, int waves, const Group *d
                               unsigned long long *d_work, bool fast);
 hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_groups,
                               const uint32_t *d_cnt_in, int32_t next_off, WaveTile *d_tiles,
-                              uint32_t *d_n_tiles, uint32_t *d_cnt_zero, uint32_t *d_tile_ctr);
+                              uint32_t *d_n_tiles, uint32_t *d_cnt_zero, uint32_t *d_tile_ctr,
+                              uint32_t *d_goff);
 hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu);
 hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, int32_t turn,
                                 const BeamPtrs &b, int64_t n, unsigned long long *d_work,
@@ -53,6 +54,7 @@ struct RunBuffers {
     uint32_t *d_list[2] = {nullptr, nullptr};
     WaveTile *d_tiles[2] = {nullptr, nullptr};
     uint32_t *d_cnt[2] = {nullptr, nullptr};
+    uint32_t *d_goff = nullptr;   // first tile of each group (n_groups + 1), tile builder scratch
     uint32_t *d_small = nullptr;  // [0..1] tile counts of the two tile lists, [2] tile counter
     void release();
     // Grows the buffers to hold the given sizes (allocates; not for use under graph capture).

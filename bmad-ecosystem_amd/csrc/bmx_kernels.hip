@@ -177,13 +177,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))
 
 // Builds the wave tiles of the next chunk from the alive counts per group (cnt_in; nullptr
 // for the first chunk: every particle of each group, indexed directly) and resets the counters
-// the next chunk appends to. One workgroup of kBuildThreads threads; groups that have no turns
-// left after next_off get no tiles.
+// the next chunk appends to. One workgroup of kBuildThreads threads: a scan of the per-group
+// tile counts (groups that have no turns left after next_off get none) into goff[0..n_groups],
+// then all threads write the tiles, each finding its group by binary search in goff.
 constexpr int kBuildThreads = 1024;
 __global__ void __launch_bounds__(kBuildThreads) k_build_tiles(
     const Group *__restrict__ groups, int32_t n_groups, const uint32_t *__restrict__ cnt_in,
     int32_t next_off, WaveTile *__restrict__ tiles, uint32_t *__restrict__ n_tiles_p,
-    uint32_t *__restrict__ cnt_zero, uint32_t *__restrict__ tile_ctr)
+    uint32_t *__restrict__ cnt_zero, uint32_t *__restrict__ tile_ctr, uint32_t *__restrict__ goff)
 {
     __shared__ uint32_t part[kBuildThreads];
     const int tid = threadIdx.x;
@@ -206,21 +207,34 @@ __global__ void __launch_bounds__(kBuildThreads) k_build_tiles(
     }
     uint32_t pos = part[tid] - mine;
     for (int g = g0; g < g1; g++) {
+        goff[g] = pos;
         const uint32_t c = cnt_in ? cnt_in[g] : groups[g].n;
-        if (groups[g].n_turns > next_off) {
-            for (uint32_t s = 0; s < c; s += 64) {
-                WaveTile w;
-                w.group = g;
-                w.start = s;
-                w.count = c - s < 64 ? c - s : 64;
-                w.pad = 0;
-                tiles[pos++] = w;
-            }
-        }
-        cnt_zero[g] = 0;
+        if (groups[g].n_turns > next_off) pos += (c + 63) / 64;
     }
-    if (tid == kBuildThreads - 1) {
-        *n_tiles_p = part[tid];
+    const uint32_t total = part[kBuildThreads - 1];
+    if (tid == 0) goff[n_groups] = total;
+    __syncthreads();
+    for (uint32_t t = tid; t < total; t += kBuildThreads) {
+        // last g with goff[g] <= t (groups without tiles have goff[g] == goff[g + 1])
+        int lo = 0, hi = n_groups - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (goff[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t c = cnt_in ? cnt_in[lo] : groups[lo].n;
+        const uint32_t s0 = (t - goff[lo]) * 64;
+        WaveTile w;
+        w.group = lo;
+        w.start = s0;
+        w.count = c - s0 < 64 ? c - s0 : 64;
+        w.pad = 0;
+        tiles[t] = w;
+    }
+    __syncthreads();
+    for (int g = tid; g < n_groups; g += kBuildThreads) cnt_zero[g] = 0;
+    if (tid == 0) {
+        *n_tiles_p = total;
         *tile_ctr = 0;
     }
 }
@@ -355,10 +369,11 @@ hipError_t launch_track_tiles(hipStream_t s, int grid, int waves, const Group *d
 
 hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_groups,
                               const uint32_t *d_cnt_in, int32_t next_off, WaveTile *d_tiles,
-                              uint32_t *d_n_tiles, uint32_t *d_cnt_zero, uint32_t *d_tile_ctr)
+                              uint32_t *d_n_tiles, uint32_t *d_cnt_zero, uint32_t *d_tile_ctr,
+                              uint32_t *d_goff)
 {
     hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(kBuildThreads), 0, s, d_groups, n_groups,
-                       d_cnt_in, next_off, d_tiles, d_n_tiles, d_cnt_zero, d_tile_ctr);
+                       d_cnt_in, next_off, d_tiles, d_n_tiles, d_cnt_zero, d_tile_ctr, d_goff);
     return hipGetLastError();
 }
 

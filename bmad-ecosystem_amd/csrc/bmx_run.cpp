@@ -26,6 +26,8 @@ void RunBuffers::release()
         d_cnt[k] = nullptr;
     }
     hipFree(d_small);
+    hipFree(d_goff);
+    d_goff = nullptr;
     d_groups = nullptr;
     d_small = nullptr;
     cap_groups = cap_particles = cap_tiles = 0;
@@ -41,11 +43,14 @@ hipError_t RunBuffers::reserve(size_t n_groups, size_t n_particles, size_t n_til
     if (n_groups > cap_groups) {
         const size_t c = std::max(n_groups, 2 * cap_groups);
         hipFree(d_groups);
+        hipFree(d_goff);
         for (int k = 0; k < 2; k++) hipFree(d_cnt[k]);
         d_groups = nullptr;
+        d_goff = nullptr;
         d_cnt[0] = d_cnt[1] = nullptr;
         cap_groups = 0;
         e = hipMalloc(&d_groups, c * sizeof(Group));
+        if (e == hipSuccess) e = hipMalloc(&d_goff, (c + 1) * sizeof(uint32_t));
         for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMalloc(&d_cnt[k], c * sizeof(uint32_t));
         if (e != hipSuccess) return e;
         cap_groups = c;
@@ -124,7 +129,7 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
     uint32_t *tile_ctr = rb.d_small + 2;
     // chunk 0 tiles index the particles directly; the chunk appends to list/cnt 1
     e = launch_build_tiles(s, rb.d_groups, n_groups, nullptr, 0, rb.d_tiles[0], n_tiles[0],
-                           rb.d_cnt[1], tile_ctr);
+                           rb.d_cnt[1], tile_ctr, rb.d_goff);
     int c = 0;
     for (int32_t off = 0; off < max_turns && e == hipSuccess; off += K, c++) {
         const int cur = c & 1, nxt = cur ^ 1;
@@ -144,7 +149,7 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
         }
         if (e == hipSuccess && off + K < max_turns)
             e = launch_build_tiles(s, rb.d_groups, n_groups, rb.d_cnt[nxt], off + K,
-                                   rb.d_tiles[nxt], n_tiles[nxt], rb.d_cnt[cur], tile_ctr);
+                                   rb.d_tiles[nxt], n_tiles[nxt], rb.d_cnt[cur], tile_ctr, rb.d_goff);
     }
     return e;
 }
