@@ -379,6 +379,16 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
         out.prm.insert(out.prm.end(), q.begin(), q.end());
         out.hdr.push_back(h);
     }
+    // Entrance tests equal to the exit test just before (a particle does not move between
+    // elements, so it passes the second if it passed the first). Element 0 keeps its test.
+    for (size_t i = 1; i < out.hdr.size(); i++) {
+        EleHdr &h = out.hdr[i];
+        const EleHdr &g = out.hdr[i - 1];
+        if (!(h.flags & F_APE_ENT) || !(g.flags & F_APE_EXIT)) continue;
+        if ((h.flags & F_APE_ELLIP) != (g.flags & F_APE_ELLIP)) continue;
+        if (std::memcmp(&out.prm[h.off], &out.prm[g.off], 4 * sizeof(double)) != 0) continue;
+        h.flags = (h.flags & ~F_APE_ENT) | F_APE_ENT_SAME;
+    }
     return BMX_OK;
 }
 

@@ -61,13 +61,14 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
     Particle p;
     load_particle(p, b, i);
     refresh_energy(p, L.sp);
-    int32_t lost_e = 0, lost_t = 0;
+    p.loss_ele = 0;
+    p.loss_turn = 0;
     // Entrance validity check of the first element (track1.f90:196 -> orbit_too_large); later
     // elements need none: a particle leaves element e with the coordinates it enters e+1 with,
     // and element e already ended with the same test (track1.f90:291).
     if (orbit_too_large(p)) {
-        lost_e = 1;
-        lost_t = turn0;
+        p.loss_ele = 1;
+        p.loss_turn = turn0;
     } else {
         const CHdr hdr = (CHdr)L.hdr;
         const CP prm = (CP)L.prm;
@@ -76,30 +77,26 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
         // shape the register assignment of the common path. The loops are wave-uniform (the
         // element index stays a scalar, so the table is read by scalar loads); a lost particle's
         // lane is masked off for the rest of the turn, and a wave stops once all its
-        // particles are lost.
+        // particles are lost. track_element records where a particle is lost.
         for (int32_t turn = 0; turn < n_turns; ++turn) {
             if (!any_lane(p.state == ST_ALIVE)) break;
+            const int32_t t_abs = turn0 + turn;
             int32_t e = 0;
             for (;;) {
                 for (; e < L.n_ele; ++e) {
+                    // e is the same on every active lane; read it from the first one so the
+                    // compiler keeps it (and the table reads) scalar whatever it makes of the
+                    // loop's control flow
+                    e = __builtin_amdgcn_readfirstlane(e);
                     const CHdr h = hdr + 4 * e;
                     if (h[0] == K_RF) break;
-                    if (p.state == ST_ALIVE) {
-                        track_element<F, MAPS_NO_RF>(p, h, prm, L.sp);
-                        if (p.state != ST_ALIVE) {
-                            lost_e = e + 1;
-                            lost_t = turn0 + turn;
-                        }
-                    }
+                    if (p.state == ST_ALIVE)
+                        track_element<F, MAPS_NO_RF>(p, h, prm, L.sp, true, e + 1, t_abs);
                 }
                 if (e >= L.n_ele) break;
-                if (p.state == ST_ALIVE) {
-                    track_element<F, MAPS_RF_ONLY>(p, hdr + 4 * e, prm, L.sp);
-                    if (p.state != ST_ALIVE) {
-                        lost_e = e + 1;
-                        lost_t = turn0 + turn;
-                    }
-                }
+                e = __builtin_amdgcn_readfirstlane(e);
+                if (p.state == ST_ALIVE)
+                    track_element<F, MAPS_RF_ONLY>(p, hdr + 4 * e, prm, L.sp, true, e + 1, t_abs);
                 ++e;
             }
         }
@@ -107,9 +104,9 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
     store_particle(p, b, i);
     if (p.state != ST_ALIVE) {
         b.state[i] = p.state;
-        b.loss_ele[i] = lost_e;
-        b.loss_turn[i] = lost_t;
-        return (unsigned long long)(lost_t - turn0) * L.n_ele + lost_e;
+        b.loss_ele[i] = p.loss_ele;
+        b.loss_turn[i] = p.loss_turn;
+        return (unsigned long long)(p.loss_turn - turn0) * L.n_ele + p.loss_ele;
     }
     return (unsigned long long)n_turns * L.n_ele;
 }
@@ -251,7 +248,8 @@ __global__ void __launch_bounds__(kBlock) k_track_element(LatDesc L, int32_t ie,
         Particle p;
         load_particle(p, b, i);
         refresh_energy(p, L.sp);
-        if (!orbit_too_large(p)) track_element<F>(p, (CHdr)L.hdr + 4 * ie, (CP)L.prm, L.sp);
+        if (!orbit_too_large(p))
+            track_element<F>(p, (CHdr)L.hdr + 4 * ie, (CP)L.prm, L.sp, false, ie + 1, turn);
         store_particle(p, b, i);
         if (p.state != ST_ALIVE) {
             b.state[i] = p.state;
@@ -300,7 +298,8 @@ __global__ void __launch_bounds__(kBlock) k_track_range(LatDesc L, int32_t e0, i
             record_orbit(rec + i * ne * 8, p, true);
         } else {
             for (int32_t e = e0; e <= e1; ++e) {
-                const bool at_entrance = track_element<F>(p, (CHdr)L.hdr + 4 * e, (CP)L.prm, L.sp);
+                const bool at_entrance =
+                    track_element<F>(p, (CHdr)L.hdr + 4 * e, (CP)L.prm, L.sp, e > e0, e + 1, turn);
                 passes++;
                 record_orbit(rec + (i * ne + (e - e0)) * 8, p, at_entrance);
                 if (p.state != ST_ALIVE) {

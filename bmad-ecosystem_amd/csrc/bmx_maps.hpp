@@ -47,6 +47,8 @@ struct Particle {
     double inv_rp;  // 1 / (1 + pz)
     double ez;      // sqrt_one(mc2^2 (2 pz + pz^2) / ((p0c rel_p)^2 + mc2^2)), track_a_drift.f90:69
     double lez;     // low_energy_z_correction per metre (low_energy_z_correction.f90:44-57)
+    // Where the particle was lost (set with the loss state): element index (1-based), turn.
+    int32_t loss_ele, loss_turn;
 };
 
 __device__ __forceinline__ double sqr(double v) { return v * v; }
@@ -64,7 +66,14 @@ __device__ __forceinline__ CP launder(CP p)
 // outside a series domain, a loss) are gated by this wave-uniform test so the common path is
 // straight-line code; inside the gated block each lane still takes its own branch, so a
 // particle's result never depends on the other particles of its wave.
+#ifndef BMX_GATE
+#define BMX_GATE 1
+#endif
+#if BMX_GATE
 __device__ __forceinline__ bool any_lane(bool c) { return __ballot(c) != 0; }
+#else
+__device__ __forceinline__ bool any_lane(bool c) { return c; }
+#endif
 
 // sim_utils/special_functions/sqrt_one.f90:18-41
 __device__ __forceinline__ double sqrt_one(double v) { return v / (sqrt(1 + v) + 1); }
@@ -182,6 +191,17 @@ __device__ __forceinline__ void drift_exact(Particle &p, double L)
     p.z = p.z + L * (p.ez + (-pxy2 / (ps + 1)) / ps);
 }
 
+// The series body of the fast form (pxy2 < kDriftSeriesMax).
+__device__ __forceinline__ void drift_series(Particle &p, double L, double px_rel, double py_rel,
+                                             double pxy2)
+{
+    const double q = inv_ps_minus_1(pxy2);
+    const double lx = L * px_rel, ly = L * py_rel;
+    p.x = p.x + fma(lx, q, lx);
+    p.y = p.y + fma(ly, q, ly);
+    p.z = p.z + L * (p.ez - q);
+}
+
 template <bool F>
 __device__ __forceinline__ void drift(Particle &p, double L)
 {
@@ -196,13 +216,9 @@ __device__ __forceinline__ void drift(Particle &p, double L)
     const bool series = pxy2 < kDriftSeriesMax;
     if (any_lane(!series)) {
         if (!series) drift_exact(p, L);
-    }
-    if (series) {
-        const double q = inv_ps_minus_1(pxy2);
-        const double lx = L * px_rel, ly = L * py_rel;
-        p.x = p.x + fma(lx, q, lx);
-        p.y = p.y + fma(ly, q, ly);
-        p.z = p.z + L * (p.ez - q);
+        else drift_series(p, L, px_rel, py_rel, pxy2);
+    } else {
+        drift_series(p, L, px_rel, py_rel, pxy2);
     }
 }
 
@@ -351,30 +367,44 @@ __device__ __forceinline__ void quadrupole(Particle &p, CP q, int32_t n_terms, c
 // ---- thick multipole ---------------------------------------------------------------------
 
 // bmad/modules/multipole_mod.f90:82-174,314-401 (ab_multipole_kicks, magnetic): kick from
-// orders n in mask with strengths (a_n, b_n) * scale; c_multi's signed binomials summed as the
-// real and imaginary parts of (x + i y)^n: kx = -b Re + a Im, ky = a Re + b Im.
-__device__ __forceinline__ void multipole_kick(Particle &p, CP ah,
-                                               CP bh, int32_t mask,
+// orders n in mask with strengths (a_n, b_n): c_multi's signed binomials summed as the real and
+// imaginary parts of (x + i y)^n, kx = -b Re + a Im, ky = a Re + b Im. The table holds the
+// half-step strengths; factor (1 or 2) scales the kick, exactly as scaling the strengths would.
+__device__ __forceinline__ void multipole_kick(Particle &p, CP ah, CP bh, int32_t mask,
                                                double factor)
 {
     double kx = 0, ky = 0;
-    double zr = 1, zi = 0;
-    const double x = p.x, y = p.y;
+    if (mask & 1) {
+        kx = -bh[0];
+        ky = ah[0];
+    }
+    if (mask >> 1) {
+        const double x = p.x, y = p.y;
+        double zr = x, zi = y;
 #pragma unroll
-    for (int n = 0; n < 4; n++) {
-        if (mask & (1 << n)) {
-            const double an = ah[n] * factor, bn = bh[n] * factor;
-            kx += -bn * zr + an * zi;
-            ky += an * zr + bn * zi;
-        }
-        if (n < 3 && (mask >> (n + 1))) {
-            const double nr = zr * x - zi * y;
-            zi = zr * y + zi * x;
-            zr = nr;
+        for (int n = 1; n < 4; n++) {
+            if (mask & (1 << n)) {
+                kx += -bh[n] * zr + ah[n] * zi;
+                ky += ah[n] * zr + bh[n] * zi;
+            }
+            if (n < 3 && (mask >> (n + 1))) {
+                const double nr = zr * x - zi * y;
+                zi = zr * y + zi * x;
+                zr = nr;
+            }
         }
     }
-    p.px = p.px + kx;
-    p.py = p.py + ky;
+    p.px = fma(factor, kx, p.px);
+    p.py = fma(factor, ky, p.py);
+}
+
+// The kick of a pure sextupole (mask = order 2 only) with its strengths held in registers.
+__device__ __forceinline__ void sextupole_kick(Particle &p, double a, double b, double factor)
+{
+    const double x = p.x, y = p.y;
+    const double re = x * x - y * y, im = x * y + y * x;
+    p.px = fma(factor, -b * re + a * im, p.px);
+    p.py = fma(factor, a * re + b * im, p.py);
 }
 
 // bmad/low_level/track_a_thick_multipole.f90:18-133 (magnetic, no misalignment):
@@ -385,6 +415,15 @@ __device__ __forceinline__ void thick_multipole(Particle &p, CP m,
                                                 int32_t n_step, int32_t mask)
 {
     const double step = m[0];
+    if (mask == 4) {
+        const double a = m[3], b = m[7];
+        sextupole_kick(p, a, b, 1.0);
+        for (int i = 1; i <= n_step; i++) {
+            drift<F>(p, step);
+            sextupole_kick(p, a, b, i == n_step ? 1.0 : 2.0);
+        }
+        return;
+    }
     multipole_kick(p, m + 1, m + 5, mask, 1.0);
     for (int i = 1; i <= n_step; i++) {
         drift<F>(p, step);
@@ -460,17 +499,19 @@ __device__ __forceinline__ double asin_over(double s2)
     return r;
 }
 
-// The same series for s^2 <= 2^-12 (|s| <= 1/64): terms up to s^8 (the next is below 2e-20),
-// in power form with the coefficients as scalar multiplicands.
-constexpr double kArcShortMax = 1.0 / 4096;
+// The same series for s^2 <= 2^-10 (|s| <= 1/32, which covers the half exit angle of bends
+// up to about 3.5 degrees): terms up to s^10, the next (231/13312 s^12) below 2e-20.
+constexpr double kArcShortMax = 1.0 / 1024;
+// Power form: the coefficients are multiplicands (scalar operands), not addends that would be
+// held in vector registers.
 __device__ __forceinline__ double asin_over_short(double s2)
 {
-    const double s4 = s2 * s2, s6 = s4 * s2, s8 = s4 * s4;
-    double r = fma(s2, 0.16666666666666666, 1.0);  // 1/6
-    r = fma(0.075, s4, r);                           // 3/40
-    r = fma(0.044642857142857144, s6, r);            // 5/112
-    r = fma(0.030381944444444444, s8, r);            // 35/1152
-    return r;
+    const double s4 = s2 * s2, s8 = s4 * s4;
+    double r = fma(0.16666666666666666, s2, 1.0);     // 1/6
+    r = fma(0.075, s4, r);                             // 3/40
+    r = fma(0.044642857142857144, s4 * s2, r);         // 5/112
+    r = fma(0.030381944444444444, s8, r);              // 35/1152
+    return fma(0.022372159090909092, s8 * s2, r);      // 63/2816
 }
 
 // sin and cos of |phi| <= 4 (an rf phase wrapped to [-pi, pi)): quadrant reduction by pi/2
@@ -504,6 +545,24 @@ __device__ __forceinline__ void sincos_reduced(double phi, double *sp, double *c
     const int q = ((int)kq) & 3;
     *sp = q == 0 ? s : (q == 1 ? co : (q == 2 ? -s : -co));
     *cp = q == 0 ? co : (q == 1 ? -s : (q == 2 ? -co : s));
+}
+
+// 1/sqrt(v) for normal v > 0: the hardware estimate y refined once as y (1 + e/2 + 3 e^2/8),
+// e = 1 - v y^2 (the series of (1 - e)^(-1/2), cubically convergent). Unlike the library
+// rsqrt it has no zero/infinity/denormal cases (the callers' arguments are near 1 or L^2).
+__device__ __forceinline__ double rsqrt_pos(double v)
+{
+    const double y = __builtin_amdgcn_rsq(v);
+    const double e = fma(-v * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
+// sqrt(v) for normal v > 0: s = v rsqrt(v) with one Newton correction of the residual v - s^2.
+__device__ __forceinline__ double sqrt_pos(double v)
+{
+    const double y = rsqrt_pos(v);
+    const double s = v * y;
+    return fma(fma(-s, s, v), 0.5 * y, s);
 }
 
 // sqrt(1 - u) for u < 2^-10 (terms after u^6 below 1e-19).
@@ -608,7 +667,7 @@ __device__ __forceinline__ void sbend_exact_fast(Particle &p, CP b)
     const double rel_p = p.rel_p;
     const double pt2 = rel_p * rel_p - py * py;
     // pt2 <= 0 gives NaN coordinates here as sqrt does in the reference form
-    const double inv_pt = rsqrt(pt2);
+    const double inv_pt = rsqrt_pos(pt2);
     const double pt = pt2 * inv_pt;
     const double g_p = g_tot * inv_pt;
     const double s1 = px * inv_pt;
@@ -623,7 +682,8 @@ __device__ __forceinline__ void sbend_exact_fast(Particle &p, CP b)
     const double u = (1 + g * x) * (L * sinc_a);
     const double alpha = 2 * sin_plus * u - g_p * (u * u);
     const double r = fma(cos_plus, cos_plus, g_p * alpha);
-    const double rad = sqrt(r);
+    // r <= 0 loses the particle below; the floor keeps the square root's argument normal
+    const double rad = sqrt_pos(fmax(r, 0x1p-1022));
     double xi = alpha / (rad + cos_plus);
     const bool turned = !(cos_plus > 0);
     if (any_lane(turned)) {
@@ -652,7 +712,7 @@ __device__ __forceinline__ void sbend_exact_fast(Particle &p, CP b)
     const double L_u = xi;
     const double mL_v = L * sinc_a + x * sin_a;  // -L_v
     const double lc2 = mL_v * mL_v + L_u * L_u;
-    const double inv_lc = rsqrt(lc2);
+    const double inv_lc = rsqrt_pos(lc2);
     const double L_c = lc2 * inv_lc;
     const double cpsi = mL_v * inv_lc, spsi = L_u * inv_lc;
     const double sh = fma(sin_plus, cpsi, -cos_plus * spsi);  // sin(angle_p/2)
@@ -838,19 +898,26 @@ __device__ __forceinline__ void rfcavity(Particle &p, CP r, int32_t n_slice,
 // aperture (location = upstream_end$).
 // MAPS selects the element kinds compiled in: MAPS_ALL, or MAPS_NO_RF / MAPS_RF_ONLY for
 // loops that keep the (rare, register-hungry) rf cavity out of the common element path.
+// from_prev: the particle has just left the element before this one in the same lattice, so
+// an entrance test that repeats that element's exit test (F_APE_ENT_SAME, F_APE_ENT cleared)
+// is skipped; otherwise it is made.
+// A particle lost here gets p.loss_ele = ix1 (the element's 1-based index), p.loss_turn = turn.
 enum MapSet : int { MAPS_ALL = 0, MAPS_NO_RF = 1, MAPS_RF_ONLY = 2 };
 
 template <bool F, int MAPS = MAPS_ALL>
-__device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, const Species &sp)
+__device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, const Species &sp,
+                                              bool from_prev, int32_t ix1, int32_t turn)
 {
     const int32_t kind = hdr[0], flags = hdr[1], n = hdr[2], off = hdr[3];
     CP ap = prm + off;
-    CP q = (flags & (F_APE_ENT | F_APE_EXIT)) ? ap + 4 : ap;
-    if (flags & F_APE_ENT) {
+    CP q = (flags & (F_APE_ENT | F_APE_EXIT | F_APE_ENT_SAME)) ? ap + 4 : ap;
+    if ((flags & F_APE_ENT) || (!from_prev && (flags & F_APE_ENT_SAME))) {
         const bool out = aperture_fails(p, flags, ap);
         if (any_lane(out)) {
             if (out) {
                 aperture(p, flags, ap);
+                p.loss_ele = ix1;
+                p.loss_turn = turn;
                 return true;
             }
         }
@@ -870,14 +937,18 @@ __device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, con
         }
     }
     // orbit_too_large (overwrites any state), else the exit aperture for live particles; both
-    // tests first without branches, the state codes only on the (rare) losing lanes
+    // tests first without branches, the state codes and the loss record only on the (rare)
+    // lanes that are lost here or were lost inside the map
     const bool bad = orbit_bad(p);
+    const bool alive = p.state == ST_ALIVE;
     const bool ape = (flags & F_APE_EXIT) ? aperture_fails(p, flags, ap) : false;
-    const bool hit = bad | (ape & (p.state == ST_ALIVE));
+    const bool hit = bad | ape | !alive;
     if (any_lane(hit)) {
         if (hit) {
             if (bad) orbit_too_large(p);
-            else aperture(p, flags, ap);
+            else if (alive) aperture(p, flags, ap);
+            p.loss_ele = ix1;
+            p.loss_turn = turn;
         }
     }
     return false;

@@ -30,6 +30,10 @@ enum Flags : int32_t {
     F_BEND_DRIFT = 1 << 6,  // g = dg = 0: the bend body is a drift (track_a_bend.f90:117-118)
     F_BEND_SERIES = 1 << 7, // |angle| small enough for the fast exact-body form
     F_ORDER_SHIFT = 8,      // K_MULT: bits 8..11 = mask of multipole orders 0..3 present
+    // The entrance test repeats the exit test of the element before (same type and values,
+    // same coordinates): a particle arriving from that element has passed it already. Set with
+    // F_APE_ENT cleared; kernels that start at this element make the test.
+    F_APE_ENT_SAME = 1 << 12,
 };
 
 struct EleHdr {
