@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--maps", choices=["fast", "reference"], default="fast",
                     help="formulation of the element maps")
-    ap.add_argument("--chunk-turns", type=int, default=10,
+    ap.add_argument("--chunk-turns", type=int, default=5,
                     help="turns between alive-list compactions (0 = none)")
     ap.add_argument("--extras", default="batched,da,served",
                     help="comma list of the other workloads to run after the main one "

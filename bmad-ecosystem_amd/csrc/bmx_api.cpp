@@ -323,7 +323,7 @@ int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t
     RunConfig cfg;
     cfg.chunk_turns = ctx->chunk_turns;
     cfg.fast = ctx->fast_maps;
-    cfg.waves = ctx->waves;
+    cfg.waves = ctx->waves ? ctx->waves : default_waves(ctx->fast_maps);
     cfg.grid = ctx->grid > 0 ? ctx->grid : default_grid(ctx->device, ctx->fast_maps, cfg.waves);
     cfg.timer = ctx->profile ? &ctx->timer : nullptr;
     HIP_TRY(enqueue_run(ctx->stream, ctx->run, &g, 1, true, n_turns, cfg, ctx->d_work), "k_track_tiles");
@@ -362,7 +362,7 @@ int32_t bmx_ctx_set_run(bmx_ctx *ctx, int32_t chunk_turns, int32_t grid, int32_t
         return set_error(BMX_ERR_ARG, "bmx_ctx_set_run: bad argument");
     ctx->chunk_turns = chunk_turns;
     ctx->grid = grid;
-    ctx->waves = waves ? waves : 3;
+    ctx->waves = waves;
     return BMX_OK;
 }
 

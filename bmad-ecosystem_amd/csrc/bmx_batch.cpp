@@ -97,7 +97,7 @@ int32_t bmx_batch_create(bmx_ctx *ctx, const bmx_group *groups, int32_t n_groups
     }
     B->cfg.chunk_turns = ctx->chunk_turns;
     B->cfg.fast = ctx->fast_maps;
-    B->cfg.waves = ctx->waves;
+    B->cfg.waves = ctx->waves ? ctx->waves : default_waves(B->cfg.fast);
     B->cfg.grid = default_grid(ctx->device, B->cfg.fast, B->cfg.waves);
     *out = B;
     return BMX_OK;
@@ -122,7 +122,7 @@ int32_t bmx_batch_set_run(bmx_batch *B, int32_t chunk_turns, int32_t grid, int32
     hipSetDevice(B->ctx->device);
     hipStreamSynchronize(B->ctx->stream);
     B->cfg.chunk_turns = chunk_turns;
-    B->cfg.waves = waves ? waves : 3;
+    B->cfg.waves = waves ? waves : default_waves(B->cfg.fast);
     B->cfg.grid = grid > 0 ? grid : default_grid(B->ctx->device, B->cfg.fast, B->cfg.waves);
     B->drop_graphs();
     return BMX_OK;

@@ -71,9 +71,9 @@ struct KernelTimer {
 
 // Parameters of one run.
 struct RunConfig {
-    int32_t chunk_turns = 10;  // turns per chunk (alive lists are rebuilt between chunks)
+    int32_t chunk_turns = 5;   // turns per chunk (alive lists are rebuilt between chunks)
     int32_t grid = 0;          // workgroups of the tracking kernel (0: occupancy limit x CUs)
-    int32_t waves = 3;         // waves per SIMD the tracking kernel is compiled for (3, 4, 5)
+    int32_t waves = 5;         // waves per SIMD the tracking kernel is compiled for (3, 4, 5)
     bool fast = true;
     KernelTimer *timer = nullptr;  // if set: an event pair around every tracking launch
 };
@@ -86,6 +86,9 @@ struct RunConfig {
 hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int32_t n_groups,
                        bool upload, int32_t max_turns, const RunConfig &cfg,
                        unsigned long long *d_work);
+// Waves per SIMD of the default tracking kernel variant: 5 for the fast maps (the fastest
+// measured), 3 for the reference maps (whose 4- and 5-wave builds spill).
+inline int32_t default_waves(bool fast) { return fast ? 5 : 3; }
 // Workgroups of the tracking kernel that fit on the device at once.
 int default_grid(int device, bool fast, int waves);
 
@@ -98,9 +101,9 @@ struct bmx_ctx {
     unsigned long long *d_work = nullptr;     // particle-element passes since the last take
     hipEvent_t events[BMX_N_EVENTS] = {};
     bool fast_maps = true;  // BMX_MAPS_FAST (default) or BMX_MAPS_REFERENCE
-    int32_t chunk_turns = 10;
+    int32_t chunk_turns = 5;
     int32_t grid = 0;       // 0: default_grid
-    int32_t waves = 3;      // waves per SIMD of the tracking kernel variant
+    int32_t waves = 0;      // waves per SIMD of the tracking kernel variant (0: default_waves)
     bmx::RunBuffers run;    // buffers of bmx_track_turns
     bmx::Group group{};     // host copy of the single group uploaded by bmx_track_turns
     bool profile = false;   // time every tracking launch of bmx_track_turns

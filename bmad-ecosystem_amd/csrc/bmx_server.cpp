@@ -274,9 +274,9 @@ int32_t bmx_server_create(bmx_ctx *ctx, const bmx_server_config *cfg, bmx_server
     S->base.loss_ele = (int32_t *)(p + 7 * nd + ni);
     S->base.loss_turn = (int32_t *)(p + 7 * nd + 2 * ni);
     S->free_list[0] = S->capacity;
-    S->cfg.chunk_turns = cfg->compact_turns ? cfg->compact_turns : 10;
+    S->cfg.chunk_turns = cfg->compact_turns ? cfg->compact_turns : 5;
     S->cfg.fast = ctx->fast_maps;
-    S->cfg.waves = ctx->waves;
+    S->cfg.waves = ctx->waves ? ctx->waves : default_waves(S->cfg.fast);
     S->cfg.grid = default_grid(S->device, S->cfg.fast, S->cfg.waves);
     S->worker = std::thread([S] { S->run(); });
     *out = S;

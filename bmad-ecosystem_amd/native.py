@@ -9,7 +9,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libbmx.so")
+# BMX_LIBRARY names another build of the library to load (same C ABI), else the in-tree one.
+LIB_PATH = os.environ.get("BMX_LIBRARY") or os.path.join(HERE, "libbmx.so")
 
 _lib = None
 

@@ -48,9 +48,10 @@ class Context:
         native.call("bmx_ctx_set_maps", self.h, {"fast": 1, "reference": 2}[form])
         self.maps = form
 
-    def set_run(self, chunk_turns: int = 10, grid: int = 0, waves: int = 0) -> None:
+    def set_run(self, chunk_turns: int = 5, grid: int = 0, waves: int = 0) -> None:
         """Turns per chunk of bmx_track_turns (0 = one chunk), persistent grid size (0 = device
-        occupancy limit) and waves per SIMD of the kernel variant (0 = default 3; 4 or 5)."""
+        occupancy limit) and waves per SIMD of the kernel variant (3, 4 or 5; 0 = default: 5 with
+        the fast maps, 3 with the reference maps)."""
         native.call("bmx_ctx_set_run", self.h, int(chunk_turns), int(grid), int(waves))
 
     def profile(self, on: bool = True) -> None:
@@ -224,7 +225,7 @@ class Batch:
     def track(self, n_turns: int, turn0: int = 0) -> None:
         native.call("bmx_batch_track", self.h, int(n_turns), int(turn0))
 
-    def set_run(self, chunk_turns: int = 10, grid: int = 0, waves: int = 0) -> None:
+    def set_run(self, chunk_turns: int = 5, grid: int = 0, waves: int = 0) -> None:
         native.call("bmx_batch_set_run", self.h, int(chunk_turns), int(grid), int(waves))
 
     def tune(self, sample_turns: int = 20) -> dict:

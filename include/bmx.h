@@ -168,9 +168,10 @@ int32_t bmx_ctx_take_work(bmx_ctx *ctx, int64_t *passes);
 #define BMX_MAPS_REFERENCE 2
 int32_t bmx_ctx_set_maps(bmx_ctx *ctx, int32_t form);
 /* Run shape of this context's bmx_track_turns calls: turns per chunk (alive particles are
- * compacted between chunks; 0 = the whole call is one chunk; default 10), workgroups of the
+ * compacted between chunks; 0 = the whole call is one chunk; default 5), workgroups of the
  * persistent tracking grid (0 = as many as fit on the device at once, the default) and the
- * waves per SIMD the tracking kernel variant is compiled for at least (0 = default 3; 4 or 5). */
+ * waves per SIMD the tracking kernel variant is compiled for at least (3, 4 or 5; 0 = the
+ * default: 5 with the fast maps, 3 with the reference maps). */
 int32_t bmx_ctx_set_run(bmx_ctx *ctx, int32_t chunk_turns, int32_t grid, int32_t waves);
 /* Kernel timing of bmx_track_turns: while on, every launch of the tracking kernel is bracketed
  * by HIP events on the context's stream; bmx_ctx_kernel_time synchronises and returns the sum
@@ -267,7 +268,7 @@ int32_t bmx_batch_tune(bmx_batch *batch, int32_t sample_turns, int32_t *chunk_tu
 typedef struct bmx_server_config {
     int64_t hbm_budget;     /* bytes of device memory for particle state (76 B per particle) */
     int32_t chunk_turns;    /* turns per scheduling chunk (0 = 100) */
-    int32_t compact_turns;  /* turns between alive-list compactions inside a chunk (0 = 10) */
+    int32_t compact_turns;  /* turns between alive-list compactions inside a chunk (0 = 5) */
 } bmx_server_config;
 typedef struct bmx_request_stats {
     int64_t submit_ns, admit_ns, done_ns;  /* steady clock, ns since the server was created */
