@@ -119,8 +119,8 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
 // tiles hold live particles only and waves stay full as the beam is lost. The order inside a
 // list depends on scheduling; each particle's result does not.
 //
-// Only the first w_eff waves take tiles, w_eff = ceil(n_tiles / ceil(n_tiles / waves)), so the
-// tiles divide evenly over the waves that run (no last partial round).
+// Every resident wave takes tiles (as many waves per SIMD as the register budget allows hide
+// each other's latencies better than fewer waves with evenly divided tiles).
 // chunk_off: turns already done by earlier chunks of this call; chunk_turns: turns per chunk.
 // W: waves per SIMD the register allocation targets at least (3: up to 168 VGPRs; 4: up to 128;
 // 5: up to 102, spilling a few values of the rf and bend paths to scratch).
@@ -134,10 +134,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))
     const uint32_t nt = *n_tiles_p;
     const int lane = threadIdx.x & 63;
     const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    const uint32_t waves = gridDim.x * (kBlock / 64);
-    const uint32_t rounds = (nt + waves - 1) / waves;
-    const uint32_t w_eff = rounds ? (nt + rounds - 1) / rounds : 0;
-    if (wave >= w_eff) return;  // whole wave
+    if (wave >= nt) return;  // whole wave
     unsigned long long passes = 0;
     for (;;) {
         uint32_t t = 0;
