@@ -162,7 +162,7 @@ def main():
             "bound": "mfma", "compute": "fp64 vector ALU", "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": profiled_traffic(),
-            "traffic_note": "HBM bytes of the first 10-turn launch over 1e6 particles, "
+            "traffic_note": "HBM bytes of the first 5-turn launch over 1e6 particles, "
                             "profiles/k_track_tiles_ring1_1e6_traffic.md",
             "flop_per_particle_element": flops,
             "kernel": "k_track_tiles", "kernel_ms_avg": kernel_ms / max(launches, 1),
