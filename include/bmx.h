@@ -248,7 +248,8 @@ int32_t bmx_batch_destroy(bmx_batch *batch);
 /* Every group n_turns turns, turn indices turn0 .. turn0+n_turns-1 (asynchronous). */
 int32_t bmx_batch_track(bmx_batch *batch, int32_t n_turns, int32_t turn0);
 /* Run shape of the batch: turns per chunk, persistent grid (0 = device limit) and the waves
- * per SIMD the tracking kernel is compiled for at least (0 = default 3; 4 or 5). */
+ * per SIMD the tracking kernel is compiled for at least (3, 4 or 5; 0 = the default: 5 with
+ * the fast maps, 3 with the reference maps). */
 int32_t bmx_batch_set_run(bmx_batch *batch, int32_t chunk_turns, int32_t grid, int32_t waves);
 /* Autotuner: times sample_turns turns of every candidate run shape (chunk length x kernel
  * occupancy x grid size) on a scratch copy of the batch's particles (the batch's beams are not
