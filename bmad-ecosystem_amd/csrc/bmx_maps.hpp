@@ -61,6 +61,13 @@ __device__ __forceinline__ CP launder(CP p)
     asm volatile("" : "+s"(p));
     return p;
 }
+// A wave-uniform integer passed through an empty asm at its point of use, so that its copy into
+// a vector register is made there (in a rare branch) and not hoisted into the common path.
+__device__ __forceinline__ int32_t uniform_here(int32_t v)
+{
+    asm volatile("" : "+s"(v));
+    return v;
+}
 
 // True on every lane if c holds on any active lane. Rare per-particle cases (a coordinate
 // outside a series domain, a loss) are gated by this wave-uniform test so the common path is
@@ -71,6 +78,12 @@ __device__ __forceinline__ CP launder(CP p)
 #endif
 #if BMX_GATE
 __device__ __forceinline__ bool any_lane(bool c) { return __ballot(c) != 0; }
+
+// The active lanes on which c holds. A compound rare condition is gated as the union of the
+// masks of its plain comparisons (what each comparison instruction writes), and re-tested per
+// lane only inside the gate, so the common path carries no per-lane form of it.
+typedef unsigned long long LaneMask;
+__device__ __forceinline__ LaneMask lanes(bool c) { return __ballot(c); }
 #else
 __device__ __forceinline__ bool any_lane(bool c) { return c; }
 #endif
@@ -119,6 +132,14 @@ __device__ __forceinline__ bool orbit_bad(const Particle &p)
     return (fabs(p.x) > kMaxAperture) | (fabs(p.y) > kMaxAperture) | (p.rel_p < 0) | (f_unstable > 0);
 }
 
+// The lanes on which orbit_bad holds.
+__device__ __forceinline__ LaneMask orbit_bad_lanes(const Particle &p)
+{
+    const double f_unstable = p.px * p.px + p.py * p.py - p.rel_p * p.rel_p;
+    return lanes(fabs(p.x) > kMaxAperture) | lanes(fabs(p.y) > kMaxAperture) |
+           lanes(p.rel_p < 0) | lanes(f_unstable > 0);
+}
+
 // True if the aperture test of check_aperture_limit.f90:256-324 loses the particle (no state
 // change): elliptical r >= 1 (alive is strictly r < 1), rectangular outside a set limit.
 __device__ __forceinline__ bool aperture_fails(const Particle &p, int32_t flags, CP ap)
@@ -129,6 +150,22 @@ __device__ __forceinline__ bool aperture_fails(const Particle &p, int32_t flags,
     }
     const double x1 = ap[0], x2 = ap[1], y1 = ap[2], y2 = ap[3];
     return (x1 != 0 && p.x < x1) | (x2 != 0 && p.x > x2) | (y1 != 0 && p.y < y1) | (y2 != 0 && p.y > y2);
+}
+
+// The lanes on which aperture_fails holds (the limits are wave-uniform).
+__device__ __forceinline__ LaneMask aperture_fail_lanes(const Particle &p, int32_t flags, CP ap)
+{
+    if (flags & F_APE_ELLIP) {
+        const double xr = (p.x - ap[0]) * ap[1], yr = (p.y - ap[2]) * ap[3];
+        return lanes(!(xr * xr + yr * yr < 1));
+    }
+    const double x1 = ap[0], x2 = ap[1], y1 = ap[2], y2 = ap[3];
+    LaneMask m = 0;
+    if (x1 != 0) m |= lanes(p.x < x1);
+    if (x2 != 0) m |= lanes(p.x > x2);
+    if (y1 != 0) m |= lanes(p.y < y1);
+    if (y2 != 0) m |= lanes(p.y > y2);
+    return m;
 }
 
 // Aperture test of check_aperture_limit.f90:256-324 (elliptical with strict r < 1, or
@@ -690,20 +727,19 @@ __device__ __forceinline__ void sbend_exact_fast(Particle &p, CP b)
         if (turned) xi = (rad - cos_plus) / g_p;
     }
     const double x_new = x * cos_a - b[22] + xi;
-    const bool lost_pz = fabs(px) > pt;
-    const bool lost_geo = (r < 0) | ((fabs(g_p) < 1e-5) & (fabs(cos_plus) < 1e-5));
-    const bool lost_x = fabs(x_new) > kMaxAperture;
-    if (any_lane(lost_pz | lost_geo | lost_x)) {
-        if (lost_pz) {
+    if (lanes(fabs(px) > pt) | lanes(r < 0) |
+        (lanes(fabs(g_p) < 1e-5) & lanes(fabs(cos_plus) < 1e-5)) |
+        lanes(fabs(x_new) > kMaxAperture)) {
+        if (fabs(px) > pt) {
             p.state = ST_LOST_PZ;
             return;
         }
-        if (lost_geo) {
+        if ((r < 0) | ((fabs(g_p) < 1e-5) & (fabs(cos_plus) < 1e-5))) {
             p.state = ST_LOST;
             p.x = 2 * kMaxAperture;
             return;
         }
-        if (lost_x) {
+        if (fabs(x_new) > kMaxAperture) {
             p.state = ST_LOST;
             p.x = x_new;
             return;
@@ -720,9 +756,8 @@ __device__ __forceinline__ void sbend_exact_fast(Particle &p, CP b)
     // L_p = L_c / sinc(h) = L_c h / sin h
     const double sh2 = sh * sh;
     double hs = asin_over_short(sh2);
-    const bool h_mid = !(sh2 <= kArcShortMax && ch > 0);
-    if (any_lane(h_mid)) {
-        if (h_mid) {
+    if (lanes(!(sh2 <= kArcShortMax)) | lanes(!(ch > 0))) {
+        if (!(sh2 <= kArcShortMax && ch > 0)) {
             if (fabs(sh) <= kArcSeriesMax && ch > 0) hs = asin_over(sh2);
             else hs = h_over_sin_far(sh, ch);
         }
@@ -742,20 +777,24 @@ __device__ __forceinline__ void sbend_exact_fast(Particle &p, CP b)
 // particle; the test is branch-free, the state assignment gated.
 __device__ __forceinline__ bool orbit_check(Particle &p)
 {
-    const bool bad = orbit_bad(p);
-    if (any_lane(bad)) {
-        if (bad) orbit_too_large(p);
+    if (orbit_bad_lanes(p)) {
+        // only live particles enter an element, so a lane not alive now was flagged here
+        if (orbit_bad(p)) orbit_too_large(p);
+        return p.state != ST_ALIVE;
     }
-    return bad;
+    return false;
 }
 
 // bmad/low_level/track_a_bend.f90:18-333: exact_multipoles = off, no body multipoles, b1 = 0,
 // n_step = 1, no misalignment. Hwang entrance kick, orbit_too_large (:54), body (exact
 // :163-209, semi-linear near the zero orbit :136-159, drift when g = dg = 0 :117-118),
 // orbit_too_large (:319), Hwang exit kick.
-template <bool F>
+// SB: both edges have the simple Hwang kick and the body is not a drift (the flags are then
+// known at compile time, so the edge code has no run-time branches to merge).
+template <bool F, bool SB>
 __device__ __forceinline__ void sbend(Particle &p, int32_t flags, CP b, const Species &sp)
 {
+    if constexpr (SB) flags = F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE;
     const double L = b[0], g = b[1], g_tot = b[2];
     const bool simple = flags & F_EDGE_SIMPLE;
     if (flags & F_EDGE_ENT) hwang_edge(p, b + 10, g_tot, true, simple);
@@ -763,9 +802,11 @@ __device__ __forceinline__ void sbend(Particle &p, int32_t flags, CP b, const Sp
     if (flags & F_BEND_DRIFT) {
         drift<F>(p, L);
     } else {
-        const bool semi = (fabs(p.x * g) < 1e-9) & (fabs(p.px) < 1e-9) & (fabs(p.py) < 1e-9) &
-                          (fabs(p.pz) < 1e-9);
-        if (any_lane(semi)) {
+        bool semi = false;
+        if (lanes(fabs(p.x * g) < 1e-9) & lanes(fabs(p.px) < 1e-9) & lanes(fabs(p.py) < 1e-9) &
+            lanes(fabs(p.pz) < 1e-9)) {
+            semi = (fabs(p.x * g) < 1e-9) & (fabs(p.px) < 1e-9) & (fabs(p.py) < 1e-9) &
+                   (fabs(p.pz) < 1e-9);
             if (semi) sbend_semilinear(p, b, sp);
         }
         if (!semi) {
@@ -916,8 +957,8 @@ __device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, con
         if (any_lane(out)) {
             if (out) {
                 aperture(p, flags, ap);
-                p.loss_ele = ix1;
-                p.loss_turn = turn;
+                p.loss_ele = uniform_here(ix1);
+                p.loss_turn = uniform_here(turn);
                 return true;
             }
         }
@@ -929,7 +970,13 @@ __device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, con
         case K_DRIFT: drift<F>(p, q[0]); break;
         case K_QUAD: quadrupole<F>(p, q, n, sp); break;
         case K_MULT: thick_multipole<F>(p, q, n, (flags >> F_ORDER_SHIFT) & 0xF); break;
-        case K_BEND: sbend<F>(p, flags, q, sp); break;
+        case K_BEND:
+            if ((flags & (F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE | F_BEND_DRIFT)) ==
+                (F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE))
+                sbend<F, true>(p, flags, q, sp);
+            else
+                sbend<F, false>(p, flags, q, sp);
+            break;
         case K_RF:
             if constexpr (MAPS == MAPS_ALL) rfcavity<F>(p, q, n, sp);
             break;
@@ -939,16 +986,17 @@ __device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, con
     // orbit_too_large (overwrites any state), else the exit aperture for live particles; both
     // tests first without branches, the state codes and the loss record only on the (rare)
     // lanes that are lost here or were lost inside the map
-    const bool bad = orbit_bad(p);
-    const bool alive = p.state == ST_ALIVE;
-    const bool ape = (flags & F_APE_EXIT) ? aperture_fails(p, flags, ap) : false;
-    const bool hit = bad | ape | !alive;
-    if (any_lane(hit)) {
-        if (hit) {
+    LaneMask hit_lanes = orbit_bad_lanes(p) | lanes(p.state != ST_ALIVE);
+    if (flags & F_APE_EXIT) hit_lanes |= aperture_fail_lanes(p, flags, ap);
+    if (hit_lanes) {
+        const bool bad = orbit_bad(p);
+        const bool alive = p.state == ST_ALIVE;
+        const bool ape = (flags & F_APE_EXIT) ? aperture_fails(p, flags, ap) : false;
+        if (bad | ape | !alive) {
             if (bad) orbit_too_large(p);
             else if (alive) aperture(p, flags, ap);
-            p.loss_ele = ix1;
-            p.loss_turn = turn;
+            p.loss_ele = uniform_here(ix1);
+            p.loss_turn = uniform_here(turn);
         }
     }
     return false;
