@@ -242,7 +242,8 @@ __device__ __forceinline__ void drift_series(Particle &p, double L, double px_re
 template <bool F>
 __device__ __forceinline__ void drift(Particle &p, double L)
 {
-    if (L == 0) return;
+    // L == 0 (either sign) tested on the wave-uniform bit pattern with scalar instructions
+    if ((__double_as_longlong(L) << 1) == 0) return;
     if (!F) {
         drift_exact(p, L);
         return;
@@ -292,6 +293,37 @@ __device__ __forceinline__ Plane quad_plane(double k, double L, double rel_p, do
 // Domain of the quadrupole's 5-term plane-function series: t = (k1 L^2/(1+pz))^2 <= 1.
 constexpr double kQuadTLim = 1.0;
 
+// The four plane functions of the fast quadrupole form for |t| <= kQuadTLim, t = W^2:
+// C(-+W) = A -+ W B and S(-+W) = A1 -+ W B1 with A = sum t^m/(4m)!, A1 = sum t^m/(4m+1)!,
+// B = sum t^m/(4m+2)!, B1 = sum t^m/(4m+3)!, m = 0..4 (the first omitted term t^5/20! is below
+// 4.2e-19 for t <= 1), in power form with the coefficients as multiplicands (scalar operands
+// of the FMAs); B1 is summed as 6 B1 so that every sum starts from an inline constant.
+struct QuadFns {
+    double cxx, cxy, sxx, sxy;
+};
+__device__ __forceinline__ QuadFns quad_series(double W, double t)
+{
+    const double t2 = t * t, t3 = t2 * t, t4 = t2 * t2;
+    double A = fma(t, 0.041666666666666664, 1.0);     // 1/4!
+    A = fma(2.48015873015873e-05, t2, A);              // 1/8!
+    A = fma(2.08767569878681e-09, t3, A);              // 1/12!
+    A = fma(4.779477332387385e-14, t4, A);             // 1/16!
+    double A1 = fma(t, 0.008333333333333333, 1.0);    // 1/5!
+    A1 = fma(2.7557319223985893e-06, t2, A1);          // 1/9!
+    A1 = fma(1.6059043836821613e-10, t3, A1);          // 1/13!
+    A1 = fma(2.8114572543455206e-15, t4, A1);          // 1/17!
+    double B = fma(t, 0.001388888888888889, 0.5);     // 1/6!, 1/2!
+    B = fma(2.755731922398589e-07, t2, B);             // 1/10!
+    B = fma(1.1470745597729725e-11, t3, B);            // 1/14!
+    B = fma(1.5619206968586225e-16, t4, B);            // 1/18!
+    double B1 = fma(t, 0.0011904761904761906, 1.0);   // 6/7!
+    B1 = fma(1.5031265031265032e-07, t2, B1);          // 6/11!
+    B1 = fma(4.58829823909189e-12, t3, B1);            // 6/15!
+    B1 = fma(4.932381147974598e-17, t4, B1);           // 6/19!
+    B1 = B1 * 0.16666666666666666;                     // 1/3!
+    return {fma(-W, B, A), fma(W, B, A), fma(-W, B1, A1), fma(W, B1, A1)};
+}
+
 // bmad/low_level/track_a_quadrupole.f90:17-148 for n_step = 1 (no multipoles other than k1),
 // no fringe, no misalignment; low_energy_z_correction.f90:22-58.
 // Fast form: with W = k1 L^2/(1+pz) the plane functions are C(-+W), L S(-+W) where
@@ -310,50 +342,28 @@ __device__ __forceinline__ void quadrupole(Particle &p, CP q, int32_t n_terms, c
         double t = W * W;
         // Argument quartering for large |W| (far off-momentum particles or strong quads):
         // C(16t) = 2 C(t)^2 - 1, S(16t) = S(t) C(t) in t = W^2, the double-angle identities of
-        // cos/cosh and sin/sinh.
-        int halvings = 0;
-        const bool big = t > kQuadTLim;
-        if (any_lane(big)) {
-            if (big) {
+        // cos/cosh and sin/sinh. Gated per wave; the common path is the series alone.
+        QuadFns f;
+        if (any_lane(t > kQuadTLim)) {
+            int halvings = 0;
+            if (t > kQuadTLim) {
                 do {
                     W *= 0.25;
                     t *= 0.0625;
                     halvings++;
                 } while (t > kQuadTLim && halvings < 64);
             }
-        }
-        // A = sum t^m/(4m)!, A1 = sum t^m/(4m+1)!, B = sum t^m/(4m+2)!, B1 = sum t^m/(4m+3)!,
-        // m = 0..4 (the first omitted term t^5/20! is below 4.2e-19 for t <= 1), in power form
-        // with the coefficients as multiplicands (scalar operands of the FMAs); B1 is summed as
-        // 6 B1 so that every sum starts from an inline constant (1 or 1/2).
-        const double t2 = t * t, t3 = t2 * t, t4 = t2 * t2;
-        double A = fma(t, 0.041666666666666664, 1.0);     // 1/4!
-        A = fma(2.48015873015873e-05, t2, A);              // 1/8!
-        A = fma(2.08767569878681e-09, t3, A);              // 1/12!
-        A = fma(4.779477332387385e-14, t4, A);             // 1/16!
-        double A1 = fma(t, 0.008333333333333333, 1.0);    // 1/5!
-        A1 = fma(2.7557319223985893e-06, t2, A1);          // 1/9!
-        A1 = fma(1.6059043836821613e-10, t3, A1);          // 1/13!
-        A1 = fma(2.8114572543455206e-15, t4, A1);          // 1/17!
-        double B = fma(t, 0.001388888888888889, 0.5);     // 1/6!, 1/2!
-        B = fma(2.755731922398589e-07, t2, B);             // 1/10!
-        B = fma(1.1470745597729725e-11, t3, B);            // 1/14!
-        B = fma(1.5619206968586225e-16, t4, B);            // 1/18!
-        double B1 = fma(t, 0.0011904761904761906, 1.0);   // 6/7!
-        B1 = fma(1.5031265031265032e-07, t2, B1);          // 6/11!
-        B1 = fma(4.58829823909189e-12, t3, B1);          // 6/15!
-        B1 = fma(4.932381147974598e-17, t4, B1);           // 6/19!
-        B1 = B1 * 0.16666666666666666;                     // 1/3!
-        double cxx = fma(-W, B, A), cxy = fma(W, B, A);
-        double sxx = fma(-W, B1, A1), sxy = fma(W, B1, A1);
-        if (any_lane(halvings > 0)) {
+            f = quad_series(W, t);
             for (int h = 0; h < halvings; h++) {
-                sxx = sxx * cxx;
-                sxy = sxy * cxy;
-                cxx = fma(2 * cxx, cxx, -1.0);
-                cxy = fma(2 * cxy, cxy, -1.0);
+                f.sxx = f.sxx * f.cxx;
+                f.sxy = f.sxy * f.cxy;
+                f.cxx = fma(2 * f.cxx, f.cxx, -1.0);
+                f.cxy = fma(2 * f.cxy, f.cxy, -1.0);
             }
+        } else {
+            f = quad_series(W, t);
         }
+        const double cxx = f.cxx, cxy = f.cxy, sxx = f.sxx, sxy = f.sxy;
         cx_x = cxx;
         cx_y = cxy;
         sx_x = L * sxx;
