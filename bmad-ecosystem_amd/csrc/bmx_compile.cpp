@@ -160,9 +160,13 @@ bool aperture_setup(const bmx_ele &e, int32_t &flags, double ap[4], std::string 
         ap[2] = yc;
         ap[3] = 1 / yw;
         f |= F_APE_ELLIP;
+        if (std::fabs(xc) + xw < 0.5 * kMaxAp && std::fabs(yc) + yw < 0.5 * kMaxAp) f |= F_APE_EXIT_COVERS;
     } else if (e.aperture_type == BMX_APERTURE_RECTANGULAR || e.aperture_type == 1 /* auto */ ||
                e.aperture_type == 0) {
         if (x1 == 0 && x2 == 0 && y1 == 0 && y2 == 0) return true;
+        if (x1 != 0 && x2 != 0 && y1 != 0 && y2 != 0 && std::fabs(x1) < 0.5 * kMaxAp &&
+            std::fabs(x2) < 0.5 * kMaxAp && std::fabs(y1) < 0.5 * kMaxAp && std::fabs(y2) < 0.5 * kMaxAp)
+            f |= F_APE_EXIT_COVERS;
         ap[0] = x1;
         ap[1] = x2;
         ap[2] = y1;
@@ -173,6 +177,7 @@ bool aperture_setup(const bmx_ele &e, int32_t &flags, double ap[4], std::string 
     }
     if (at == BMX_ENTRANCE_END || at == BMX_BOTH_ENDS) f |= F_APE_ENT;
     if (at == BMX_EXIT_END || at == BMX_BOTH_ENDS) f |= F_APE_EXIT;
+    if (!(f & F_APE_EXIT)) f &= ~F_APE_EXIT_COVERS;
     flags |= f;
     return true;
 }

@@ -996,7 +996,14 @@ __device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, con
     // orbit_too_large (overwrites any state), else the exit aperture for live particles; both
     // tests first without branches, the state codes and the loss record only on the (rare)
     // lanes that are lost here or were lost inside the map
-    LaneMask hit_lanes = orbit_bad_lanes(p) | lanes(p.state != ST_ALIVE);
+    // The gate leaves out what cannot newly hold here: rel_p < 0 outside rf cavities (pz only
+    // changes in them, and a particle enters with rel_p >= 0), and |x| or |y| above the limit
+    // when the exit aperture fails for those anyway (F_APE_EXIT_COVERS).
+    const double f_unstable = p.px * p.px + p.py * p.py - p.rel_p * p.rel_p;
+    LaneMask hit_lanes = lanes(f_unstable > 0) | lanes(p.state != ST_ALIVE);
+    if constexpr (MAPS != MAPS_NO_RF) hit_lanes |= lanes(p.rel_p < 0);
+    if (!(flags & F_APE_EXIT_COVERS))
+        hit_lanes |= lanes(fabs(p.x) > kMaxAperture) | lanes(fabs(p.y) > kMaxAperture);
     if (flags & F_APE_EXIT) hit_lanes |= aperture_fail_lanes(p, flags, ap);
     if (hit_lanes) {
         const bool bad = orbit_bad(p);

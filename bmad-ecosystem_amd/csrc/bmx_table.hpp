@@ -34,6 +34,9 @@ enum Flags : int32_t {
     // same coordinates): a particle arriving from that element has passed it already. Set with
     // F_APE_ENT cleared; kernels that start at this element make the test.
     F_APE_ENT_SAME = 1 << 12,
+    // The exit aperture fails for every |x| or |y| above orbit_too_large's limit (its limits all
+    // lie well inside it), so that part of the exit validity test need not be gated separately.
+    F_APE_EXIT_COVERS = 1 << 13,
 };
 
 struct EleHdr {
