@@ -268,58 +268,55 @@ struct Plane {
     double m11, m12, m21, m22, z1, z2, z3;
 };
 
-__device__ __forceinline__ Plane quad_plane(double k, double L, double rel_p, double inv_rp,
-                                            double cx, double sx, bool F)
+__device__ __forceinline__ Plane quad_plane(double k, double L, double rel_p, double cx, double sx)
 {
     Plane q;
     q.m11 = cx;
     q.m22 = cx;
-    if (F) {
-        q.m12 = sx * inv_rp;
-        q.m21 = k * sx * rel_p;
-        q.z1 = k * (L - cx * sx) * 0.25;
-        q.z2 = -k * sx * sx * inv_rp * 0.5;
-        q.z3 = -(cx * sx + L) * (inv_rp * inv_rp) * 0.25;
-    } else {
-        q.m12 = sx / rel_p;
-        q.m21 = k * sx * rel_p;
-        q.z1 = k * (-cx * sx + L) / 4;
-        q.z2 = -k * sx * sx / (2 * rel_p);
-        q.z3 = -(cx * sx + L) / (4 * rel_p * rel_p);
-    }
+    q.m12 = sx / rel_p;
+    q.m21 = k * sx * rel_p;
+    q.z1 = k * (-cx * sx + L) / 4;
+    q.z2 = -k * sx * sx / (2 * rel_p);
+    q.z3 = -(cx * sx + L) / (4 * rel_p * rel_p);
     return q;
 }
 
 // Domain of the quadrupole's 5-term plane-function series: t = (k1 L^2/(1+pz))^2 <= 1.
 constexpr double kQuadTLim = 1.0;
 
-// The four plane functions of the fast quadrupole form for |t| <= kQuadTLim, t = W^2:
-// C(-+W) = A -+ W B and S(-+W) = A1 -+ W B1 with A = sum t^m/(4m)!, A1 = sum t^m/(4m+1)!,
-// B = sum t^m/(4m+2)!, B1 = sum t^m/(4m+3)!, m = 0..4 (the first omitted term t^5/20! is below
-// 4.2e-19 for t <= 1), in power form with the coefficients as multiplicands (scalar operands
-// of the FMAs); B1 is summed as 6 B1 so that every sum starts from an inline constant.
+// The four plane functions of the fast quadrupole form, t = W^2: C(-+W) = A -+ W B and
+// S(-+W) = A1 -+ W B1 with A = sum t^m/(4m)!, A1 = sum t^m/(4m+1)!, B = sum t^m/(4m+2)!,
+// B1 = sum t^m/(4m+3)!, m = 0..N-1, in power form with the coefficients as multiplicands
+// (scalar operands of the FMAs); B1 is summed as 6 B1 so that every sum starts from an inline
+// constant. N = 5 for t <= kQuadTLim (the first omitted term t^5/20! is below 4.2e-19), N = 4
+// for t <= kQuadShortMax (t^4/16! below 7.3e-19).
+constexpr double kQuadShortMax = 1.0 / 16;
 struct QuadFns {
     double cxx, cxy, sxx, sxy;
 };
+template <int N>
 __device__ __forceinline__ QuadFns quad_series(double W, double t)
 {
-    const double t2 = t * t, t3 = t2 * t, t4 = t2 * t2;
+    const double t2 = t * t, t3 = t2 * t;
     double A = fma(t, 0.041666666666666664, 1.0);     // 1/4!
     A = fma(2.48015873015873e-05, t2, A);              // 1/8!
     A = fma(2.08767569878681e-09, t3, A);              // 1/12!
-    A = fma(4.779477332387385e-14, t4, A);             // 1/16!
     double A1 = fma(t, 0.008333333333333333, 1.0);    // 1/5!
     A1 = fma(2.7557319223985893e-06, t2, A1);          // 1/9!
     A1 = fma(1.6059043836821613e-10, t3, A1);          // 1/13!
-    A1 = fma(2.8114572543455206e-15, t4, A1);          // 1/17!
     double B = fma(t, 0.001388888888888889, 0.5);     // 1/6!, 1/2!
     B = fma(2.755731922398589e-07, t2, B);             // 1/10!
     B = fma(1.1470745597729725e-11, t3, B);            // 1/14!
-    B = fma(1.5619206968586225e-16, t4, B);            // 1/18!
     double B1 = fma(t, 0.0011904761904761906, 1.0);   // 6/7!
     B1 = fma(1.5031265031265032e-07, t2, B1);          // 6/11!
     B1 = fma(4.58829823909189e-12, t3, B1);            // 6/15!
-    B1 = fma(4.932381147974598e-17, t4, B1);           // 6/19!
+    if constexpr (N == 5) {
+        const double t4 = t2 * t2;
+        A = fma(4.779477332387385e-14, t4, A);         // 1/16!
+        A1 = fma(2.8114572543455206e-15, t4, A1);      // 1/17!
+        B = fma(1.5619206968586225e-16, t4, B);        // 1/18!
+        B1 = fma(4.932381147974598e-17, t4, B1);       // 6/19!
+    }
     B1 = B1 * 0.16666666666666666;                     // 1/3!
     return {fma(-W, B, A), fma(W, B, A), fma(-W, B1, A1), fma(W, B1, A1)};
 }
@@ -340,28 +337,32 @@ __device__ __forceinline__ void quadrupole(Particle &p, CP q, int32_t n_terms, c
         k1 = q[3] * p.inv_rp;                 // b1/L / (1 + pz)
         double W = q[2] * p.inv_rp;           // b1 L / (1 + pz)
         double t = W * W;
-        // Argument quartering for large |W| (far off-momentum particles or strong quads):
+        // The 4-term series for t <= kQuadShortMax, else the 5-term one, with argument
+        // quartering for t > kQuadTLim (far off-momentum particles or strong quads):
         // C(16t) = 2 C(t)^2 - 1, S(16t) = S(t) C(t) in t = W^2, the double-angle identities of
-        // cos/cosh and sin/sinh. Gated per wave; the common path is the series alone.
+        // cos/cosh and sin/sinh. Chosen per lane, gated per wave: the common path is the short
+        // series alone.
         QuadFns f;
-        if (any_lane(t > kQuadTLim)) {
-            int halvings = 0;
-            if (t > kQuadTLim) {
-                do {
+        if (any_lane(t > kQuadShortMax)) {
+            if (t > kQuadShortMax) {
+                int halvings = 0;
+                while (t > kQuadTLim && halvings < 64) {
                     W *= 0.25;
                     t *= 0.0625;
                     halvings++;
-                } while (t > kQuadTLim && halvings < 64);
-            }
-            f = quad_series(W, t);
-            for (int h = 0; h < halvings; h++) {
-                f.sxx = f.sxx * f.cxx;
-                f.sxy = f.sxy * f.cxy;
-                f.cxx = fma(2 * f.cxx, f.cxx, -1.0);
-                f.cxy = fma(2 * f.cxy, f.cxy, -1.0);
+                }
+                f = quad_series<5>(W, t);
+                for (int h = 0; h < halvings; h++) {
+                    f.sxx = f.sxx * f.cxx;
+                    f.sxy = f.sxy * f.cxy;
+                    f.cxx = fma(2 * f.cxx, f.cxx, -1.0);
+                    f.cxy = fma(2 * f.cxy, f.cxy, -1.0);
+                }
+            } else {
+                f = quad_series<4>(W, t);
             }
         } else {
-            f = quad_series(W, t);
+            f = quad_series<4>(W, t);
         }
         const double cxx = f.cxx, cxy = f.cxy, sxx = f.sxx, sxy = f.sxy;
         cx_x = cxx;
@@ -390,18 +391,34 @@ __device__ __forceinline__ void quadrupole(Particle &p, CP q, int32_t n_terms, c
             }
         }
     }
-    const Plane a = quad_plane(-k1, L, p.rel_p, p.inv_rp, cx_x, sx_x, F);
-    const Plane b = quad_plane(k1, L, p.rel_p, p.inv_rp, cx_y, sx_y, F);
-    const double x = p.x, px = p.px, y = p.y, py = p.py;
-    p.z = p.z + (a.z1 * x * x + a.z2 * x * px + a.z3 * px * px + b.z1 * y * y + b.z2 * y * py +
-                 b.z3 * py * py);
-    p.x = a.m11 * x + a.m12 * px;
-    p.px = a.m21 * x + a.m22 * px;
-    p.y = b.m11 * y + b.m12 * py;
-    p.py = b.m21 * y + b.m22 * py;
-    if (F) {
-        p.z = p.z + L * p.lez;
+    if constexpr (F) {
+        // Per plane (strength k, C = cx, S = sx) with u = p/(1+pz): x' = C x + S u,
+        // p' = k S (1+pz) x + C p, dz = [k (L - C S) x^2 - 2 k S^2 x u - (C S + L) u^2] / 4 --
+        // quad_mat2_calc.f90:56-61 and the update of track_a_quadrupole.f90:104-112 with the
+        // (1+pz) factors of m12, z2 and z3 folded into u.
+        const double x = p.x, px = p.px, y = p.y, py = p.py;
+        const double ux = px * p.inv_rp, uy = py * p.inv_rp;
+        const double wx = sx_x * ux, wy = sx_y * uy;
+        const double kxx = -k1 * x, kyy = k1 * y;
+        const double dzx = fma(kxx, fma(-2 * sx_x, wx, fma(-cx_x, sx_x, L) * x),
+                               -(fma(cx_x, sx_x, L) * ux) * ux);
+        const double dzy = fma(kyy, fma(-2 * sx_y, wy, fma(-cx_y, sx_y, L) * y),
+                               -(fma(cx_y, sx_y, L) * uy) * uy);
+        p.x = fma(cx_x, x, wx);
+        p.px = fma(kxx, sx_x * p.rel_p, cx_x * px);
+        p.y = fma(cx_y, y, wy);
+        p.py = fma(kyy, sx_y * p.rel_p, cx_y * py);
+        p.z = p.z + fma(0.25, dzx + dzy, L * p.lez);
     } else {
+        const Plane a = quad_plane(-k1, L, p.rel_p, cx_x, sx_x);
+        const Plane b = quad_plane(k1, L, p.rel_p, cx_y, sx_y);
+        const double x = p.x, px = p.px, y = p.y, py = p.py;
+        p.z = p.z + (a.z1 * x * x + a.z2 * x * px + a.z3 * px * px + b.z1 * y * y + b.z2 * y * py +
+                     b.z3 * py * py);
+        p.x = a.m11 * x + a.m12 * px;
+        p.px = a.m21 * x + a.m22 * px;
+        p.y = b.m11 * y + b.m12 * py;
+        p.py = b.m21 * y + b.m22 * py;
         const double beta0 = sp.beta0, me2 = sp.me2, f = sp.lez_f, thr = sp.lez_thr;
         const double pz = p.pz;
         if (sp.mc2 * sqr(beta0 * pz) < thr)
