@@ -349,6 +349,9 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             edge_constants(e, false, &q[16]);
             q[22] = L * L * g * q[7];
             q[23] = L / q[8];
+            q[24] = L * q[6];
+            q[25] = 0.5 * q[12];
+            q[26] = 0.5 * q[18];
             if (std::fabs(angle) <= 0.1) h.flags |= F_BEND_SERIES;
             break;
         }

@@ -499,22 +499,27 @@ __device__ __forceinline__ void thick_multipole(Particle &p, CP m,
 
 // bmad/modules/fringe_mod.f90:241-419 hwang_bend_edge_kick (time_dir = +1), orbit only.
 // e[] = gt, gt2, gs2, k1_tane, fg_factor, tan_e^2 of that face; g_tot of the bend.
-__device__ __forceinline__ void hwang_edge(Particle &p, CP e, double g_tot,
+// half_gs2 = gs2/2 of that face.
+__device__ __forceinline__ void hwang_edge(Particle &p, CP e, double half_gs2, double g_tot,
                                            bool entering, bool simple)
 {
     const double ef = p.inv_rp;
     const double x = p.x, px = p.px, y = p.y, py = p.py;
     if (simple) {
-        // e = 0, fint*hgap = 0, k1 = 0: gt = gt2 = k1_tane = fg = 0, gs2 = g_tot.
-        const double gs2 = e[2];
+        // e = 0, fint*hgap = 0, k1 = 0: gt = gt2 = k1_tane = fg = 0, gs2 = g_tot:
+        // x += gs2 y^2 ef/2, py -= g_tot px y ef, z -= gs2 px y^2 ef^2/2 (signs flipped at the
+        // exit face)
+        const double yy = y * y;
+        const double a = half_gs2 * ef;
+        const double gef = g_tot * ef;
         if (entering) {
-            p.x = x + (gs2 * y * y) * ef / 2;
-            p.py = py + (-(g_tot)*px * y) * ef;
-            p.z = p.z + ef * ef * 0.5 * (-(px * y * y * gs2));
+            p.x = fma(a, yy, x);
+            p.py = fma(-gef, px * y, py);
+            p.z = fma(-(a * ef), px * yy, p.z);
         } else {
-            p.x = x + (-gs2 * y * y) * ef / 2;
-            p.py = py + ((g_tot)*px * y) * ef;
-            p.z = p.z + ef * ef * 0.5 * (px * y * y * gs2);
+            p.x = fma(-a, yy, x);
+            p.py = fma(gef, px * y, py);
+            p.z = fma(a * ef, px * yy, p.z);
         }
         return;
     }
@@ -619,6 +624,17 @@ __device__ __forceinline__ double rsqrt_pos(double v)
     const double y = __builtin_amdgcn_rsq(v);
     const double e = fma(-v * y, y, 1.0);
     return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
+// a / d for a normal d: the hardware reciprocal estimate refined by two Newton steps, the
+// quotient corrected once with its residual (within an ulp of the rounded quotient).
+__device__ __forceinline__ double div_normal(double a, double d)
+{
+    double y = __builtin_amdgcn_rcp(d);
+    y = fma(fma(-d, y, 1.0), y, y);
+    y = fma(fma(-d, y, 1.0), y, y);
+    const double q = a * y;
+    return fma(fma(-d, q, a), y, q);
 }
 
 // sqrt(v) for normal v > 0: s = v rsqrt(v) with one Newton correction of the residual v - s^2.
@@ -743,12 +759,12 @@ __device__ __forceinline__ void sbend_exact_fast(Particle &p, CP b)
     }
     const double sin_plus = fma(sin_a, c1, cos_a * s1);
     const double cos_plus = fma(cos_a, c1, -sin_a * s1);
-    const double u = (1 + g * x) * (L * sinc_a);
-    const double alpha = 2 * sin_plus * u - g_p * (u * u);
+    const double u = fma(g, x, 1.0) * b[24];                 // (1 + g x) L sinc_a
+    const double alpha = u * fma(-g_p, u, 2 * sin_plus);     // 2 sin_plus u - g_p u^2
     const double r = fma(cos_plus, cos_plus, g_p * alpha);
     // r <= 0 loses the particle below; the floor keeps the square root's argument normal
     const double rad = sqrt_pos(fmax(r, 0x1p-1022));
-    double xi = alpha / (rad + cos_plus);
+    double xi = div_normal(alpha, rad + cos_plus);
     const bool turned = !(cos_plus > 0);
     if (any_lane(turned)) {
         if (turned) xi = (rad - cos_plus) / g_p;
@@ -773,7 +789,7 @@ __device__ __forceinline__ void sbend_exact_fast(Particle &p, CP b)
         }
     }
     const double L_u = xi;
-    const double mL_v = L * sinc_a + x * sin_a;  // -L_v
+    const double mL_v = fma(x, sin_a, b[24]);  // -L_v = L sinc_a + x sin_a
     const double lc2 = mL_v * mL_v + L_u * L_u;
     const double inv_lc = rsqrt_pos(lc2);
     const double L_c = lc2 * inv_lc;
@@ -824,7 +840,7 @@ __device__ __forceinline__ void sbend(Particle &p, int32_t flags, CP b, const Sp
     if constexpr (SB) flags = F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE;
     const double L = b[0], g = b[1], g_tot = b[2];
     const bool simple = flags & F_EDGE_SIMPLE;
-    if (flags & F_EDGE_ENT) hwang_edge(p, b + 10, g_tot, true, simple);
+    if (flags & F_EDGE_ENT) hwang_edge(p, b + 10, b[25], g_tot, true, simple);
     if (orbit_check(p)) return;
     if (flags & F_BEND_DRIFT) {
         drift<F>(p, L);
@@ -843,7 +859,7 @@ __device__ __forceinline__ void sbend(Particle &p, int32_t flags, CP b, const Sp
         if (p.state != ST_ALIVE) return;
     }
     if (orbit_check(p)) return;
-    if (flags & F_EDGE_EXIT) hwang_edge(p, b + 16, g_tot, false, simple);
+    if (flags & F_EDGE_EXIT) hwang_edge(p, b + 16, b[26], g_tot, false, simple);
 }
 
 // ---- rf cavity ---------------------------------------------------------------------------

@@ -54,14 +54,15 @@ struct EleHdr {
 //  K_BEND  : [0] L, [1] g, [2] g_tot, [3] angle = g L, [4] cos_a, [5] sin_a, [6] sinc_a,
 //            [7] cosc_a, [8] beta_ref, [9] sincc_a,
 //            [10..15] entrance gt, gt2, gs2, k1_tane, fg_factor, tan_e^2, [16..21] exit,
-//            [22] L^2 g cosc_a, [23] L / beta_ref
+//            [22] L^2 g cosc_a, [23] L / beta_ref, [24] L sinc_a, [25] gs2/2 entrance,
+//            [26] gs2/2 exit
 //  K_RF    : [0] L/n_slice, [1] voltage * charge_dir, [2] phi0, [3] rf_frequency,
 //            [4] 0.5/rf_frequency, [5] twopi * rf_frequency, [6] twopi * phi0_multipass
 //  K_MARKER: (none)
 // preceded, when F_APE_ENT|F_APE_EXIT, by 4 aperture values (hdr.off points at them):
 //  elliptical : x_center, 1/x_width2, y_center, 1/y_width2
 //  rectangular: x1_lim (= -x1_limit), x2_lim, y1_lim, y2_lim (0 = no limit)
-constexpr int kParams[K_NKIND] = {1, 5, 9, 24, 7, 0};
+constexpr int kParams[K_NKIND] = {1, 5, 9, 27, 7, 0};
 
 struct Species {
     double mc2;      // [eV]
