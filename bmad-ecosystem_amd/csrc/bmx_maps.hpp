@@ -242,8 +242,11 @@ __device__ __forceinline__ void drift_series(Particle &p, double L, double px_re
 template <bool F>
 __device__ __forceinline__ void drift(Particle &p, double L)
 {
-    // L == 0 (either sign) tested on the wave-uniform bit pattern with scalar instructions
-    if ((__double_as_longlong(L) << 1) == 0) return;
+    // L == 0 (either sign) tested on the wave-uniform bit pattern with scalar instructions (the
+    // empty asm keeps the compiler from turning the test back into a vector compare)
+    unsigned long long bits = __double_as_longlong(L);
+    asm volatile("" : "+s"(bits));
+    if ((bits << 1) == 0) return;
     if (!F) {
         drift_exact(p, L);
         return;
