@@ -284,6 +284,7 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
                 for (int n = 0; n < 4; n++) q.push_back(half * b[n]);
                 h.n = n_step;
                 h.flags |= mask << F_ORDER_SHIFT;
+                if (a[0] == 0 && a[1] == 0 && a[2] == 0 && a[3] == 0) h.flags |= F_MULT_NORMAL;
             }
             break;
         }

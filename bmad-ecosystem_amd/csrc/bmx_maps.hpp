@@ -465,30 +465,45 @@ __device__ __forceinline__ void multipole_kick(Particle &p, CP ah, CP bh, int32_
     p.py = fma(factor, ky, p.py);
 }
 
-// The kick of a pure sextupole (mask = order 2 only) with its strengths held in registers.
+// The kick of a pure sextupole (mask = order 2 only) with its strengths held in registers;
+// NORMAL: no skew strength (a = 0).
+template <bool NORMAL>
 __device__ __forceinline__ void sextupole_kick(Particle &p, double a, double b, double factor)
 {
     const double x = p.x, y = p.y;
     const double re = x * x - y * y, im = x * y + y * x;
-    p.px = fma(factor, -b * re + a * im, p.px);
-    p.py = fma(factor, a * re + b * im, p.py);
+    if constexpr (NORMAL) {
+        p.px = fma(factor, -b * re, p.px);
+        p.py = fma(factor, b * im, p.py);
+    } else {
+        p.px = fma(factor, -b * re + a * im, p.px);
+        p.py = fma(factor, a * re + b * im, p.py);
+    }
 }
 
 // bmad/low_level/track_a_thick_multipole.f90:18-133 (magnetic, no misalignment):
 // half kick, then n_step x (drift, kick), the last kick halved. As in the reference, a drift
 // that loses the particle does not end the loop (track_a_drift does not test the state).
-template <bool F>
-__device__ __forceinline__ void thick_multipole(Particle &p, CP m,
-                                                int32_t n_step, int32_t mask)
+template <bool F, bool NORMAL>
+__device__ __forceinline__ void sextupole_body(Particle &p, CP m, int32_t n_step)
 {
     const double step = m[0];
+    const double a = m[3], b = m[7];
+    sextupole_kick<NORMAL>(p, a, b, 1.0);
+    for (int i = 1; i <= n_step; i++) {
+        drift<F>(p, step);
+        sextupole_kick<NORMAL>(p, a, b, i == n_step ? 1.0 : 2.0);
+    }
+}
+
+template <bool F>
+__device__ __forceinline__ void thick_multipole(Particle &p, CP m, int32_t n_step, int32_t flags)
+{
+    const double step = m[0];
+    const int32_t mask = (flags >> F_ORDER_SHIFT) & 0xF;
     if (mask == 4) {
-        const double a = m[3], b = m[7];
-        sextupole_kick(p, a, b, 1.0);
-        for (int i = 1; i <= n_step; i++) {
-            drift<F>(p, step);
-            sextupole_kick(p, a, b, i == n_step ? 1.0 : 2.0);
-        }
+        if (flags & F_MULT_NORMAL) sextupole_body<F, true>(p, m, n_step);
+        else sextupole_body<F, false>(p, m, n_step);
         return;
     }
     multipole_kick(p, m + 1, m + 5, mask, 1.0);
@@ -823,7 +838,11 @@ __device__ __forceinline__ void sbend_exact_fast(Particle &p, CP b)
 // particle; the test is branch-free, the state assignment gated.
 __device__ __forceinline__ bool orbit_check(Particle &p)
 {
-    if (orbit_bad_lanes(p)) {
+    // rel_p < 0 cannot newly hold inside a bend (pz does not change there), so the gate tests
+    // the rest of orbit_bad only
+    const double f_unstable = p.px * p.px + p.py * p.py - p.rel_p * p.rel_p;
+    if (lanes(fabs(p.x) > kMaxAperture) | lanes(fabs(p.y) > kMaxAperture) |
+        lanes(f_unstable > 0)) {
         // only live particles enter an element, so a lane not alive now was flagged here
         if (orbit_bad(p)) orbit_too_large(p);
         return p.state != ST_ALIVE;
@@ -1015,7 +1034,7 @@ __device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, con
         switch (kind) {
         case K_DRIFT: drift<F>(p, q[0]); break;
         case K_QUAD: quadrupole<F>(p, q, n, sp); break;
-        case K_MULT: thick_multipole<F>(p, q, n, (flags >> F_ORDER_SHIFT) & 0xF); break;
+        case K_MULT: thick_multipole<F>(p, q, n, flags); break;
         case K_BEND:
             if ((flags & (F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE | F_BEND_DRIFT)) ==
                 (F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE))

@@ -37,6 +37,7 @@ enum Flags : int32_t {
     // The exit aperture fails for every |x| or |y| above orbit_too_large's limit (its limits all
     // lie well inside it), so that part of the exit validity test need not be gated separately.
     F_APE_EXIT_COVERS = 1 << 13,
+    F_MULT_NORMAL = 1 << 14,  // K_MULT: no skew strengths (every a_n = 0)
 };
 
 struct EleHdr {
