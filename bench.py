@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--served-requests", type=int, default=24)
     ap.add_argument("--waves", type=int, default=5,
                     help="waves per SIMD of the tracking kernel variant (3, 4 or 5)")
+    ap.add_argument("--specialize", choices=["on", "off"], default="on",
+                    help="run-time compiled kernels specialised to the ring's repeated cell "
+                         "(fast maps only; off = the generic table-driven kernel)")
     return ap.parse_args()
 
 
@@ -105,6 +108,10 @@ def main():
     ctx.set_maps(args.maps)
     ctx.set_run(args.chunk_turns, 0, args.waves)
     lat = T.Lattice(ctx, arr)
+    t0 = time.perf_counter()
+    special = args.specialize == "on" and args.maps == "fast" and lat.specialize()
+    t_special = time.perf_counter() - t0
+    kernel_name = f"bmx_cell_tiles_{args.waves}" if special else "k_track_tiles"
     tps = args.turns_per_step
     if args.warmup > 0:
         wb = T.Beam.from_host(ctx, beam)
@@ -156,7 +163,8 @@ def main():
                         f"{args.steps * tps} turns in steps of {tps}",
             "particles": args.particles, "elements": len(arr), "turns": args.steps * tps,
             "turns_per_step": tps, "alive_at_end": alive, "wall_s": t_wall,
-            "maps": args.maps,
+            "maps": args.maps, "cell_specialized": bool(special),
+            "specialize_s": t_special,
         },
         "roofline": {
             "bound": "mfma", "compute": "fp64 vector ALU", "achieved": achieved,
@@ -165,7 +173,7 @@ def main():
             "traffic_note": "HBM bytes of the first 5-turn launch over 1e6 particles, "
                             "profiles/k_track_tiles_ring1_1e6_traffic.md",
             "flop_per_particle_element": flops,
-            "kernel": "k_track_tiles", "kernel_ms_avg": kernel_ms / max(launches, 1),
+            "kernel": kernel_name, "kernel_ms_avg": kernel_ms / max(launches, 1),
             "launches": launches, "step_ms": [float(x) for x in launch_ms],
         },
     }
@@ -174,7 +182,7 @@ def main():
     extras = [e for e in args.extras.split(",") if e and e != "none"]
     out["extras"] = {}
     if "batched" in extras:
-        out["extras"]["batched"] = run_batched(bmx, ctx, flops)
+        out["extras"]["batched"] = run_batched(bmx, ctx, flops, specialize=bool(special))
     if "da" in extras:
         out["extras"]["dynamic_aperture"] = run_da(bmx, ctx)
     if "served" in extras:
@@ -182,7 +190,7 @@ def main():
     print(json.dumps(out))
 
 
-def run_batched(bmx, ctx, flops, n_var=512, turns=500, seed=2):
+def run_batched(bmx, ctx, flops, n_var=512, turns=500, seed=2, specialize=True):
     """512 error variants of the ring (sigma 1e-3 on every k1, k2), log-uniform particle
     counts in [256, 65536], 500 turns: one batch, autotuned, replayed as one captured graph."""
     L, T = bmx.lattices, bmx.tracking
@@ -191,6 +199,9 @@ def run_batched(bmx, ctx, flops, n_var=512, turns=500, seed=2):
     total = int(counts.sum())
     beam = L.gaussian_beam(total, seed=seed, sigma=3 * L.SIGMA0)
     lats = [T.Lattice(ctx, bmx.elements.to_array(v)) for v in variants]
+    if specialize:
+        for lt in lats:
+            lt.specialize()
     b = T.Beam.from_host(ctx, beam)
     starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
     batch = T.Batch(ctx, [(lats[i], b, int(starts[i]), int(counts[i])) for i in range(n_var)])

@@ -195,6 +195,17 @@ int32_t bmx_lattice_validate(const bmx_ele *eles, int32_t n_ele, double mc2, int
     return BMX_OK;
 }
 
+int32_t bmx_lattice_specialize(bmx_lattice *L)
+{
+    if (!L) return set_error(BMX_ERR_ARG, "bmx_lattice_specialize: null lattice");
+    std::string err;
+    const JitKernels *j = nullptr;
+    const int rc = jit_specialize(L->device, L->host, &j, err);
+    if (rc != BMX_OK) return set_error(rc, "bmx_lattice_specialize: " + err);
+    L->jit = j;
+    return BMX_OK;
+}
+
 int32_t bmx_lattice_info(const bmx_lattice *L, int32_t *n_ele, uint64_t *kind_mask)
 {
     if (!L) return set_error(BMX_ERR_ARG, "bmx_lattice_info: null lattice");
@@ -323,8 +334,9 @@ int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t
     RunConfig cfg;
     cfg.chunk_turns = ctx->chunk_turns;
     cfg.fast = ctx->fast_maps;
+    cfg.jit = ctx->fast_maps ? L->jit : nullptr;
     cfg.waves = ctx->waves ? ctx->waves : default_waves(ctx->fast_maps);
-    cfg.grid = ctx->grid > 0 ? ctx->grid : default_grid(ctx->device, ctx->fast_maps, cfg.waves);
+    cfg.grid = ctx->grid > 0 ? ctx->grid : default_grid(ctx->device, ctx->fast_maps, cfg.waves, cfg.jit);
     cfg.timer = ctx->profile ? &ctx->timer : nullptr;
     HIP_TRY(enqueue_run(ctx->stream, ctx->run, &g, 1, true, n_turns, cfg, ctx->d_work), "k_track_tiles");
     return BMX_OK;

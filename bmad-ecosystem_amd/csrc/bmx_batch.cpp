@@ -20,10 +20,33 @@ struct bmx_batch {
     RunBuffers rb;
     RunConfig cfg;
     std::map<int32_t, hipGraphExec_t> graphs;  // by turn count, for the current cfg
+    bool grid_auto = true;                      // grid follows the kernel's occupancy
     void drop_graphs()
     {
         for (auto &kv : graphs) hipGraphExecDestroy(kv.second);
         graphs.clear();
+    }
+    // The cell-specialised kernels, when every group's lattice carries the same ones and the
+    // fast maps are selected; otherwise the generic kernel.
+    const JitKernels *shared_jit() const
+    {
+        if (!cfg.fast || lats.empty()) return nullptr;
+        const JitKernels *j = lats[0]->jit;
+        for (const bmx_lattice *L : lats)
+            if (L->jit != j) return nullptr;
+        return j;
+    }
+    // Follow the context's map choice and the lattices' specialisation (which may be set after
+    // the batch was made); a change invalidates the captured graphs.
+    void refresh(bool fast)
+    {
+        const bool fast_changed = cfg.fast != fast;
+        cfg.fast = fast;
+        const JitKernels *j = shared_jit();
+        if (!fast_changed && j == cfg.jit) return;
+        cfg.jit = j;
+        if (grid_auto) cfg.grid = default_grid(device, cfg.fast, cfg.waves, cfg.jit);
+        drop_graphs();
     }
 };
 
@@ -98,7 +121,8 @@ int32_t bmx_batch_create(bmx_ctx *ctx, const bmx_group *groups, int32_t n_groups
     B->cfg.chunk_turns = ctx->chunk_turns;
     B->cfg.fast = ctx->fast_maps;
     B->cfg.waves = ctx->waves ? ctx->waves : default_waves(B->cfg.fast);
-    B->cfg.grid = default_grid(ctx->device, B->cfg.fast, B->cfg.waves);
+    B->cfg.jit = B->shared_jit();
+    B->cfg.grid = default_grid(ctx->device, B->cfg.fast, B->cfg.waves, B->cfg.jit);
     *out = B;
     return BMX_OK;
 }
@@ -123,7 +147,9 @@ int32_t bmx_batch_set_run(bmx_batch *B, int32_t chunk_turns, int32_t grid, int32
     hipStreamSynchronize(B->ctx->stream);
     B->cfg.chunk_turns = chunk_turns;
     B->cfg.waves = waves ? waves : default_waves(B->cfg.fast);
-    B->cfg.grid = grid > 0 ? grid : default_grid(B->ctx->device, B->cfg.fast, B->cfg.waves);
+    B->cfg.jit = B->shared_jit();
+    B->grid_auto = grid <= 0;
+    B->cfg.grid = grid > 0 ? grid : default_grid(B->ctx->device, B->cfg.fast, B->cfg.waves, B->cfg.jit);
     B->drop_graphs();
     return BMX_OK;
 }
@@ -134,10 +160,7 @@ int32_t bmx_batch_track(bmx_batch *B, int32_t n_turns, int32_t turn0)
     if (n_turns == 0 || B->groups.empty()) return BMX_OK;
     bmx_ctx *ctx = B->ctx;
     hipSetDevice(ctx->device);
-    if (B->cfg.fast != ctx->fast_maps) {
-        B->cfg.fast = ctx->fast_maps;
-        B->drop_graphs();
-    }
+    B->refresh(ctx->fast_maps != 0);
     // the previous call's upload of the host group array must be complete before it changes
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(e, "bmx_batch_track");
@@ -177,6 +200,7 @@ int32_t bmx_batch_tune(bmx_batch *B, int32_t sample_turns, int32_t *chunk_out, i
     if (!B || sample_turns <= 0) return set_error(BMX_ERR_ARG, "bmx_batch_tune: bad argument");
     bmx_ctx *ctx = B->ctx;
     hipSetDevice(ctx->device);
+    B->refresh(ctx->fast_maps != 0);
     const size_t n = B->n_particles;
     if (n == 0) return BMX_OK;
     // scratch SoA copy of every group's particles, restored before each candidate
@@ -217,7 +241,7 @@ int32_t bmx_batch_tune(bmx_batch *B, int32_t sample_turns, int32_t *chunk_out, i
     const int32_t chunks[] = {5, 10, 25, 50};
     const int32_t waves[] = {3, 4, 5};
     for (int32_t w : waves) {
-        const int full = default_grid(ctx->device, B->cfg.fast, w);
+        const int full = default_grid(ctx->device, B->cfg.fast, w, B->cfg.jit);
         for (int32_t K : chunks) {
             if (K > 2 * sample_turns && K != chunks[0]) continue;
             for (int gsel = 0; gsel < 2 && e == hipSuccess; gsel++) {
@@ -249,6 +273,7 @@ int32_t bmx_batch_tune(bmx_batch *B, int32_t sample_turns, int32_t *chunk_out, i
     hipFree(work);
     if (e != hipSuccess) return fail(e, "bmx_batch_tune");
     B->cfg = best;
+    B->grid_auto = false;
     B->drop_graphs();
     if (chunk_out) *chunk_out = best.chunk_turns;
     if (grid_out) *grid_out = best.grid;

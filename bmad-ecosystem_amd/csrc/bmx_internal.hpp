@@ -69,6 +69,21 @@ struct KernelTimer {
     void release();
 };
 
+// Threads per workgroup of the tile kernels.
+constexpr int kTileBlock = 256;
+
+// Lattice-specialised tile kernels (bmx_jit.cpp): compiled at run time for a lattice that is
+// mostly one repeated cell; shared by every lattice with the same cell structure.
+struct JitKernels;
+int jit_specialize(int device, const HostTable &t, const JitKernels **out, std::string &err);
+hipError_t jit_occupancy(const JitKernels *j, int waves, int *blocks_per_cu);
+hipError_t launch_track_tiles_jit(const JitKernels *j, hipStream_t s, int grid, int waves,
+                                  const Group *d_groups, const WaveTile *d_tiles,
+                                  const uint32_t *d_n_tiles, const uint32_t *d_list_in,
+                                  uint32_t *d_list_out, uint32_t *d_cnt_out, uint32_t *d_tile_ctr,
+                                  int32_t chunk_off, int32_t chunk_turns,
+                                  unsigned long long *d_work);
+
 // Parameters of one run.
 struct RunConfig {
     int32_t chunk_turns = 5;   // turns per chunk (alive lists are rebuilt between chunks)
@@ -76,6 +91,7 @@ struct RunConfig {
     int32_t waves = 5;         // waves per SIMD the tracking kernel is compiled for (3, 4, 5)
     bool fast = true;
     KernelTimer *timer = nullptr;  // if set: an event pair around every tracking launch
+    const JitKernels *jit = nullptr;  // cell-specialised kernels shared by every group's lattice
 };
 
 // Uploads the groups (asynchronously, from host memory that must stay valid until the stream
@@ -89,8 +105,9 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
 // Waves per SIMD of the default tracking kernel variant: 5 for the fast maps (the fastest
 // measured), 3 for the reference maps (whose 4- and 5-wave builds spill).
 inline int32_t default_waves(bool fast) { return fast ? 5 : 3; }
-// Workgroups of the tracking kernel that fit on the device at once.
-int default_grid(int device, bool fast, int waves);
+// Workgroups of the tracking kernel that fit on the device at once (of the cell-specialised
+// kernels when jit is set).
+int default_grid(int device, bool fast, int waves, const JitKernels *jit = nullptr);
 
 }  // namespace bmx
 
@@ -118,6 +135,7 @@ struct bmx_lattice {
     double *d_prm = nullptr;
     bmx::LatDesc desc{};              // device pointers inside
     bmx::LatDesc *d_desc = nullptr;   // one-element array for the tile kernel
+    const bmx::JitKernels *jit = nullptr;  // set by bmx_lattice_specialize
 };
 
 struct bmx_beam {

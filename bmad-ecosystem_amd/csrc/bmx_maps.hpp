@@ -29,6 +29,19 @@ namespace bmx {
 typedef const double BMX_CONST *CP;
 typedef const int32_t BMX_CONST *CHdr;
 
+// One element's header values (bmx_table.hpp EleHdr). A specialised build passes them as
+// compile-time constants, so every test on the kind and flags folds away.
+struct EleRef {
+    int32_t kind, flags, n, off;
+};
+typedef int32_t EleVec __attribute__((ext_vector_type(4)));
+// The header of element h, read with one 16-byte scalar load.
+__device__ __forceinline__ EleRef load_ele(CHdr h)
+{
+    const EleVec v = *(const EleVec BMX_CONST *)h;
+    return {v.x, v.y, v.z, v.w};
+}
+
 constexpr double kPi = 3.141592653589793238462643383279;
 constexpr double kTwoPi = 2 * kPi;
 constexpr double kCLight = 2.99792458e8;
@@ -999,8 +1012,8 @@ __device__ __forceinline__ void rfcavity(Particle &p, CP r, int32_t n_slice,
 
 // One element for one particle: the body of track1 (bmad/code/track1.f90:196-329) after the
 // entrance validity check, i.e. entrance aperture, element map, orbit_too_large (:291),
-// exit aperture (:329). hdr points at the element's 4-int header (bmx_table.hpp); prm at the
-// lattice's parameter array. Returns true when the particle was lost at the entrance
+// exit aperture (:329). h is the element's header (bmx_table.hpp); prm the lattice's parameter
+// array. Returns true when the particle was lost at the entrance
 // aperture (location = upstream_end$).
 // MAPS selects the element kinds compiled in: MAPS_ALL, or MAPS_NO_RF / MAPS_RF_ONLY for
 // loops that keep the (rare, register-hungry) rf cavity out of the common element path.
@@ -1011,10 +1024,10 @@ __device__ __forceinline__ void rfcavity(Particle &p, CP r, int32_t n_slice,
 enum MapSet : int { MAPS_ALL = 0, MAPS_NO_RF = 1, MAPS_RF_ONLY = 2 };
 
 template <bool F, int MAPS = MAPS_ALL>
-__device__ __forceinline__ bool track_element(Particle &p, CHdr hdr, CP prm, const Species &sp,
+__device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, const Species &sp,
                                               bool from_prev, int32_t ix1, int32_t turn)
 {
-    const int32_t kind = hdr[0], flags = hdr[1], n = hdr[2], off = hdr[3];
+    const int32_t kind = h.kind, flags = h.flags, n = h.n, off = h.off;
     CP ap = prm + off;
     CP q = (flags & (F_APE_ENT | F_APE_EXIT | F_APE_ENT_SAME)) ? ap + 4 : ap;
     if ((flags & F_APE_ENT) || (!from_prev && (flags & F_APE_ENT_SAME))) {

@@ -104,10 +104,12 @@ void KernelTimer::release()
     pool.clear();
 }
 
-int default_grid(int device, bool fast, int waves)
+int default_grid(int device, bool fast, int waves, const JitKernels *jit)
 {
     int blocks = 0, cus = 0;
-    if (track_tiles_occupancy(fast, waves, &blocks) != hipSuccess || blocks <= 0) blocks = 1;
+    const hipError_t e =
+        jit ? jit_occupancy(jit, waves, &blocks) : track_tiles_occupancy(fast, waves, &blocks);
+    if (e != hipSuccess || blocks <= 0) blocks = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
         cus = 1;
     return blocks * cus;
@@ -140,9 +142,14 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
             if (e == hipSuccess) e = hipEventRecord(ev0, s);
             if (e != hipSuccess) break;
         }
-        e = launch_track_tiles(s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur], n_tiles[cur],
-                               c == 0 ? nullptr : rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt],
-                               tile_ctr, off, K, d_work, cfg.fast);
+        if (cfg.jit)
+            e = launch_track_tiles_jit(cfg.jit, s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur],
+                                       n_tiles[cur], c == 0 ? nullptr : rb.d_list[cur],
+                                       rb.d_list[nxt], rb.d_cnt[nxt], tile_ctr, off, K, d_work);
+        else
+            e = launch_track_tiles(s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur], n_tiles[cur],
+                                   c == 0 ? nullptr : rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt],
+                                   tile_ctr, off, K, d_work, cfg.fast);
         if (cfg.timer && e == hipSuccess) {
             e = hipEventRecord(ev1, s);
             cfg.timer->pending.emplace_back(ev0, ev1);

@@ -1,0 +1,179 @@
+// Device code of the persistent tile kernel, shared by the library's build of it
+// (bmx_kernels.hip) and by lattice-specialised builds compiled at run time (bmx_jit.cpp), which
+// differ only in the turn policy: how one particle is taken once around the ring.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "bmx_maps.hpp"
+
+namespace bmx {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ void load_particle(Particle &p, const BeamPtrs &b, int64_t i)
+{
+    p.x = b.x[i];
+    p.px = b.px[i];
+    p.y = b.y[i];
+    p.py = b.py[i];
+    p.z = b.z[i];
+    p.pz = b.pz[i];
+    p.beta = b.beta[i];
+    p.state = ST_ALIVE;
+}
+
+__device__ __forceinline__ void store_particle(const Particle &p, const BeamPtrs &b, int64_t i)
+{
+    b.x[i] = p.x;
+    b.px[i] = p.px;
+    b.y[i] = p.y;
+    b.py[i] = p.py;
+    b.z[i] = p.z;
+    b.pz[i] = p.pz;
+    b.beta[i] = p.beta;
+}
+
+// Adds the number of particle-element passes tracked by this wave to *work (if non-null):
+// element passes of live particles only, as the reference's loop skips dead ones
+// (beam_utils.f90:83).
+__device__ __forceinline__ void add_work(unsigned long long *work, unsigned long long mine)
+{
+    if (!work) return;
+    unsigned long long s = mine;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(work, s);
+}
+
+// The generic turn policy: elements [e0, e1) of the lattice read from its table. The rf
+// cavities sit outside the inner element loop: the loop over the other kinds runs until the
+// next cavity, which is tracked on its own, so the cavity's code does not shape the register
+// assignment of the common path. The loops are wave-uniform (the element index stays a
+// scalar, so the table is read by scalar loads); a lost particle's lane is masked off for the
+// rest of the turn.
+template <bool F>
+__device__ __forceinline__ void track_range(Particle &p, const LatDesc &L, int32_t e0, int32_t e1,
+                                            int32_t t_abs)
+{
+    const CHdr hdr = (CHdr)L.hdr;
+    const CP prm = (CP)L.prm;
+    int32_t e = e0;
+    for (;;) {
+        for (; e < e1; ++e) {
+            // e is the same on every active lane; read it from the first one so the compiler
+            // keeps it (and the table reads) scalar whatever it makes of the loop's control flow
+            e = __builtin_amdgcn_readfirstlane(e);
+            const CHdr h = hdr + 4 * e;
+            if (h[0] == K_RF) break;
+            if (p.state == ST_ALIVE)
+                track_element<F, MAPS_NO_RF>(p, load_ele(h), prm, L.sp, true, e + 1, t_abs);
+        }
+        if (e >= e1) break;
+        e = __builtin_amdgcn_readfirstlane(e);
+        if (p.state == ST_ALIVE)
+            track_element<F, MAPS_RF_ONLY>(p, load_ele(hdr + 4 * e), prm, L.sp, true, e + 1, t_abs);
+        ++e;
+    }
+}
+
+template <bool F>
+struct GenericTurn {
+    __device__ __forceinline__ static void turn(Particle &p, const LatDesc &L, int32_t t_abs)
+    {
+        track_range<F>(p, L, 0, L.n_ele, t_abs);
+    }
+};
+
+// Tracks one particle through up to n_turns full turns starting at turn index turn0 (Turn:
+// the turn policy); records the loss element (1-based) and turn. Returns the number of
+// element passes made.
+template <bool F, class Turn>
+__device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, const BeamPtrs &b,
+                                                             uint32_t i, int32_t n_turns,
+                                                             int32_t turn0)
+{
+    Particle p;
+    load_particle(p, b, i);
+    refresh_energy(p, L.sp);
+    p.loss_ele = 0;
+    p.loss_turn = 0;
+    // Entrance validity check of the first element (track1.f90:196 -> orbit_too_large); later
+    // elements need none: a particle leaves element e with the coordinates it enters e+1 with,
+    // and element e already ended with the same test (track1.f90:291).
+    if (orbit_too_large(p)) {
+        p.loss_ele = 1;
+        p.loss_turn = turn0;
+    } else {
+        // a wave stops once all its particles are lost
+        for (int32_t turn = 0; turn < n_turns; ++turn) {
+            if (!any_lane(p.state == ST_ALIVE)) break;
+            Turn::turn(p, L, turn0 + turn);
+        }
+    }
+    store_particle(p, b, i);
+    if (p.state != ST_ALIVE) {
+        b.state[i] = p.state;
+        b.loss_ele[i] = p.loss_ele;
+        b.loss_turn[i] = p.loss_turn;
+        return (unsigned long long)(p.loss_turn - turn0) * L.n_ele + p.loss_ele;
+    }
+    return (unsigned long long)n_turns * L.n_ele;
+}
+
+// Persistent tracking of one chunk of turns over a list of wave tiles.
+//
+// Each wave repeatedly takes the next tile (atomic counter) and tracks its particles for the
+// chunk's turns with the state in registers. Particles still alive at the end are appended to
+// their group's segment of the next alive list (one atomic per wave), so the next chunk's
+// tiles hold live particles only and waves stay full as the beam is lost. The order inside a
+// list depends on scheduling; each particle's result does not.
+//
+// Every resident wave takes tiles (as many waves per SIMD as the register budget allows hide
+// each other's latencies better than fewer waves with evenly divided tiles).
+// chunk_off: turns already done by earlier chunks of this call; chunk_turns: turns per chunk.
+template <bool F, class Turn>
+__device__ __forceinline__ void track_tiles(
+    const Group *__restrict__ groups, const WaveTile *__restrict__ tiles,
+    const uint32_t *__restrict__ n_tiles_p, const uint32_t *__restrict__ list_in,
+    uint32_t *__restrict__ list_out, uint32_t *__restrict__ cnt_out, uint32_t *tile_ctr,
+    int32_t chunk_off, int32_t chunk_turns, unsigned long long *work)
+{
+    const uint32_t nt = *n_tiles_p;
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (wave >= nt) return;  // whole wave
+    unsigned long long passes = 0;
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(tile_ctr, 1u);
+        t = __builtin_amdgcn_readfirstlane(t);
+        if (t >= nt) break;
+        const WaveTile T = tiles[t];
+        const Group &G = groups[T.group];
+        const int32_t rem = G.n_turns - chunk_off;
+        const int32_t n_turns = rem < chunk_turns ? rem : chunk_turns;
+        const int32_t turn0 = G.turn0 + chunk_off;
+        bool alive_end = false;
+        uint32_t idx = 0;
+        if ((uint32_t)lane < T.count) {
+            idx = list_in ? list_in[G.list_off + T.start + lane] : T.start + lane;
+            if (G.b.state[idx] == ST_ALIVE) {
+                passes += track_particle<F, Turn>(G.lat, G.b, idx, n_turns, turn0);
+                alive_end = G.b.state[idx] == ST_ALIVE;
+            }
+        }
+        const unsigned long long m = __ballot(alive_end);
+        if (m) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&cnt_out[T.group], (uint32_t)__popcll(m));
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (alive_end) {
+                const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1));
+                list_out[G.list_off + base + rank] = idx;
+            }
+        }
+    }
+    add_work(work, passes);
+}
+
+}  // namespace bmx

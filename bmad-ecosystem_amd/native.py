@@ -14,6 +14,8 @@ LIB_PATH = os.environ.get("BMX_LIBRARY") or os.path.join(HERE, "libbmx.so")
 
 _lib = None
 
+ERR_UNSUPPORTED = 2  # BMX_ERR_UNSUPPORTED
+
 
 class BmxError(RuntimeError):
     def __init__(self, code: int, msg: str):
@@ -69,6 +71,7 @@ _SIGS = {
     "bmx_lattice_create": (_I32, [_P, _P, _I32, ctypes.c_double, _I32, _PP]),
     "bmx_lattice_destroy": (_I32, [_P]),
     "bmx_lattice_validate": (_I32, [_P, _I32, ctypes.c_double, _I32, ctypes.POINTER(_I32)]),
+    "bmx_lattice_specialize": (_I32, [_P]),
     "bmx_lattice_info": (_I32, [_P, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_uint64)]),
     "bmx_beam_create": (_I32, [_P, _I64, _PP]),
     "bmx_beam_wrap": (_I32, [_P, ctypes.POINTER(BeamView), _I64, _PP]),

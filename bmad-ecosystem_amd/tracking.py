@@ -105,6 +105,19 @@ class Lattice:
         native.call("bmx_lattice_info", self.h, ctypes.byref(n), ctypes.byref(mask))
         self.n_ele = n.value
         self.kind_mask = mask.value
+        self.specialized = False
+
+    def specialize(self) -> bool:
+        """Compile tile kernels specialised to this lattice's repeated cell (bmx_lattice_specialize).
+
+        Returns False when the lattice has no cell repeated over three quarters of it; the
+        generic kernel is then used. Other errors raise."""
+        rc = native.load().bmx_lattice_specialize(self.h)
+        if rc == native.ERR_UNSUPPORTED:
+            return False
+        native.check(rc)
+        self.specialized = True
+        return True
 
     def close(self):
         if self.h:
@@ -193,11 +206,15 @@ def track_beam_plain(ctx: Context, lat: Lattice, beam: Beam, n_turns: int, turn0
 
 
 def track_beam_host(eles, beam: dict, n_turns: int, device: int = 0, turn0: int = 0,
-                    mc2: float = E.M_ELECTRON, charge: int = -1, maps: str = "fast") -> dict:
-    """Convenience: host SoA dict in, host SoA dict out (tracks a copy)."""
+                    mc2: float = E.M_ELECTRON, charge: int = -1, maps: str = "fast",
+                    specialize: bool = False) -> dict:
+    """Convenience: host SoA dict in, host SoA dict out (tracks a copy). specialize: use the
+    lattice's cell-specialised kernels when it has a repeated cell."""
     ctx = Context(device)
     ctx.set_maps(maps)
     lat = Lattice(ctx, eles, mc2, charge)
+    if specialize:
+        lat.specialize()
     b = Beam.from_host(ctx, beam)
     track_beam(ctx, lat, b, n_turns, turn0)
     ctx.synchronize()

@@ -198,6 +198,15 @@ int32_t bmx_lattice_destroy(bmx_lattice *lat);
  * the offending element in bmx_last_error(). n_params receives the table size in doubles. */
 int32_t bmx_lattice_validate(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge,
                              int32_t *n_params);
+/* Compile, at run time for this device, tile kernels specialised to the lattice's repeated
+ * cell: the shortest run of elements whose kinds, flags and parameter-block sizes repeat over at
+ * least three quarters of the table, written out once with every kind dispatch and flag test
+ * resolved. Strengths stay in the table, so lattices differing only in strengths (error
+ * variants) share one compiled kernel. Used by the fast-map runs of bmx_track_turns and
+ * bmx_batch_run; the results are those of the generic kernel. BMX_ERR_UNSUPPORTED when no
+ * such cell exists (the generic kernel is then used). No counterpart in the reference, whose
+ * element loop is track_all (bmad/code/track_all.f90:1) over the generic element dispatch. */
+int32_t bmx_lattice_specialize(bmx_lattice *lat);
 /* Number of elements and bit set of element kinds present (bit = key). */
 int32_t bmx_lattice_info(const bmx_lattice *lat, int32_t *n_ele, uint64_t *kind_mask);
 

@@ -1,0 +1,150 @@
+"""Cell-specialised tile kernels (bmx_lattice_specialize): run-time compiled per repeated
+lattice cell, held to the same bars as the generic kernel against the CPU restatement."""
+import numpy as np
+import pytest
+
+import parity as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(bmx):
+    c = bmx.tracking.Context(0)
+    c.set_maps("fast")
+    return c
+
+
+def _track(bmx, ctx, arr, beam, n_turns, specialize, turn0=0):
+    T = bmx.tracking
+    lat = T.Lattice(ctx, arr)
+    if specialize:
+        assert lat.specialize()
+    b = T.Beam.from_host(ctx, beam)
+    T.track_beam(ctx, lat, b, n_turns, turn0)
+    ctx.synchronize()
+    out = b.download()
+    b.close()
+    lat.close()
+    return out
+
+
+def _within_spread(out, ref, noise):
+    alive = ref["state"] == 1
+    dev = P.rel_dev(out, ref, alive)[alive]
+    nz = noise[alive]
+    print("deviation / fma spread percentiles 50, 90, 99:",
+          [(float(np.percentile(dev, q)), float(np.percentile(nz, q))) for q in (50, 90, 99)])
+    for q in (50, 90, 99):
+        assert np.percentile(dev, q) <= 3 * np.percentile(nz, q) + 1e-13, (
+            q, np.percentile(dev, q), np.percentile(nz, q))
+
+
+def test_ring0_1000_turns_specialized(bmx, oracle, ctx):
+    """700-element ring (cell of 7), 1-sigma beam, 1000 turns: within 1e-9 of the CPU
+    restatement for every particle, like the generic kernel."""
+    L = bmx.lattices
+    arr = bmx.elements.to_array(L.ring0())
+    beam = L.gaussian_beam(512, seed=7, sigma=L.SIGMA0)
+    ref = P.copy_beam(beam)
+    oracle.track_bunch(arr, ref, n_turns=1000)
+    print("ring0 1000 turns, specialised", flush=True)
+    out = _track(bmx, ctx, arr, beam, 1000, True)
+    assert np.all(out["state"] == 1) and np.all(ref["state"] == 1)
+    assert P.rel_err(out, ref) <= 1e-9
+
+
+@pytest.mark.parametrize("waves", [3, 4, 5])
+def test_ring1_losses_specialized(bmx, oracle, ctx, waves):
+    """901-element ring (cell of 9, rf cavity in the generic tail), a beam losing about half of
+    its particles in 30 turns: loss state, element and turn identical to the CPU restatement for
+    every particle, for each of the three kernel variants."""
+    L = bmx.lattices
+    arr = bmx.elements.to_array(L.ring1())
+    beam = P.random_beam(bmx, 3000, seed=21, scale=25.0, pz_sigma=4e-3)
+    ref = P.copy_beam(beam)
+    oracle.track_bunch(arr, ref, n_turns=30)
+    print(f"ring1 losses, specialised, waves {waves}", flush=True)
+    try:
+        ctx.set_run(5, 0, waves)
+        out = _track(bmx, ctx, arr, beam, 30, True)
+    finally:
+        ctx.set_run(5, 0, 0)
+    lost = ref["state"] != 1
+    assert lost.sum() > 500
+    bad = P.loss_mismatch(out, ref)
+    assert bad.size == 0, (bad[:10], out["loss_ele"][bad[:10]], ref["loss_ele"][bad[:10]])
+
+
+def test_head_cell_and_tail(bmx, oracle, ctx):
+    """A marker in front of the ring shifts the repeated run: a generic head cell, 99 rotated
+    cells and a two-element generic tail. After 20 turns: losses bit-exact; the survivors'
+    deviations from the CPU restatement within 3x of the spread that FMA contraction alone
+    causes in it (median, 90th, 99th percentile), as for the generic kernel: these
+    large-amplitude orbits amplify any rounding difference."""
+    L, E = bmx.lattices, bmx.elements
+    eles = L.ring1()
+    p0c, et = L.reference()
+    eles = [E.make_element(E.MARKER, p0c, et)] + list(eles)
+    arr = E.to_array(eles)
+    beam = P.random_beam(bmx, 2000, seed=3, scale=15.0, pz_sigma=3e-3)
+    ref, noise = P.fma_noise(oracle, arr, beam, 20)
+    print("head cell + tail, specialised", flush=True)
+    out = _track(bmx, ctx, arr, beam, 20, True)
+    assert (ref["state"] != 1).sum() > 100
+    bad = P.loss_mismatch(out, ref)
+    assert bad.size == 0, bad[:10]
+    _within_spread(out, ref, noise)
+
+
+def test_batch_of_specialized_variants(bmx, oracle, ctx):
+    """Eight error variants of a 10-cell ring share one specialised kernel (strengths stay in
+    the table); the batch equals the CPU restatement of each variant after 25 turns: losses
+    bit-exact, the survivors' deviations within 3x of the FMA-contraction spread (median, 90th,
+    99th percentile over all variants)."""
+    L, T = bmx.lattices, bmx.tracking
+    arrs = [bmx.elements.to_array(e) for e in L.ring1_variants(5, 8, n_cells=10)]
+    counts = [700, 64, 1, 333, 1000, 129, 64, 500]
+    beam = P.random_beam(bmx, sum(counts), seed=9, scale=15.0, pz_sigma=3e-3)
+    lats = [T.Lattice(ctx, a) for a in arrs]
+    for lt in lats:
+        assert lt.specialize()
+    b = T.Beam.from_host(ctx, beam)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    batch = T.Batch(ctx, [(lats[i], b, int(starts[i]), counts[i]) for i in range(8)])
+    print("batch of specialised variants", flush=True)
+    batch.track(25, 0)
+    ctx.synchronize()
+    out = b.download()
+    batch.close()
+    b.close()
+    lost = 0
+    devs, noises = [], []
+    for i in range(8):
+        sl = slice(int(starts[i]), int(starts[i]) + counts[i])
+        part, noise = P.fma_noise(oracle, arrs[i], {k: v[sl].copy() for k, v in beam.items()}, 25)
+        mine = {k: v[sl].copy() for k, v in out.items()}
+        bad = P.loss_mismatch(mine, part)
+        assert bad.size == 0, (i, bad[:5])
+        alive = part["state"] == 1
+        devs.append(P.rel_dev(mine, part, alive)[alive])
+        noises.append(noise[alive])
+        lost += int((part["state"] != 1).sum())
+    assert lost > 100
+    dev, nz = np.concatenate(devs), np.concatenate(noises)
+    for q in (50, 90, 99):
+        assert np.percentile(dev, q) <= 3 * np.percentile(nz, q) + 1e-13, q
+    for lt in lats:
+        lt.close()
+
+
+def test_lattice_without_repeated_cell(bmx, ctx):
+    """No cell repeats over three quarters of the table: specialize reports it and tracking
+    keeps the generic kernel."""
+    L, E, T = bmx.lattices, bmx.elements, bmx.tracking
+    p0c, et = L.reference()
+    eles = [E.make_element(E.DRIFT, p0c, et, l=1.0 + 0.1 * i) for i in range(3)]
+    eles += [E.make_element(E.QUADRUPOLE, p0c, et, l=0.3, k1=0.2), E.make_element(E.MARKER, p0c, et)]
+    lat = T.Lattice(ctx, E.to_array(eles))
+    assert lat.specialize() is False
+    lat.close()
