@@ -49,19 +49,25 @@ def parse():
                     help="comma list of the other workloads to run after the main one "
                          "(batched, da, served; empty = none)")
     ap.add_argument("--served-requests", type=int, default=24)
-    ap.add_argument("--waves", type=int, default=5,
-                    help="waves per SIMD of the tracking kernel variant (3, 4 or 5)")
+    ap.add_argument("--waves", type=int, default=0,
+                    help="waves per SIMD of the tracking kernel variant (3, 4 or 5; 0 = the "
+                         "library's default for the kernel in use)")
     ap.add_argument("--specialize", choices=["on", "off"], default="on",
                     help="run-time compiled kernels specialised to the ring's repeated cell "
                          "(fast maps only; off = the generic table-driven kernel)")
     return ap.parse_args()
 
 
-def profiled_traffic():
+def traffic_file(special: bool) -> str:
+    name = "bmx_cell_tiles" if special else "k_track_tiles"
+    return os.path.join("profiles", f"{name}_ring1_1e6_traffic.md")
+
+
+def profiled_traffic(special: bool):
     """HBM bytes per tracking-kernel launch from the committed rocprofv3 summary (FETCH_SIZE and
     WRITE_SIZE runs scaled by a copy-kernel calibration), or None if it is absent."""
     import re
-    path = os.path.join(ROOT, "profiles", "k_track_tiles_ring1_1e6_traffic.md")
+    path = os.path.join(ROOT, traffic_file(special))
     try:
         m = re.search(r"\*\*([0-9.]+) MB per launch\*\*", open(path).read())
     except OSError:
@@ -111,7 +117,8 @@ def main():
     t0 = time.perf_counter()
     special = args.specialize == "on" and args.maps == "fast" and lat.specialize()
     t_special = time.perf_counter() - t0
-    kernel_name = f"bmx_cell_tiles_{args.waves}" if special else "k_track_tiles"
+    waves = args.waves or (3 if special else (5 if args.maps == "fast" else 3))
+    kernel_name = f"bmx_cell_tiles_{waves}" if special else f"k_track_tiles<{str(args.maps == 'fast').lower()}, {waves}>"
     tps = args.turns_per_step
     if args.warmup > 0:
         wb = T.Beam.from_host(ctx, beam)
@@ -163,15 +170,15 @@ def main():
                         f"{args.steps * tps} turns in steps of {tps}",
             "particles": args.particles, "elements": len(arr), "turns": args.steps * tps,
             "turns_per_step": tps, "alive_at_end": alive, "wall_s": t_wall,
-            "maps": args.maps, "cell_specialized": bool(special),
+            "maps": args.maps, "cell_specialized": bool(special), "waves_per_simd": waves,
             "specialize_s": t_special,
         },
         "roofline": {
             "bound": "mfma", "compute": "fp64 vector ALU", "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": profiled_traffic(),
+            "traffic": profiled_traffic(special),
             "traffic_note": "HBM bytes of the first 5-turn launch over 1e6 particles, "
-                            "profiles/k_track_tiles_ring1_1e6_traffic.md",
+                            + traffic_file(special),
             "flop_per_particle_element": flops,
             "kernel": kernel_name, "kernel_ms_avg": kernel_ms / max(launches, 1),
             "launches": launches, "step_ms": [float(x) for x in launch_ms],
@@ -184,9 +191,9 @@ def main():
     if "batched" in extras:
         out["extras"]["batched"] = run_batched(bmx, ctx, flops, specialize=bool(special))
     if "da" in extras:
-        out["extras"]["dynamic_aperture"] = run_da(bmx, ctx)
+        out["extras"]["dynamic_aperture"] = run_da(bmx, ctx, specialize=bool(special))
     if "served" in extras:
-        out["extras"]["served"] = run_served(bmx, ctx, args.served_requests)
+        out["extras"]["served"] = run_served(bmx, ctx, args.served_requests, specialize=bool(special))
     print(json.dumps(out))
 
 
@@ -228,12 +235,15 @@ def run_batched(bmx, ctx, flops, n_var=512, turns=500, seed=2, specialize=True):
             "tuned": shape, "tune_s": t_tune}
 
 
-def run_da(bmx, ctx, n_var=4, turns=10_000, seed=3):
+def run_da(bmx, ctx, n_var=4, turns=10_000, seed=3, specialize=True):
     """Dynamic-aperture grid (11 angles x 64 radii x 4 pz) on 4 error variants, 1e4 turns:
     survival turn per particle."""
     L, T = bmx.lattices, bmx.tracking
     variants = L.ring1_variants(seed, n_var)
     lats = [T.Lattice(ctx, bmx.elements.to_array(v)) for v in variants]
+    if specialize:
+        for lt in lats:
+            lt.specialize()
     grid = L.da_grid()
     ctx.take_work()
     t0 = time.perf_counter()
@@ -248,7 +258,7 @@ def run_da(bmx, ctx, n_var=4, turns=10_000, seed=3):
             "median_survival_turn": float(np.median(surv))}
 
 
-def run_served(bmx, ctx, n_requests, seed=4, load=0.8, budget=16 << 30):
+def run_served(bmx, ctx, n_requests, seed=4, load=0.8, budget=16 << 30, specialize=True):
     """Served stream against one-job-at-a-time tracking: Poisson arrivals of requests on a pool
     of 32 error variants, particles log-uniform [1e3, 1e6], turns log-uniform [10, 1e4],
     100-turn chunks, HBM budget 16 GiB. The arrival rate is set to `load` times the served
@@ -256,6 +266,9 @@ def run_served(bmx, ctx, n_requests, seed=4, load=0.8, budget=16 << 30):
     L, T = bmx.lattices, bmx.tracking
     S = bmx.serving
     lats = [T.Lattice(ctx, bmx.elements.to_array(v)) for v in L.ring1_variants(seed, 32)]
+    if specialize:
+        for lt in lats:
+            lt.specialize()
     reqs, gaps = S.make_stream(seed, n_requests)
     # capacity: everything submitted at t = 0
     cap = S.run_served(ctx, lats, reqs, np.zeros(len(reqs)), budget)

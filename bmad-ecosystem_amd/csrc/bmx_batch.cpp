@@ -21,6 +21,7 @@ struct bmx_batch {
     RunConfig cfg;
     std::map<int32_t, hipGraphExec_t> graphs;  // by turn count, for the current cfg
     bool grid_auto = true;                      // grid follows the kernel's occupancy
+    bool waves_auto = true;                     // waves per SIMD follow the kernel
     void drop_graphs()
     {
         for (auto &kv : graphs) hipGraphExecDestroy(kv.second);
@@ -45,6 +46,7 @@ struct bmx_batch {
         const JitKernels *j = shared_jit();
         if (!fast_changed && j == cfg.jit) return;
         cfg.jit = j;
+        if (waves_auto) cfg.waves = default_waves(cfg.fast, cfg.jit);
         if (grid_auto) cfg.grid = default_grid(device, cfg.fast, cfg.waves, cfg.jit);
         drop_graphs();
     }
@@ -120,8 +122,9 @@ int32_t bmx_batch_create(bmx_ctx *ctx, const bmx_group *groups, int32_t n_groups
     }
     B->cfg.chunk_turns = ctx->chunk_turns;
     B->cfg.fast = ctx->fast_maps;
-    B->cfg.waves = ctx->waves ? ctx->waves : default_waves(B->cfg.fast);
     B->cfg.jit = B->shared_jit();
+    B->waves_auto = ctx->waves == 0;
+    B->cfg.waves = ctx->waves ? ctx->waves : default_waves(B->cfg.fast, B->cfg.jit);
     B->cfg.grid = default_grid(ctx->device, B->cfg.fast, B->cfg.waves, B->cfg.jit);
     *out = B;
     return BMX_OK;
@@ -146,8 +149,9 @@ int32_t bmx_batch_set_run(bmx_batch *B, int32_t chunk_turns, int32_t grid, int32
     hipSetDevice(B->ctx->device);
     hipStreamSynchronize(B->ctx->stream);
     B->cfg.chunk_turns = chunk_turns;
-    B->cfg.waves = waves ? waves : default_waves(B->cfg.fast);
     B->cfg.jit = B->shared_jit();
+    B->waves_auto = waves == 0;
+    B->cfg.waves = waves ? waves : default_waves(B->cfg.fast, B->cfg.jit);
     B->grid_auto = grid <= 0;
     B->cfg.grid = grid > 0 ? grid : default_grid(B->ctx->device, B->cfg.fast, B->cfg.waves, B->cfg.jit);
     B->drop_graphs();
@@ -274,6 +278,7 @@ int32_t bmx_batch_tune(bmx_batch *B, int32_t sample_turns, int32_t *chunk_out, i
     if (e != hipSuccess) return fail(e, "bmx_batch_tune");
     B->cfg = best;
     B->grid_auto = false;
+    B->waves_auto = false;
     B->drop_graphs();
     if (chunk_out) *chunk_out = best.chunk_turns;
     if (grid_out) *grid_out = best.grid;

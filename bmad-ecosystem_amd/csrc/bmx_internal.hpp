@@ -104,7 +104,12 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
                        unsigned long long *d_work);
 // Waves per SIMD of the default tracking kernel variant: 5 for the fast maps (the fastest
 // measured), 3 for the reference maps (whose 4- and 5-wave builds spill).
-inline int32_t default_waves(bool fast) { return fast ? 5 : 3; }
+// Default waves per SIMD of the tracking kernel: the variant measured fastest on the 901-element
+// ring for each kernel (cell-specialised 3, generic fast maps 5, reference maps 3).
+inline int32_t default_waves(bool fast, const JitKernels *jit = nullptr)
+{
+    return jit ? 3 : (fast ? 5 : 3);
+}
 // Workgroups of the tracking kernel that fit on the device at once (of the cell-specialised
 // kernels when jit is set).
 int default_grid(int device, bool fast, int waves, const JitKernels *jit = nullptr);

@@ -51,7 +51,8 @@ struct bmx_server {
     BeamPtrs base{};
     std::map<int64_t, int64_t> free_list;  // offset -> length
     RunBuffers rb;
-    RunConfig cfg;
+    RunConfig cfg;          // run shape of the generic kernel
+    bool waves_auto = true;  // waves per SIMD follow the kernel in use
     int32_t chunk_turns = 100;
     std::chrono::steady_clock::time_point t0;
 
@@ -199,9 +200,22 @@ void bmx_server::run()
             max_turns = std::max(max_turns, g.n_turns);
             tiles += (size_t)(r->n + 63) / 64;
         }
+        // the cell-specialised kernels when every request of this chunk tracks through lattices
+        // that share them
+        RunConfig run = cfg;
+        if (cfg.fast) {
+            const JitKernels *j = run_now[0]->lat->jit;
+            for (Request *r : run_now)
+                if (r->lat->jit != j) j = nullptr;
+            if (j) {
+                run.jit = j;
+                if (waves_auto) run.waves = default_waves(true, j);
+                run.grid = default_grid(device, true, run.waves, j);
+            }
+        }
         e = rb.reserve(groups.size(), (size_t)capacity, tiles + 1);
         if (e == hipSuccess)
-            e = enqueue_run(stream, rb, groups.data(), (int32_t)groups.size(), true, max_turns, cfg, d_work);
+            e = enqueue_run(stream, rb, groups.data(), (int32_t)groups.size(), true, max_turns, run, d_work);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         // finished requests: results to the host, pool space back
         std::vector<Request *> finished;
@@ -276,6 +290,7 @@ int32_t bmx_server_create(bmx_ctx *ctx, const bmx_server_config *cfg, bmx_server
     S->free_list[0] = S->capacity;
     S->cfg.chunk_turns = cfg->compact_turns ? cfg->compact_turns : 5;
     S->cfg.fast = ctx->fast_maps;
+    S->waves_auto = ctx->waves == 0;
     S->cfg.waves = ctx->waves ? ctx->waves : default_waves(S->cfg.fast);
     S->cfg.grid = default_grid(S->device, S->cfg.fast, S->cfg.waves);
     S->worker = std::thread([S] { S->run(); });

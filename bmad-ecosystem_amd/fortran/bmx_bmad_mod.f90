@@ -129,6 +129,12 @@ interface
     integer(c_int32_t) :: st
   end function
 
+  function c_lattice_specialize(lat) bind(c, name = 'bmx_lattice_specialize') result(st)
+    import :: c_int32_t, c_ptr
+    type(c_ptr), value :: lat
+    integer(c_int32_t) :: st
+  end function
+
   function c_lattice_destroy(lat) bind(c, name = 'bmx_lattice_destroy') result(st)
     import :: c_int32_t, c_ptr
     type(c_ptr), value :: lat
@@ -219,20 +225,23 @@ function bmx_last_error_message() result(msg)
 end function
 
 !+
-! Subroutine bmx_lat_init (lat, eles, mc2, charge, err_flag, device)
+! Subroutine bmx_lat_init (lat, eles, mc2, charge, err_flag, device, specialize)
 !
 ! Compiles the element table eles(1:n) (the tracking part of a lattice branch, in order) on
 ! device `device` (default 0). mc2 [eV] and charge are the species' (particle_species_mod.f90).
 ! err_flag is set, and the message is in bmx_last_error_message(), if an element is outside
-! the supported set or the reference momenta differ.
+! the supported set or the reference momenta differ. specialize (default false): also compile
+! tracking kernels for the lattice's repeated cell (bmx_lattice_specialize); a lattice without
+! one keeps the generic kernel.
 !-
-subroutine bmx_lat_init (lat, eles, mc2, charge, err_flag, device)
+subroutine bmx_lat_init (lat, eles, mc2, charge, err_flag, device, specialize)
   type(bmx_lat_struct), intent(inout) :: lat
   type(bmx_ele_struct), intent(in) :: eles(:)
   real(bmx_rp), intent(in) :: mc2
   integer, intent(in) :: charge
   logical, intent(out) :: err_flag
   integer, optional, intent(in) :: device
+  logical, optional, intent(in) :: specialize
   integer(c_int32_t) :: dev, st
 
   err_flag = .true.
@@ -249,6 +258,12 @@ subroutine bmx_lat_init (lat, eles, mc2, charge, err_flag, device)
   st = c_lattice_create(lat%ctx, eles, int(size(eles), c_int32_t), real(mc2, c_double), &
                         int(charge, c_int32_t), lat%lat)
   if (st /= bmx_ok) return
+  if (present(specialize)) then
+    if (specialize) then
+      st = c_lattice_specialize(lat%lat)
+      if (st /= bmx_ok .and. st /= bmx_err_unsupported) return
+    endif
+  endif
   lat%n_ele = size(eles)
   lat%p0c = 0
   if (size(eles) > 0) lat%p0c = eles(1)%p0c

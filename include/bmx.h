@@ -171,7 +171,8 @@ int32_t bmx_ctx_set_maps(bmx_ctx *ctx, int32_t form);
  * compacted between chunks; 0 = the whole call is one chunk; default 5), workgroups of the
  * persistent tracking grid (0 = as many as fit on the device at once, the default) and the
  * waves per SIMD the tracking kernel variant is compiled for at least (3, 4 or 5; 0 = the
- * default: 5 with the fast maps, 3 with the reference maps). */
+ * default: 3 for a cell-specialised lattice, else 5 with the fast maps, 3 with the reference
+ * maps). */
 int32_t bmx_ctx_set_run(bmx_ctx *ctx, int32_t chunk_turns, int32_t grid, int32_t waves);
 /* Kernel timing of bmx_track_turns: while on, every launch of the tracking kernel is bracketed
  * by HIP events on the context's stream; bmx_ctx_kernel_time synchronises and returns the sum
@@ -257,8 +258,8 @@ int32_t bmx_batch_destroy(bmx_batch *batch);
 /* Every group n_turns turns, turn indices turn0 .. turn0+n_turns-1 (asynchronous). */
 int32_t bmx_batch_track(bmx_batch *batch, int32_t n_turns, int32_t turn0);
 /* Run shape of the batch: turns per chunk, persistent grid (0 = device limit) and the waves
- * per SIMD the tracking kernel is compiled for at least (3, 4 or 5; 0 = the default: 5 with
- * the fast maps, 3 with the reference maps). */
+ * per SIMD the tracking kernel is compiled for at least (3, 4 or 5; 0 = the default: 3 for
+ * cell-specialised lattices, else 5 with the fast maps, 3 with the reference maps). */
 int32_t bmx_batch_set_run(bmx_batch *batch, int32_t chunk_turns, int32_t grid, int32_t waves);
 /* Autotuner: times sample_turns turns of every candidate run shape (chunk length x kernel
  * occupancy x grid size) on a scratch copy of the batch's particles (the batch's beams are not

@@ -148,3 +148,32 @@ def test_lattice_without_repeated_cell(bmx, ctx):
     lat = T.Lattice(ctx, E.to_array(eles))
     assert lat.specialize() is False
     lat.close()
+
+
+def test_server_with_specialized_lattices(bmx, ctx):
+    """Requests on three specialised variants through the continuous-batching server (37-turn
+    chunks): every particle equals tracking its request alone with the specialised kernel, bit
+    for bit (losses included)."""
+    T, L = bmx.tracking, bmx.lattices
+    arrs = [bmx.elements.to_array(e) for e in L.ring1_variants(4, 3, n_cells=10)]
+    lats = [T.Lattice(ctx, a) for a in arrs]
+    for lt in lats:
+        assert lt.specialize()
+    reqs = [(i % 3, 500 + 700 * i, 5 + 17 * i) for i in range(6)]
+    beams = [P.random_beam(bmx, n, seed=70 + i, scale=12.0, pz_sigma=4e-3)
+             for i, (_, n, _) in enumerate(reqs)]
+    print("server with specialised lattices", flush=True)
+    srv = T.Server(ctx, hbm_budget=1 << 30, chunk_turns=37, compact_turns=10)
+    ids = [srv.submit(lats[v], beams[i], t, 3) for i, (v, n, t) in enumerate(reqs)]
+    results = [srv.wait(r) for r in ids]
+    srv.close()
+    lost = 0
+    for i, (v, n, t) in enumerate(reqs):
+        out, _ = results[i]
+        alone = T.track_beam_host(arrs[v], beams[i], t, turn0=3, specialize=True)
+        for k in alone:
+            assert np.array_equal(out[k], alone[k]), (i, k)
+        lost += int((alone["state"] != 1).sum())
+    assert lost > 500
+    for lt in lats:
+        lt.close()
