@@ -160,6 +160,7 @@ bool aperture_setup(const bmx_ele &e, int32_t &flags, double ap[4], std::string 
         ap[2] = yc;
         ap[3] = 1 / yw;
         f |= F_APE_ELLIP;
+        if (xc == 0 && yc == 0) f |= F_APE_CENTERED;
         if (std::fabs(xc) + xw < 0.5 * kMaxAp && std::fabs(yc) + yw < 0.5 * kMaxAp) f |= F_APE_EXIT_COVERS;
     } else if (e.aperture_type == BMX_APERTURE_RECTANGULAR || e.aperture_type == 1 /* auto */ ||
                e.aperture_type == 0) {

@@ -39,7 +39,8 @@ typedef int32_t EleVec __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ EleRef load_ele(CHdr h)
 {
     const EleVec v = *(const EleVec BMX_CONST *)h;
-    return {v.x, v.y, v.z, v.w};
+    // F_APE_CENTERED only pays where the flags are constants; cleared here, its test folds away
+    return {v.x, v.y & ~F_APE_CENTERED, v.z, v.w};
 }
 
 constexpr double kPi = 3.141592653589793238462643383279;
@@ -155,10 +156,26 @@ __device__ __forceinline__ LaneMask orbit_bad_lanes(const Particle &p)
 
 // True if the aperture test of check_aperture_limit.f90:256-324 loses the particle (no state
 // change): elliptical r >= 1 (alive is strictly r < 1), rectangular outside a set limit.
+// Offsets from the centre of an elliptical aperture (x - 0 is x exactly, so a centred ellipse
+// skips the subtraction with identical results).
+__device__ __forceinline__ void ellipse_offsets(const Particle &p, int32_t flags, CP ap, double &xp,
+                                                double &yp)
+{
+    if (flags & F_APE_CENTERED) {
+        xp = p.x;
+        yp = p.y;
+    } else {
+        xp = p.x - ap[0];
+        yp = p.y - ap[2];
+    }
+}
+
 __device__ __forceinline__ bool aperture_fails(const Particle &p, int32_t flags, CP ap)
 {
     if (flags & F_APE_ELLIP) {
-        const double xr = (p.x - ap[0]) * ap[1], yr = (p.y - ap[2]) * ap[3];
+        double xp, yp;
+        ellipse_offsets(p, flags, ap, xp, yp);
+        const double xr = xp * ap[1], yr = yp * ap[3];
         return !(xr * xr + yr * yr < 1);
     }
     const double x1 = ap[0], x2 = ap[1], y1 = ap[2], y2 = ap[3];
@@ -169,7 +186,9 @@ __device__ __forceinline__ bool aperture_fails(const Particle &p, int32_t flags,
 __device__ __forceinline__ LaneMask aperture_fail_lanes(const Particle &p, int32_t flags, CP ap)
 {
     if (flags & F_APE_ELLIP) {
-        const double xr = (p.x - ap[0]) * ap[1], yr = (p.y - ap[2]) * ap[3];
+        double xp, yp;
+        ellipse_offsets(p, flags, ap, xp, yp);
+        const double xr = xp * ap[1], yr = yp * ap[3];
         return lanes(!(xr * xr + yr * yr < 1));
     }
     const double x1 = ap[0], x2 = ap[1], y1 = ap[2], y2 = ap[3];
@@ -186,7 +205,8 @@ __device__ __forceinline__ LaneMask aperture_fail_lanes(const Particle &p, int32
 __device__ __forceinline__ void aperture(Particle &p, int32_t flags, CP ap)
 {
     if (flags & F_APE_ELLIP) {
-        const double xp = p.x - ap[0], yp = p.y - ap[2];
+        double xp, yp;
+        ellipse_offsets(p, flags, ap, xp, yp);
         const double xr = xp * ap[1], yr = yp * ap[3];
         const double r = xr * xr + yr * yr;
         if (r < 1) return;
@@ -268,12 +288,12 @@ __device__ __forceinline__ void drift(Particle &p, double L)
     const double py_rel = p.py * p.inv_rp;
     const double pxy2 = px_rel * px_rel + py_rel * py_rel;
     const bool series = pxy2 < kDriftSeriesMax;
+    // the exact body for the rare lanes outside the series domain (gated per wave), then the
+    // series body, written once, for the rest
     if (any_lane(!series)) {
         if (!series) drift_exact(p, L);
-        else drift_series(p, L, px_rel, py_rel, pxy2);
-    } else {
-        drift_series(p, L, px_rel, py_rel, pxy2);
     }
+    if (series) drift_series(p, L, px_rel, py_rel, pxy2);
 }
 
 // ---- quadrupole --------------------------------------------------------------------------

@@ -38,6 +38,9 @@ enum Flags : int32_t {
     // lie well inside it), so that part of the exit validity test need not be gated separately.
     F_APE_EXIT_COVERS = 1 << 13,
     F_MULT_NORMAL = 1 << 14,  // K_MULT: no skew strengths (every a_n = 0)
+    // Elliptical aperture centred on the axis (x0 = y0 = 0). Used where the flags are
+    // compile-time constants (cell-specialised kernels); the table-driven loop masks it out.
+    F_APE_CENTERED = 1 << 15,
 };
 
 struct EleHdr {
