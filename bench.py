@@ -225,6 +225,10 @@ def run_batched(bmx, ctx, flops, n_var=512, turns=500, seed=2, specialize=True):
     alive = b.count_alive()
     batch.close()
     b.close()
+    # 6-D closed orbit and one-turn matrix of every variant, all in one call
+    t0 = time.perf_counter()
+    _, _, co_status = T.closed_orbit(ctx, lats, n_dim=6, max_iter=40)
+    t_co = time.perf_counter() - t0
     for lt in lats:
         lt.close()
     rate = passes / (ms * 1e-3)
@@ -232,7 +236,9 @@ def run_batched(bmx, ctx, flops, n_var=512, turns=500, seed=2, specialize=True):
                         f"({total} total), {turns} turns, one captured graph",
             "value": rate, "unit": UNIT, "ms": ms, "alive_at_end": alive,
             "tflops_fp64": rate * flops / 1e12, "frac_fp64_peak": rate * flops / 1e12 / FP64_PEAK_TFLOPS,
-            "tuned": shape, "tune_s": t_tune}
+            "tuned": shape, "tune_s": t_tune,
+            "closed_orbits": {"variants": n_var, "n_dim": 6, "s": t_co,
+                              "converged": int((co_status == 0).sum())}}
 
 
 def run_da(bmx, ctx, n_var=4, turns=10_000, seed=3, specialize=True):

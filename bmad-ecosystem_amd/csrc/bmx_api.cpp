@@ -138,14 +138,23 @@ int32_t bmx_lattice_create(bmx_ctx *ctx, const bmx_ele *eles, int32_t n_ele, dou
 {
     if (!ctx || !out) return set_error(BMX_ERR_ARG, "bmx_lattice_create: null argument");
     *out = nullptr;
+    HostTable t;
+    std::string err;
+    int rc = compile_table(eles, n_ele, mc2, charge, t, err);
+    if (rc != BMX_OK) return set_error(rc, "bmx_lattice_create: " + err);
+    return lattice_upload(ctx, std::move(t), out);
+}
+
+}  // extern "C"
+
+namespace bmx {
+
+int32_t lattice_upload(bmx_ctx *ctx, HostTable &&table, bmx_lattice **out)
+{
+    *out = nullptr;
     bmx_lattice *L = new (std::nothrow) bmx_lattice();
     if (!L) return set_error(BMX_ERR_NOMEM, "bmx_lattice_create: host allocation");
-    std::string err;
-    int rc = compile_table(eles, n_ele, mc2, charge, L->host, err);
-    if (rc != BMX_OK) {
-        delete L;
-        return set_error(rc, "bmx_lattice_create: " + err);
-    }
+    L->host = std::move(table);
     L->ctx = ctx;
     L->device = ctx->device;
     hipSetDevice(ctx->device);
@@ -170,6 +179,10 @@ int32_t bmx_lattice_create(bmx_ctx *ctx, const bmx_ele *eles, int32_t n_ele, dou
     *out = L;
     return BMX_OK;
 }
+
+}  // namespace bmx
+
+extern "C" {
 
 int32_t bmx_lattice_destroy(bmx_lattice *L)
 {

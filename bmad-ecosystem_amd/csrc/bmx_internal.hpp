@@ -151,3 +151,10 @@ struct bmx_beam {
     void *block = nullptr;  // single allocation when owned
     bmx::BeamPtrs d{};
 };
+
+namespace bmx {
+// Uploads a compiled element table as a new lattice on ctx's device (bmx_lattice_create's
+// device half).
+int32_t lattice_upload(bmx_ctx *ctx, HostTable &&table, bmx_lattice **out);
+}  // namespace bmx
+

@@ -72,6 +72,9 @@ _SIGS = {
     "bmx_lattice_destroy": (_I32, [_P]),
     "bmx_lattice_validate": (_I32, [_P, _I32, ctypes.c_double, _I32, ctypes.POINTER(_I32)]),
     "bmx_lattice_specialize": (_I32, [_P]),
+    "bmx_closed_orbit": (_I32, [_P, _PP, _I32, _I32, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double), _I32, ctypes.c_double,
+                                ctypes.c_double, ctypes.POINTER(_I32)]),
     "bmx_lattice_info": (_I32, [_P, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_uint64)]),
     "bmx_beam_create": (_I32, [_P, _I64, _PP]),
     "bmx_beam_wrap": (_I32, [_P, ctypes.POINTER(BeamView), _I64, _PP]),

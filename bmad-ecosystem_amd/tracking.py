@@ -205,6 +205,26 @@ def track_beam_plain(ctx: Context, lat: Lattice, beam: Beam, n_turns: int, turn0
     native.call("bmx_track_turns_plain", ctx.h, lat.h, beam.h, int(n_turns), int(turn0))
 
 
+def closed_orbit(ctx: Context, lats, n_dim: int = 4, guess=None, max_iter: int = 20,
+                 rel_tol: float = 1e-12, abs_tol: float = 1e-15):
+    """Closed orbits and one-turn matrices of several lattices at once (bmx_closed_orbit;
+    bmad/code/closed_orbit_calc.f90:192). guess: (n, 6) starting orbits (zeros by default; with
+    n_dim = 4 its pz is the energy the orbit is closed at). Returns (orbit (n, 6), one-turn matrix
+    (n, 6, 6), status (n,): 0 converged, 1 not converged, 2 probe lost, 3 singular)."""
+    lats = list(lats)
+    n = len(lats)
+    orb = np.zeros((n, 6)) if guess is None else np.array(guess, dtype=np.float64).reshape(n, 6)
+    orb = np.ascontiguousarray(orb)
+    mat = np.zeros((n, 6, 6))
+    status = np.zeros(n, dtype=np.int32)
+    handles = (ctypes.c_void_p * max(n, 1))(*[lt.h.value for lt in lats])
+    dp = ctypes.POINTER(ctypes.c_double)
+    native.call("bmx_closed_orbit", ctx.h, handles, n, int(n_dim), orb.ctypes.data_as(dp),
+                mat.ctypes.data_as(dp), int(max_iter), float(rel_tol), float(abs_tol),
+                status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return orb, mat, status
+
+
 def track_beam_host(eles, beam: dict, n_turns: int, device: int = 0, turn0: int = 0,
                     mc2: float = E.M_ELECTRON, charge: int = -1, maps: str = "fast",
                     specialize: bool = False) -> dict:

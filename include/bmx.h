@@ -267,6 +267,20 @@ int32_t bmx_batch_set_run(bmx_batch *batch, int32_t chunk_turns, int32_t grid, i
 int32_t bmx_batch_tune(bmx_batch *batch, int32_t sample_turns, int32_t *chunk_turns,
                        int32_t *grid, int32_t *waves, float *best_ms);
 
+/* ---- closed orbit and one-turn matrix (bmad/code/closed_orbit_calc.f90:192) ----------------- *
+ * For each of n_lat lattices: the fixed point of the one-turn map, found by Newton steps whose
+ * Jacobian is the one-turn matrix from central differences of tracked probes (13 particles per
+ * lattice per iteration, all lattices in one batch). n_dim = 4 closes x, px, y, py at the pz of
+ * the guess with the rf cavities off (as closed_orbit_calc's i_dim = 4); n_dim = 6 closes all
+ * six coordinates with the rf on. orbit[6 i .. 6 i + 5] holds lattice i's guess on entry and its
+ * closed orbit on return; mat (optional, 36 per lattice, row-major) receives the one-turn matrix
+ * d out_r / d in_c at the last iterate (finite differences, about 1e-9 relative). Converged when
+ * every Newton step is below |orbit| rel_tol + abs_tol (closed_orbit_calc.f90:534). status[i]:
+ * 0 converged, 1 not within max_iter, 2 a probe particle was lost, 3 singular (M - 1). */
+int32_t bmx_closed_orbit(bmx_ctx *ctx, bmx_lattice *const *lats, int32_t n_lat, int32_t n_dim,
+                         double *orbit, double *mat, int32_t max_iter, double rel_tol,
+                         double abs_tol, int32_t *status);
+
 /* ---- server: continuous batching of tracking requests ---------------------------------------- *
  * A server owns a worker thread, a HIP stream and a device pool of hbm_budget bytes for
  * particle state. Submitted requests (a lattice, n particles in caller-owned host arrays, a
