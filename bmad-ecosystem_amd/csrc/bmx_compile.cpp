@@ -42,6 +42,48 @@ bool misaligned(const bmx_ele &e)
            e.y_pitch != 0 || e.tilt != 0 || e.ref_tilt != 0;
 }
 
+// bmad/geometry/floor_angles_to_w_mat.f90:17-57 for (theta, phi, psi = 0): the rotation of
+// offset_particle's frame change (inverse: entering the element), row-major.
+void pitch_matrix(double theta, double phi, bool inverse, double *w)
+{
+    const double s_the = std::sin(theta), c_the = std::cos(theta);
+    const double s_phi = std::sin(phi), c_phi = std::cos(phi);
+    const double s_psi = std::sin(0.0), c_psi = std::cos(0.0);
+    if (!inverse) {
+        const double m[9] = {c_the * c_psi - s_the * s_phi * s_psi, -c_the * s_psi - s_the * s_phi * c_psi,
+                             s_the * c_phi, c_phi * s_psi, c_phi * c_psi, s_phi,
+                             -s_the * c_psi - c_the * s_phi * s_psi, s_the * s_psi - c_the * s_phi * c_psi,
+                             c_the * c_phi};
+        std::memcpy(w, m, sizeof m);
+    } else {
+        const double m[9] = {c_the * c_psi - s_the * s_phi * s_psi, c_phi * s_psi,
+                             -s_the * c_psi - c_the * s_phi * s_psi, -c_the * s_psi - s_the * s_phi * c_psi,
+                             c_phi * c_psi, s_the * s_psi - c_the * s_phi * c_psi, s_the * c_phi, s_phi,
+                             c_the * c_phi};
+        std::memcpy(w, m, sizeof m);
+    }
+}
+
+// The offset block of a misaligned or tilted element (bmx_table.hpp), offset_particle's
+// non-bend branch (offset_particle.f90:183-492).
+void offset_block(const bmx_ele &e, int32_t &flags, double *ob)
+{
+    const bool mis = e.x_offset != 0 || e.y_offset != 0 || e.z_offset != 0 || e.x_pitch != 0 ||
+                     e.y_pitch != 0;
+    for (int k = 0; k < kOffsetLen; k++) ob[k] = 0;
+    ob[0] = e.x_offset;
+    ob[1] = e.y_offset;
+    ob[2] = e.z_offset;
+    ob[3] = 0.5 * e.l;
+    pitch_matrix(e.x_pitch, e.y_pitch, true, ob + 4);
+    pitch_matrix(e.x_pitch, e.y_pitch, false, ob + 13);
+    ob[22] = std::cos(e.tilt);
+    ob[23] = std::sin(e.tilt);
+    flags |= F_OFFSET;
+    if (mis) flags |= F_MISALIGN;
+    if (e.tilt != 0) flags |= F_TILT;
+}
+
 std::string where(int i, const bmx_ele &e)
 {
     char buf[96];
@@ -242,9 +284,17 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             if (key == BMX_SBEND) { err = where(i, e) + "switched-off bends are not supported"; return BMX_ERR_UNSUPPORTED; }
             key = BMX_DRIFT;  // track1_bmad.f90:51-60
         }
-        // Misalignments: only drifts ignore them in the reference (track1_bmad.f90:93-94).
-        if (key != BMX_DRIFT && key != BMX_MARKER && misaligned(e)) {
-            err = where(i, e) + "offsets, pitches and tilts are not supported on the GPU path";
+        // Misalignments: drifts and markers ignore them in the reference (track1_bmad.f90:93-94);
+        // the other non-bend kinds get offset_particle's frame change around their map; the
+        // bend branch of offset_particle is not implemented here.
+        const bool offset = key != BMX_DRIFT && key != BMX_MARKER && misaligned(e);
+        if (offset && (key == BMX_SBEND || e.ref_tilt != 0)) {
+            err = where(i, e) + "bend offsets, pitches and tilts are not supported on the GPU path";
+            return BMX_ERR_UNSUPPORTED;
+        }
+        if (offset && e.offset_moves_aperture &&
+            (e.x1_limit != 0 || e.x2_limit != 0 || e.y1_limit != 0 || e.y2_limit != 0)) {
+            err = where(i, e) + "apertures that move with the element offsets are not supported";
             return BMX_ERR_UNSUPPORTED;
         }
         double a[BMX_N_POLE], b[BMX_N_POLE];
@@ -385,6 +435,11 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
         }
         if ((int)q.size() != kParams[h.kind]) { err = where(i, e) + "internal parameter count"; return BMX_ERR_ARG; }
         h.off = (int32_t)out.prm.size();
+        if (offset) {
+            double ob[kOffsetLen];
+            offset_block(e, h.flags, ob);
+            out.prm.insert(out.prm.end(), ob, ob + kOffsetLen);
+        }
         if (h.flags & (F_APE_ENT | F_APE_EXIT)) out.prm.insert(out.prm.end(), ap, ap + 4);
         out.prm.insert(out.prm.end(), q.begin(), q.end());
         out.hdr.push_back(h);
@@ -396,7 +451,9 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
         const EleHdr &g = out.hdr[i - 1];
         if (!(h.flags & F_APE_ENT) || !(g.flags & F_APE_EXIT)) continue;
         if ((h.flags & F_APE_ELLIP) != (g.flags & F_APE_ELLIP)) continue;
-        if (std::memcmp(&out.prm[h.off], &out.prm[g.off], 4 * sizeof(double)) != 0) continue;
+        const int32_t ha = h.off + ((h.flags & F_OFFSET) ? kOffsetLen : 0);
+        const int32_t ga = g.off + ((g.flags & F_OFFSET) ? kOffsetLen : 0);
+        if (std::memcmp(&out.prm[ha], &out.prm[ga], 4 * sizeof(double)) != 0) continue;
         h.flags = (h.flags & ~F_APE_ENT) | F_APE_ENT_SAME;
     }
     return BMX_OK;

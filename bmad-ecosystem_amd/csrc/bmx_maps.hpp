@@ -1043,20 +1043,103 @@ __device__ __forceinline__ void rfcavity(Particle &p, CP r, int32_t n_slice,
 // exit aperture (:329). h is the element's header (bmx_table.hpp); prm the lattice's parameter
 // array. Returns true when the particle was lost at the entrance
 // aperture (location = upstream_end$).
-// MAPS selects the element kinds compiled in: MAPS_ALL, or MAPS_NO_RF / MAPS_RF_ONLY for
-// loops that keep the (rare, register-hungry) rf cavity out of the common element path.
+// MAPS selects the code compiled in: MAPS_ALL (every kind, offsets), or the pair MAPS_NO_RF /
+// MAPS_RARE for loops that keep the rare, register-hungry cases out of the common element
+// path: MAPS_NO_RF has no rf cavity and no offset_particle frame change; MAPS_RARE has only
+// the kinds that can carry them (rf cavity, quadrupole, thick multipole) with both.
 // from_prev: the particle has just left the element before this one in the same lattice, so
 // an entrance test that repeats that element's exit test (F_APE_ENT_SAME, F_APE_ENT cleared)
 // is skipped; otherwise it is made.
 // A particle lost here gets p.loss_ele = ix1 (the element's 1-based index), p.loss_turn = turn.
-enum MapSet : int { MAPS_ALL = 0, MAPS_NO_RF = 1, MAPS_RF_ONLY = 2 };
+enum MapSet : int { MAPS_ALL = 0, MAPS_NO_RF = 1, MAPS_RARE = 2 };
+
+// ---- offset_particle, non-bend branch (bmad/code/offset_particle.f90:183-492) -------------
+// Element orientation +1, particle direction +1; ob is the offset block (bmx_table.hpp).
+
+// bmad/low_level/track_a_drift.f90:25-116 with include_ref_motion = False (dz = -L/ps): the
+// drift to the element edge after the frame change.
+__device__ __forceinline__ void drift_edge(Particle &p, double ds)
+{
+    const double px_rel = p.px / p.rel_p;
+    const double py_rel = p.py / p.rel_p;
+    const double pxy2 = px_rel * px_rel + py_rel * py_rel;
+    if (pxy2 >= 1) {
+        p.state = ST_LOST_PZ;
+        return;
+    }
+    const double ps = sqrt(1 - pxy2);
+    p.x = p.x + ds * px_rel / ps;
+    p.y = p.y + ds * py_rel / ps;
+    p.z = p.z + (-ds / ps);
+}
+
+// The frame change of a misaligned element (:183-230 entering, :430-470 leaving): position
+// (r0, r1, r2) and momentum (px, py, ps) rotated by w (3x3, row-major), then (add0, add1) added
+// to the new x, y; s_edge receives the rotated longitudinal position. A particle with
+// px^2 + py^2 >= (1 + pz)^2 is lost with its coordinates unchanged.
+__device__ __forceinline__ void offset_frame(Particle &p, CP w, double r0, double r1, double r2,
+                                             double add0, double add1, double &s_edge)
+{
+    const double s0 = w[0] * r0 + w[1] * r1 + w[2] * r2;
+    const double s1 = w[3] * r0 + w[4] * r1 + w[5] * r2;
+    s_edge = w[6] * r0 + w[7] * r1 + w[8] * r2;
+    const double pz2 = p.rel_p * p.rel_p - p.px * p.px - p.py * p.py;
+    if (pz2 <= 0) {
+        p.state = ST_LOST_PZ;
+        return;
+    }
+    const double ps = sqrt(pz2);
+    const double q0 = w[0] * p.px + w[1] * p.py + w[2] * ps;
+    const double q1 = w[3] * p.px + w[4] * p.py + w[5] * ps;
+    p.px = q0;
+    p.py = q1;
+    p.x = s0 + add0;
+    p.y = s1 + add1;
+}
+
+// bmad/code/tilt_coords.f90:18-52 with (cos, sin) of the angle.
+__device__ __forceinline__ void tilt_rotate(Particle &p, double c, double s)
+{
+    const double x = p.x, y = p.y, px = p.px, py = p.py;
+    p.x = c * x + s * y;
+    p.y = -s * x + c * y;
+    p.px = c * px + s * py;
+    p.py = -s * px + c * py;
+}
+
+__device__ __forceinline__ void offset_set(Particle &p, CP ob, int32_t flags)
+{
+    if (flags & F_MISALIGN) {
+        double s_body;
+        offset_frame(p, ob + 4, p.x - ob[0], p.y - ob[1], 0.0 - (ob[2] + ob[3]), 0.0, 0.0, s_body);
+        s_body = s_body + ob[3];
+        // drift from s_body to the entrance face (s = 0)
+        const double ds = 0.0 - s_body;
+        if (p.state == ST_ALIVE && fabs(ds) > 1e-14) drift_edge(p, ds);
+    }
+    if (flags & F_TILT) tilt_rotate(p, ob[22], ob[23]);
+}
+
+__device__ __forceinline__ void offset_unset(Particle &p, CP ob, int32_t flags)
+{
+    if (flags & F_TILT) tilt_rotate(p, ob[22], -ob[23]);
+    if (flags & F_MISALIGN) {
+        double s_lab;
+        // the exit face sits at ds_center = L/2 - L = -L/2 from the element centre
+        offset_frame(p, ob + 13, p.x, p.y, 0.0 - (-ob[3]), ob[0], ob[1], s_lab);
+        s_lab = s_lab + (ob[2] + ob[3]);
+        const double ds = (ob[3] + ob[3]) - s_lab;
+        if (p.state == ST_ALIVE && fabs(ds) > 1e-14) drift_edge(p, ds);
+    }
+}
 
 template <bool F, int MAPS = MAPS_ALL>
 __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, const Species &sp,
                                               bool from_prev, int32_t ix1, int32_t turn)
 {
     const int32_t kind = h.kind, flags = h.flags, n = h.n, off = h.off;
-    CP ap = prm + off;
+    CP ob = prm + off;
+    CP ap = (flags & F_OFFSET) ? ob + kOffsetLen : ob;
     CP q = (flags & (F_APE_ENT | F_APE_EXIT | F_APE_ENT_SAME)) ? ap + 4 : ap;
     if ((flags & F_APE_ENT) || (!from_prev && (flags & F_APE_ENT_SAME))) {
         const bool out = aperture_fails(p, flags, ap);
@@ -1069,24 +1152,34 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
             }
         }
     }
-    if constexpr (MAPS == MAPS_RF_ONLY) {
-        rfcavity<F>(p, q, n, sp);
-    } else {
-        switch (kind) {
-        case K_DRIFT: drift<F>(p, q[0]); break;
-        case K_QUAD: quadrupole<F>(p, q, n, sp); break;
-        case K_MULT: thick_multipole<F>(p, q, n, flags); break;
-        case K_BEND:
+    constexpr bool kOffsets = MAPS != MAPS_NO_RF;
+    if constexpr (kOffsets) {
+        if (flags & F_OFFSET) offset_set(p, ob, flags);
+    }
+    switch (kind) {
+    case K_DRIFT:
+        if constexpr (MAPS != MAPS_RARE) drift<F>(p, q[0]);
+        break;
+    case K_QUAD: quadrupole<F>(p, q, n, sp); break;
+    case K_MULT: thick_multipole<F>(p, q, n, flags); break;
+    case K_BEND:
+        if constexpr (MAPS != MAPS_RARE) {
             if ((flags & (F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE | F_BEND_DRIFT)) ==
                 (F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE))
                 sbend<F, true>(p, flags, q, sp);
             else
                 sbend<F, false>(p, flags, q, sp);
-            break;
-        case K_RF:
-            if constexpr (MAPS == MAPS_ALL) rfcavity<F>(p, q, n, sp);
-            break;
-        default: break;
+        }
+        break;
+    case K_RF:
+        if constexpr (MAPS != MAPS_NO_RF) rfcavity<F>(p, q, n, sp);
+        break;
+    default: break;
+    }
+    if constexpr (kOffsets) {
+        // the rf cavity returns before its exit offset when it loses the particle (pz = -1)
+        if (flags & F_OFFSET) {
+            if (kind != K_RF || p.state == ST_ALIVE) offset_unset(p, ob, flags);
         }
     }
     // orbit_too_large (overwrites any state), else the exit aperture for live particles; both

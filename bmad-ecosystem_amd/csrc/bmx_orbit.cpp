@@ -98,7 +98,8 @@ int32_t bmx_closed_orbit(bmx_ctx *ctx, bmx_lattice *const *lats, int32_t n_lat, 
             bool has_rf = false;
             for (const EleHdr &h : t.hdr)
                 if (h.kind == K_RF) {
-                    const int32_t q = h.off + ((h.flags & (F_APE_ENT | F_APE_EXIT | F_APE_ENT_SAME)) ? 4 : 0);
+                    const int32_t q = h.off + ((h.flags & F_OFFSET) ? kOffsetLen : 0) +
+                                      ((h.flags & (F_APE_ENT | F_APE_EXIT | F_APE_ENT_SAME)) ? 4 : 0);
                     t.prm[q + 1] = 0;  // voltage
                     has_rf = true;
                 }

@@ -41,7 +41,14 @@ enum Flags : int32_t {
     // Elliptical aperture centred on the axis (x0 = y0 = 0). Used where the flags are
     // compile-time constants (cell-specialised kernels); the table-driven loop masks it out.
     F_APE_CENTERED = 1 << 15,
+    // offset_particle's non-bend branch around the element map (offset_particle.f90:183-492):
+    // a block of kOffsetLen values precedes the aperture values. F_MISALIGN: offsets or pitches
+    // (the frame change with the drift to the element edge); F_TILT: a tilt rotation.
+    F_OFFSET = 1 << 16,
+    F_MISALIGN = 1 << 17,
+    F_TILT = 1 << 18,
 };
+constexpr int kOffsetLen = 24;
 
 struct EleHdr {
     int32_t kind;
@@ -63,7 +70,11 @@ struct EleHdr {
 //  K_RF    : [0] L/n_slice, [1] voltage * charge_dir, [2] phi0, [3] rf_frequency,
 //            [4] 0.5/rf_frequency, [5] twopi * rf_frequency, [6] twopi * phi0_multipass
 //  K_MARKER: (none)
-// preceded, when F_APE_ENT|F_APE_EXIT, by 4 aperture values (hdr.off points at them):
+// preceded, when F_OFFSET, by the offset block (hdr.off points at it; then the aperture values,
+// then the parameters):
+//  [0] x_offset, [1] y_offset, [2] z_offset, [3] L/2, [4..12] W^-1 (entrance frame change,
+//  row-major), [13..21] W (exit), [22] cos tilt, [23] sin tilt
+// and, when F_APE_ENT|F_APE_EXIT, by 4 aperture values (hdr.off points at them otherwise):
 //  elliptical : x_center, 1/x_width2, y_center, 1/y_width2
 //  rectangular: x1_lim (= -x1_limit), x2_lim, y1_lim, y2_lim (0 = no limit)
 constexpr int kParams[K_NKIND] = {1, 5, 9, 27, 7, 0};

@@ -64,14 +64,14 @@ __device__ __forceinline__ void track_range(Particle &p, const LatDesc &L, int32
             // keeps it (and the table reads) scalar whatever it makes of the loop's control flow
             e = __builtin_amdgcn_readfirstlane(e);
             const CHdr h = hdr + 4 * e;
-            if (h[0] == K_RF) break;
+            if (h[0] == K_RF || (h[1] & F_OFFSET)) break;
             if (p.state == ST_ALIVE)
                 track_element<F, MAPS_NO_RF>(p, load_ele(h), prm, L.sp, true, e + 1, t_abs);
         }
         if (e >= e1) break;
         e = __builtin_amdgcn_readfirstlane(e);
         if (p.state == ST_ALIVE)
-            track_element<F, MAPS_RF_ONLY>(p, load_ele(hdr + 4 * e), prm, L.sp, true, e + 1, t_abs);
+            track_element<F, MAPS_RARE>(p, load_ele(hdr + 4 * e), prm, L.sp, true, e + 1, t_abs);
         ++e;
     }
 }

@@ -1,0 +1,82 @@
+"""The reference's own known-answer vectors run on the GPU (through the C-ABI), and misaligned
+rings against the CPU restatement."""
+import numpy as np
+import pytest
+
+import kat
+import parity as P
+
+pytestmark = pytest.mark.gpu
+
+# not on the GPU path: apertures that move with the offsets, rf cavity multipoles
+OFF_PATH = {"RCOLLIMATOR1", "RFCAVITY3"}
+
+
+@pytest.mark.parametrize("maps", ["fast", "reference"])
+def test_tracking_method_known_answers_on_gpu(bmx, maps):
+    """tracking_method_test "<ELE>: Bmad_Standard" lines (ABS 2e-14, written es18.10): drift,
+    thick multipoles (sextupole, octupole, thick_multipole, kickers, collimator, instrument,
+    monitor) with offsets, pitch and tilt, and the rf cavity, each tracked by the GPU kernel
+    from the fixture's particle_start as a one-element lattice."""
+    E, T = bmx.elements, bmx.tracking
+    fx = kat.read_fixture("tracking_method_test.correct")
+    eles = kat.tracking_method_elements(E)
+    beta0 = kat.tm_start_beta(E)
+    failures = []
+    ran = 0
+    for name, ele in eles.items():
+        if name in OFF_PATH:
+            continue
+        typ, tol, expect = fx[f"{name}: Bmad_Standard"]
+        beam = {k: np.array([v]) for k, v in zip(P.COORDS, kat.TM_START)}
+        beam["beta"] = np.array([beta0])
+        beam["state"] = np.ones(1, dtype=np.int32)
+        beam["loss_ele"] = np.zeros(1, dtype=np.int32)
+        beam["loss_turn"] = np.zeros(1, dtype=np.int32)
+        out = T.track_beam_host(E.to_array([ele]), beam, 1, maps=maps)
+        assert out["state"][0] == E.STATE_ALIVE, name
+        vec = [float(out[k][0]) for k in P.COORDS]
+        ok, worst = kat.abs_match(vec, expect[:6], tol, 10)
+        if not ok:
+            failures.append((name, worst, [kat.fortran_es(v, 10) for v in vec], expect[:6]))
+        ran += 1
+    assert not failures, failures
+    assert ran == len(eles) - len(OFF_PATH)
+
+
+def _misaligned_ring(bmx, seed, n_cells=10):
+    """ring1 of n_cells with every quadrupole and sextupole offset, pitched and tilted at random
+    (sigma 1e-4 m, 1e-4 rad, 1e-3 rad)."""
+    E, L = bmx.elements, bmx.lattices
+    rng = np.random.default_rng(seed)
+    arr = E.to_array(L.ring1(n_cells))
+    sel = (arr["key"] == E.QUADRUPOLE) | (arr["key"] == E.SEXTUPOLE)
+    k = int(sel.sum())
+    arr["x_offset"][sel] = 1e-4 * rng.standard_normal(k)
+    arr["y_offset"][sel] = 1e-4 * rng.standard_normal(k)
+    arr["x_pitch"][sel] = 1e-4 * rng.standard_normal(k)
+    arr["y_pitch"][sel] = 1e-4 * rng.standard_normal(k)
+    arr["tilt"][sel] = 1e-3 * rng.standard_normal(k)
+    return arr
+
+
+@pytest.mark.parametrize("specialize", [False, True])
+def test_misaligned_ring_against_oracle(bmx, oracle, specialize):
+    """A 10-cell ring with misaligned, pitched and tilted quadrupoles and sextupoles, 20 turns,
+    generic and cell-specialised kernels: losses bit-exact and the survivors within 3x of the
+    spread FMA contraction alone causes in the CPU restatement (median, 90th, 99th
+    percentile)."""
+    T = bmx.tracking
+    arr = _misaligned_ring(bmx, 3)
+    beam = P.random_beam(bmx, 2000, seed=8, scale=15.0, pz_sigma=3e-3)
+    ref, noise = P.fma_noise(oracle, arr, beam, 20)
+    out = T.track_beam_host(arr, beam, 20, specialize=specialize)
+    assert (ref["state"] != 1).sum() > 50
+    bad = P.loss_mismatch(out, ref)
+    assert bad.size == 0, bad[:10]
+    alive = ref["state"] == 1
+    dev = P.rel_dev(out, ref, alive)[alive]
+    nz = noise[alive]
+    for q in (50, 90, 99):
+        assert np.percentile(dev, q) <= 3 * np.percentile(nz, q) + 1e-13, (
+            q, np.percentile(dev, q), np.percentile(nz, q))

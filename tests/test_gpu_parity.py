@@ -295,7 +295,7 @@ def test_edge_sizes(bmx, oracle, ctx):
 def test_unsupported_element_raises(bmx, ctx):
     E = bmx.elements
     p0c, et = bmx.lattices.reference()
-    bad = [E.make_element(E.QUADRUPOLE, p0c, et, l=0.3, k1=0.5, x_offset=1e-3)]
+    bad = [E.make_element(E.SBEND, p0c, et, l=1.0, g=0.01, x_offset=1e-3)]
     with pytest.raises(bmx.native.BmxError) as ei:
         bmx.tracking.Lattice(ctx, bad)
     assert ei.value.code == 2

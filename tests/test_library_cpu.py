@@ -59,8 +59,9 @@ def test_benchmark_rings_compile(bmx):
 
 
 @pytest.mark.parametrize("attrs,code", [
-    (dict(key=3, l=0.3, k1=0.5, x_offset=1e-3), 2),       # misaligned quad
-    (dict(key=3, l=0.3, k1=0.5, tilt=0.1), 2),            # tilted quad
+    (dict(key=2, l=1.0, g=0.01, x_offset=1e-3), 2),       # misaligned bend
+    (dict(key=2, l=1.0, g=0.01, ref_tilt=0.1), 2),        # tilted bend
+    (dict(key=3, l=0.3, k1=0.5, x_offset=1e-3, x_limit=0.01, offset_moves_aperture=1), 2),
     (dict(key=2, l=1.0, g=0.01, k1=0.1), 2),              # bend with k1
     (dict(key=2, l=1.0, g=0.01, fringe_type=4), 2),       # full bend fringe
     (dict(key=9, l=0.5, voltage=1e6, rf_frequency=0.0), 2),
@@ -84,6 +85,18 @@ def test_p0c_mismatch_rejected(bmx):
     b = E.make_element(E.DRIFT, p0c * 1.01, et, l=1.0)
     rc, _, msg = _validate(bmx, [a, b])
     assert rc == 4 and "element 2" in msg
+
+
+def test_misaligned_non_bend_elements_accepted(bmx):
+    """Quadrupoles, thick multipoles and rf cavities with offsets, pitches or tilt carry the
+    24-value offset_particle block ahead of their parameters (quadrupole: 5 + 24)."""
+    E = bmx.elements
+    p0c, et = bmx.lattices.reference()
+    for attrs in (dict(x_offset=1e-3), dict(y_pitch=1e-3), dict(tilt=0.1), dict(z_offset=0.01)):
+        rc, nprm, msg = _validate(bmx, [E.make_element(E.QUADRUPOLE, p0c, et, l=0.3, k1=0.5, **attrs)])
+        assert rc == 0 and nprm == 29, (attrs, msg)
+    rc, nprm, msg = _validate(bmx, [E.make_element(E.QUADRUPOLE, p0c, et, l=0.3, k1=0.5)])
+    assert rc == 0 and nprm == 5
 
 
 def test_misaligned_drift_accepted(bmx):
