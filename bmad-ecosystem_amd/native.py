@@ -72,6 +72,12 @@ _SIGS = {
     "bmx_lattice_destroy": (_I32, [_P]),
     "bmx_lattice_validate": (_I32, [_P, _I32, ctypes.c_double, _I32, ctypes.POINTER(_I32)]),
     "bmx_lattice_specialize": (_I32, [_P]),
+    "bmx_beam_file_info": (_I32, [ctypes.c_char_p, ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),
+    "bmx_beam_file_read": (_I32, [ctypes.c_char_p, _I32, ctypes.POINTER(BeamView), _I64,
+                                  ctypes.c_double, ctypes.c_double, ctypes.POINTER(_I64),
+                                  ctypes.POINTER(ctypes.c_double)]),
+    "bmx_beam_file_write": (_I32, [ctypes.c_char_p, ctypes.POINTER(BeamView), _I64, ctypes.c_char_p,
+                                   ctypes.c_double, ctypes.c_double, _I32, _I32]),
     "bmx_closed_orbit": (_I32, [_P, _PP, _I32, _I32, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double), _I32, ctypes.c_double,
                                 ctypes.c_double, ctypes.POINTER(_I32)]),

@@ -205,6 +205,28 @@ def track_beam_plain(ctx: Context, lat: Lattice, beam: Beam, n_turns: int, turn0
     native.call("bmx_track_turns_plain", ctx.h, lat.h, beam.h, int(n_turns), int(turn0))
 
 
+def read_beam_file(path: str, p0c: float, mc2: float = E.M_ELECTRON, ix_bunch: int = 1):
+    """Bmad ASCII beam file (bmx_beam_file_read) -> (host SoA dict, charge_tot)."""
+    nb, npart = ctypes.c_int32(), ctypes.c_int64()
+    native.call("bmx_beam_file_info", path.encode(), ctypes.byref(nb), ctypes.byref(npart))
+    cap = int(npart.value) if ix_bunch == 1 else 1 << 26
+    out = {k: np.zeros(cap, dtype=np.float64) for k in FLOAT_KEYS}
+    out.update({k: np.zeros(cap, dtype=np.int32) for k in INT_KEYS})
+    v = _host_view(out)
+    n, q = ctypes.c_int64(), ctypes.c_double()
+    native.call("bmx_beam_file_read", path.encode(), int(ix_bunch), ctypes.byref(v), cap, float(p0c),
+                float(mc2), ctypes.byref(n), ctypes.byref(q))
+    return {k: a[:n.value].copy() for k, a in out.items()}, q.value
+
+
+def write_beam_file(path: str, beam: dict, p0c: float, species: str = "Electron",
+                    charge: float = 0.0, ix_ele: int = 0, append: bool = False) -> None:
+    """Host SoA dict -> Bmad ASCII column-layout beam file (bmx_beam_file_write)."""
+    v = _host_view(beam)
+    native.call("bmx_beam_file_write", path.encode(), ctypes.byref(v), int(beam["x"].shape[0]),
+                species.encode(), float(charge), float(p0c), int(ix_ele), int(bool(append)))
+
+
 def closed_orbit(ctx: Context, lats, n_dim: int = 4, guess=None, max_iter: int = 20,
                  rel_tol: float = 1e-12, abs_tol: float = 1e-15):
     """Closed orbits and one-turn matrices of several lattices at once (bmx_closed_orbit;

@@ -220,6 +220,25 @@ int32_t bmx_beam_wrap(bmx_ctx *ctx, const bmx_beam_view *device, int64_t n, bmx_
 int32_t bmx_beam_upload(bmx_beam *beam, const bmx_beam_view *host, int64_t n);
 int32_t bmx_beam_download(bmx_beam *beam, const bmx_beam_view *host, int64_t n);
 int32_t bmx_beam_destroy(bmx_beam *beam);
+/* ---- beam files (bmad/multiparticle/beam_file_io.f90), host memory, no device needed ------ *
+ * Read Bmad's ASCII beam files: the old "!ASCII::3" layout (read_beam_file, :380-804) or the
+ * column layout (read_beam_ascii, :834-1040). bmx_beam_file_info gives the bunch count and the
+ * first bunch's particle count. bmx_beam_file_read fills bunch ix_bunch (1-based) into host
+ * arrays of capacity cap: x..pz and state from the file, loss_ele = the file's ix_ele for lost
+ * particles (0 for live ones), loss_turn = 0, beta from a beta column or else from pz, p0c and
+ * mc2 as init_coord sets it; n_out and charge_tot (sum of particle charges or the file's
+ * charge_tot) are optional. */
+int32_t bmx_beam_file_info(const char *path, int32_t *n_bunch, int64_t *n_particle);
+int32_t bmx_beam_file_read(const char *path, int32_t ix_bunch, const bmx_beam_view *host, int64_t cap,
+                           double p0c, double mc2, int64_t *n_out, double *charge_tot);
+/* Write n particles as one bunch in the column layout of write_ascii_beam_file (:123-350):
+ * constant columns as '#' parameters (charge per particle, species, location Downstream_End,
+ * ix_branch 0, direction and time_dir 1), then index, x..pz, p0c, state name and ix_ele (the
+ * given ix_ele for live particles, loss_ele for lost ones). Coordinates carry 17 significant
+ * digits (the reference writes 16), so reading a written file back is bit-identical: a
+ * tracking checkpoint. append != 0 adds the bunch to an existing file. */
+int32_t bmx_beam_file_write(const char *path, const bmx_beam_view *host, int64_t n, const char *species,
+                            double charge, double p0c, int32_t ix_ele, int32_t append);
 /* Count of particles with state == alive (lt_tracking_mod.f90:1672, bunch%n_live). */
 int32_t bmx_beam_count_alive(bmx_beam *beam, int64_t *n_alive);
 
