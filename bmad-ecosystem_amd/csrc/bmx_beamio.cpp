@@ -319,14 +319,15 @@ int32_t bmx_beam_file_write(const char *path, const bmx_beam_view *host, int64_t
     std::fprintf(f, "# %-15s= %d\n", "time_dir", 1);
     std::fprintf(f, "# %-15s= %s\n", "species", species);
     std::fprintf(f, "# %-15s= %s\n", "location", kLocationName[2]);
-    std::fprintf(f, "#!%6s%24s%24s%24s%24s%24s%24s%24s%14s%8s\n", "index", "x", "px", "y", "py", "z", "pz",
-                 "p0c", "state", "ix_ele");
+    std::fprintf(f, "#!%6s%24s%24s%24s%24s%24s%24s%24s%24s%14s%8s\n", "index", "x", "px", "y", "py", "z",
+                 "pz", "p0c", "beta", "state", "ix_ele");
     for (int64_t i = 0; i < n; i++) {
         const int s = st[i];
         const char *sn = (s >= 0 && s <= 8) ? kStateName[s] : "Lost";
         std::fprintf(f, "%8lld", (long long)(i + 1));
         for (int k = 0; k < 6; k++) std::fprintf(f, "%24.16E", v[k][i]);
-        std::fprintf(f, "%24.16E%14s%8d\n", p0c, sn, s == BMX_STATE_ALIVE ? ix_ele : le[i]);
+        std::fprintf(f, "%24.16E%24.16E%14s%8d\n", p0c, ((const double *)host->beta)[i], sn,
+                     s == BMX_STATE_ALIVE ? ix_ele : le[i]);
     }
     const bool ok = std::ferror(f) == 0;
     std::fclose(f);

@@ -233,7 +233,7 @@ int32_t bmx_beam_file_read(const char *path, int32_t ix_bunch, const bmx_beam_vi
                            double p0c, double mc2, int64_t *n_out, double *charge_tot);
 /* Write n particles as one bunch in the column layout of write_ascii_beam_file (:123-350):
  * constant columns as '#' parameters (charge per particle, species, location Downstream_End,
- * ix_branch 0, direction and time_dir 1), then index, x..pz, p0c, state name and ix_ele (the
+ * ix_branch 0, direction and time_dir 1), then index, x..pz, p0c, beta, state name and ix_ele (the
  * given ix_ele for live particles, loss_ele for lost ones). Coordinates carry 17 significant
  * digits (the reference writes 16), so reading a written file back is bit-identical: a
  * tracking checkpoint. append != 0 adds the bunch to an existing file. */

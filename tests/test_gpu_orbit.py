@@ -135,3 +135,34 @@ def test_closed_orbit_6d_batch_of_variants(bmx, oracle, ctx):
         assert _symplectic_defect(mat[v]) < 1e-6
     # the variants differ: their orbits are not all the same
     assert np.ptp(orb[:, 0]) > 0
+
+
+def test_checkpoint_resume_is_bit_identical(bmx, ctx, tmp_path):
+    """Track 12 turns, write the beam file, read it back and track 13 more from turn 12: every
+    particle equals 25 turns in one call, bit for bit (losses included)."""
+    T, L = bmx.tracking, bmx.lattices
+    arr = bmx.elements.to_array(L.ring1(n_cells=10))
+    p0c, _ = L.reference()
+    beam = P.random_beam(bmx, 3000, seed=12, scale=12.0, pz_sigma=4e-3)
+    lat = T.Lattice(ctx, arr)
+    b = T.Beam.from_host(ctx, beam)
+    T.track_beam(ctx, lat, b, 25)
+    ctx.synchronize()
+    straight = b.download()
+    b.upload(beam)
+    T.track_beam(ctx, lat, b, 12)
+    ctx.synchronize()
+    mid = b.download()
+    path = str(tmp_path / "checkpoint.dat")
+    T.write_beam_file(path, mid, p0c, ix_ele=len(arr))
+    back, _ = T.read_beam_file(path, p0c)
+    back["loss_turn"] = mid["loss_turn"]  # the file has no loss turn
+    b.upload(back)
+    T.track_beam(ctx, lat, b, 13, 12)
+    ctx.synchronize()
+    resumed = b.download()
+    b.close()
+    lat.close()
+    assert (straight["state"] != 1).sum() > 100
+    for k in straight:
+        assert np.array_equal(resumed[k], straight[k]), k
