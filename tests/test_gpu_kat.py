@@ -80,3 +80,31 @@ def test_misaligned_ring_against_oracle(bmx, oracle, specialize):
     for q in (50, 90, 99):
         assert np.percentile(dev, q) <= 3 * np.percentile(nz, q) + 1e-13, (
             q, np.percentile(dev, q), np.percentile(nz, q))
+
+
+def test_lattice_file_ring_against_oracle(bmx, oracle):
+    """The test ring written in the Bmad language (tests/golden/lattices/ring_small.bmad: rbends
+    with edge angles, sextupoles, a corrector, monitors, an rf cavity given by frequency), loaded
+    by the lattice-file loader and tracked 30 turns: losses bit-exact and the survivors within 3x
+    of the oracle's FMA spread."""
+    import importlib
+    import os
+    lf = importlib.import_module(bmx.__name__ + ".lattice_file")
+    eles, mc2, charge = lf.load(os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                             "golden", "lattices", "ring_small.bmad"))
+    E, T, L = bmx.elements, bmx.tracking, bmx.lattices
+    arr = E.to_array(eles)
+    beam = P.random_beam(bmx, 2000, seed=4, scale=30.0, pz_sigma=3e-3)
+    p0c, et = float(arr[0]["p0c"]), float(arr[0]["e_tot"])
+    beam["beta"] = L.init_beta(beam["pz"], p0c, et, mc2)
+    ref, noise = P.fma_noise(oracle, arr, beam, 30)
+    out = T.track_beam_host(arr, beam, 30, mc2=mc2, charge=charge)
+    assert (ref["state"] != 1).sum() > 20
+    bad = P.loss_mismatch(out, ref)
+    assert bad.size == 0, bad[:10]
+    alive = ref["state"] == 1
+    dev = P.rel_dev(out, ref, alive)[alive]
+    nz = noise[alive]
+    for q in (50, 90, 99):
+        assert np.percentile(dev, q) <= 3 * np.percentile(nz, q) + 1e-13, (
+            q, np.percentile(dev, q), np.percentile(nz, q))
