@@ -115,7 +115,12 @@ def main():
     ctx.set_run(args.chunk_turns, 0, args.waves)
     lat = T.Lattice(ctx, arr)
     t0 = time.perf_counter()
-    special = args.specialize == "on" and args.maps == "fast" and lat.specialize()
+    special, special_err = False, None
+    if args.specialize == "on" and args.maps == "fast":
+        try:
+            special = lat.specialize()
+        except bmx.native.BmxError as e:  # reported in the JSON; the generic kernel runs
+            special_err = str(e)
     t_special = time.perf_counter() - t0
     waves = args.waves or (3 if special else (5 if args.maps == "fast" else 3))
     kernel_name = f"bmx_cell_tiles_{waves}" if special else f"k_track_tiles<{str(args.maps == 'fast').lower()}, {waves}>"
@@ -171,7 +176,7 @@ def main():
             "particles": args.particles, "elements": len(arr), "turns": args.steps * tps,
             "turns_per_step": tps, "alive_at_end": alive, "wall_s": t_wall,
             "maps": args.maps, "cell_specialized": bool(special), "waves_per_simd": waves,
-            "specialize_s": t_special,
+            "specialize_s": t_special, "specialize_error": special_err,
         },
         "roofline": {
             "bound": "mfma", "compute": "fp64 vector ALU", "achieved": achieved,
