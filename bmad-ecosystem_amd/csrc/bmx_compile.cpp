@@ -174,23 +174,32 @@ int ele_to_ab(const bmx_ele &e, double *a, double *b)
 }
 
 // Aperture flags and the 4 aperture values; returns false if unsupported.
-bool aperture_setup(const bmx_ele &e, int32_t &flags, double ap[4], std::string &why)
+bool aperture_setup(const bmx_ele &e, int32_t &flags, double ap[4], double mom[8], std::string &why)
 {
     int at = e.aperture_at;
     if (at == BMX_CONTINUOUS) { why = "continuous apertures"; return false; }
     if (at != BMX_ENTRANCE_END && at != BMX_EXIT_END && at != BMX_BOTH_ENDS) return true;
-    if (e.key == BMX_ECOLLIMATOR || e.key == BMX_RCOLLIMATOR) {
-        if (e.px_aperture_width2 != 0 || e.py_aperture_width2 != 0 || e.z_aperture_width2 != 0 ||
-            e.pz_aperture_width2 != 0) {
-            why = "collimator momentum apertures";
-            return false;
-        }
+    int32_t f = 0;
+    // collimator momentum windows (check_aperture_limit.f90, tested before the x-y aperture)
+    const bool windows = (e.key == BMX_ECOLLIMATOR || e.key == BMX_RCOLLIMATOR) &&
+                         (e.px_aperture_width2 != 0 || e.py_aperture_width2 != 0 ||
+                          e.z_aperture_width2 != 0 || e.pz_aperture_width2 != 0);
+    if (windows) {
+        const double m[8] = {e.px_aperture_width2, e.px_aperture_center, e.py_aperture_width2,
+                             e.py_aperture_center, e.z_aperture_width2,  e.z_aperture_center,
+                             e.pz_aperture_width2, e.pz_aperture_center};
+        std::memcpy(mom, m, sizeof m);
+        f |= F_APE_MOMENTUM;
+        if (e.key == BMX_ECOLLIMATOR) f |= F_APE_MOM_ELLIP;
     }
     const double x1 = -e.x1_limit, x2 = e.x2_limit, y1 = -e.y1_limit, y2 = e.y2_limit;
-    int32_t f = 0;
     if (e.aperture_type == BMX_APERTURE_ELLIPTICAL) {
-        // elliptical_params_setup: a half-set pair makes the reference skip the test.
-        if ((x1 == 0) != (x2 == 0) || (y1 == 0) != (y2 == 0)) return true;
+        // elliptical_params_setup: a half-set pair makes the reference skip the x-y test.
+        if ((x1 == 0) != (x2 == 0) || (y1 == 0) != (y2 == 0)) {
+            if (!windows) return true;
+            why = "momentum windows with a half-set elliptical aperture";
+            return false;
+        }
         double xw = (x2 - x1) / 2, xc = (x2 + x1) / 2;
         double yw = (y2 - y1) / 2, yc = (y2 + y1) / 2;
         if (xw == 0) xw = kMaxAp;
@@ -206,7 +215,7 @@ bool aperture_setup(const bmx_ele &e, int32_t &flags, double ap[4], std::string 
         if (std::fabs(xc) + xw < 0.5 * kMaxAp && std::fabs(yc) + yw < 0.5 * kMaxAp) f |= F_APE_EXIT_COVERS;
     } else if (e.aperture_type == BMX_APERTURE_RECTANGULAR || e.aperture_type == 1 /* auto */ ||
                e.aperture_type == 0) {
-        if (x1 == 0 && x2 == 0 && y1 == 0 && y2 == 0) return true;
+        if (x1 == 0 && x2 == 0 && y1 == 0 && y2 == 0 && !windows) return true;
         if (x1 != 0 && x2 != 0 && y1 != 0 && y2 != 0 && std::fabs(x1) < 0.5 * kMaxAp &&
             std::fabs(x2) < 0.5 * kMaxAp && std::fabs(y1) < 0.5 * kMaxAp && std::fabs(y2) < 0.5 * kMaxAp)
             f |= F_APE_EXIT_COVERS;
@@ -428,8 +437,8 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             err = where(i, e) + "element kind is not implemented on the GPU path";
             return BMX_ERR_UNSUPPORTED;
         }
-        double ap[4] = {0, 0, 0, 0};
-        if (!aperture_setup(e, h.flags, ap, why)) {
+        double ap[4] = {0, 0, 0, 0}, mom[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (!aperture_setup(e, h.flags, ap, mom, why)) {
             err = where(i, e) + why + " are not supported";
             return BMX_ERR_UNSUPPORTED;
         }
@@ -441,6 +450,7 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             out.prm.insert(out.prm.end(), ob, ob + kOffsetLen);
         }
         if (h.flags & (F_APE_ENT | F_APE_EXIT)) out.prm.insert(out.prm.end(), ap, ap + 4);
+        if (h.flags & F_APE_MOMENTUM) out.prm.insert(out.prm.end(), mom, mom + 8);
         out.prm.insert(out.prm.end(), q.begin(), q.end());
         out.hdr.push_back(h);
     }
@@ -450,6 +460,7 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
         EleHdr &h = out.hdr[i];
         const EleHdr &g = out.hdr[i - 1];
         if (!(h.flags & F_APE_ENT) || !(g.flags & F_APE_EXIT)) continue;
+        if ((h.flags | g.flags) & F_APE_MOMENTUM) continue;
         if ((h.flags & F_APE_ELLIP) != (g.flags & F_APE_ELLIP)) continue;
         const int32_t ha = h.off + ((h.flags & F_OFFSET) ? kOffsetLen : 0);
         const int32_t ga = g.off + ((g.flags & F_OFFSET) ? kOffsetLen : 0);

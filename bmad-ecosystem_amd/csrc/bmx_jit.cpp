@@ -81,9 +81,10 @@ std::string generate(const HostTable &t, const Period &pd, std::string &sig)
         const EleHdr &h = t.hdr[pd.H + j];
         const int32_t rel = h.off - t.hdr[pd.H].off;
         key << '/' << h.kind << ',' << h.flags << ',' << h.n << ',' << rel;
-        // the rare set only where the element is a cavity or carries offsets: elsewhere pz
-        // cannot change and the exit test for rel_p < 0 drops out, as in the generic loop
-        const char *maps = (h.kind == K_RF || (h.flags & F_OFFSET)) ? "MAPS_RARE" : "MAPS_NO_RF";
+        // the rare set only where the element is a cavity or carries offsets or momentum
+        // windows: elsewhere pz cannot change and the exit test for rel_p < 0 drops out, as in
+        // the generic loop
+        const char *maps = (h.kind == K_RF || (h.flags & (F_OFFSET | F_APE_MOMENTUM))) ? "MAPS_RARE" : "MAPS_NO_RF";
         body << "            if (p.state == ST_ALIVE)\n"
              << "                track_element<true, " << maps << ">(p, EleRef{" << h.kind << ", " << h.flags
              << ", " << h.n << ", " << rel << "}, base, L.sp, true, e1 + " << j << ", t_abs);\n";

@@ -178,8 +178,47 @@ __device__ __forceinline__ void ellipse_offsets(const Particle &p, int32_t flags
     }
 }
 
+// Collimator momentum windows (check_aperture_limit.f90, ecollimator / rcollimator branches):
+// m[] = px, py, z, pz (half width, centre) pairs; an ecollimator with both widths of a pair set
+// tests that pair as an ellipse. True if the particle is outside; u receives the unstable
+// factor as the reference computes it (each failing test overwrites it).
+__device__ __forceinline__ bool window_out(double v, double w, double c, double &u)
+{
+    if (w == 0 || (v > c - w && v < c + w)) return false;
+    const double a = (c - w) - v, b = v - (c + w);
+    u = a > b ? a : b;
+    return true;
+}
+
+__device__ __forceinline__ bool window_pair_out(double v0, double v1, CP m, bool ellip, double &u)
+{
+    const double w0 = m[0], c0 = m[1], w1 = m[2], c1 = m[3];
+    if (ellip && w0 != 0 && w1 != 0) {
+        const double a = (v0 - c0) / w0, b = (v1 - c1) / w1;
+        const double r = a * a + b * b;
+        if (!(r > 1)) return false;
+        u = sqrt(r) - 1;
+        return true;
+    }
+    const bool o0 = window_out(v0, w0, c0, u);
+    const bool o1 = window_out(v1, w1, c1, u);
+    return o0 | o1;
+}
+
+__device__ __forceinline__ bool momentum_out(const Particle &p, int32_t flags, CP m, double &u)
+{
+    const bool ellip = flags & F_APE_MOM_ELLIP;
+    const bool a = window_pair_out(p.px, p.py, m, ellip, u);
+    const bool b = window_pair_out(p.z, p.pz, m + 4, ellip, u);
+    return a | b;
+}
+
 __device__ __forceinline__ bool aperture_fails(const Particle &p, int32_t flags, CP ap)
 {
+    if (flags & F_APE_MOMENTUM) {
+        double u;
+        if (momentum_out(p, flags, ap + 4, u)) return true;
+    }
     if (flags & F_APE_ELLIP) {
         double xp, yp;
         ellipse_offsets(p, flags, ap, xp, yp);
@@ -193,11 +232,16 @@ __device__ __forceinline__ bool aperture_fails(const Particle &p, int32_t flags,
 // The lanes on which aperture_fails holds (the limits are wave-uniform).
 __device__ __forceinline__ LaneMask aperture_fail_lanes(const Particle &p, int32_t flags, CP ap)
 {
+    LaneMask mom = 0;
+    if (flags & F_APE_MOMENTUM) {
+        double u;
+        mom = lanes(momentum_out(p, flags, ap + 4, u));
+    }
     if (flags & F_APE_ELLIP) {
         double xp, yp;
         ellipse_offsets(p, flags, ap, xp, yp);
         const double xr = xp * ap[1], yr = yp * ap[3];
-        return lanes(!(xr * xr + yr * yr < 1));
+        return mom | lanes(!(xr * xr + yr * yr < 1));
     }
     const double x1 = ap[0], x2 = ap[1], y1 = ap[2], y2 = ap[3];
     LaneMask m = 0;
@@ -205,13 +249,19 @@ __device__ __forceinline__ LaneMask aperture_fail_lanes(const Particle &p, int32
     if (x2 != 0) m |= lanes(p.x > x2);
     if (y1 != 0) m |= lanes(p.y < y1);
     if (y2 != 0) m |= lanes(p.y > y2);
-    return m;
+    return m | mom;
 }
 
 // Aperture test of check_aperture_limit.f90:256-324 (elliptical with strict r < 1, or
 // rectangular). ap[] layout in bmx_table.hpp.
 __device__ __forceinline__ void aperture(Particle &p, int32_t flags, CP ap)
 {
+    // momentum windows first: lost (state lost$) with their unstable factor, which the
+    // spatial test below can override
+    double u = 0;
+    if (flags & F_APE_MOMENTUM) {
+        if (momentum_out(p, flags, ap + 4, u)) p.state = ST_LOST;
+    }
     if (flags & F_APE_ELLIP) {
         double xp, yp;
         ellipse_offsets(p, flags, ap, xp, yp);
@@ -225,7 +275,7 @@ __device__ __forceinline__ void aperture(Particle &p, int32_t flags, CP ap)
         const bool out = (x1 != 0 && p.x < x1) | (x2 != 0 && p.x > x2) | (y1 != 0 && p.y < y1) |
                          (y2 != 0 && p.y > y2);
         if (!out) return;
-        double u = 0, f;
+        double f;
         if (x1 != 0 && p.x < x1) { f = fabs((p.x - x1) / (x2 - x1)); if (f > u) { p.state = ST_LOST_NEG_X; u = f; } }
         if (x2 != 0 && p.x > x2) { f = fabs((p.x - x2) / (x2 - x1)); if (f > u) { p.state = ST_LOST_POS_X; u = f; } }
         if (y1 != 0 && p.y < y1) { f = fabs((p.y - y1) / (y2 - y1)); if (f > u) { p.state = ST_LOST_NEG_Y; u = f; } }
@@ -1045,8 +1095,8 @@ __device__ __forceinline__ void rfcavity(Particle &p, CP r, int32_t n_slice,
 // aperture (location = upstream_end$).
 // MAPS selects the code compiled in: MAPS_ALL (every kind, offsets), or the pair MAPS_NO_RF /
 // MAPS_RARE for loops that keep the rare, register-hungry cases out of the common element
-// path: MAPS_NO_RF has no rf cavity and no offset_particle frame change; MAPS_RARE has only
-// the kinds that can carry them (rf cavity, quadrupole, thick multipole) with both.
+// path: MAPS_NO_RF has no rf cavity, no offset_particle frame change and no collimator
+// momentum windows; MAPS_RARE has only the kinds that can carry them (no bends) with all three.
 // from_prev: the particle has just left the element before this one in the same lattice, so
 // an entrance test that repeats that element's exit test (F_APE_ENT_SAME, F_APE_ENT cleared)
 // is skipped; otherwise it is made.
@@ -1137,15 +1187,21 @@ template <bool F, int MAPS = MAPS_ALL>
 __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, const Species &sp,
                                               bool from_prev, int32_t ix1, int32_t turn)
 {
-    const int32_t kind = h.kind, flags = h.flags, n = h.n, off = h.off;
+    const int32_t kind = h.kind, n = h.n, off = h.off;
+    // The common element path (MAPS_NO_RF) never meets offsets or momentum windows: the loops
+    // send those elements to the rare path, so here their bits are known to be clear and every
+    // test and table offset that depends on them folds away.
+    const int32_t flags = MAPS != MAPS_NO_RF ? h.flags : (h.flags & ~(F_OFFSET | F_APE_MOMENTUM));
     CP ob = prm + off;
     CP ap = (flags & F_OFFSET) ? ob + kOffsetLen : ob;
     CP q = (flags & (F_APE_ENT | F_APE_EXIT | F_APE_ENT_SAME)) ? ap + 4 : ap;
+    if (flags & F_APE_MOMENTUM) q += 8;
+    const int32_t aflags = flags;
     if ((flags & F_APE_ENT) || (!from_prev && (flags & F_APE_ENT_SAME))) {
-        const bool out = aperture_fails(p, flags, ap);
+        const bool out = aperture_fails(p, aflags, ap);
         if (any_lane(out)) {
             if (out) {
-                aperture(p, flags, ap);
+                aperture(p, aflags, ap);
                 p.loss_ele = uniform_here(ix1);
                 p.loss_turn = uniform_here(turn);
                 return true;
@@ -1157,9 +1213,7 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
         if (flags & F_OFFSET) offset_set(p, ob, flags);
     }
     switch (kind) {
-    case K_DRIFT:
-        if constexpr (MAPS != MAPS_RARE) drift<F>(p, q[0]);
-        break;
+    case K_DRIFT: drift<F>(p, q[0]); break;
     case K_QUAD: quadrupole<F>(p, q, n, sp); break;
     case K_MULT: thick_multipole<F>(p, q, n, flags); break;
     case K_BEND:
@@ -1193,14 +1247,14 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
     if constexpr (MAPS != MAPS_NO_RF) hit_lanes |= lanes(p.rel_p < 0);
     if (!(flags & F_APE_EXIT_COVERS))
         hit_lanes |= lanes(fabs(p.x) > kMaxAperture) | lanes(fabs(p.y) > kMaxAperture);
-    if (flags & F_APE_EXIT) hit_lanes |= aperture_fail_lanes(p, flags, ap);
+    if (flags & F_APE_EXIT) hit_lanes |= aperture_fail_lanes(p, aflags, ap);
     if (hit_lanes) {
         const bool bad = orbit_bad(p);
         const bool alive = p.state == ST_ALIVE;
-        const bool ape = (flags & F_APE_EXIT) ? aperture_fails(p, flags, ap) : false;
+        const bool ape = (flags & F_APE_EXIT) ? aperture_fails(p, aflags, ap) : false;
         if (bad | ape | !alive) {
             if (bad) orbit_too_large(p);
-            else if (alive) aperture(p, flags, ap);
+            else if (alive) aperture(p, aflags, ap);
             p.loss_ele = uniform_here(ix1);
             p.loss_turn = uniform_here(turn);
         }

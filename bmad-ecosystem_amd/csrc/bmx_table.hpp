@@ -47,6 +47,10 @@ enum Flags : int32_t {
     F_OFFSET = 1 << 16,
     F_MISALIGN = 1 << 17,
     F_TILT = 1 << 18,
+    // collimator momentum windows: 8 values after the aperture values (px, py, z, pz half
+    // width and centre); F_APE_MOM_ELLIP: the ecollimator's elliptical pairs
+    F_APE_MOMENTUM = 1 << 19,
+    F_APE_MOM_ELLIP = 1 << 20,
 };
 constexpr int kOffsetLen = 24;
 
@@ -74,7 +78,8 @@ struct EleHdr {
 // then the parameters):
 //  [0] x_offset, [1] y_offset, [2] z_offset, [3] L/2, [4..12] W^-1 (entrance frame change,
 //  row-major), [13..21] W (exit), [22] cos tilt, [23] sin tilt
-// and, when F_APE_ENT|F_APE_EXIT, by 4 aperture values (hdr.off points at them otherwise):
+// and, when F_APE_ENT|F_APE_EXIT, by 4 aperture values (hdr.off points at them otherwise), then
+// when F_APE_MOMENTUM by 8 momentum-window values:
 //  elliptical : x_center, 1/x_width2, y_center, 1/y_width2
 //  rectangular: x1_lim (= -x1_limit), x2_lim, y1_lim, y2_lim (0 = no limit)
 constexpr int kParams[K_NKIND] = {1, 5, 9, 27, 7, 0};

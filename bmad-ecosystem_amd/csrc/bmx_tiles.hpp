@@ -64,7 +64,7 @@ __device__ __forceinline__ void track_range(Particle &p, const LatDesc &L, int32
             // keeps it (and the table reads) scalar whatever it makes of the loop's control flow
             e = __builtin_amdgcn_readfirstlane(e);
             const CHdr h = hdr + 4 * e;
-            if (h[0] == K_RF || (h[1] & F_OFFSET)) break;
+            if (h[0] == K_RF || (h[1] & (F_OFFSET | F_APE_MOMENTUM))) break;
             if (p.state == ST_ALIVE)
                 track_element<F, MAPS_NO_RF>(p, load_ele(h), prm, L.sp, true, e + 1, t_abs);
         }

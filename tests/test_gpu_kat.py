@@ -108,3 +108,29 @@ def test_lattice_file_ring_against_oracle(bmx, oracle):
     for q in (50, 90, 99):
         assert np.percentile(dev, q) <= 3 * np.percentile(nz, q) + 1e-13, (
             q, np.percentile(dev, q), np.percentile(nz, q))
+
+
+@pytest.mark.parametrize("maps", ["fast", "reference"])
+def test_aperture_known_answers_on_gpu(bmx, maps):
+    """aperture_test EC_S_* / RC_S_* lines: the state codes of 54 particles per collimator
+    (x-y apertures with px, py, z, pz windows, elliptical pairs for the ecollimator) after the
+    exit-end test of a one-element lattice on the GPU, exactly as the fixture lists them."""
+    E, T, L = bmx.elements, bmx.tracking, bmx.lattices
+    fx = kat.read_fixture("aperture_test.correct")
+    mc2 = 105.6583755e6
+    for ename, ele in kat.aperture_test_elements(E).items():
+        cases = kat.aperture_test_cases()
+        vecs = np.array([v for _, vs in cases for v in vs], dtype=np.float64)
+        beam = {k: np.ascontiguousarray(vecs[:, i]) for i, k in enumerate(P.COORDS)}
+        beam["beta"] = L.init_beta(beam["pz"], ele.p0c, ele.e_tot, mc2)
+        n = len(vecs)
+        beam["state"] = np.ones(n, dtype=np.int32)
+        beam["loss_ele"] = np.zeros(n, dtype=np.int32)
+        beam["loss_turn"] = np.zeros(n, dtype=np.int32)
+        out = T.track_beam_host(E.to_array([ele]), beam, 1, mc2=mc2, charge=1, maps=maps)
+        k = 0
+        for sfx, vs in cases:
+            _, _, exp_s = fx[f"{ename}_S_{sfx}"]
+            got = [int(s) for s in out["state"][k:k + len(vs)]]
+            assert got == [int(x) for x in exp_s], (ename, sfx, got, exp_s)
+            k += len(vs)

@@ -99,6 +99,20 @@ def test_misaligned_non_bend_elements_accepted(bmx):
     assert rc == 0 and nprm == 5
 
 
+def test_collimator_momentum_windows_accepted(bmx):
+    """Collimators with px/py/z/pz windows carry 8 window values after their 4 aperture
+    values."""
+    E = bmx.elements
+    p0c, et = bmx.lattices.reference()
+    for key in (E.ECOLLIMATOR, E.RCOLLIMATOR):
+        plain = E.make_element(key, p0c, et, l=0.5, x_limit=0.02, y_limit=0.015)
+        e = E.make_element(key, p0c, et, l=0.5, x_limit=0.02, y_limit=0.015, px_aperture_width2=0.01,
+                           pz_aperture_width2=0.03, pz_aperture_center=0.02)
+        rc0, n0, msg0 = _validate(bmx, [plain])
+        rc, nprm, msg = _validate(bmx, [e])
+        assert rc0 == 0 and rc == 0 and nprm == n0 + 8, (msg0, msg)
+
+
 def test_misaligned_drift_accepted(bmx):
     """track1_bmad tracks drifts without offset_particle (track1_bmad.f90:93-94)."""
     E = bmx.elements
