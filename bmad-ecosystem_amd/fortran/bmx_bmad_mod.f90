@@ -129,6 +129,19 @@ interface
     integer(c_int32_t) :: st
   end function
 
+  function c_closed_orbit(ctx, lats, n_lat, n_dim, orbit, mat, max_iter, rel_tol, abs_tol, status) &
+                          bind(c, name = 'bmx_closed_orbit') result(st)
+    import :: c_int32_t, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    type(c_ptr), intent(in) :: lats(*)
+    integer(c_int32_t), value :: n_lat, n_dim, max_iter
+    real(c_double), intent(inout) :: orbit(*)
+    real(c_double), intent(out) :: mat(*)
+    real(c_double), value :: rel_tol, abs_tol
+    integer(c_int32_t), intent(out) :: status(*)
+    integer(c_int32_t) :: st
+  end function
+
   function c_lattice_specialize(lat) bind(c, name = 'bmx_lattice_specialize') result(st)
     import :: c_int32_t, c_ptr
     type(c_ptr), value :: lat
@@ -199,6 +212,7 @@ interface
 end interface
 
 public :: bmx_lat_init, bmx_lat_free, bmx_track_bunch, bmx_track_all, bmx_last_error_message
+public :: bmx_closed_orbit_calc
 
 contains
 
@@ -413,6 +427,45 @@ subroutine bmx_track_all (lat, orbit, track_state, err_flag)
       err_flag = .true.
     endif
   enddo
+end subroutine
+
+!+
+! Subroutine bmx_closed_orbit_calc (lat, closed_orb, i_dim, err_flag, mat6, rel_tol, abs_tol)
+!
+! closed_orbit_calc (bmad/code/closed_orbit_calc.f90:192) on the GPU: closed_orb%vec is the
+! starting guess on entry (its pz is the energy for i_dim = 4, which closes x, px, y, py with
+! the rf off) and the closed orbit on return (i_dim = 6 closes all six with the rf on). mat6
+! (optional) receives the one-turn matrix at the orbit. The search ends when every Newton step
+! is below |orbit| rel_tol + abs_tol (defaults 1e-8 and 1e-11, bmad_com's rel_tol_tracking and
+! abs_tol_tracking). err_flag is set if it does not converge in 100 steps or a probe particle
+! is lost.
+!-
+subroutine bmx_closed_orbit_calc (lat, closed_orb, i_dim, err_flag, mat6, rel_tol, abs_tol)
+  type(bmx_lat_struct), intent(in) :: lat
+  type(bmx_coord_struct), intent(inout) :: closed_orb
+  integer, intent(in) :: i_dim
+  logical, intent(out) :: err_flag
+  real(bmx_rp), optional, intent(out) :: mat6(6, 6)
+  real(bmx_rp), optional, intent(in) :: rel_tol, abs_tol
+
+  type(c_ptr) :: lats(1)
+  real(c_double) :: orb(6), m(6, 6), rt, at
+  integer(c_int32_t) :: st, status(1)
+
+  err_flag = .true.
+  rt = 1e-8_c_double
+  at = 1e-11_c_double
+  if (present(rel_tol)) rt = rel_tol
+  if (present(abs_tol)) at = abs_tol
+  lats(1) = lat%lat
+  orb = closed_orb%vec
+  st = c_closed_orbit(lat%ctx, lats, 1_c_int32_t, int(i_dim, c_int32_t), orb, m, 100_c_int32_t, &
+                      rt, at, status)
+  if (st /= bmx_ok) return
+  closed_orb%vec = orb
+  ! the library's matrix is row-major: m(c, r) = d out_r / d in_c
+  if (present(mat6)) mat6 = transpose(m)
+  err_flag = status(1) /= 0
 end subroutine
 
 subroutine fill_view (view, x, px, y, py, z, pz, beta, state, loss_ele, loss_turn)

@@ -11,6 +11,8 @@
 !                       each) and state (n int32)
 !   <dir>/out.bin     : x .. beta (n real64 each), state, ix_ele, ix_turn (n int32 each)
 !   <dir>/orbit.bin   : int32 track_state, then for i = 0..n_ele: vec(6), beta, real(state)
+!   <dir>/co.bin      : int32 err (0/1), 4-D closed orbit vec(6) from zero, one-turn matrix
+!                       (6x6, row-major), with rel_tol 1e-12 and abs_tol 1e-15
 !-
 program ftrack
 
@@ -20,6 +22,8 @@ implicit none
 
 type(bmx_ele_struct), allocatable :: eles(:)
 type(bmx_coord_struct), allocatable :: p(:), orbit(:)
+type(bmx_coord_struct) :: co
+real(c_double) :: mat6(6, 6)
 type(bmx_lat_struct) :: lat
 integer(c_int32_t) :: n_ele, n, n_turns
 integer(c_int32_t), allocatable :: ibuf(:)
@@ -87,6 +91,15 @@ write(u) int(track_state, c_int32_t)
 do i = 0, n_ele
   write(u) orbit(i)%vec, orbit(i)%beta, real(orbit(i)%state, c_double)
 enddo
+close(u)
+
+co = bmx_coord_struct()
+co%vec = 0
+call bmx_closed_orbit_calc(lat, co, 4, err, mat6, 1e-12_c_double, 1e-15_c_double)
+open(newunit = u, file = trim(dir) // '/co.bin', access = 'stream', form = 'unformatted', status = 'replace')
+write(u) int(merge(1, 0, err), c_int32_t)
+write(u) co%vec
+write(u) transpose(mat6)
 close(u)
 
 call bmx_lat_free(lat)

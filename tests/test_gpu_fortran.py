@@ -120,3 +120,29 @@ def test_fortran_track_all_loss(bmx, oracle, tmp_path):
     assert np.all(orbit[track_state + 1:, 7] == -999)
     assert np.all(orbit[track_state + 1:, 6] == -1)
     assert np.all(orbit[track_state + 1:, :6] == 0)
+
+
+def test_fortran_closed_orbit_calc(bmx, tmp_path):
+    """bmx_closed_orbit_calc (closed_orbit_calc's calling convention) from the Fortran host on a
+    kicked 10-cell ring (rf on in the table, off for i_dim = 4): the same orbit and one-turn
+    matrix, bit for bit, as the Python host's closed_orbit with the same tolerances."""
+    L, E, T = bmx.lattices, bmx.elements, bmx.tracking
+    p0c, et = L.reference()
+    eles = L.ring1(10)
+    eles.insert(1, E.make_element(E.HKICKER, p0c, et, l=0.5, kick=2e-5))
+    arr = E.to_array(eles)
+    beam = P.random_beam(bmx, 16, seed=5)
+    _write_inputs(tmp_path, arr, beam, 1, E.M_ELECTRON)
+    _run(tmp_path)
+    raw = open(os.path.join(tmp_path, "co.bin"), "rb").read()
+    err = int(np.frombuffer(raw, np.int32, 1, 0)[0])
+    vec = np.frombuffer(raw, np.float64, 6, 4)
+    mat = np.frombuffer(raw, np.float64, 36, 52).reshape(6, 6)
+    assert err == 0
+    ctx = T.Context(0)
+    lat = T.Lattice(ctx, arr)
+    orb, M, st = T.closed_orbit(ctx, [lat], n_dim=4, max_iter=100, rel_tol=1e-12, abs_tol=1e-15)
+    lat.close()
+    assert st[0] == 0 and abs(vec[0]) > 1e-5
+    assert np.array_equal(vec, orb[0])
+    assert np.array_equal(mat, M[0])
