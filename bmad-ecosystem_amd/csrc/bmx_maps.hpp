@@ -1240,10 +1240,15 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
     // tests first without branches, the state codes and the loss record only on the (rare)
     // lanes that are lost here or were lost inside the map
     // The gate leaves out what cannot newly hold here: rel_p < 0 outside rf cavities (pz only
-    // changes in them, and a particle enters with rel_p >= 0), and |x| or |y| above the limit
-    // when the exit aperture fails for those anyway (F_APE_EXIT_COVERS).
-    const double f_unstable = p.px * p.px + p.py * p.py - p.rel_p * p.rel_p;
-    LaneMask hit_lanes = lanes(f_unstable > 0) | lanes(p.state != ST_ALIVE);
+    // changes in them, and a particle enters with rel_p >= 0), px^2 + py^2 > (1 + pz)^2 after a
+    // drift or marker (neither changes px, py or pz, and the particle entered without it), and
+    // |x| or |y| above the limit when the exit aperture fails for those anyway
+    // (F_APE_EXIT_COVERS).
+    LaneMask hit_lanes = lanes(p.state != ST_ALIVE);
+    if (kind != K_DRIFT && kind != K_MARKER) {
+        const double f_unstable = p.px * p.px + p.py * p.py - p.rel_p * p.rel_p;
+        hit_lanes |= lanes(f_unstable > 0);
+    }
     if constexpr (MAPS != MAPS_NO_RF) hit_lanes |= lanes(p.rel_p < 0);
     if (!(flags & F_APE_EXIT_COVERS))
         hit_lanes |= lanes(fabs(p.x) > kMaxAperture) | lanes(fabs(p.y) > kMaxAperture);
