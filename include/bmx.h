@@ -204,7 +204,8 @@ int32_t bmx_lattice_validate(const bmx_ele *eles, int32_t n_ele, double mc2, int
  * least three quarters of the table, written out once with every kind dispatch and flag test
  * resolved. Strengths stay in the table, so lattices differing only in strengths (error
  * variants) share one compiled kernel. Used by the fast-map runs of bmx_track_turns and
- * bmx_batch_run; the results are those of the generic kernel. BMX_ERR_UNSUPPORTED when no
+ * bmx_batch_track; the maps are the generic kernel's, so results agree with it to rounding
+ * (losses identical in the parity tests). BMX_ERR_UNSUPPORTED when no
  * such cell exists (the generic kernel is then used). No counterpart in the reference, whose
  * element loop is track_all (bmad/code/track_all.f90:1) over the generic element dispatch. */
 int32_t bmx_lattice_specialize(bmx_lattice *lat);
