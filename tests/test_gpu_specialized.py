@@ -177,3 +177,21 @@ def test_server_with_specialized_lattices(bmx, ctx):
     assert lost > 500
     for lt in lats:
         lt.close()
+
+
+def test_specialized_agrees_with_generic(bmx, ctx):
+    """Same maps, other code: on the 700-element ring (every orbit regular) the specialised and
+    the generic kernel agree to 1e-12 after 20 turns, and on the 901-element ring their loss
+    records are identical for a beam that loses a third of its particles in 20 turns."""
+    L = bmx.lattices
+    arr0 = bmx.elements.to_array(L.ring0())
+    beam = L.gaussian_beam(2000, seed=3, sigma=3 * L.SIGMA0)
+    a = _track(bmx, ctx, arr0, beam, 20, True)
+    b = _track(bmx, ctx, arr0, beam, 20, False)
+    assert P.rel_err(a, b) <= 1e-12
+    arr1 = bmx.elements.to_array(L.ring1())
+    beam = P.random_beam(bmx, 3000, seed=17, scale=12.0, pz_sigma=4e-3)
+    a = _track(bmx, ctx, arr1, beam, 20, True)
+    b = _track(bmx, ctx, arr1, beam, 20, False)
+    assert (b["state"] != 1).sum() > 300
+    assert P.loss_mismatch(a, b).size == 0
