@@ -142,6 +142,17 @@ interface
     integer(c_int32_t) :: st
   end function
 
+  function c_beam_file_write(path, host, n, species, charge, p0c, ix_ele, append) &
+                             bind(c, name = 'bmx_beam_file_write') result(st)
+    import :: c_int32_t, c_int64_t, c_char, c_double, bmx_beam_view
+    character(kind = c_char), intent(in) :: path(*), species(*)
+    type(bmx_beam_view), intent(in) :: host
+    integer(c_int64_t), value :: n
+    real(c_double), value :: charge, p0c
+    integer(c_int32_t), value :: ix_ele, append
+    integer(c_int32_t) :: st
+  end function
+
   function c_lattice_specialize(lat) bind(c, name = 'bmx_lattice_specialize') result(st)
     import :: c_int32_t, c_ptr
     type(c_ptr), value :: lat
@@ -212,7 +223,7 @@ interface
 end interface
 
 public :: bmx_lat_init, bmx_lat_free, bmx_track_bunch, bmx_track_all, bmx_last_error_message
-public :: bmx_closed_orbit_calc
+public :: bmx_closed_orbit_calc, bmx_write_beam_file
 
 contains
 
@@ -466,6 +477,42 @@ subroutine bmx_closed_orbit_calc (lat, closed_orb, i_dim, err_flag, mat6, rel_to
   ! the library's matrix is row-major: m(c, r) = d out_r / d in_c
   if (present(mat6)) mat6 = transpose(m)
   err_flag = status(1) /= 0
+end subroutine
+
+!+
+! Subroutine bmx_write_beam_file (file_name, particle, species, charge, ix_ele, err_flag)
+!
+! write_beam_file (bmad/multiparticle/beam_file_io.f90:29) for one bunch in the column layout:
+! vec, p0c, beta, state and ix_ele of every particle (ix_ele for live ones, the particle's ix_ele
+! -- its loss element -- for lost ones), charge per particle and species as parameters; 17
+! significant digits, so the file is a bit-identical checkpoint.
+!-
+subroutine bmx_write_beam_file (file_name, particle, species, charge, ix_ele, err_flag)
+  character(*), intent(in) :: file_name, species
+  type(bmx_coord_struct), intent(in) :: particle(:)
+  real(bmx_rp), intent(in) :: charge
+  integer, intent(in) :: ix_ele
+  logical, intent(out) :: err_flag
+
+  real(c_double), allocatable, target :: x(:), px(:), y(:), py(:), z(:), pz(:), beta(:)
+  integer(c_int32_t), allocatable, target :: state(:), loss_ele(:), loss_turn(:)
+  type(bmx_beam_view) :: view
+  integer(c_int32_t) :: st
+  integer :: n
+
+  n = size(particle)
+  allocate(x(n), px(n), y(n), py(n), z(n), pz(n), beta(n), state(n), loss_ele(n), loss_turn(n))
+  x = particle%vec(1); px = particle%vec(2); y = particle%vec(3)
+  py = particle%vec(4); z = particle%vec(5); pz = particle%vec(6)
+  beta = particle%beta
+  state = particle%state
+  loss_ele = particle%ix_ele
+  loss_turn = particle%ix_turn
+  call fill_view(view, x, px, y, py, z, pz, beta, state, loss_ele, loss_turn)
+  st = c_beam_file_write(trim(file_name) // c_null_char, view, int(n, c_int64_t), &
+                         trim(species) // c_null_char, real(charge, c_double), &
+                         real(particle(1)%p0c, c_double), int(ix_ele, c_int32_t), 0_c_int32_t)
+  err_flag = st /= bmx_ok
 end subroutine
 
 subroutine fill_view (view, x, px, y, py, z, pz, beta, state, loss_ele, loss_turn)

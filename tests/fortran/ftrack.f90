@@ -11,6 +11,7 @@
 !                       each) and state (n int32)
 !   <dir>/out.bin     : x .. beta (n real64 each), state, ix_ele, ix_turn (n int32 each)
 !   <dir>/orbit.bin   : int32 track_state, then for i = 0..n_ele: vec(6), beta, real(state)
+!   <dir>/beam_out.dat: the tracked bunch as a Bmad beam file (bmx_write_beam_file)
 !   <dir>/co.bin      : int32 err (0/1), 4-D closed orbit vec(6) from zero, one-turn matrix
 !                       (6x6, row-major), with rel_tol 1e-12 and abs_tol 1e-15
 !-
@@ -69,6 +70,12 @@ call bmx_track_bunch(lat, p, n_turns, err)
 if (err) then
   print '(a)', 'bmx_track_bunch failed: ' // bmx_last_error_message()
   stop 2
+endif
+
+call bmx_write_beam_file(trim(dir) // '/beam_out.dat', p, 'Electron', 1e-12_c_double, int(n_ele), err)
+if (err) then
+  print '(a)', 'bmx_write_beam_file failed: ' // bmx_last_error_message()
+  stop 4
 endif
 
 open(newunit = u, file = trim(dir) // '/out.bin', access = 'stream', form = 'unformatted', status = 'replace')

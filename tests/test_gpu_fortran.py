@@ -84,6 +84,12 @@ def test_fortran_track_bunch_and_track_all(bmx, oracle, tmp_path):
     assert np.array_equal(out["ix_turn"][lost], py["loss_turn"][lost])
     assert np.all(out["ix_ele"][~lost] == n_ele)
     assert np.all(out["ix_turn"][~lost] == n_turns - 1)
+    # the bunch the Fortran host wrote as a Bmad beam file reads back bit for bit
+    back, q = bmx.tracking.read_beam_file(os.path.join(tmp_path, "beam_out.dat"), L.reference()[0])
+    for k in P.COORDS + ("beta", "state"):
+        assert np.array_equal(back[k], out[k]), k
+    assert np.array_equal(back["loss_ele"][lost], out["ix_ele"][lost])
+    assert abs(q - 2000 * 1e-12) < 1e-22
 
     ref = P.copy_beam(beam)
     oracle.track_bunch(arr, ref, n_turns)
