@@ -2,13 +2,14 @@
 
 Reads ``parameter[...]`` (p0c or e_tot, particle), variables, element definitions (by type or
 by inheriting another element's attributes), ``name[attr] = value`` settings, lines with
-repetition (``3*cell``) and nesting, ``use`` and ``call, file = ...``, with arithmetic
+repetition (``3*cell``), reflection (``-cell``) and nesting, ``use`` and ``call, file = ...``, with arithmetic
 expressions over numbers, constants, variables and ``ele[attr]`` references. Element types are
 those of this path (drift, quadrupole, sbend, rbend, sextupole, octupole, marker, rfcavity,
 kickers, monitor, instrument, collimators, pipe, thick_multipole), matched by unique prefix as
 the reference parser does. An rbend becomes an sbend as the reference's bookkeeping makes it
 (arc length from the chord, edge angles increased by half the bend angle). Superposition,
-overlays, groups, reflected lines and attributes outside this path raise ``LatticeFileError``.
+overlays, groups, reversed elements (``--``) and attributes outside this path raise
+``LatticeFileError``.
 The result goes through ``elements.make_element`` (Bmad defaults and step bookkeeping).
 """
 from __future__ import annotations
@@ -234,13 +235,18 @@ class _Parser:
                 continue
             m = re.match(r"(\d+)\s*\*\s*(.+)$", it)
             n, body = (int(m.group(1)), m.group(2).strip()) if m else (1, it)
-            if body.startswith("-"):
-                raise LatticeFileError(f"reflected lines are not supported: {item!r}")
+            if body.startswith("--"):
+                raise LatticeFileError(f"reversed elements are not supported: {item!r}")
+            reflect = body.startswith("-")
+            if reflect:
+                body = body[1:].strip()
             if body.startswith("("):
                 sub = f"__anon{len(self.lines)}"
                 self.lines[sub] = _split_top(body[1:-1])
                 body = sub
-            out += self.expand(body, depth + 1) * n
+            seq = self.expand(body, depth + 1)
+            # reflection (a leading '-'): the same elements in reverse order
+            out += (seq[::-1] if reflect else seq) * n
         return out
 
 

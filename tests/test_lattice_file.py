@@ -52,7 +52,7 @@ def test_small_ring(bmx, lf):
 
 
 @pytest.mark.parametrize("text", [
-    "parameter[e_tot] = 1e9\nd: drift, l = 1\nr: line = (d, -d)\nuse, r\n",
+    "parameter[e_tot] = 1e9\nd: drift, l = 1\nr: line = (d, --d)\nuse, r\n",
     "parameter[e_tot] = 1e9\nd: drift, l = 1\no: overlay = {d[l]: x}\nr: line = (d)\nuse, r\n",
     "parameter[e_tot] = 1e9\nd: drift, l = 1, superimpose, ref = d\nr: line = (d)\nuse, r\n",
     "parameter[e_tot] = 1e9\ns: solenoid, l = 1, ks = 0.1\nr: line = (s)\nuse, r\n",
@@ -63,6 +63,16 @@ def test_outside_the_subset_raises(lf, tmp_path, text):
     p.write_text(text)
     with pytest.raises(lf.LatticeFileError):
         lf.load(str(p))
+
+
+def test_reflection_and_repetition(lf, tmp_path):
+    """'-line' is the line in reverse order (the reference's reflection); 'n*line' repeats."""
+    p = tmp_path / "lat.bmad"
+    p.write_text("parameter[e_tot] = 1e9\n"
+                 "a: drift, l = 1\nb: drift, l = 2\nc: drift, l = 3\nd: drift, l = 4\n"
+                 "s: line = (c, d)\nr: line = (a, b, -s, 2*(a, -(b, c)))\nuse, r\n")
+    eles, _, _ = lf.load(str(p))
+    assert [e.l for e in eles] == [1, 2, 4, 3, 1, 3, 2, 1, 3, 2]
 
 
 @pytest.mark.skipif(not os.path.isdir(TUTORIAL), reason="reference tree not present")
