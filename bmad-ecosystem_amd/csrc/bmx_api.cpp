@@ -324,14 +324,22 @@ int32_t bmx_beam_count_alive(bmx_beam *B, int64_t *n_alive)
     return BMX_OK;
 }
 
-int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t n_turns, int32_t turn0)
+}  // extern "C"
+
+namespace {
+
+// Persistent tracking of one group: n_turns turns of `count` elements from element `first`
+// (0-based), turn indices turn0 ... (bmx_table.hpp Group).
+int32_t track_group(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t first, int32_t count,
+                    int32_t n_turns, int32_t turn0, const char *who)
 {
-    if (!ctx || !L || !B) return set_error(BMX_ERR_ARG, "bmx_track_turns: null argument");
-    if (n_turns < 0) return set_error(BMX_ERR_ARG, "bmx_track_turns: negative turn count");
+    const std::string w(who);
+    if (!ctx || !L || !B) return set_error(BMX_ERR_ARG, w + ": null argument");
+    if (n_turns < 0) return set_error(BMX_ERR_ARG, w + ": negative turn count");
     if (L->ctx->device != ctx->device || B->ctx->device != ctx->device)
-        return set_error(BMX_ERR_ARG, "bmx_track_turns: handles on different devices");
-    if (B->n >= (int64_t)UINT32_MAX) return set_error(BMX_ERR_ARG, "bmx_track_turns: more than 2^32-1 particles");
-    if (n_turns == 0 || B->n == 0) return BMX_OK;
+        return set_error(BMX_ERR_ARG, w + ": handles on different devices");
+    if (B->n >= (int64_t)UINT32_MAX) return set_error(BMX_ERR_ARG, w + ": more than 2^32-1 particles");
+    if (n_turns == 0 || B->n == 0 || count == 0) return BMX_OK;
     hipSetDevice(ctx->device);
     // the previous call's group upload must have been consumed before the host copy changes
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
@@ -344,6 +352,8 @@ int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t
     g.list_off = 0;
     g.n_turns = n_turns;
     g.turn0 = turn0;
+    g.first = first;
+    g.count = count;
     RunConfig cfg;
     cfg.chunk_turns = ctx->chunk_turns;
     cfg.fast = ctx->fast_maps;
@@ -353,6 +363,37 @@ int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t
     cfg.timer = ctx->profile ? &ctx->timer : nullptr;
     HIP_TRY(enqueue_run(ctx->stream, ctx->run, &g, 1, true, n_turns, cfg, ctx->d_work), "k_track_tiles");
     return BMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t n_turns, int32_t turn0)
+{
+    return track_group(ctx, L, B, 0, L ? L->desc.n_ele : 0, n_turns, turn0, "bmx_track_turns");
+}
+
+int32_t bmx_track_turns_from(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t ix_ele_start,
+                             int32_t n_turns, int32_t turn0)
+{
+    if (L && (ix_ele_start < 0 || ix_ele_start > L->desc.n_ele))
+        return set_error(BMX_ERR_ARG, "bmx_track_turns_from: ix_ele_start outside 0..n_ele");
+    const int32_t n = L ? L->desc.n_ele : 0;
+    return track_group(ctx, L, B, n ? ix_ele_start % n : 0, n, n_turns, turn0, "bmx_track_turns_from");
+}
+
+int32_t bmx_track_bunch(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t ix_ele1,
+                        int32_t ix_ele2, int32_t turn)
+{
+    if (!L) return set_error(BMX_ERR_ARG, "bmx_track_bunch: null lattice");
+    const int32_t n = L->desc.n_ele;
+    if (ix_ele1 < 0 || ix_ele1 > n || ix_ele2 < 0 || ix_ele2 > n)
+        return set_error(BMX_ERR_ARG, "bmx_track_bunch: element index outside 0..n_ele");
+    // beam_mod.f90:148-166: ix_ele1+1 .. ix_ele2, or through the end and round to ix_ele2 when
+    // ix_ele2 is at or before ix_ele1
+    const int32_t count = ix_ele1 < ix_ele2 ? ix_ele2 - ix_ele1 : n - ix_ele1 + ix_ele2;
+    return track_group(ctx, L, B, n ? ix_ele1 % n : 0, count, 1, turn, "bmx_track_bunch");
 }
 
 int32_t bmx_ctx_profile(bmx_ctx *ctx, int32_t on)

@@ -102,6 +102,8 @@ int32_t bmx_batch_create(bmx_ctx *ctx, const bmx_group *groups, int32_t n_groups
         G.b = offset(g.beam->d, g.first);
         G.n = (uint32_t)g.count;
         G.list_off = (uint32_t)off;
+        G.first = 0;
+        G.count = G.lat.n_ele;
         off += (size_t)g.count;
         B->groups.push_back(G);
         B->lats.push_back(g.lat);

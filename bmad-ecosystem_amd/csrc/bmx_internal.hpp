@@ -20,6 +20,9 @@ hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_gr
                               const uint32_t *d_cnt_in, int32_t next_off, WaveTile *d_tiles,
                               uint32_t *d_n_tiles, uint32_t *d_cnt_zero, uint32_t *d_tile_ctr,
                               uint32_t *d_goff);
+hipError_t launch_collect_alive(hipStream_t s, int grid, const Group *d_groups,
+                                const WaveTile *d_tiles, const uint32_t *d_n_tiles,
+                                uint32_t *d_list_out, uint32_t *d_cnt_out);
 hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu);
 hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, int32_t turn,
                                 const BeamPtrs &b, int64_t n, unsigned long long *d_work,

@@ -91,6 +91,33 @@ __global__ void __launch_bounds__(kBuildThreads) k_build_tiles(
     }
 }
 
+// The alive list of a run's first chunk: one wave per tile of every group's particles (tiles
+// built from the group sizes), a ballot of state == alive and one atomic per wave appending
+// the live indices to the group's segment of list_out (counts in cnt_out, zeroed by the tile
+// build). Dead particles of the caller's beam thus take no lane in the first chunk, as
+// track1_bunch_hom skips them (beam_utils.f90:83). Grid-stride over the tiles.
+__global__ void __launch_bounds__(kBlock) k_collect_alive(
+    const Group *__restrict__ groups, const WaveTile *__restrict__ tiles,
+    const uint32_t *__restrict__ n_tiles_p, uint32_t *__restrict__ list_out,
+    uint32_t *__restrict__ cnt_out)
+{
+    const uint32_t nt = *n_tiles_p;
+    const int lane = threadIdx.x & 63;
+    const uint32_t n_waves = gridDim.x * (kBlock / 64);
+    for (uint32_t t = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); t < nt; t += n_waves) {
+        const WaveTile T = tiles[t];
+        const Group &G = groups[T.group];
+        const uint32_t idx = T.start + lane;
+        const bool alive = (uint32_t)lane < T.count && G.b.state[idx] == ST_ALIVE;
+        const unsigned long long m = __ballot(alive);
+        if (!m) continue;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&cnt_out[T.group], (uint32_t)__popcll(m));
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (alive) list_out[G.list_off + base + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = idx;
+    }
+}
+
 // One element, whole bunch: track1 (track1.f90:44-343) per live particle.
 template <bool F>
 __global__ void __launch_bounds__(kBlock) k_track_element(LatDesc L, int32_t ie, int32_t turn,
@@ -229,6 +256,16 @@ hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_gr
 {
     hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(kBuildThreads), 0, s, d_groups, n_groups,
                        d_cnt_in, next_off, d_tiles, d_n_tiles, d_cnt_zero, d_tile_ctr, d_goff);
+    return hipGetLastError();
+}
+
+hipError_t launch_collect_alive(hipStream_t s, int grid, const Group *d_groups,
+                                const WaveTile *d_tiles, const uint32_t *d_n_tiles,
+                                uint32_t *d_list_out, uint32_t *d_cnt_out)
+{
+    if (grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_collect_alive, dim3((unsigned)grid), dim3(kBlock), 0, s, d_groups, d_tiles,
+                       d_n_tiles, d_list_out, d_cnt_out);
     return hipGetLastError();
 }
 

@@ -129,9 +129,21 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
     const int32_t K = cfg.chunk_turns > 0 ? cfg.chunk_turns : max_turns;
     uint32_t *n_tiles[2] = {rb.d_small, rb.d_small + 1};
     uint32_t *tile_ctr = rb.d_small + 2;
-    // chunk 0 tiles index the particles directly; the chunk appends to list/cnt 1
-    e = launch_build_tiles(s, rb.d_groups, n_groups, nullptr, 0, rb.d_tiles[0], n_tiles[0],
-                           rb.d_cnt[1], tile_ctr, rb.d_goff);
+    // Chunk 0's alive list (list/cnt 0) from the particles' states: tiles over every particle
+    // of every group, a ballot pass over them, then the chunk's tiles from the live counts; the
+    // chunk appends to list/cnt 1.
+    size_t all_tiles = 0;
+    for (int32_t g = 0; g < n_groups; g++)
+        if (h_groups[g].n_turns > 0) all_tiles += (h_groups[g].n + 63) / 64;
+    const int collect_grid = (int)std::min<size_t>((all_tiles + 3) / 4, 4096);
+    e = launch_build_tiles(s, rb.d_groups, n_groups, nullptr, 0, rb.d_tiles[1], n_tiles[1],
+                           rb.d_cnt[0], tile_ctr, rb.d_goff);
+    if (e == hipSuccess)
+        e = launch_collect_alive(s, collect_grid, rb.d_groups, rb.d_tiles[1], n_tiles[1],
+                                 rb.d_list[0], rb.d_cnt[0]);
+    if (e == hipSuccess)
+        e = launch_build_tiles(s, rb.d_groups, n_groups, rb.d_cnt[0], 0, rb.d_tiles[0], n_tiles[0],
+                               rb.d_cnt[1], tile_ctr, rb.d_goff);
     int c = 0;
     for (int32_t off = 0; off < max_turns && e == hipSuccess; off += K, c++) {
         const int cur = c & 1, nxt = cur ^ 1;
@@ -144,12 +156,12 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
         }
         if (cfg.jit)
             e = launch_track_tiles_jit(cfg.jit, s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur],
-                                       n_tiles[cur], c == 0 ? nullptr : rb.d_list[cur],
-                                       rb.d_list[nxt], rb.d_cnt[nxt], tile_ctr, off, K, d_work);
+                                       n_tiles[cur], rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt],
+                                       tile_ctr, off, K, d_work);
         else
             e = launch_track_tiles(s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur], n_tiles[cur],
-                                   c == 0 ? nullptr : rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt],
-                                   tile_ctr, off, K, d_work, cfg.fast);
+                                   rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt], tile_ctr, off, K,
+                                   d_work, cfg.fast);
         if (cfg.timer && e == hipSuccess) {
             e = hipEventRecord(ev1, s);
             cfg.timer->pending.emplace_back(ev0, ev1);

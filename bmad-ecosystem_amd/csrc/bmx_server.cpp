@@ -159,9 +159,16 @@ void bmx_server::run()
         }
         const int64_t t_start = now_ns();
         hipError_t e = hipSuccess;
+        bool zero_turn_copies = false;
         for (Request *r : admitted) {
             if (r->n > 0 && e == hipSuccess) e = copy(r, true);
+            // a request of no turns returns its particles unchanged
+            if (r->n > 0 && r->n_turns == 0 && e == hipSuccess) {
+                e = copy(r, false);
+                zero_turn_copies = true;
+            }
         }
+        if (zero_turn_copies && e == hipSuccess) e = hipStreamSynchronize(stream);
         std::vector<Request *> run_now;
         {
             std::lock_guard<std::mutex> lk(mu);
@@ -195,6 +202,8 @@ void bmx_server::run()
             g.b = at(r->off);
             g.n = (uint32_t)r->n;
             g.list_off = (uint32_t)r->off;
+            g.first = 0;
+            g.count = g.lat.n_ele;
             g.n_turns = std::min(chunk_turns, r->n_turns - r->done_turns);
             g.turn0 = r->turn0 + r->done_turns;
             max_turns = std::max(max_turns, g.n_turns);

@@ -113,6 +113,11 @@ struct BeamPtrs {
 // A group: particles [0, n) of one beam view tracked through one lattice for turns
 // turn0 .. turn0 + n_turns - 1. A single-lattice call is one group; batched variants and served
 // requests are one group each. list_off locates the group's segment in the alive lists.
+// Each turn passes through `count` elements starting at element `first` (0-based) and wrapping
+// past the last element to element 0: first = 0, count = n_ele is the full turn from the start
+// of the lattice; a turn that starts after element s (LTT's track_bunch(ele_start, ele_start),
+// lt_tracking_mod.f90:3223) is first = s, count = n_ele; track_bunch's ele1 -> ele2 span
+// (beam_mod.f90:148-166) is one turn of first = ix_ele1, count < n_ele possibly.
 struct Group {
     LatDesc lat;
     BeamPtrs b;
@@ -120,6 +125,8 @@ struct Group {
     uint32_t list_off;
     int32_t n_turns;
     int32_t turn0;
+    int32_t first;
+    int32_t count;
 };
 
 // One wave's unit of work: up to 64 particles of one group for one chunk of turns. start

@@ -84,30 +84,68 @@ struct GenericTurn {
     }
 };
 
-// Tracks one particle through up to n_turns full turns starting at turn index turn0 (Turn:
-// the turn policy); records the loss element (1-based) and turn. Returns the number of
-// element passes made.
+// `count` elements from element `first` (0-based), wrapping past the last element to element 0
+// (beam_mod.f90:159-166, track_bunch's forward loops): the turn of a group that does not start
+// at element 0 or does not pass every element.
+template <bool F>
+__device__ __forceinline__ void track_span(Particle &p, const LatDesc &L, int32_t first,
+                                           int32_t count, int32_t t_abs)
+{
+    const int32_t n = L.n_ele;
+    const int32_t end1 = count < n - first ? first + count : n;
+    track_range<F>(p, L, first, end1, t_abs);
+    const int32_t rest = count - (end1 - first);
+    if (rest > 0) track_range<F>(p, L, 0, rest, t_abs);
+}
+
+// The entrance aperture test of element e for a particle that does not arrive from element
+// e - 1 in this launch (it resumes at e): made here when the table marks it as a repeat of the
+// previous element's exit test (F_APE_ENT_SAME), which the element loop skips. A particle that
+// did pass that exit test passes this one too, so making it again changes no result.
+__device__ __forceinline__ void entrance_retest(Particle &p, const LatDesc &L, int32_t e,
+                                                int32_t turn)
+{
+    const EleRef h = load_ele((CHdr)L.hdr + 4 * e);
+    if (!(h.flags & F_APE_ENT_SAME)) return;
+    CP ap = (CP)L.prm + h.off + ((h.flags & F_OFFSET) ? kOffsetLen : 0);
+    const bool out = aperture_fails(p, h.flags, ap);
+    if (any_lane(out)) {
+        if (out) {
+            aperture(p, h.flags, ap);
+            p.loss_ele = uniform_here(e + 1);
+            p.loss_turn = uniform_here(turn);
+        }
+    }
+}
+
+// Tracks one particle through up to n_turns turns of the group's element span (first, count)
+// starting at turn index turn0 (Turn: the policy of the full turn from element 0); records the
+// loss element (1-based) and turn. Returns the number of element passes made.
 template <bool F, class Turn>
 __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, const BeamPtrs &b,
                                                              uint32_t i, int32_t n_turns,
-                                                             int32_t turn0)
+                                                             int32_t turn0, int32_t first,
+                                                             int32_t count)
 {
     Particle p;
     load_particle(p, b, i);
     refresh_energy(p, L.sp);
     p.loss_ele = 0;
     p.loss_turn = 0;
+    const bool full = first == 0 && count == L.n_ele;  // wave-uniform
     // Entrance validity check of the first element (track1.f90:196 -> orbit_too_large); later
     // elements need none: a particle leaves element e with the coordinates it enters e+1 with,
     // and element e already ended with the same test (track1.f90:291).
     if (orbit_too_large(p)) {
-        p.loss_ele = 1;
+        p.loss_ele = first + 1;
         p.loss_turn = turn0;
     } else {
+        if (!full && first > 0) entrance_retest(p, L, first, turn0);
         // a wave stops once all its particles are lost
         for (int32_t turn = 0; turn < n_turns; ++turn) {
             if (!any_lane(p.state == ST_ALIVE)) break;
-            Turn::turn(p, L, turn0 + turn);
+            if (full) Turn::turn(p, L, turn0 + turn);
+            else track_span<F>(p, L, first, count, turn0 + turn);
         }
     }
     store_particle(p, b, i);
@@ -115,9 +153,11 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
         b.state[i] = p.state;
         b.loss_ele[i] = p.loss_ele;
         b.loss_turn[i] = p.loss_turn;
-        return (unsigned long long)(p.loss_turn - turn0) * L.n_ele + p.loss_ele;
+        // passes up to and including the loss element, counted from `first`
+        const int32_t pos = p.loss_ele - 1 - first;
+        return (unsigned long long)(p.loss_turn - turn0) * count + (pos < 0 ? pos + L.n_ele : pos) + 1;
     }
-    return (unsigned long long)n_turns * L.n_ele;
+    return (unsigned long long)n_turns * count;
 }
 
 // Persistent tracking of one chunk of turns over a list of wave tiles.
@@ -156,9 +196,9 @@ __device__ __forceinline__ void track_tiles(
         bool alive_end = false;
         uint32_t idx = 0;
         if ((uint32_t)lane < T.count) {
-            idx = list_in ? list_in[G.list_off + T.start + lane] : T.start + lane;
+            idx = list_in[G.list_off + T.start + lane];
             if (G.b.state[idx] == ST_ALIVE) {
-                passes += track_particle<F, Turn>(G.lat, G.b, idx, n_turns, turn0);
+                passes += track_particle<F, Turn>(G.lat, G.b, idx, n_turns, turn0, G.first, G.count);
                 alive_end = G.b.state[idx] == ST_ALIVE;
             }
         }

@@ -6,6 +6,8 @@
   track_beam       n_turns full turns, persistent kernel
                    (bmad/multiparticle/beam_mod.f90:40 track_beam, per turn as in
                    bsim/modules/lt_tracking_mod.f90:1631-1716)
+  track_beam_from  the same with every turn starting after a given element (LTT's ele_start)
+  track_bunch      one element span ele1 -> ele2 (beam_mod.f90:103), wrapping round the ring
   track1_bunch     one element for the whole bunch (beam_mod.f90:238 -> beam_utils.f90:32)
   track_beam_plain n_turns as n_turns x n_ele element launches (the reference loop nest)
   track_all        single particles through the lattice, stopping at loss
@@ -193,6 +195,23 @@ class Beam:
 def track_beam(ctx: Context, lat: Lattice, beam: Beam, n_turns: int, turn0: int = 0) -> None:
     """Persistent tracking of n_turns full turns (asynchronous on ctx's stream)."""
     native.call("bmx_track_turns", ctx.h, lat.h, beam.h, int(n_turns), int(turn0))
+
+
+def track_beam_from(ctx: Context, lat: Lattice, beam: Beam, ix_ele_start: int, n_turns: int,
+                    turn0: int = 0) -> None:
+    """n_turns turns that each start after element ix_ele_start: LTT's
+    track_bunch(lat, bunch, ele_start, ele_start) per turn (bmx_track_turns_from; asynchronous)."""
+    native.call("bmx_track_turns_from", ctx.h, lat.h, beam.h, int(ix_ele_start), int(n_turns),
+                int(turn0))
+
+
+def track_bunch(ctx: Context, lat: Lattice, beam: Beam, ix_ele1: int = 0, ix_ele2: int | None = None,
+                turn: int = 0) -> None:
+    """track_bunch (beam_mod.f90:103): from the end of element ix_ele1 (not tracked) to the end
+    of ix_ele2 (default: the last element), wrapping round the ring when ix_ele2 <= ix_ele1
+    (bmx_track_bunch; asynchronous)."""
+    e2 = lat.n_ele if ix_ele2 is None else int(ix_ele2)
+    native.call("bmx_track_bunch", ctx.h, lat.h, beam.h, int(ix_ele1), e2, int(turn))
 
 
 def track1_bunch(ctx: Context, lat: Lattice, beam: Beam, ix_ele: int, turn: int = 0) -> None:

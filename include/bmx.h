@@ -251,6 +251,20 @@ int32_t bmx_beam_count_alive(bmx_beam *beam, int64_t *n_alive);
  * At most 2^32-1 particles. Asynchronous on the context's stream. */
 int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam, int32_t n_turns,
                         int32_t turn0);
+/* The same turn loop with every turn starting after element ix_ele_start (0..n_ele; 0 and
+ * n_ele are bmx_track_turns): elements ix_ele_start+1 .. n_ele, then 1 .. ix_ele_start, all
+ * with the turn's index. This is LTT's turn, track_bunch(lat, bunch, ele_start, ele_start)
+ * (bsim/modules/lt_tracking_mod.f90:3223, called from the turn loop at :1652). A particle that
+ * resumes at element ix_ele_start+1 has its entrance aperture tested there. Asynchronous. */
+int32_t bmx_track_turns_from(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam,
+                             int32_t ix_ele_start, int32_t n_turns, int32_t turn0);
+/* track_bunch (bmad/multiparticle/beam_mod.f90:103, forward direction): the bunch at the end of
+ * element ix_ele1 (0..n_ele; not tracked through) to the end of element ix_ele2 (0..n_ele):
+ * elements ix_ele1+1 .. ix_ele2, or, when ix_ele2 <= ix_ele1, ix_ele1+1 .. n_ele then
+ * 1 .. ix_ele2 (beam_mod.f90:148-166). Losses record loss_turn = turn. Persistent kernel;
+ * asynchronous. */
+int32_t bmx_track_bunch(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam, int32_t ix_ele1,
+                        int32_t ix_ele2, int32_t turn);
 /* One element for the whole bunch: track1_bunch_hom (beam_utils.f90:32-90) on the GPU, state
  * read from and written back to HBM. ix_ele is 1-based. Asynchronous. */
 int32_t bmx_track_element(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam, int32_t ix_ele,
