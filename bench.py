@@ -2,8 +2,9 @@
 
 Workload (default): the 901-element ring with sextupoles, one rf cavity and elliptical
 apertures on every element; 10^6 electrons at 5 GeV from a seeded 6-D Gaussian with 3x the
-base sigmas; 1000 turns. One step = 100 turns of the whole beam in one persistent launch
-(state resident in HBM between steps), so the default 10 steps are the full 1000 turns.
+base sigmas; 1000 turns. The 1000 turns are split evenly over the --steps steps (one step =
+1000/steps turns of the whole beam in one bmx_track_turns call, state resident in HBM between
+steps), so every run is the same 1000-turn workload whatever the step count.
 Throughput counts particle-element passes of live particles only, as the reference's loop
 skips dead particles (bmad/multiparticle/beam_utils.f90:83).
 
@@ -37,8 +38,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--particles", type=int, default=1_000_000)
-    ap.add_argument("--turns-per-step", type=int, default=100)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+    ap.add_argument("--turns", type=int, default=1000,
+                    help="turns of the timed run, split evenly over the steps")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time of the cpu_baseline sample (0 = skip)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--maps", choices=["fast", "reference"], default="fast",
@@ -75,25 +77,73 @@ def profiled_traffic(special: bool):
     return float(m.group(1)) * 1e6 if m else None
 
 
+def host_topology() -> dict:
+    """Sockets, cores per socket and threads per core of the host (lscpu)."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    keys = {"Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "Model name": "model", "CPU(s)": "cpus"}
+    for line in out.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in keys:
+            v = v.strip()
+            info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    return info
+
+
 def cpu_baseline(bmx, arr, beam, target_s):
-    """The CPU restatement (OpenMP over particles per element, as track1_bunch_hom) on a
-    bounded sample of the same workload: a particle subset over a few turns."""
+    """The CPU restatement (OpenMP over particles per element, as track1_bunch_hom,
+    beam_utils.f90:81-86) on a bounded sample of the same workload: a particle subset over a
+    prefix of the turns, threads spread over the host cores. Two builds of the same source: -O2
+    (the reference's optional optimisation level) and -O3 -march=native (compiled here, for
+    this host); value is the -O2 leg."""
+    import tempfile
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     n = 20000
     sample = {k: np.ascontiguousarray(v[:n]) for k, v in beam.items()}
-    # calibrate with one turn, then run enough turns for ~target_s
-    t0 = time.perf_counter()
-    done = oracle.track_bunch(arr, {k: v.copy() for k, v in sample.items()}, 1, n_threads=threads)
-    t1 = time.perf_counter()
-    rate1 = done / max(t1 - t0, 1e-9)
-    turns = max(1, int(target_s * rate1 / (n * len(arr))))
-    t0 = time.perf_counter()
-    done = oracle.track_bunch(arr, sample, turns, n_threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": UNIT, "cores": threads, "kind": "port",
-            "sample": f"{n} of the {beam['x'].shape[0]} particles x {len(arr)} elements x "
-                      f"{turns} turns ({dt:.1f} s, C restatement, -O2 -fopenmp)"}
+    native_path = os.path.join(tempfile.mkdtemp(prefix="bmx_cpu_"), "libbmad_oracle_native.so")
+    try:
+        oracle.build_native(native_path)
+    except Exception as e:  # reported; the -O2 leg still runs
+        native_path, native_err = None, str(e)
+    else:
+        native_err = None
+
+    def leg(lib_path):
+        # calibrate with one turn, then run enough turns for ~target_s
+        t0 = time.perf_counter()
+        done = oracle.track_bunch(arr, {k: v.copy() for k, v in sample.items()}, 1,
+                                  n_threads=threads, lib_path=lib_path)
+        rate1 = done / max(time.perf_counter() - t0, 1e-9)
+        turns = max(1, int(target_s * rate1 / (n * len(arr))))
+        b = {k: v.copy() for k, v in sample.items()}
+        t0 = time.perf_counter()
+        done = oracle.track_bunch(arr, b, turns, n_threads=threads, lib_path=lib_path)
+        dt = time.perf_counter() - t0
+        return done / dt, turns, dt, float(np.mean(b["state"] == 1))
+
+    v2, turns, dt, live = leg(None)
+    topo = host_topology()
+    out = {"value": v2, "unit": UNIT, "cores": threads, "kind": "port",
+           "sample": f"{n} of the {beam['x'].shape[0]} particles x {len(arr)} elements x the "
+                     f"first {turns} turns ({dt:.1f} s; {live:.3f} of the sample alive at its end), "
+                     f"C restatement -O2 -fopenmp -ffp-contract=off",
+           "threads": threads, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+           "host": topo}
+    if native_path:
+        v3, turns3, dt3, _ = leg(native_path)
+        out["o3_native"] = {"value": v3, "turns": turns3, "s": dt3,
+                            "build": "-O3 -march=native -fopenmp -ffp-contract=off"}
+    else:
+        out["o3_native"] = {"error": native_err}
+    return out
 
 
 def main():
@@ -124,11 +174,14 @@ def main():
     t_special = time.perf_counter() - t0
     waves = args.waves or (3 if special else (5 if args.maps == "fast" else 3))
     kernel_name = f"bmx_cell_tiles_{waves}" if special else f"k_track_tiles<{str(args.maps == 'fast').lower()}, {waves}>"
-    tps = args.turns_per_step
+    # the turns of each step: the run's turns split evenly (the first steps take the remainder)
+    step_turns = [args.turns // args.steps + (1 if s < args.turns % args.steps else 0)
+                  for s in range(args.steps)]
+    step_turn0 = [int(x) for x in np.concatenate([[0], np.cumsum(step_turns)[:-1]])]
     if args.warmup > 0:
         wb = T.Beam.from_host(ctx, beam)
         for w in range(args.warmup):
-            T.track_beam(ctx, lat, wb, tps, w * tps)
+            T.track_beam(ctx, lat, wb, step_turns[0], w * step_turns[0])
         ctx.synchronize()
         wb.close()
     b = T.Beam.from_host(ctx, beam)
@@ -140,7 +193,7 @@ def main():
     ctx.record(0)
     for s in range(args.steps):
         ctx.record(1 + 2 * s)
-        T.track_beam(ctx, lat, b, tps, s * tps)
+        T.track_beam(ctx, lat, b, step_turns[s], step_turn0[s])
         ctx.record(2 + 2 * s)
     ctx.record(63)
     ctx.synchronize()
@@ -172,14 +225,15 @@ def main():
         "config": {
             "workload": f"ring of 100 FODO cells + sextupoles + rf cavity, {len(arr)} elements, "
                         f"elliptical apertures; {args.particles} e- at 5 GeV, 3x sigmas; "
-                        f"{args.steps * tps} turns in steps of {tps}",
-            "particles": args.particles, "elements": len(arr), "turns": args.steps * tps,
-            "turns_per_step": tps, "alive_at_end": alive, "wall_s": t_wall,
+                        f"{args.turns} turns in {args.steps} steps",
+            "particles": args.particles, "elements": len(arr), "turns": args.turns,
+            "turns_per_step": step_turns[0], "alive_at_end": alive, "wall_s": t_wall,
             "maps": args.maps, "cell_specialized": bool(special), "waves_per_simd": waves,
             "specialize_s": t_special, "specialize_error": special_err,
         },
         "roofline": {
-            "bound": "mfma", "compute": "fp64 vector ALU", "achieved": achieved,
+            "bound": "valu", "compute": "fp64 vector ALU (no MFMA: the maps are not contractions)",
+            "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": profiled_traffic(special),
             "traffic_note": "HBM bytes of the first 5-turn launch over 1e6 particles, "

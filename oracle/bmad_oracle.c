@@ -34,6 +34,15 @@ static const double C_LIGHT = 2.99792458e8;
 /* bmad_struct.f90:2314 bmad_com%max_aperture_limit */
 static const double MAX_APERTURE_LIMIT = 1e3;
 
+/* A validity test that repeats one made just before on the same coordinates (the entrance and
+ * exit aperture checks call orbit_too_large again, check_aperture_limit.f90:61; a drift or a
+ * marker changes none of px, py, pz). The FLOP-counting build (count_flops.cpp) defines this to
+ * leave the expression's operations uncounted, as the GPU kernels do not repeat these tests;
+ * here it is the expression itself. */
+#ifndef BMO_REPEATED_TEST
+#define BMO_REPEATED_TEST(expr) (expr)
+#endif
+
 #define FIRST_TRACK_EDGE 11  /* bmad_struct.f90:379 */
 #define SECOND_TRACK_EDGE 12
 
@@ -1060,7 +1069,7 @@ static void check_aperture_limit(coord_t *o, const bmx_ele *e, int particle_at, 
 {
     double u = 0;
     *uf = 0;
-    if (orbit_too_large(o, uf)) return;
+    if (BMO_REPEATED_TEST(orbit_too_large(o, uf))) return;
     int physical_end = particle_at == FIRST_TRACK_EDGE ? BMX_ENTRANCE_END : BMX_EXIT_END;
     int at = e->aperture_at;
     int here;
@@ -1193,7 +1202,9 @@ static int track1(coord_t *o, const bmx_ele *e)
     if (o->state != BMX_STATE_ALIVE) return 0;
     int err = track1_bmad(o, e);
     if (err) return err;
-    if (orbit_too_large(o, NULL)) return 0;
+    const int no_momentum_change = e->key == BMX_DRIFT || e->key == BMX_MARKER;
+    if (no_momentum_change ? BMO_REPEATED_TEST(orbit_too_large(o, NULL)) : orbit_too_large(o, NULL))
+        return 0;
     if (o->state != BMX_STATE_ALIVE) return 0;
     check_aperture_limit(o, e, SECOND_TRACK_EDGE, &u);
     return 0;

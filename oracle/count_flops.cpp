@@ -4,8 +4,9 @@
 // that carries a "depends on the particle" flag. An arithmetic operation is counted only when
 // an operand depends on the particle, so per-element constants that the reference recomputes
 // per particle are excluded automatically. Convention: add/sub/mul/div = 1, sqrt = 1, each libm
-// transcendental (sin, cos, asin, atan2, sinh, cosh, fmod) = 1; sign changes, fabs, comparisons
-// and multiplications by an exact constant 1 are free. The rf phase is computed in quad
+// transcendental (sin, cos, asin, atan2, sinh, cosh, fmod) = 1; sign changes, fabs, comparisons,
+// multiplications by an exact constant 1 and operations with an exact constant 0 are free, and so
+// are validity tests that repeat one made on the same coordinates (BMO_REPEATED_TEST). The rf phase is computed in quad
 // precision by the reference; its 4 operations are counted at the conversion back to double.
 #include <math.h>
 #include <quadmath.h>
@@ -33,22 +34,39 @@ struct cdouble {
 };
 
 inline bool one(const cdouble &a) { return !a.dyn && a.v == 1.0; }
+// An exact constant zero: an operation with it as an operand does no work (x + 0 = x, x * 0 = 0,
+// 0 / x = 0), so it is not counted and a product or quotient with it is a constant, which makes
+// every operation on that term free as well (the general Hwang edge kick with e = 0, for example,
+// is the simple one).
+inline bool zero(const cdouble &a) { return !a.dyn && a.v == 0.0; }
 inline cdouble op(double r, const cdouble &a, const cdouble &b)
 {
     bool d = a.dyn || b.dyn;
     if (d) g_ops++;
     return cdouble(r, d);
 }
-inline cdouble operator+(const cdouble &a, const cdouble &b) { return op(a.v + b.v, a, b); }
-inline cdouble operator-(const cdouble &a, const cdouble &b) { return op(a.v - b.v, a, b); }
+inline cdouble operator+(const cdouble &a, const cdouble &b)
+{
+    if (zero(a)) return cdouble(a.v + b.v, b.dyn);
+    if (zero(b)) return cdouble(a.v + b.v, a.dyn);
+    return op(a.v + b.v, a, b);
+}
+inline cdouble operator-(const cdouble &a, const cdouble &b)
+{
+    if (zero(a)) return cdouble(a.v - b.v, b.dyn);  // a sign change
+    if (zero(b)) return cdouble(a.v - b.v, a.dyn);
+    return op(a.v - b.v, a, b);
+}
 inline cdouble operator*(const cdouble &a, const cdouble &b)
 {
+    if (zero(a) || zero(b)) return cdouble(a.v * b.v, false);
     if (one(a)) return b;
     if (one(b)) return a;
     return op(a.v * b.v, a, b);
 }
 inline cdouble operator/(const cdouble &a, const cdouble &b)
 {
+    if (zero(a)) return cdouble(a.v / b.v, false);
     if (one(b)) return a;
     return op(a.v / b.v, a, b);
 }
@@ -78,7 +96,20 @@ inline cdouble fabs(const cdouble &a) { return cdouble(::fabs(a.v), a.dyn); }
 inline cdouble atan2(const cdouble &a, const cdouble &b) { return op(::atan2(a.v, b.v), a, b); }
 inline cdouble fmod(const cdouble &a, const cdouble &b) { return op(::fmod(a.v, b.v), a, b); }
 inline long lround(const cdouble &a) { return ::lround(a.v); }
+
+// Runs f without counting its operations: validity tests that repeat one made just before on
+// the same coordinates, which the GPU kernels do not repeat (BMO_REPEATED_TEST in the source).
+template <class Fn>
+inline auto uncounted(Fn f)
+{
+    const long long saved = g_ops;
+    auto r = f();
+    g_ops = saved;
+    return r;
+}
 }  // namespace cnt
+
+#define BMO_REPEATED_TEST(expr) (cnt::uncounted([&]() { return (expr); }))
 
 using cnt::cdouble;
 using cnt::sqrt;

@@ -72,8 +72,12 @@ def main(n=4000):
     kinds["aperture_rectangular"] = {"flops": round(float(avg_count(rect) - marker), 2)}
     out = {
         "convention": "add/sub/mul/div = 1, sqrt = 1, libm call = 1; per-particle work only "
-                      "(operations with a particle-dependent operand); counted by running the "
-                      "CPU restatement with a counting FP64 type (oracle/count_flops.cpp)",
+                      "(operations with a particle-dependent operand); operations with an exact "
+                      "constant 0 operand are free (x + 0, x * 0, 0 / x), and so are validity "
+                      "tests that repeat one made on the same coordinates (the aperture checks' "
+                      "orbit_too_large, the exit test of drifts and markers), which the kernels "
+                      "do not repeat; counted by running the CPU restatement with a counting "
+                      "FP64 type (oracle/count_flops.cpp)",
         "beam": f"{n} particles, 5 GeV e-, seeded Gaussian with 3x base sigmas",
         "per_kind": kinds,
     }

@@ -27,10 +27,18 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
-def lib(fma: bool = False):
+def build_native(out_path: str) -> str:
+    """The restatement at -O3 -march=native for this host (the CPU baseline's second leg),
+    built into out_path."""
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "native", f"NATIVE_OUT={out_path}"], check=True)
+    return out_path
+
+
+def lib(fma: bool = False, path: str | None = None):
     """The restatement; fma=True: the build with FMA contraction (same algorithm, other
-    rounding)."""
-    path = LIB_FMA_PATH if fma else LIB_PATH
+    rounding); path: another build of the same source (build_native)."""
+    if path is None:
+        path = LIB_FMA_PATH if fma else LIB_PATH
     if path not in _libs:
         if not os.path.exists(path):
             build()
@@ -91,9 +99,10 @@ def track1(ele, vec, beta, state=1, mc2=0.51099895069e6, charge=-1, p0c=None):
 
 
 def track_bunch(ele_arr: np.ndarray, beam: dict, n_turns: int, turn0: int = 0, p0c=None,
-                mc2=0.51099895069e6, charge=-1, n_threads: int = 0, fma: bool = False) -> int:
+                mc2=0.51099895069e6, charge=-1, n_threads: int = 0, fma: bool = False,
+                lib_path: str | None = None) -> int:
     """Track a SoA beam dict in place; returns the number of particle-element passes.
-    fma=True runs the FMA-contracted build of the same source."""
+    fma=True runs the FMA-contracted build of the same source; lib_path another build."""
     n = beam["x"].shape[0]
     if p0c is None:
         p0c = float(ele_arr[0]["p0c"])
@@ -102,7 +111,7 @@ def track_bunch(ele_arr: np.ndarray, beam: dict, n_turns: int, turn0: int = 0, p
     for k in ("state", "loss_ele", "loss_turn"):
         assert beam[k].dtype == np.int32 and beam[k].flags.c_contiguous
     tracked = ctypes.c_int64(0)
-    err = lib(fma).bmo_track_bunch(
+    err = lib(fma, lib_path).bmo_track_bunch(
         ele_arr.ctypes.data, len(ele_arr), mc2, charge, p0c, n,
         *[_ptr(beam[k], _D) for k in ("x", "px", "y", "py", "z", "pz", "beta")],
         *[_ptr(beam[k], _I) for k in ("state", "loss_ele", "loss_turn")],

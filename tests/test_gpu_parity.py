@@ -169,9 +169,6 @@ def test_ring0_1000_turns(bmx, oracle, ctx):
     assert err <= 1e-9, err
 
 
-_CPU_1000 = {}
-
-
 def test_ring1_no_rf_1000_turns(bmx, oracle, ctx):
     """Sextupoles and apertures (the 901-element ring with its rf cavity switched off), a
     1-sigma beam, 1000 turns without resynchronisation. The amplitude-dependent tunes turn
@@ -179,20 +176,11 @@ def test_ring1_no_rf_1000_turns(bmx, oracle, ctx):
     CPU restatement built with FMA contraction differs from the plain build by ~3e-10 (median)
     to ~3e-8 (max) after 1000 turns, so that is the resolution at which any FP64
     implementation can match it. The GPU must stay within 3x of that spread at the median,
-    90th and 99th percentile, and within 1e-9 at the median."""
-    if "norf" not in _CPU_1000:
-        L = bmx.lattices
-        arr = bmx.elements.to_array(L.ring1(rf=False))
-        beam = L.gaussian_beam(1024, seed=1, sigma=L.SIGMA0)
-        ref, noise = P.fma_noise(oracle, arr, beam, 1000)
-        _CPU_1000["norf"] = (arr, beam, ref, noise)
-    arr, beam, ref, noise = _CPU_1000["norf"]
+    90th and 99th percentile, within 1e-9 at the median, and every particle within
+    max(1e-9, 100x its envelope) (parity.envelope)."""
+    arr, beam, ref, noise, env = P.ring1_norf_1000(bmx, oracle)
     out = _host(bmx, ctx, arr, beam, 1000)
-    assert np.all(out["state"] == 1) and np.all(ref["state"] == 1) and np.all(np.isfinite(noise))
-    dev = P.rel_dev(out, ref, ref["state"] == 1)
-    for q in (50, 90, 99):
-        assert np.percentile(dev, q) <= 3 * np.percentile(noise, q), (q, np.percentile(dev, q), np.percentile(noise, q))
-    assert np.median(dev) <= 1e-9
+    P.check_norf_1000(out, ref, noise, env)
 
 
 def test_ring1_chaotic_1000_turns(bmx, oracle, ctx):
@@ -205,26 +193,9 @@ def test_ring1_chaotic_1000_turns(bmx, oracle, ctx):
     (per-turn agreement is checked by test_ring1_loss_decisions_per_turn). Checked here:
     losses in the first 10 turns are identical, and the surviving sets differ by no more than
     a small multiple of what a (1 + 1e-15) rescaling alone changes in the CPU restatement."""
-    if "full" not in _CPU_1000:
-        L = bmx.lattices
-        arr = bmx.elements.to_array(L.ring1())
-        beam = L.gaussian_beam(1024, seed=1, sigma=3 * L.SIGMA0)
-        ref = P.copy_beam(beam)
-        oracle.track_bunch(arr, ref, n_turns=1000)
-        pert = P.copy_beam(beam)
-        for k in P.COORDS:
-            pert[k] = pert[k] * (1 + 1e-15)
-        oracle.track_bunch(arr, pert, n_turns=1000)
-        self_flips = int(np.sum((ref["state"] == 1) ^ (pert["state"] == 1)))
-        _CPU_1000["full"] = (arr, beam, ref, self_flips)
-    arr, beam, ref, self_flips = _CPU_1000["full"]
+    arr, beam, ref, self_flips = P.ring1_full_1000(bmx, oracle)
     out = _host(bmx, ctx, arr, beam, 1000)
-    early = (ref["state"] != 1) & (ref["loss_turn"] < 10)
-    assert early.sum() > 20
-    for k in ("state", "loss_ele", "loss_turn"):
-        assert np.array_equal(out[k][early], ref[k][early]), k
-    flips = int(np.sum((ref["state"] == 1) ^ (out["state"] == 1)))
-    assert flips <= 3 * max(self_flips, 5), (flips, self_flips)
+    P.check_full_1000(out, ref, self_flips)
 
 
 def test_chunked_equals_single_call(bmx, ctx):

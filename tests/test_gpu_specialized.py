@@ -29,15 +29,16 @@ def _track(bmx, ctx, arr, beam, n_turns, specialize, turn0=0):
     return out
 
 
-def _within_spread(out, ref, noise):
+def _within_envelope(out, ref, env, floor=1e-10):
+    """Every survivor within max(floor, 100 x its envelope) of the restatement (parity.envelope:
+    the response of the restatement itself to FMA contraction and to 1e-14 rescalings of the
+    input, so the bar is what a 1e-12 change of the input moves that particle by)."""
+    bad, dev = P.outside_envelope(out, ref, env, floor=floor)
     alive = ref["state"] == 1
-    dev = P.rel_dev(out, ref, alive)[alive]
-    nz = noise[alive]
-    print("deviation / fma spread percentiles 50, 90, 99:",
-          [(float(np.percentile(dev, q)), float(np.percentile(nz, q))) for q in (50, 90, 99)])
-    for q in (50, 90, 99):
-        assert np.percentile(dev, q) <= 3 * np.percentile(nz, q) + 1e-13, (
-            q, np.percentile(dev, q), np.percentile(nz, q))
+    ratio = dev[alive] / np.maximum(env[alive], 1e-300)
+    print("deviation percentiles 50, 90, 99, max:", np.percentile(dev[alive], [50, 90, 99, 100]),
+          "deviation / envelope max:", float(np.max(ratio)), flush=True)
+    assert bad.size == 0, (bad[:10], dev[bad[:10]], env[bad[:10]])
 
 
 def test_ring0_1000_turns_specialized(bmx, oracle, ctx):
@@ -52,6 +53,26 @@ def test_ring0_1000_turns_specialized(bmx, oracle, ctx):
     out = _track(bmx, ctx, arr, beam, 1000, True)
     assert np.all(out["state"] == 1) and np.all(ref["state"] == 1)
     assert P.rel_err(out, ref) <= 1e-9
+
+
+def test_ring1_no_rf_1000_turns_specialized(bmx, oracle, ctx):
+    """The bench's kernel on the 901-element ring without rf, a 1-sigma beam, 1000 turns: the
+    bars of the generic kernel's test (parity.check_norf_1000): the FMA-spread percentiles, the
+    median within 1e-9 and every particle within max(1e-9, 100x its envelope)."""
+    arr, beam, ref, noise, env = P.ring1_norf_1000(bmx, oracle)
+    print("ring1 without rf, 1000 turns, specialised", flush=True)
+    out = _track(bmx, ctx, arr, beam, 1000, True)
+    P.check_norf_1000(out, ref, noise, env)
+
+
+def test_ring1_chaotic_1000_turns_specialized(bmx, oracle, ctx):
+    """The bench's kernel on the full 901-element ring, 1000 turns: losses of the first 10 turns
+    identical to the restatement and the surviving set within 3x of the restatement's own
+    sensitivity (parity.check_full_1000)."""
+    arr, beam, ref, self_flips = P.ring1_full_1000(bmx, oracle)
+    print("ring1 full, 1000 turns, specialised", flush=True)
+    out = _track(bmx, ctx, arr, beam, 1000, True)
+    P.check_full_1000(out, ref, self_flips)
 
 
 @pytest.mark.parametrize("waves", [3, 4, 5])
@@ -78,30 +99,30 @@ def test_ring1_losses_specialized(bmx, oracle, ctx, waves):
 
 def test_head_cell_and_tail(bmx, oracle, ctx):
     """A marker in front of the ring shifts the repeated run: a generic head cell, 99 rotated
-    cells and a two-element generic tail. After 20 turns: losses bit-exact; the survivors'
-    deviations from the CPU restatement within 3x of the spread that FMA contraction alone
-    causes in it (median, 90th, 99th percentile), as for the generic kernel: these
-    large-amplitude orbits amplify any rounding difference."""
+    cells and a two-element generic tail. After 20 turns: losses bit-exact; every survivor
+    within 100x of its envelope (the restatement's own response to FMA contraction and to 1e-14
+    rescalings of its input, parity.envelope): these large-amplitude orbits amplify any
+    rounding difference."""
     L, E = bmx.lattices, bmx.elements
     eles = L.ring1()
     p0c, et = L.reference()
     eles = [E.make_element(E.MARKER, p0c, et)] + list(eles)
     arr = E.to_array(eles)
     beam = P.random_beam(bmx, 2000, seed=3, scale=15.0, pz_sigma=3e-3)
-    ref, noise = P.fma_noise(oracle, arr, beam, 20)
+    ref, env = P.envelope(oracle, arr, beam, 20)
     print("head cell + tail, specialised", flush=True)
     out = _track(bmx, ctx, arr, beam, 20, True)
     assert (ref["state"] != 1).sum() > 100
     bad = P.loss_mismatch(out, ref)
     assert bad.size == 0, bad[:10]
-    _within_spread(out, ref, noise)
+    _within_envelope(out, ref, env)
 
 
 def test_batch_of_specialized_variants(bmx, oracle, ctx):
     """Eight error variants of a 10-cell ring share one specialised kernel (strengths stay in
     the table); the batch equals the CPU restatement of each variant after 25 turns: losses
-    bit-exact, the survivors' deviations within 3x of the FMA-contraction spread (median, 90th,
-    99th percentile over all variants)."""
+    bit-exact, every survivor within 100x of its envelope (the restatement's own response to FMA
+    contraction and to 1e-14 rescalings of its input) or 1e-10."""
     L, T = bmx.lattices, bmx.tracking
     arrs = [bmx.elements.to_array(e) for e in L.ring1_variants(5, 8, n_cells=10)]
     counts = [700, 64, 1, 333, 1000, 129, 64, 500]
@@ -119,21 +140,16 @@ def test_batch_of_specialized_variants(bmx, oracle, ctx):
     batch.close()
     b.close()
     lost = 0
-    devs, noises = [], []
     for i in range(8):
         sl = slice(int(starts[i]), int(starts[i]) + counts[i])
-        part, noise = P.fma_noise(oracle, arrs[i], {k: v[sl].copy() for k, v in beam.items()}, 25)
+        part, env = P.envelope(oracle, arrs[i], {k: v[sl].copy() for k, v in beam.items()}, 25)
         mine = {k: v[sl].copy() for k, v in out.items()}
         bad = P.loss_mismatch(mine, part)
         assert bad.size == 0, (i, bad[:5])
-        alive = part["state"] == 1
-        devs.append(P.rel_dev(mine, part, alive)[alive])
-        noises.append(noise[alive])
+        far, _ = P.outside_envelope(mine, part, env)
+        assert far.size == 0, (i, far[:5])
         lost += int((part["state"] != 1).sum())
     assert lost > 100
-    dev, nz = np.concatenate(devs), np.concatenate(noises)
-    for q in (50, 90, 99):
-        assert np.percentile(dev, q) <= 3 * np.percentile(nz, q) + 1e-13, q
     for lt in lats:
         lt.close()
 
