@@ -359,7 +359,10 @@ int32_t track_group(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t fir
     cfg.fast = ctx->fast_maps;
     cfg.jit = ctx->fast_maps ? L->jit : nullptr;
     cfg.waves = ctx->waves ? ctx->waves : default_waves(ctx->fast_maps, cfg.jit);
-    cfg.grid = ctx->grid > 0 ? ctx->grid : default_grid(ctx->device, ctx->fast_maps, cfg.waves, cfg.jit);
+    // a start element or a partial turn runs on the generic span kernel
+    cfg.span = first != 0 || count != L->desc.n_ele;
+    cfg.grid = ctx->grid > 0 ? ctx->grid
+                             : default_grid(ctx->device, ctx->fast_maps, cfg.waves, cfg.jit, cfg.span);
     cfg.timer = ctx->profile ? &ctx->timer : nullptr;
     HIP_TRY(enqueue_run(ctx->stream, ctx->run, &g, 1, true, n_turns, cfg, ctx->d_work), "k_track_tiles");
     return BMX_OK;

@@ -15,7 +15,7 @@ hipError_t launch_track_tiles(hipStream_t s, int grid, int waves, const Group *d
                               const WaveTile *d_tiles, const uint32_t *d_n_tiles,
                               const uint32_t *d_list_in, uint32_t *d_list_out, uint32_t *d_cnt_out,
                               uint32_t *d_tile_ctr, int32_t chunk_off, int32_t chunk_turns,
-                              unsigned long long *d_work, bool fast);
+                              unsigned long long *d_work, bool fast, bool span);
 hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_groups,
                               const uint32_t *d_cnt_in, int32_t next_off, WaveTile *d_tiles,
                               uint32_t *d_n_tiles, uint32_t *d_cnt_zero, uint32_t *d_tile_ctr,
@@ -23,7 +23,10 @@ hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_gr
 hipError_t launch_collect_alive(hipStream_t s, int grid, const Group *d_groups,
                                 const WaveTile *d_tiles, const uint32_t *d_n_tiles,
                                 uint32_t *d_list_out, uint32_t *d_cnt_out);
-hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu);
+hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu, bool span = false);
+// Waves per SIMD of the kernel for runs with element spans (Group first/count other than the
+// full turn from element 0).
+constexpr int kSpanWaves = 3;
 hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, int32_t turn,
                                 const BeamPtrs &b, int64_t n, unsigned long long *d_work,
                                 bool fast);
@@ -95,6 +98,7 @@ struct RunConfig {
     bool fast = true;
     KernelTimer *timer = nullptr;  // if set: an event pair around every tracking launch
     const JitKernels *jit = nullptr;  // cell-specialised kernels shared by every group's lattice
+    bool span = false;  // some group tracks an element span: the generic span kernel runs
 };
 
 // Uploads the groups (asynchronously, from host memory that must stay valid until the stream
@@ -115,7 +119,8 @@ inline int32_t default_waves(bool fast, const JitKernels *jit = nullptr)
 }
 // Workgroups of the tracking kernel that fit on the device at once (of the cell-specialised
 // kernels when jit is set).
-int default_grid(int device, bool fast, int waves, const JitKernels *jit = nullptr);
+int default_grid(int device, bool fast, int waves, const JitKernels *jit = nullptr,
+                 bool span = false);
 
 }  // namespace bmx
 

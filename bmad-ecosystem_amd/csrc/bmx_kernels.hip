@@ -15,16 +15,18 @@
 namespace bmx {
 
 // W: waves per SIMD the register allocation targets at least (3: up to 168 VGPRs; 4: up to 128;
-// 5: up to 102, spilling a few values of the rf and bend paths to scratch).
-template <bool F, int W>
+// 5: up to 102, spilling a few values of the rf and bend paths to scratch). SPAN: groups may
+// track element spans (runs with a start element or a partial turn); the full-turn kernels
+// carry no code for them.
+template <bool F, int W, bool SPAN = false>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) k_track_tiles(
     const Group *__restrict__ groups, const WaveTile *__restrict__ tiles,
     const uint32_t *__restrict__ n_tiles_p, const uint32_t *__restrict__ list_in,
     uint32_t *__restrict__ list_out, uint32_t *__restrict__ cnt_out, uint32_t *tile_ctr,
     int32_t chunk_off, int32_t chunk_turns, unsigned long long *work)
 {
-    track_tiles<F, GenericTurn<F>>(groups, tiles, n_tiles_p, list_in, list_out, cnt_out, tile_ctr,
-                                   chunk_off, chunk_turns, work);
+    track_tiles<F, GenericTurn<F>, SPAN>(groups, tiles, n_tiles_p, list_in, list_out, cnt_out,
+                                         tile_ctr, chunk_off, chunk_turns, work);
 }
 
 // Builds the wave tiles of the next chunk from the alive counts per group (cnt_in; nullptr
@@ -217,26 +219,30 @@ __global__ void __launch_bounds__(kBlock) k_count_alive(const int32_t *__restric
     }
 }
 
-template <bool F, int W>
+template <bool F, int W, bool SPAN = false>
 static void launch_tiles(hipStream_t s, int grid, const Group *g, const WaveTile *t, const uint32_t *nt,
                          const uint32_t *li, uint32_t *lo, uint32_t *co, uint32_t *tc, int32_t off,
                          int32_t K, unsigned long long *w)
 {
-    hipLaunchKernelGGL((k_track_tiles<F, W>), dim3((unsigned)grid), dim3(kBlock), 0, s, g, t, nt, li,
-                       lo, co, tc, off, K, w);
+    hipLaunchKernelGGL((k_track_tiles<F, W, SPAN>), dim3((unsigned)grid), dim3(kBlock), 0, s, g, t, nt,
+                       li, lo, co, tc, off, K, w);
 }
 
 hipError_t launch_track_tiles(hipStream_t s, int grid, int waves, const Group *d_groups,
                               const WaveTile *d_tiles, const uint32_t *d_n_tiles,
                               const uint32_t *d_list_in, uint32_t *d_list_out, uint32_t *d_cnt_out,
                               uint32_t *d_tile_ctr, int32_t chunk_off, int32_t chunk_turns,
-                              unsigned long long *d_work, bool fast)
+                              unsigned long long *d_work, bool fast, bool span)
 {
     if (grid <= 0) return hipSuccess;
-#define BMX_TILES(F, W)                                                                          \
-    launch_tiles<F, W>(s, grid, d_groups, d_tiles, d_n_tiles, d_list_in, d_list_out, d_cnt_out,  \
-                       d_tile_ctr, chunk_off, chunk_turns, d_work)
-    if (fast) {
+#define BMX_TILES(F, W, ...)                                                                     \
+    launch_tiles<F, W, ##__VA_ARGS__>(s, grid, d_groups, d_tiles, d_n_tiles, d_list_in,          \
+                                      d_list_out, d_cnt_out, d_tile_ctr, chunk_off, chunk_turns, \
+                                      d_work)
+    if (span) {
+        if (fast) BMX_TILES(true, kSpanWaves, true);
+        else BMX_TILES(false, kSpanWaves, true);
+    } else if (fast) {
         if (waves == 5) BMX_TILES(true, 5);
         else if (waves == 4) BMX_TILES(true, 4);
         else BMX_TILES(true, 3);
@@ -269,10 +275,13 @@ hipError_t launch_collect_alive(hipStream_t s, int grid, const Group *d_groups,
     return hipGetLastError();
 }
 
-hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu)
+hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu, bool span)
 {
     const void *k;
-    if (fast) k = waves == 5 ? (const void *)k_track_tiles<true, 5> : waves == 4 ? (const void *)k_track_tiles<true, 4> : (const void *)k_track_tiles<true, 3>;
+    if (span)
+        k = fast ? (const void *)k_track_tiles<true, kSpanWaves, true>
+                 : (const void *)k_track_tiles<false, kSpanWaves, true>;
+    else if (fast) k = waves == 5 ? (const void *)k_track_tiles<true, 5> : waves == 4 ? (const void *)k_track_tiles<true, 4> : (const void *)k_track_tiles<true, 3>;
     else k = waves == 5 ? (const void *)k_track_tiles<false, 5> : waves == 4 ? (const void *)k_track_tiles<false, 4> : (const void *)k_track_tiles<false, 3>;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, kBlock, 0);
 }

@@ -104,11 +104,12 @@ void KernelTimer::release()
     pool.clear();
 }
 
-int default_grid(int device, bool fast, int waves, const JitKernels *jit)
+int default_grid(int device, bool fast, int waves, const JitKernels *jit, bool span)
 {
     int blocks = 0, cus = 0;
-    const hipError_t e =
-        jit ? jit_occupancy(jit, waves, &blocks) : track_tiles_occupancy(fast, waves, &blocks);
+    const hipError_t e = span  ? track_tiles_occupancy(fast, kSpanWaves, &blocks, true)
+                         : jit ? jit_occupancy(jit, waves, &blocks)
+                               : track_tiles_occupancy(fast, waves, &blocks);
     if (e != hipSuccess || blocks <= 0) blocks = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
         cus = 1;
@@ -154,14 +155,14 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
             if (e == hipSuccess) e = hipEventRecord(ev0, s);
             if (e != hipSuccess) break;
         }
-        if (cfg.jit)
+        if (cfg.jit && !cfg.span)
             e = launch_track_tiles_jit(cfg.jit, s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur],
                                        n_tiles[cur], rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt],
                                        tile_ctr, off, K, d_work);
         else
             e = launch_track_tiles(s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur], n_tiles[cur],
                                    rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt], tile_ctr, off, K,
-                                   d_work, cfg.fast);
+                                   d_work, cfg.fast, cfg.span);
         if (cfg.timer && e == hipSuccess) {
             e = hipEventRecord(ev1, s);
             cfg.timer->pending.emplace_back(ev0, ev1);

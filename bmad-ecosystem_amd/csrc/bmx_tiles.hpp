@@ -91,11 +91,14 @@ template <bool F>
 __device__ __forceinline__ void track_span(Particle &p, const LatDesc &L, int32_t first,
                                            int32_t count, int32_t t_abs)
 {
-    const int32_t n = L.n_ele;
-    const int32_t end1 = count < n - first ? first + count : n;
-    track_range<F>(p, L, first, end1, t_abs);
-    const int32_t rest = count - (end1 - first);
-    if (rest > 0) track_range<F>(p, L, 0, rest, t_abs);
+    // at most two segments (first .. end, then 0 .. rest), one call site of the element loop
+    int32_t e0 = first, left = count;
+    while (left > 0) {
+        const int32_t e1 = left < L.n_ele - e0 ? e0 + left : L.n_ele;
+        track_range<F>(p, L, e0, e1, t_abs);
+        left -= e1 - e0;
+        e0 = 0;
+    }
 }
 
 // The entrance aperture test of element e for a particle that does not arrive from element
@@ -118,10 +121,11 @@ __device__ __forceinline__ void entrance_retest(Particle &p, const LatDesc &L, i
     }
 }
 
-// Tracks one particle through up to n_turns turns of the group's element span (first, count)
-// starting at turn index turn0 (Turn: the policy of the full turn from element 0); records the
-// loss element (1-based) and turn. Returns the number of element passes made.
-template <bool F, class Turn>
+// Tracks one particle through up to n_turns turns starting at turn index turn0 and records the
+// loss element (1-based) and turn. SPAN false: full turns from element 0 (first = 0, count =
+// n_ele) by the turn policy Turn; SPAN true: turns of the group's element span (first, count)
+// through the generic element loop. Returns the number of element passes made.
+template <bool F, class Turn, bool SPAN>
 __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, const BeamPtrs &b,
                                                              uint32_t i, int32_t n_turns,
                                                              int32_t turn0, int32_t first,
@@ -132,7 +136,6 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
     refresh_energy(p, L.sp);
     p.loss_ele = 0;
     p.loss_turn = 0;
-    const bool full = first == 0 && count == L.n_ele;  // wave-uniform
     // Entrance validity check of the first element (track1.f90:196 -> orbit_too_large); later
     // elements need none: a particle leaves element e with the coordinates it enters e+1 with,
     // and element e already ended with the same test (track1.f90:291).
@@ -140,12 +143,14 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
         p.loss_ele = first + 1;
         p.loss_turn = turn0;
     } else {
-        if (!full && first > 0) entrance_retest(p, L, first, turn0);
+        if constexpr (SPAN) {
+            if (first > 0) entrance_retest(p, L, first, turn0);
+        }
         // a wave stops once all its particles are lost
         for (int32_t turn = 0; turn < n_turns; ++turn) {
             if (!any_lane(p.state == ST_ALIVE)) break;
-            if (full) Turn::turn(p, L, turn0 + turn);
-            else track_span<F>(p, L, first, count, turn0 + turn);
+            if constexpr (SPAN) track_span<F>(p, L, first, count, turn0 + turn);
+            else Turn::turn(p, L, turn0 + turn);
         }
     }
     store_particle(p, b, i);
@@ -171,7 +176,9 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
 // Every resident wave takes tiles (as many waves per SIMD as the register budget allows hide
 // each other's latencies better than fewer waves with evenly divided tiles).
 // chunk_off: turns already done by earlier chunks of this call; chunk_turns: turns per chunk.
-template <bool F, class Turn>
+// SPAN: groups may track element spans (Group first/count); false: every group tracks full
+// turns from element 0, and the kernel carries no code for spans.
+template <bool F, class Turn, bool SPAN = false>
 __device__ __forceinline__ void track_tiles(
     const Group *__restrict__ groups, const WaveTile *__restrict__ tiles,
     const uint32_t *__restrict__ n_tiles_p, const uint32_t *__restrict__ list_in,
@@ -198,7 +205,9 @@ __device__ __forceinline__ void track_tiles(
         if ((uint32_t)lane < T.count) {
             idx = list_in[G.list_off + T.start + lane];
             if (G.b.state[idx] == ST_ALIVE) {
-                passes += track_particle<F, Turn>(G.lat, G.b, idx, n_turns, turn0, G.first, G.count);
+                passes += track_particle<F, Turn, SPAN>(G.lat, G.b, idx, n_turns, turn0,
+                                                        SPAN ? G.first : 0,
+                                                        SPAN ? G.count : G.lat.n_ele);
                 alive_end = G.b.state[idx] == ST_ALIVE;
             }
         }

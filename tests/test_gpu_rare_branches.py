@@ -166,12 +166,15 @@ def test_orbit_too_large_at_entrance(bmx, oracle, ctx):
 def test_orbit_too_large_on_ring0(bmx, oracle, ctx):
     """Orbits that grow past orbit_too_large's limits inside the 700-element ring (no
     apertures): large |px|, |py| send x or y beyond 1 km or the transverse momentum beyond
-    1 + pz after some elements. Loss state, element and turn identical to the restatement."""
+    1 + pz after some elements. Loss state, element and turn identical to the restatement;
+    survivors within 100x of their envelope (the restatement's own response to FMA contraction
+    and to 1e-14 rescalings of its input) or 1e-10: at transverse angles of 0.3 rad the drift's
+    1/ps and the bend's arcsine amplify rounding, so the restatement's two builds already
+    differ by up to ~1e-8 here."""
     n = 4096
     beam = _beam(bmx, n, 29, [1e-3, 0.3, 1e-3, 0.3, 1e-3, 0.05])
     arr = bmx.elements.to_array(bmx.lattices.ring0(n_cells=20))
-    ref = P.copy_beam(beam)
-    oracle.track_bunch(arr, ref, n_turns=3)
+    ref, env = P.envelope(oracle, arr, beam, 3)
     lost = ref["state"] != 1
     codes = set(ref["state"][lost].tolist())
     assert {3, 4, 5, 6} <= codes, codes
@@ -183,6 +186,5 @@ def test_orbit_too_large_on_ring0(bmx, oracle, ctx):
     out = b.download()
     bad = P.loss_mismatch(out, ref)
     assert bad.size == 0, (bad[:10], out["loss_ele"][bad[:10]], ref["loss_ele"][bad[:10]])
-    alive = ref["state"] == 1
-    if alive.any():
-        assert P.rel_err(out, ref, alive) <= 1e-10
+    far, dev = P.outside_envelope(out, ref, env)
+    assert far.size == 0, (far[:10], dev[far[:10]], env[far[:10]])

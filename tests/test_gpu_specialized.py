@@ -122,7 +122,12 @@ def test_batch_of_specialized_variants(bmx, oracle, ctx):
     """Eight error variants of a 10-cell ring share one specialised kernel (strengths stay in
     the table); the batch equals the CPU restatement of each variant after 25 turns: losses
     bit-exact, every survivor within 100x of its envelope (the restatement's own response to FMA
-    contraction and to 1e-14 rescalings of its input) or 1e-10."""
+    contraction and to 1e-14 rescalings of its input) or 1e-9, the multi-turn bar. The envelope
+    can miss a direction along which an orbit amplifies errors: the sector bend computes px from
+    sums of angles of the order of the bend angle, whose last-bit rounding (~1e-17 absolute)
+    contraction does not change, and four random rescalings need not excite it; one orbit of
+    these 8000 large-amplitude ones grows such a difference to ~1.4e-10 in 25 turns (the
+    reference-form maps, evaluating the same formulas as the restatement, reach 1.6e-10 on it)."""
     L, T = bmx.lattices, bmx.tracking
     arrs = [bmx.elements.to_array(e) for e in L.ring1_variants(5, 8, n_cells=10)]
     counts = [700, 64, 1, 333, 1000, 129, 64, 500]
@@ -146,8 +151,8 @@ def test_batch_of_specialized_variants(bmx, oracle, ctx):
         mine = {k: v[sl].copy() for k, v in out.items()}
         bad = P.loss_mismatch(mine, part)
         assert bad.size == 0, (i, bad[:5])
-        far, _ = P.outside_envelope(mine, part, env)
-        assert far.size == 0, (i, far[:5])
+        far, dev = P.outside_envelope(mine, part, env, floor=1e-9)
+        assert far.size == 0, (i, far[:5], dev[far[:5]], env[far[:5]])
         lost += int((part["state"] != 1).sum())
     assert lost > 100
     for lt in lats:
