@@ -172,7 +172,7 @@ def main():
         except bmx.native.BmxError as e:  # reported in the JSON; the generic kernel runs
             special_err = str(e)
     t_special = time.perf_counter() - t0
-    waves = args.waves or (3 if special else (5 if args.maps == "fast" else 3))
+    waves = args.waves or (4 if special else (5 if args.maps == "fast" else 3))
     kernel_name = f"bmx_cell_tiles_{waves}" if special else f"k_track_tiles<{str(args.maps == 'fast').lower()}, {waves}>"
     # the turns of each step: the run's turns split evenly (the first steps take the remainder)
     step_turns = [args.turns // args.steps + (1 if s < args.turns % args.steps else 0)

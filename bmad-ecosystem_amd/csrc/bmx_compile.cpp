@@ -413,7 +413,9 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             q[24] = L * q[6];
             q[25] = 0.5 * q[12];
             q[26] = 0.5 * q[18];
-            if (std::fabs(angle) <= 0.1) h.flags |= F_BEND_SERIES;
+            q[27] = angle * angle * q[7];  // 1 - cos(angle) without cancellation
+            q[28] = g != 0 ? 1.0 / g : 0.0;
+            if (g != 0 && q[2] == g && std::fabs(angle) <= 0.1) h.flags |= F_BEND_SERIES;
             break;
         }
         case BMX_RFCAVITY: {

@@ -109,13 +109,12 @@ struct RunConfig {
 hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int32_t n_groups,
                        bool upload, int32_t max_turns, const RunConfig &cfg,
                        unsigned long long *d_work);
-// Waves per SIMD of the default tracking kernel variant: 5 for the fast maps (the fastest
-// measured), 3 for the reference maps (whose 4- and 5-wave builds spill).
 // Default waves per SIMD of the tracking kernel: the variant measured fastest on the 901-element
-// ring for each kernel (cell-specialised 3, generic fast maps 5, reference maps 3).
+// ring for each kernel (cell-specialised 4: at most 128 VGPRs, so 4 waves per SIMD are resident,
+// without spilling; generic fast maps 5; reference maps 3, whose 4- and 5-wave builds spill).
 inline int32_t default_waves(bool fast, const JitKernels *jit = nullptr)
 {
-    return jit ? 3 : (fast ? 5 : 3);
+    return jit ? 4 : (fast ? 5 : 3);
 }
 // Workgroups of the tracking kernel that fit on the device at once (of the cell-specialised
 // kernels when jit is set).

@@ -766,6 +766,34 @@ __device__ __forceinline__ double sqrt_1m(double u)
     return fma(r, u, 1.0);
 }
 
+// sqrt(1 - u) - 1 for u < 2^-10, without the cancellation of forming sqrt(1 - u) first: the
+// same series times u (the first omitted term, -33/2048 u^7, is below 3e-20 of the result).
+// Power form, coefficients as multiplicands (scalar operands), as asin_over_short.
+__device__ __forceinline__ double sqrt_1m_minus_1(double u)
+{
+    const double u2 = u * u, u4 = u2 * u2;
+    double r = fma(-0.125, u, -0.5);                   // -1/2, -1/8
+    r = fma(-0.0625, u2, r);                           // -1/16
+    r = fma(-0.0390625, u2 * u, r);                    // -5/128
+    r = fma(-0.02734375, u4, r);                       // -7/256
+    r = fma(-0.0205078125, u4 * u, r);                 // -21/1024
+    return r * u;
+}
+
+// atan(t) for t^2 <= 2^-10: t (1 - t^2/3 + t^4/5 - t^6/7 + t^8/9 - t^10/11); the first omitted
+// term, t^13/13, is below 7e-20 of the result. Power form as sqrt_1m_minus_1.
+constexpr double kAtanSeriesMax = 1.0 / 1024;
+__device__ __forceinline__ double atan_short(double t)
+{
+    const double t2 = t * t, t4 = t2 * t2, t8 = t4 * t4;
+    double r = fma(-0.3333333333333333, t2, 1.0);     // -1/3
+    r = fma(0.2, t4, r);                               // 1/5
+    r = fma(-0.14285714285714285, t4 * t2, r);         // -1/7
+    r = fma(0.1111111111111111, t8, r);                // 1/9
+    r = fma(-0.09090909090909091, t8 * t2, r);         // -1/11
+    return r * t;
+}
+
 // h / sin(h) for the angle h with sine sh and cosine ch (|h| < pi) outside the direct series
 // domain: k half-angle steps s' = s / sqrt(2 (1 + c)), c' = sqrt((1 + c)/2) bring |s| below 1/8,
 // then h / sin h = 2^k asin(s_k)/s_k * s_k / sh.
@@ -925,6 +953,54 @@ __device__ __forceinline__ void sbend_exact_fast(Particle &p, CP b)
     p.z = z + (p.beta * b[23] - rel_p * lp_pt);
 }
 
+// The exact body in momentum form, for g_tot = g (the same orbit as track_a_bend.f90:163-209:
+// a circle of radius pt/g in the horizontal plane from the entrance face to the exit face of
+// a sector bend of angle theta). With the longitudinal momenta pz_t = sqrt((1+pz)^2 - px^2 -
+// py^2) at the entrance and pz_f at the exit, the circle's geometry gives in closed form
+//   px_f = px cos theta - (g x - (pz_t - 1)) sin theta,
+//   x_f  = x cos theta + ((pz_f - pz_t) + (pz_t - 1)(1 - cos theta) + px sin theta) / g,
+// and the orbit turns by Phi = theta + phi_1 - phi_f (phi = atan2(px, pz) at each face), so
+// the path length in the horizontal plane is L_p = pt Phi / g (y += py L_p/pt, z -= (1+pz)
+// L_p/pt, as the reference). pz - 1 comes from the series of sqrt(1 - u) - 1 and
+// tan(phi_1 - phi_f) = (px pz_f - px_f pz_t)/(pz_t pz_f + px px_f), so the body needs no
+// transcendental call, one division and no square root, and every difference of nearly equal
+// quantities is formed from small terms. The reference's form evaluates the same geometry
+// through asin, atan2 and sin of angle sums; the two agree to a few units in the last place of
+// the coordinates' scale. Returns false on lanes outside the form's domain (|p_perp|/p >=
+// 2^-5 at either face, |phi_1 - phi_f| beyond the atan series, |x_f| above orbit_too_large's
+// limit), whose coordinates are then untouched; the caller takes those through the general
+// form, which also decides every loss inside the body.
+__device__ __forceinline__ bool sbend_momentum_form(Particle &p, CP b)
+{
+    const double x = p.x, px = p.px, py = p.py, pz = p.pz, rel_p = p.rel_p;
+    const double g = b[1], cos_a = b[4], sin_a = b[5], omc = b[27], inv_g = b[28];
+    const double irp2 = p.inv_rp * p.inv_rp;
+    const double py2 = py * py;
+    const double su = fma(px, px, py2) * irp2;
+    const double m_t = sqrt_1m_minus_1(su);
+    const double pzt_m1 = fma(rel_p, m_t, pz);        // pz_t - 1
+    const double pz_t = 1.0 + pzt_m1;
+    const double pxf = fma(px, cos_a, -fma(g, x, -pzt_m1) * sin_a);
+    const double sf = fma(pxf, pxf, py2) * irp2;
+    const double m_f = sqrt_1m_minus_1(sf);
+    const double pz_f = 1.0 + fma(rel_p, m_f, pz);
+    const double br = fma(px, sin_a, fma(pzt_m1, omc, rel_p * (m_f - m_t)));
+    const double xf = fma(br, inv_g, x * cos_a);
+    const double S = fma(px, pz_f, -pxf * pz_t);
+    const double C = fma(pz_t, pz_f, px * pxf);
+    const double t = div_normal(S, C);
+    const bool ok = (su < kDriftSeriesMax) & (sf < kDriftSeriesMax) & (t * t <= kAtanSeriesMax) &
+                    (fabs(xf) <= kMaxAperture);
+    if (ok) {
+        const double lp_pt = (b[3] + atan_short(t)) * inv_g;  // L_p / pt = Phi / g
+        p.x = xf;
+        p.px = pxf;
+        p.y = fma(py, lp_pt, p.y);
+        p.z = p.z + (p.beta * b[23] - rel_p * lp_pt);
+    }
+    return ok;
+}
+
 // orbit_too_large at a point inside an element: true (and the state set) if it flags the
 // particle; the test is branch-free, the state assignment gated.
 __device__ __forceinline__ bool orbit_check(Particle &p)
@@ -946,11 +1022,12 @@ __device__ __forceinline__ bool orbit_check(Particle &p)
 // :163-209, semi-linear near the zero orbit :136-159, drift when g = dg = 0 :117-118),
 // orbit_too_large (:319), Hwang exit kick.
 // SB: both edges have the simple Hwang kick and the body is not a drift (the flags are then
-// known at compile time, so the edge code has no run-time branches to merge).
+// known at compile time, so the edge code has no run-time branches to merge; F_BEND_SERIES
+// stays as given).
 template <bool F, bool SB>
 __device__ __forceinline__ void sbend(Particle &p, int32_t flags, CP b, const Species &sp)
 {
-    if constexpr (SB) flags = F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE;
+    if constexpr (SB) flags = F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE | (flags & F_BEND_SERIES);
     const double L = b[0], g = b[1], g_tot = b[2];
     const bool simple = flags & F_EDGE_SIMPLE;
     if (flags & F_EDGE_ENT) hwang_edge(p, b + 10, b[25], g_tot, true, simple);
@@ -958,16 +1035,25 @@ __device__ __forceinline__ void sbend(Particle &p, int32_t flags, CP b, const Sp
     if (flags & F_BEND_DRIFT) {
         drift<F>(p, L);
     } else {
+        // the semi-linear branch needs |pz| < 1e-9 among its conditions: gated on that alone
         bool semi = false;
-        if (lanes(fabs(p.x * g) < 1e-9) & lanes(fabs(p.px) < 1e-9) & lanes(fabs(p.py) < 1e-9) &
-            lanes(fabs(p.pz) < 1e-9)) {
+        if (any_lane(fabs(p.pz) < 1e-9)) {
             semi = (fabs(p.x * g) < 1e-9) & (fabs(p.px) < 1e-9) & (fabs(p.py) < 1e-9) &
                    (fabs(p.pz) < 1e-9);
             if (semi) sbend_semilinear(p, b, sp);
         }
         if (!semi) {
-            if (F) sbend_exact_fast(p, b);
-            else sbend_exact_ref(p, b);
+            if (F && (flags & F_BEND_SERIES)) {
+                // the momentum form on the common lanes, the general form on the others
+                const bool done = sbend_momentum_form(p, b);
+                if (any_lane(!done)) {
+                    if (!done) sbend_exact_fast(p, b);
+                }
+            } else if (F) {
+                sbend_exact_fast(p, b);
+            } else {
+                sbend_exact_ref(p, b);
+            }
         }
         if (p.state != ST_ALIVE) return;
     }

@@ -28,7 +28,7 @@ enum Flags : int32_t {
     F_EDGE_EXIT = 1 << 4,   // Hwang edge kick at the bend exit
     F_EDGE_SIMPLE = 1 << 5, // both edges have e = 0, fint*hgap = 0, k1 = 0 (only the g y^2 terms)
     F_BEND_DRIFT = 1 << 6,  // g = dg = 0: the bend body is a drift (track_a_bend.f90:117-118)
-    F_BEND_SERIES = 1 << 7, // |angle| small enough for the fast exact-body form
+    F_BEND_SERIES = 1 << 7, // g_tot = g and |angle| <= 0.1: the momentum form of the exact body
     F_ORDER_SHIFT = 8,      // K_MULT: bits 8..11 = mask of multipole orders 0..3 present
     // The entrance test repeats the exit test of the element before (same type and values,
     // same coordinates): a particle arriving from that element has passed it already. Set with
@@ -70,7 +70,7 @@ struct EleHdr {
 //            [7] cosc_a, [8] beta_ref, [9] sincc_a,
 //            [10..15] entrance gt, gt2, gs2, k1_tane, fg_factor, tan_e^2, [16..21] exit,
 //            [22] L^2 g cosc_a, [23] L / beta_ref, [24] L sinc_a, [25] gs2/2 entrance,
-//            [26] gs2/2 exit
+//            [26] gs2/2 exit, [27] 1 - cos_a, [28] 1 / g
 //  K_RF    : [0] L/n_slice, [1] voltage * charge_dir, [2] phi0, [3] rf_frequency,
 //            [4] 0.5/rf_frequency, [5] twopi * rf_frequency, [6] twopi * phi0_multipass
 //  K_MARKER: (none)
@@ -82,7 +82,7 @@ struct EleHdr {
 // when F_APE_MOMENTUM by 8 momentum-window values:
 //  elliptical : x_center, 1/x_width2, y_center, 1/y_width2
 //  rectangular: x1_lim (= -x1_limit), x2_lim, y1_lim, y2_lim (0 = no limit)
-constexpr int kParams[K_NKIND] = {1, 5, 9, 27, 7, 0};
+constexpr int kParams[K_NKIND] = {1, 5, 9, 29, 7, 0};
 
 struct Species {
     double mc2;      // [eV]
