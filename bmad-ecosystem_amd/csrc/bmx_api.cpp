@@ -60,6 +60,8 @@ int32_t bmx_ctx_create(int32_t device, bmx_ctx **out)
         c->d_work = c->d_counter + 1;
         e = hipMemset(c->d_counter, 0, 2 * sizeof(unsigned long long));
     }
+    // the device address of this device's fixed scratch array (bmx_track_turns_noalloc)
+    if (e == hipSuccess) e = noalloc_init(device);
     if (e != hipSuccess) {
         hipStreamDestroy(c->stream);
         delete c;
@@ -326,12 +328,14 @@ int32_t bmx_beam_count_alive(bmx_beam *B, int64_t *n_alive)
 
 }  // extern "C"
 
-namespace {
+namespace bmx {
 
 // Persistent tracking of one group: n_turns turns of `count` elements from element `first`
-// (0-based), turn indices turn0 ... (bmx_table.hpp Group).
+// (0-based), turn indices turn0 ... (bmx_table.hpp Group). fixed: run buffers of a fixed size
+// to use instead of the context's (which grow by allocation); the caller has checked that the
+// beam fits them.
 int32_t track_group(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t first, int32_t count,
-                    int32_t n_turns, int32_t turn0, const char *who)
+                    int32_t n_turns, int32_t turn0, const char *who, RunBuffers *fixed)
 {
     const std::string w(who);
     if (!ctx || !L || !B) return set_error(BMX_ERR_ARG, w + ": null argument");
@@ -343,8 +347,12 @@ int32_t track_group(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t fir
     hipSetDevice(ctx->device);
     // the previous call's group upload must have been consumed before the host copy changes
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    ctx->run.device = ctx->device;
-    HIP_TRY(ctx->run.reserve(1, (size_t)B->n, (size_t)(B->n + 63) / 64), "run buffers");
+    RunBuffers *rb = fixed;
+    if (!rb) {
+        rb = &ctx->run;
+        rb->device = ctx->device;
+        HIP_TRY(rb->reserve(1, (size_t)B->n, (size_t)(B->n + 63) / 64), "run buffers");
+    }
     Group &g = ctx->group;
     g.lat = L->desc;
     g.b = B->d;
@@ -364,17 +372,17 @@ int32_t track_group(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t fir
     cfg.grid = ctx->grid > 0 ? ctx->grid
                              : default_grid(ctx->device, ctx->fast_maps, cfg.waves, cfg.jit, cfg.span);
     cfg.timer = ctx->profile ? &ctx->timer : nullptr;
-    HIP_TRY(enqueue_run(ctx->stream, ctx->run, &g, 1, true, n_turns, cfg, ctx->d_work), "k_track_tiles");
+    HIP_TRY(enqueue_run(ctx->stream, *rb, &g, 1, true, n_turns, cfg, ctx->d_work), "k_track_tiles");
     return BMX_OK;
 }
 
-}  // namespace
+}  // namespace bmx
 
 extern "C" {
 
 int32_t bmx_track_turns(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t n_turns, int32_t turn0)
 {
-    return track_group(ctx, L, B, 0, L ? L->desc.n_ele : 0, n_turns, turn0, "bmx_track_turns");
+    return track_group(ctx, L, B, 0, L ? L->desc.n_ele : 0, n_turns, turn0, "bmx_track_turns", nullptr);
 }
 
 int32_t bmx_track_turns_from(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t ix_ele_start,
@@ -383,7 +391,8 @@ int32_t bmx_track_turns_from(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, in
     if (L && (ix_ele_start < 0 || ix_ele_start > L->desc.n_ele))
         return set_error(BMX_ERR_ARG, "bmx_track_turns_from: ix_ele_start outside 0..n_ele");
     const int32_t n = L ? L->desc.n_ele : 0;
-    return track_group(ctx, L, B, n ? ix_ele_start % n : 0, n, n_turns, turn0, "bmx_track_turns_from");
+    return track_group(ctx, L, B, n ? ix_ele_start % n : 0, n, n_turns, turn0, "bmx_track_turns_from",
+                       nullptr);
 }
 
 int32_t bmx_track_bunch(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t ix_ele1,
@@ -396,7 +405,7 @@ int32_t bmx_track_bunch(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t
     // beam_mod.f90:148-166: ix_ele1+1 .. ix_ele2, or through the end and round to ix_ele2 when
     // ix_ele2 is at or before ix_ele1
     const int32_t count = ix_ele1 < ix_ele2 ? ix_ele2 - ix_ele1 : n - ix_ele1 + ix_ele2;
-    return track_group(ctx, L, B, n ? ix_ele1 % n : 0, count, 1, turn, "bmx_track_bunch");
+    return track_group(ctx, L, B, n ? ix_ele1 % n : 0, count, 1, turn, "bmx_track_bunch", nullptr);
 }
 
 int32_t bmx_ctx_profile(bmx_ctx *ctx, int32_t on)

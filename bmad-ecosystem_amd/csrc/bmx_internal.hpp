@@ -163,5 +163,12 @@ namespace bmx {
 // Uploads a compiled element table as a new lattice on ctx's device (bmx_lattice_create's
 // device half).
 int32_t lattice_upload(bmx_ctx *ctx, HostTable &&table, bmx_lattice **out);
+// Persistent tracking of one group (bmx_api.cpp); fixed: run buffers to use instead of the
+// context's growing ones (nullptr: the context's).
+int32_t track_group(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t first, int32_t count,
+                    int32_t n_turns, int32_t turn0, const char *who, RunBuffers *fixed);
+// Records the device address of the fixed scratch array of the allocation-free entry point on
+// `device`, which must be current (bmx_noalloc.hip); once per device.
+hipError_t noalloc_init(int device);
 }  // namespace bmx
 

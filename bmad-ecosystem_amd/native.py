@@ -90,6 +90,7 @@ _SIGS = {
     "bmx_beam_count_alive": (_I32, [_P, ctypes.POINTER(_I64)]),
     "bmx_track_turns": (_I32, [_P, _P, _P, _I32, _I32]),
     "bmx_track_turns_from": (_I32, [_P, _P, _P, _I32, _I32, _I32]),
+    "bmx_track_turns_noalloc": (_I32, [_P, _P, _P, _I32, _I32]),
     "bmx_track_bunch": (_I32, [_P, _P, _P, _I32, _I32, _I32]),
     "bmx_track_element": (_I32, [_P, _P, _P, _I32, _I32]),
     "bmx_track_turns_plain": (_I32, [_P, _P, _P, _I32, _I32]),

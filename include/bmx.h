@@ -265,6 +265,15 @@ int32_t bmx_track_turns_from(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *bea
  * asynchronous. */
 int32_t bmx_track_bunch(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam, int32_t ix_ele1,
                         int32_t ix_ele2, int32_t turn);
+/* bmx_track_turns without any device allocation. No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Its run buffers (alive lists, wave tiles, counters) are a
+ * fixed device array of about 8.5 MiB per GPU, held until the process ends; beams of more than
+ * 2^20 particles are refused with BMX_ERR_ARG. All calls on one GPU share that array, so calls
+ * that overlap on one GPU (other contexts, other host threads) race: the caller serialises
+ * them. Same results as bmx_track_turns. Asynchronous. */
+int32_t bmx_track_turns_noalloc(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam,
+                                int32_t n_turns, int32_t turn0);
 /* One element for the whole bunch: track1_bunch_hom (beam_utils.f90:32-90) on the GPU, state
  * read from and written back to HBM. ix_ele is 1-based. Asynchronous. */
 int32_t bmx_track_element(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam, int32_t ix_ele,
