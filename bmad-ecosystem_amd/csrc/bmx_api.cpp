@@ -29,6 +29,21 @@ static int hip_fail(hipError_t e, const char *what)
         if (_e != hipSuccess) return bmx::hip_fail(_e, what); \
     } while (0)
 
+// Equal kernel arguments: the same device arrays and values (a captured launch holds them).
+static bool same_launch(const LatDesc &a, const LatDesc &b)
+{
+    return a.hdr == b.hdr && a.prm == b.prm && a.n_ele == b.n_ele && a.sp.mc2 == b.sp.mc2 &&
+           a.sp.p0c == b.sp.p0c && a.sp.mc2_sq == b.sp.mc2_sq && a.sp.beta0 == b.sp.beta0 &&
+           a.sp.me2 == b.sp.me2 && a.sp.lez_f == b.sp.lez_f && a.sp.lez_thr == b.sp.lez_thr;
+}
+
+static bool same_launch(const BeamPtrs &a, const BeamPtrs &b)
+{
+    return a.x == b.x && a.px == b.px && a.y == b.y && a.py == b.py && a.z == b.z && a.pz == b.pz &&
+           a.beta == b.beta && a.state == b.state && a.loss_ele == b.loss_ele &&
+           a.loss_turn == b.loss_turn;
+}
+
 }  // namespace bmx
 
 using namespace bmx;
@@ -55,10 +70,13 @@ int32_t bmx_ctx_create(int32_t device, bmx_ctx **out)
         delete c;
         return hip_fail(e, "hipStreamCreateWithFlags");
     }
-    e = hipMalloc(&c->d_counter, 2 * sizeof(unsigned long long));
+    // [0] reduction scratch, [1 .. kWorkSlots] pass counter, then the plain path's turn index
+    constexpr size_t kWords = 2 + kWorkSlots;
+    e = hipMalloc(&c->d_counter, kWords * sizeof(unsigned long long));
     if (e == hipSuccess) {
         c->d_work = c->d_counter + 1;
-        e = hipMemset(c->d_counter, 0, 2 * sizeof(unsigned long long));
+        c->d_turn = reinterpret_cast<int32_t *>(c->d_counter + 1 + kWorkSlots);
+        e = hipMemset(c->d_counter, 0, kWords * sizeof(unsigned long long));
     }
     // the device address of this device's fixed scratch array (bmx_track_turns_noalloc)
     if (e == hipSuccess) e = noalloc_init(device);
@@ -78,6 +96,7 @@ int32_t bmx_ctx_destroy(bmx_ctx *ctx)
     hipStreamSynchronize(ctx->stream);
     ctx->run.release();
     ctx->timer.release();
+    if (ctx->plain.exec) hipGraphExecDestroy(ctx->plain.exec);
     for (auto &ev : ctx->events)
         if (ev) hipEventDestroy(ev);
     hipFree(ctx->d_counter);
@@ -127,11 +146,13 @@ int32_t bmx_ctx_take_work(bmx_ctx *ctx, int64_t *passes)
 {
     if (!ctx || !passes) return set_error(BMX_ERR_ARG, "bmx_ctx_take_work: null argument");
     hipSetDevice(ctx->device);
-    unsigned long long v = 0;
-    HIP_TRY(hipMemcpyAsync(&v, ctx->d_work, sizeof v, hipMemcpyDeviceToHost, ctx->stream), "copy");
+    unsigned long long v[kWorkSlots] = {};
+    HIP_TRY(hipMemcpyAsync(v, ctx->d_work, sizeof v, hipMemcpyDeviceToHost, ctx->stream), "copy");
     HIP_TRY(hipMemsetAsync(ctx->d_work, 0, sizeof v, ctx->stream), "memset");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
-    *passes = (int64_t)v;
+    unsigned long long sum = 0;
+    for (unsigned long long x : v) sum += x;
+    *passes = (int64_t)sum;
     return BMX_OK;
 }
 
@@ -459,10 +480,41 @@ int32_t bmx_track_turns_plain(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, i
     if (!ctx || !L || !B) return set_error(BMX_ERR_ARG, "bmx_track_turns_plain: null argument");
     if (n_turns < 0) return set_error(BMX_ERR_ARG, "bmx_track_turns_plain: negative turn count");
     hipSetDevice(ctx->device);
-    for (int32_t t = 0; t < n_turns; t++)
-        for (int32_t e = 0; e < L->desc.n_ele; e++)
-            HIP_TRY(launch_track_element(ctx->stream, L->desc, e, turn0 + t, B->d, B->n, ctx->d_work,
-                                         ctx->fast_maps), "k_track_element");
+    if (n_turns == 0 || B->n == 0) return BMX_OK;
+    // One turn of element launches is captured in a graph (kept in the context for the same
+    // lattice, beam and maps) and replayed once per turn; the launches read the turn index from
+    // device memory, which the graph's last node advances.
+    bmx_ctx::PlainGraph &pg = ctx->plain;
+    if (!pg.exec || !same_launch(pg.lat, L->desc) || !same_launch(pg.beam, B->d) || pg.n != B->n ||
+        pg.fast != ctx->fast_maps) {
+        if (pg.exec) {
+            // replays of the previous graph may still be queued
+            HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+            hipGraphExecDestroy(pg.exec);
+            pg.exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipError_t e = hipSuccess;
+        for (int32_t ie = 0; ie < L->desc.n_ele && e == hipSuccess; ie++)
+            e = launch_track_element_at(ctx->stream, L->desc, ie, ctx->d_turn, B->d, B->n, ctx->d_work,
+                                        ctx->fast_maps);
+        if (e == hipSuccess) e = launch_step_turn(ctx->stream, ctx->d_turn, false, 0);
+        const hipError_t e_end = hipStreamEndCapture(ctx->stream, &graph);
+        if (e == hipSuccess) e = e_end;
+        if (e == hipSuccess) e = hipGraphInstantiate(&pg.exec, graph, nullptr, nullptr, 0);
+        if (graph) hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            pg.exec = nullptr;
+            return hip_fail(e, "bmx_track_turns_plain: turn graph");
+        }
+        pg.lat = L->desc;
+        pg.beam = B->d;
+        pg.n = B->n;
+        pg.fast = ctx->fast_maps;
+    }
+    HIP_TRY(launch_step_turn(ctx->stream, ctx->d_turn, true, turn0), "k_step_turn");
+    for (int32_t t = 0; t < n_turns; t++) HIP_TRY(hipGraphLaunch(pg.exec, ctx->stream), "hipGraphLaunch");
     return BMX_OK;
 }
 

@@ -30,6 +30,10 @@ constexpr int kSpanWaves = 3;
 hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, int32_t turn,
                                 const BeamPtrs &b, int64_t n, unsigned long long *d_work,
                                 bool fast);
+hipError_t launch_track_element_at(hipStream_t s, const LatDesc &lat, int32_t ie, const int32_t *d_turn,
+                                   const BeamPtrs &b, int64_t n, unsigned long long *d_work,
+                                   bool fast);
+hipError_t launch_step_turn(hipStream_t s, int32_t *d_turn, bool set, int32_t v);
 hipError_t launch_track_range(hipStream_t s, const LatDesc &lat, int32_t e0, int32_t e1,
                               int32_t turn, const BeamPtrs &b, int64_t n, double *d_rec,
                               unsigned long long *d_work, bool fast);
@@ -137,6 +141,16 @@ struct bmx_ctx {
     bmx::Group group{};     // host copy of the single group uploaded by bmx_track_turns
     bool profile = false;   // time every tracking launch of bmx_track_turns
     bmx::KernelTimer timer;
+    int32_t *d_turn = nullptr;  // turn index of the plain path's captured turn (in d_counter)
+    // the plain path's turn of element launches, captured once per (lattice, beam, maps) and
+    // replayed for every turn
+    struct PlainGraph {
+        bmx::LatDesc lat{};    // the kernel arguments the graph was captured with
+        bmx::BeamPtrs beam{};
+        int64_t n = -1;
+        bool fast = true;
+        hipGraphExec_t exec = nullptr;
+    } plain;
 };
 
 struct bmx_lattice {

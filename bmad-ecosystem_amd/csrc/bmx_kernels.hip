@@ -122,9 +122,9 @@ __global__ void __launch_bounds__(kBlock) k_collect_alive(
 
 // One element, whole bunch: track1 (track1.f90:44-343) per live particle.
 template <bool F>
-__global__ void __launch_bounds__(kBlock) k_track_element(LatDesc L, int32_t ie, int32_t turn,
-                                                          BeamPtrs b, int64_t n,
-                                                          unsigned long long *work)
+__device__ __forceinline__ void track_element_pass(const LatDesc &L, int32_t ie, int32_t turn,
+                                                   const BeamPtrs &b, int64_t n,
+                                                   unsigned long long *work)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool active = i < n && b.state[i] == ST_ALIVE;
@@ -141,7 +141,44 @@ __global__ void __launch_bounds__(kBlock) k_track_element(LatDesc L, int32_t ie,
             b.loss_turn[i] = turn;
         }
     }
-    add_work(work, active ? 1ull : 0ull);
+    // one atomic per workgroup: the waves' counts summed in LDS
+    if (!work) return;
+    unsigned long long s = active ? 1ull : 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    __shared__ unsigned long long part[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kBlock / 64; w++) t += part[w];
+        if (t) atomicAdd(work + blockIdx.x % kWorkSlots, t);
+    }
+}
+
+template <bool F>
+__global__ void __launch_bounds__(kBlock) k_track_element(LatDesc L, int32_t ie, int32_t turn,
+                                                          BeamPtrs b, int64_t n,
+                                                          unsigned long long *work)
+{
+    track_element_pass<F>(L, ie, turn, b, n, work);
+}
+
+// The same with the turn index read from device memory: the launches of one turn captured once
+// in a graph serve every turn, the turn advanced by k_step_turn at the end of each replay.
+template <bool F>
+__global__ void __launch_bounds__(kBlock) k_track_element_at(LatDesc L, int32_t ie,
+                                                             const int32_t *__restrict__ turn_p,
+                                                             BeamPtrs b, int64_t n,
+                                                             unsigned long long *work)
+{
+    track_element_pass<F>(L, ie, *turn_p, b, n, work);
+}
+
+// turn_p = v (set_v != 0) or turn_p += 1: one thread.
+__global__ void k_step_turn(int32_t *turn_p, int32_t set_v, int32_t v)
+{
+    if (threadIdx.x == 0) *turn_p = set_v ? v : *turn_p + 1;
 }
 
 // Elements e0..e1 (0-based, inclusive) once for every live particle, recording the state at
@@ -298,6 +335,27 @@ hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, i
     else
         hipLaunchKernelGGL(k_track_element<false>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat, ie,
                            turn, b, n, d_work);
+    return hipGetLastError();
+}
+
+hipError_t launch_track_element_at(hipStream_t s, const LatDesc &lat, int32_t ie, const int32_t *d_turn,
+                                   const BeamPtrs &b, int64_t n, unsigned long long *d_work,
+                                   bool fast)
+{
+    int64_t grid = (n + kBlock - 1) / kBlock;
+    if (grid <= 0) return hipSuccess;
+    if (fast)
+        hipLaunchKernelGGL(k_track_element_at<true>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat,
+                           ie, d_turn, b, n, d_work);
+    else
+        hipLaunchKernelGGL(k_track_element_at<false>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat,
+                           ie, d_turn, b, n, d_work);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_turn(hipStream_t s, int32_t *d_turn, bool set, int32_t v)
+{
+    hipLaunchKernelGGL(k_step_turn, dim3(1), dim3(64), 0, s, d_turn, set ? 1 : 0, v);
     return hipGetLastError();
 }
 

@@ -236,10 +236,12 @@ void bmx_server::run()
         }
         for (Request *r : finished)
             if (e == hipSuccess) e = copy(r, false);
-        unsigned long long passes = 0;
-        if (e == hipSuccess) e = hipMemcpyAsync(&passes, d_work, sizeof passes, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipMemsetAsync(d_work, 0, sizeof passes, stream);
+        unsigned long long slots[kWorkSlots] = {};
+        if (e == hipSuccess) e = hipMemcpyAsync(slots, d_work, sizeof slots, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipMemsetAsync(d_work, 0, sizeof slots, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        unsigned long long passes = 0;
+        for (unsigned long long x : slots) passes += x;
         {
             std::lock_guard<std::mutex> lk(mu);
             total_chunks++;
@@ -275,8 +277,8 @@ int32_t bmx_server_create(bmx_ctx *ctx, const bmx_server_config *cfg, bmx_server
     hipSetDevice(S->device);
     const size_t nd = (size_t)S->capacity * 8, ni = (size_t)S->capacity * 4;
     hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&S->d_work, sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(S->d_work, 0, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&S->d_work, kWorkSlots * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(S->d_work, 0, kWorkSlots * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&S->pool, 7 * nd + 3 * ni);
     if (e == hipSuccess) {
         S->rb.device = S->device;

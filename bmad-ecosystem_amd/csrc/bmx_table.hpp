@@ -129,6 +129,10 @@ struct Group {
     int32_t count;
 };
 
+// Slots of a particle-element pass counter: kernels add to slot blockIdx.x % kWorkSlots and the
+// host reads their sum.
+constexpr int kWorkSlots = 16;
+
 // One wave's unit of work: up to 64 particles of one group for one chunk of turns. start
 // indexes the group's alive-list segment (or the particles themselves in the first chunk).
 struct WaveTile {

@@ -33,16 +33,17 @@ __device__ __forceinline__ void store_particle(const Particle &p, const BeamPtrs
     b.beta[i] = p.beta;
 }
 
-// Adds the number of particle-element passes tracked by this wave to *work (if non-null):
-// element passes of live particles only, as the reference's loop skips dead ones
-// (beam_utils.f90:83).
+// Adds the number of particle-element passes tracked by this wave to the pass counter (if
+// non-null): element passes of live particles only, as the reference's loop skips dead ones
+// (beam_utils.f90:83). The counter is kWorkSlots slots that the reader sums (bmx_table.hpp), so
+// the atomics of many waves do not all meet at one address.
 __device__ __forceinline__ void add_work(unsigned long long *work, unsigned long long mine)
 {
     if (!work) return;
     unsigned long long s = mine;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(work, s);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(work + blockIdx.x % kWorkSlots, s);
 }
 
 // The generic turn policy: elements [e0, e1) of the lattice read from its table. The rf

@@ -279,7 +279,9 @@ int32_t bmx_track_turns_noalloc(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *
 int32_t bmx_track_element(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam, int32_t ix_ele,
                           int32_t turn);
 /* Plain path: n_turns x n_ele launches of the per-element kernel (the reference's loop nest:
- * turn -> element -> particles, beam_mod.f90:159-166 + beam_utils.f90:81-86). Asynchronous. */
+ * turn -> element -> particles, beam_mod.f90:159-166 + beam_utils.f90:81-86), state through HBM
+ * at every element. One turn of launches is captured as a HIP graph (kept in the context for
+ * the same lattice, beam and maps) and replayed per turn. Asynchronous. */
 int32_t bmx_track_turns_plain(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam,
                               int32_t n_turns, int32_t turn0);
 

@@ -112,6 +112,34 @@ def test_ring0_turns(bmx, oracle, ctx):
         assert np.array_equal(out2[k], out[k]), k
 
 
+def test_plain_path_turn_graph(bmx, oracle, ctx):
+    """The plain path (one launch per element, one turn captured as a graph and replayed) on
+    the 91-element ring with apertures and rf: 12 turns from turn 5 in two calls (7, then 5
+    more with the cached graph) give the restatement's losses -- state, element and turn index
+    -- bit for bit, and survivors within 100x of their envelope (the restatement's own response
+    to FMA contraction and to 1e-14 rescalings of its input) or 1e-9; a second beam of another
+    size recaptures the graph."""
+    L = bmx.lattices
+    arr = bmx.elements.to_array(L.ring1(n_cells=10))
+    lat = bmx.tracking.Lattice(ctx, arr)
+    for n, seed in ((3000, 71), (1100, 72)):
+        beam = P.random_beam(bmx, n, seed=seed, scale=25.0, pz_sigma=4e-3)
+        ref, env = P.envelope(oracle, arr, beam, 12, turn0=5)
+        b = bmx.tracking.Beam.from_host(ctx, beam)
+        print(f"plain path, {n} particles", flush=True)
+        bmx.tracking.track_beam_plain(ctx, lat, b, 7, 5)
+        bmx.tracking.track_beam_plain(ctx, lat, b, 5, 12)
+        ctx.synchronize()
+        out = b.download()
+        b.close()
+        lost = ref["state"] != 1
+        assert lost.sum() > 100 and len(np.unique(ref["loss_turn"][lost])) > 5
+        bad = P.loss_mismatch(out, ref)
+        assert bad.size == 0, (bad[:10], out["loss_turn"][bad[:10]], ref["loss_turn"][bad[:10]])
+        far, dev = P.outside_envelope(out, ref, env, floor=1e-9)
+        assert far.size == 0, (far[:10], dev[far[:10]], env[far[:10]])
+
+
 def test_ring1_losses_bit_exact(bmx, oracle, ctx):
     """901-element ring with apertures and RF, a beam wide enough to lose about half of its
     particles in 30 turns: state code, loss element and loss turn identical for every particle.
