@@ -47,10 +47,28 @@ def parse():
                     help="formulation of the element maps")
     ap.add_argument("--chunk-turns", type=int, default=5,
                     help="turns between alive-list compactions (0 = none)")
-    ap.add_argument("--extras", default="batched,da,served",
+    ap.add_argument("--extras", default="batched,da,served,plain",
                     help="comma list of the other workloads to run after the main one "
-                         "(batched, da, served; empty = none)")
-    ap.add_argument("--served-requests", type=int, default=24)
+                         "(batched, da, served, plain; empty = none)")
+    ap.add_argument("--main", choices=["on", "off"], default="on",
+                    help="off: skip the single-lattice workload and the CPU baseline (runs of "
+                         "the other workloads alone)")
+    ap.add_argument("--served-requests", type=int, default=200,
+                    help="requests per served stream")
+    ap.add_argument("--served-streams", default="full,reduced",
+                    help="comma list of served streams: full (particles log-uniform [1e3, 1e6], "
+                         "turns [10, 1e4]) and reduced (particles [1e3, 1e5], turns [10, 300]: a "
+                         "stream the baselines can track in full within minutes)")
+    ap.add_argument("--served-full-baselines", default="none",
+                    help="baselines tracked over the full stream's arrivals (comma list of "
+                         "one_job, plain; none)")
+    ap.add_argument("--served-reduced-baselines", default="one_job,plain",
+                    help="baselines tracked over the reduced stream's arrivals: one_job "
+                         "(persistent kernel, one request at a time), plain (one launch per "
+                         "element per turn), each request tracked in full")
+    ap.add_argument("--plain-graph", choices=["on", "off"], default="on",
+                    help="plain leg: on = graph-replayed turns (bmx_track_turns_plain); off = "
+                         "one host launch per element (bmx_track_element), for kernel traces")
     ap.add_argument("--waves", type=int, default=0,
                     help="waves per SIMD of the tracking kernel variant (3, 4 or 5; 0 = the "
                          "library's default for the kernel in use)")
@@ -154,6 +172,9 @@ def main():
               "ranges, one process per GPU).", file=sys.stderr)
         sys.exit(2)
     bmx = load_package()
+    if args.main == "off":
+        extras_only(bmx, args)
+        return
     L, E, T = bmx.lattices, bmx.elements, bmx.tracking
     eles = L.ring1()
     arr = E.to_array(eles)
@@ -245,15 +266,84 @@ def main():
     }
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(bmx, arr, beam, args.cpu_seconds)
-    extras = [e for e in args.extras.split(",") if e and e != "none"]
-    out["extras"] = {}
-    if "batched" in extras:
-        out["extras"]["batched"] = run_batched(bmx, ctx, flops, specialize=bool(special))
-    if "da" in extras:
-        out["extras"]["dynamic_aperture"] = run_da(bmx, ctx, specialize=bool(special))
-    if "served" in extras:
-        out["extras"]["served"] = run_served(bmx, ctx, args.served_requests, specialize=bool(special))
+    out["extras"] = run_extras(bmx, ctx, args, flops, bool(special))
     print(json.dumps(out))
+
+
+def run_extras(bmx, ctx, args, flops, special):
+    extras = [e for e in args.extras.split(",") if e and e != "none"]
+    out = {}
+    if "batched" in extras:
+        out["batched"] = run_batched(bmx, ctx, flops, specialize=special)
+    if "da" in extras:
+        out["dynamic_aperture"] = run_da(bmx, ctx, specialize=special)
+    if "served" in extras:
+        streams = [x for x in args.served_streams.split(",") if x]
+        for stream, bl, key in (("full", args.served_full_baselines, "served"),
+                                ("reduced", args.served_reduced_baselines, "served_reduced")):
+            if stream in streams:
+                baselines = [b for b in bl.split(",") if b and b != "none"]
+                out[key] = run_served(bmx, ctx, args.served_requests, specialize=special,
+                                      stream=stream, baselines=baselines)
+    if "plain" in extras:
+        out["plain"] = run_plain(bmx, ctx, graph=args.plain_graph == "on")
+    return out
+
+
+def extras_only(bmx, args):
+    """The other workloads alone (--main off): one JSON line with their results."""
+    T = bmx.tracking
+    ctx = T.Context(0)
+    ctx.set_maps(args.maps)
+    flops = bmx.flops.per_element_pass(bmx.elements.to_array(bmx.lattices.ring1()))
+    special = args.specialize == "on" and args.maps == "fast"
+    out = {"metric": METRIC, "unit": UNIT, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "extras only: " + args.extras, "maps": args.maps},
+           "extras": run_extras(bmx, ctx, args, flops, special)}
+    print(json.dumps(out))
+
+
+# HBM bytes of one element pass of a live particle on the plain path: 7 FP64 coordinates and
+# the state read, 7 coordinates written (the loss record only on the rare lost lanes)
+PLAIN_BYTES_PER_PASS = 7 * 8 + 4 + 7 * 8
+HBM_PEAK_GBS = 8000.0
+
+
+def run_plain(bmx, ctx, graph=True):
+    """The plain path (one launch per element per turn, state through HBM; the reference's
+    loop nest) on the 901-element ring: 1e4 particles x 20 turns and 1e6 particles x 5 turns,
+    3x-sigma beams. graph: the turn's launches captured once and replayed per turn
+    (bmx_track_turns_plain); else one host launch per element (bmx_track_element)."""
+    L, E, T = bmx.lattices, bmx.elements, bmx.tracking
+    lat = T.Lattice(ctx, E.to_array(L.ring1()))
+    out = {"path": "bmx_track_turns_plain (graph-replayed turns)" if graph else
+           "bmx_track_element per element (host launches)", "legs": []}
+    for n, turns in ((10_000, 20), (1_000_000, 5)):
+        beam = L.gaussian_beam(n, seed=1, sigma=3 * L.SIGMA0)
+        b = T.Beam.from_host(ctx, beam)
+        if graph:
+            T.track_beam_plain(ctx, lat, b, 1, 0)  # capture and warm-up turn
+        ctx.synchronize()
+        ctx.take_work()
+        ctx.record(20)
+        if graph:
+            T.track_beam_plain(ctx, lat, b, turns, 1)
+        else:
+            for t in range(turns):
+                for e in range(1, lat.n_ele + 1):
+                    T.track1_bunch(ctx, lat, b, e, 1 + t)
+        ctx.record(21)
+        ctx.synchronize()
+        ms = ctx.elapsed_ms(20, 21)
+        passes = ctx.take_work()
+        b.close()
+        gbs = passes * PLAIN_BYTES_PER_PASS / (ms * 1e-3) / 1e9
+        out["legs"].append({"particles": n, "turns": turns, "value": passes / (ms * 1e-3),
+                            "unit": UNIT, "ms_per_turn": ms / turns, "hbm_gbs": gbs,
+                            "hbm_frac": gbs / HBM_PEAK_GBS,
+                            "bytes_per_pass": PLAIN_BYTES_PER_PASS})
+    lat.close()
+    return out
 
 
 def run_batched(bmx, ctx, flops, n_var=512, turns=500, seed=2, specialize=True):
@@ -323,46 +413,55 @@ def run_da(bmx, ctx, n_var=4, turns=10_000, seed=3, specialize=True):
             "median_survival_turn": float(np.median(surv))}
 
 
-def run_served(bmx, ctx, n_requests, seed=4, load=0.8, budget=16 << 30, specialize=True):
-    """Served stream against one-job-at-a-time tracking: Poisson arrivals of requests on a pool
-    of 32 error variants, particles log-uniform [1e3, 1e6], turns log-uniform [10, 1e4],
-    100-turn chunks, HBM budget 16 GiB. The arrival rate is set to `load` times the served
-    capacity measured on the same requests submitted at once."""
+def run_served(bmx, ctx, n_requests, seed=4, load=0.8, budget=16 << 30, specialize=True,
+               stream="full", baselines=("one_job",)):
+    """Served stream: Poisson arrivals of requests on a pool of 32 error variants, 100-turn
+    chunks, HBM budget 16 GiB; full stream: particles log-uniform [1e3, 1e6], turns
+    log-uniform [10, 1e4] (reduced: [1e3, 1e5] and [10, 300]). The arrival rate is `load`
+    times the served capacity measured on the same requests submitted at once. The baselines
+    track the same requests over the same arrival times, one at a time, each in full."""
     L, T = bmx.lattices, bmx.tracking
     S = bmx.serving
     lats = [T.Lattice(ctx, bmx.elements.to_array(v)) for v in L.ring1_variants(seed, 32)]
     if specialize:
         for lt in lats:
             lt.specialize()
-    reqs, gaps = S.make_stream(seed, n_requests)
+    n_hi, t_hi = (1_000_000, 10_000) if stream == "full" else (100_000, 300)
+    reqs, gaps = S.make_stream(seed, n_requests, n_hi=n_hi, t_hi=t_hi)
     # capacity: everything submitted at t = 0
     cap = S.run_served(ctx, lats, reqs, np.zeros(len(reqs)), budget)
     mean_gap = cap["makespan_s"] / len(reqs) / load
     arrivals = np.cumsum(gaps) * mean_gap
     arrivals -= arrivals[0]
     served = S.run_served(ctx, lats, reqs, arrivals, budget)
-    one = S.run_plain(ctx, lats, reqs, arrivals)
-    plain = S.run_plain_elementwise(ctx, lats, reqs, arrivals)
+    sv = served["passes"] / served["makespan_s"]
+    out = {"workload": f"{n_requests} requests, Poisson arrivals at {load} of served capacity "
+                       f"(mean gap {mean_gap:.3f} s), 32 variants, N log-uniform[1e3,{n_hi:.0e}], "
+                       f"T log-uniform[10,{t_hi:.0e}], 100-turn chunks, budget {budget} B",
+           "stream": stream,
+           "served": {"value": sv, **S.percentiles(served["latency_s"]),
+                      "makespan_s": served["makespan_s"], "chunks": served["chunks"]},
+           "served_capacity": cap["passes"] / cap["makespan_s"],
+           "passes": served["passes"], "unit": UNIT}
+    if "one_job" in baselines:
+        one = S.run_plain(ctx, lats, reqs, arrivals)
+        on = one["passes"] / one["makespan_s"]
+        out["one_job_at_a_time"] = {"path": "bmx_track_turns (persistent, compacted) per request, "
+                                            "no batching across requests",
+                                    "value": on, **S.percentiles(one["latency_s"]),
+                                    "makespan_s": one["makespan_s"], "passes": one["passes"]}
+        out["speedup_vs_one_job"] = sv / on
+    if "plain" in baselines:
+        plain = S.run_plain_elementwise(ctx, lats, reqs, arrivals)
+        pl = plain["passes"] / plain["makespan_s"]
+        out["plain"] = {"path": "bmx_track_turns_plain: one launch per element per turn, state in "
+                                "HBM, one request at a time, each tracked in full",
+                        "value": pl, **S.percentiles(plain["latency_s"]),
+                        "makespan_s": plain["makespan_s"], "passes": plain["passes"]}
+        out["speedup_vs_plain"] = sv / pl
     for lt in lats:
         lt.close()
-    sv = served["passes"] / served["makespan_s"]
-    on = one["passes"] / one["makespan_s"]
-    pl = served["passes"] / plain["makespan_s"]
-    return {"workload": f"{n_requests} requests, Poisson arrivals at {load} of served capacity "
-                        f"(mean gap {mean_gap:.3f} s), 32 variants, N log-uniform[1e3,1e6], "
-                        f"T log-uniform[10,1e4], 100-turn chunks, budget {budget} B",
-            "served": {"value": sv, **S.percentiles(served["latency_s"]),
-                       "makespan_s": served["makespan_s"], "chunks": served["chunks"]},
-            "plain": {"path": "bmx_track_turns_plain: one launch per element per turn, state in "
-                              "HBM, one request at a time",
-                      "value": pl, **S.percentiles(plain["latency_s"]),
-                      "makespan_s": plain["makespan_s"], "method": plain["method"]},
-            "one_job_at_a_time": {"path": "bmx_track_turns (persistent, compacted) per request, "
-                                          "no batching across requests",
-                                  "value": on, **S.percentiles(one["latency_s"]),
-                                  "makespan_s": one["makespan_s"]},
-            "speedup_vs_plain": sv / pl, "speedup_vs_one_job": sv / on,
-            "served_capacity": cap["passes"] / cap["makespan_s"], "unit": UNIT}
+    return out
 
 
 if __name__ == "__main__":

@@ -5,6 +5,11 @@
 // resident in a device pool and advances all active requests together, chunk by chunk, with
 // the same tile run (enqueue_run) that batches lattice variants. Admission is FIFO by pool
 // space; the pool is a first-fit free list over particle slots of one SoA block.
+//
+// The worker thread keeps one chunk in flight on the compute stream. While it runs, requests
+// that arrive are admitted and uploaded on a second stream (same priority), and the results of
+// the requests the previous chunk finished are downloaded there; the next chunk starts as soon
+// as the running one ends, waiting only for the uploads of the requests it adds.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -44,8 +49,12 @@ struct Request {
 struct bmx_server {
     bmx_ctx *ctx = nullptr;
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // chunks (tracking kernels)
+    hipStream_t copy_stream = nullptr;  // request uploads and downloads, overlapping the chunks
+    hipEvent_t ev_chunk = nullptr;      // end of the chunk in flight
+    hipEvent_t ev_up = nullptr;         // end of the uploads the next chunk waits for
     unsigned long long *d_work = nullptr;
+    unsigned long long *h_slots = nullptr;  // pinned: the last chunk's pass counter
     int64_t capacity = 0;   // particle slots
     char *pool = nullptr;
     BeamPtrs base{};
@@ -63,6 +72,7 @@ struct bmx_server {
     std::unordered_map<int64_t, Request *> all;
     int64_t next_id = 1;
     bool stop = false;
+    bool arrivals = false;  // a request was submitted since the worker last tried admission
     int64_t total_chunks = 0, busy_ns = 0;
     unsigned long long total_passes = 0;
     std::thread worker;
@@ -108,7 +118,7 @@ struct bmx_server {
                         base.beta + off, base.state + off, base.loss_ele + off, base.loss_turn + off};
     }
 
-    hipError_t copy(Request *r, bool up)
+    hipError_t copy(Request *r, bool up, hipStream_t s)
     {
         const BeamPtrs d = at(r->off);
         const bmx_beam_view &h = up ? r->in : r->out;
@@ -117,8 +127,8 @@ struct bmx_server {
         hipError_t e = hipSuccess;
         for (int k = 0; k < 10 && e == hipSuccess; k++) {
             const size_t bytes = (size_t)r->n * (k < 7 ? 8 : 4);
-            e = up ? hipMemcpyAsync(dp[k], hp[k], bytes, hipMemcpyHostToDevice, stream)
-                   : hipMemcpyAsync(hp[k], dp[k], bytes, hipMemcpyDeviceToHost, stream);
+            e = up ? hipMemcpyAsync(dp[k], hp[k], bytes, hipMemcpyHostToDevice, s)
+                   : hipMemcpyAsync(hp[k], dp[k], bytes, hipMemcpyDeviceToHost, s);
         }
         return e;
     }
@@ -139,14 +149,36 @@ void bmx_server::run()
 {
     hipSetDevice(device);
     std::vector<Group> groups;
-    std::vector<Request *> admitted;
+    std::vector<Request *> running;   // requests of the chunk in flight on the compute stream
+    std::vector<Request *> ready;     // admitted, uploads queued on the copy stream: next chunk
+    std::vector<Request *> copied;    // results queued for download on the copy stream
+    bool in_flight = false;           // a chunk is running (ev_chunk not yet reached)
+    bool uploads_queued = false;      // ev_up recorded after uploads the next chunk must wait for
+    int64_t t_launch = 0;
+    hipError_t e = hipSuccess;
+    auto fail_all = [&](std::unique_lock<std::mutex> &) {
+        for (Request *r : active) finish(r, BMX_ERR_HIP);
+        for (Request *r : copied)
+            if (!r->done) finish(r, BMX_ERR_HIP);
+        active.clear();
+        running.clear();
+        ready.clear();
+        copied.clear();
+        in_flight = false;
+        uploads_queued = false;
+        e = hipSuccess;
+        (void)hipGetLastError();
+    };
     for (;;) {
-        admitted.clear();
+        std::vector<Request *> admitted;
         {
             std::unique_lock<std::mutex> lk(mu);
-            cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
-            if (stop && pending.empty() && active.empty()) break;
-            // FIFO admission at the chunk boundary while the head request fits
+            if (!in_flight && ready.empty() && copied.empty()) {
+                cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+                if (stop && pending.empty()) break;
+            }
+            // FIFO admission while the head request fits in the pool
+            arrivals = false;
             while (!pending.empty()) {
                 Request *r = pending.front();
                 const int64_t off = r->n > 0 ? alloc(r->n) : 0;
@@ -155,109 +187,132 @@ void bmx_server::run()
                 r->off = r->n > 0 ? off : -1;
                 r->t_admit = now_ns();
                 admitted.push_back(r);
+                if (r->n > 0 && r->n_turns > 0) active.push_back(r);
             }
         }
-        const int64_t t_start = now_ns();
-        hipError_t e = hipSuccess;
-        bool zero_turn_copies = false;
+        // uploads of the admitted requests on the copy stream, while a chunk may be running; a
+        // request of no turns returns its particles unchanged
         for (Request *r : admitted) {
-            if (r->n > 0 && e == hipSuccess) e = copy(r, true);
-            // a request of no turns returns its particles unchanged
-            if (r->n > 0 && r->n_turns == 0 && e == hipSuccess) {
-                e = copy(r, false);
-                zero_turn_copies = true;
+            if (r->n == 0) {
+                copied.push_back(r);
+                continue;
             }
-        }
-        if (zero_turn_copies && e == hipSuccess) e = hipStreamSynchronize(stream);
-        std::vector<Request *> run_now;
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            for (Request *r : admitted) {
-                if (r->n == 0 || r->n_turns == 0) {
-                    finish(r, BMX_OK);
-                    continue;
-                }
-                active.push_back(r);
-            }
-            run_now = active;
-        }
-        if (e != hipSuccess) {
-            std::lock_guard<std::mutex> lk(mu);
-            for (Request *r : run_now) finish(r, BMX_ERR_HIP);
-            active.clear();
-            cv_done.notify_all();
-            continue;
-        }
-        if (run_now.empty()) {
-            cv_done.notify_all();
-            continue;
-        }
-        groups.resize(run_now.size());
-        int32_t max_turns = 0;
-        size_t tiles = 0;
-        for (size_t i = 0; i < run_now.size(); i++) {
-            Request *r = run_now[i];
-            Group &g = groups[i];
-            g.lat = r->lat->desc;
-            g.b = at(r->off);
-            g.n = (uint32_t)r->n;
-            g.list_off = (uint32_t)r->off;
-            g.first = 0;
-            g.count = g.lat.n_ele;
-            g.n_turns = std::min(chunk_turns, r->n_turns - r->done_turns);
-            g.turn0 = r->turn0 + r->done_turns;
-            max_turns = std::max(max_turns, g.n_turns);
-            tiles += (size_t)(r->n + 63) / 64;
-        }
-        // the cell-specialised kernels when every request of this chunk tracks through lattices
-        // that share them
-        RunConfig run = cfg;
-        if (cfg.fast) {
-            const JitKernels *j = run_now[0]->lat->jit;
-            for (Request *r : run_now)
-                if (r->lat->jit != j) j = nullptr;
-            if (j) {
-                run.jit = j;
-                if (waves_auto) run.waves = default_waves(true, j);
-                run.grid = default_grid(device, true, run.waves, j);
-            }
-        }
-        e = rb.reserve(groups.size(), (size_t)capacity, tiles + 1);
-        if (e == hipSuccess)
-            e = enqueue_run(stream, rb, groups.data(), (int32_t)groups.size(), true, max_turns, run, d_work);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        // finished requests: results to the host, pool space back
-        std::vector<Request *> finished;
-        for (size_t i = 0; i < run_now.size(); i++) {
-            Request *r = run_now[i];
-            r->done_turns += groups[i].n_turns;
-            r->chunks++;
-            if (r->done_turns >= r->n_turns) finished.push_back(r);
-        }
-        for (Request *r : finished)
-            if (e == hipSuccess) e = copy(r, false);
-        unsigned long long slots[kWorkSlots] = {};
-        if (e == hipSuccess) e = hipMemcpyAsync(slots, d_work, sizeof slots, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipMemsetAsync(d_work, 0, sizeof slots, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        unsigned long long passes = 0;
-        for (unsigned long long x : slots) passes += x;
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            total_chunks++;
-            total_passes += passes;
-            busy_ns += now_ns() - t_start;
-            if (e != hipSuccess) {
-                for (Request *r : active) finish(r, BMX_ERR_HIP);
-                active.clear();
+            if (e == hipSuccess) e = copy(r, true, copy_stream);
+            if (r->n_turns == 0) {
+                if (e == hipSuccess) e = copy(r, false, copy_stream);
+                copied.push_back(r);
             } else {
-                for (Request *r : finished) finish(r, BMX_OK);
+                ready.push_back(r);
+            }
+        }
+        if (!admitted.empty() && e == hipSuccess) {
+            e = hipEventRecord(ev_up, copy_stream);
+            uploads_queued = true;
+        }
+        if (in_flight && e == hipSuccess) {
+            const hipError_t q = hipEventQuery(ev_chunk);
+            if (q == hipErrorNotReady) {
+                // the chunk is still running: admit arrivals meanwhile
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait_for(lk, std::chrono::microseconds(200), [&] { return stop || arrivals; });
+                continue;
+            }
+            e = q;
+            if (e == hipSuccess) {
+                unsigned long long passes = 0;
+                for (int k = 0; k < kWorkSlots; k++) passes += h_slots[k];
+                std::vector<Request *> next;
+                for (Request *r : running) {
+                    r->done_turns += std::min(chunk_turns, r->n_turns - r->done_turns);
+                    r->chunks++;
+                    if (r->done_turns >= r->n_turns) {
+                        // results down on the copy stream, behind the uploads queued above
+                        if (e == hipSuccess) e = copy(r, false, copy_stream);
+                        copied.push_back(r);
+                    } else {
+                        next.push_back(r);
+                    }
+                }
+                running.swap(next);
+                in_flight = false;
+                std::lock_guard<std::mutex> lk(mu);
+                total_chunks++;
+                total_passes += passes;
+                busy_ns += now_ns() - t_launch;
+            }
+        }
+        // the next chunk: the requests still running plus the uploaded ones
+        if (!in_flight && e == hipSuccess && (!running.empty() || !ready.empty())) {
+            running.insert(running.end(), ready.begin(), ready.end());
+            ready.clear();
+            if (uploads_queued) {
+                e = hipStreamWaitEvent(stream, ev_up, 0);
+                uploads_queued = false;
+            }
+            groups.resize(running.size());
+            int32_t max_turns = 0;
+            size_t tiles = 0;
+            for (size_t i = 0; i < running.size(); i++) {
+                Request *r = running[i];
+                Group &g = groups[i];
+                g.lat = r->lat->desc;
+                g.b = at(r->off);
+                g.n = (uint32_t)r->n;
+                g.list_off = (uint32_t)r->off;
+                g.first = 0;
+                g.count = g.lat.n_ele;
+                g.n_turns = std::min(chunk_turns, r->n_turns - r->done_turns);
+                g.turn0 = r->turn0 + r->done_turns;
+                max_turns = std::max(max_turns, g.n_turns);
+                tiles += (size_t)(r->n + 63) / 64;
+            }
+            // the cell-specialised kernels when every request of this chunk tracks through
+            // lattices that share them
+            RunConfig run = cfg;
+            if (cfg.fast) {
+                const JitKernels *j = running[0]->lat->jit;
+                for (Request *r : running)
+                    if (r->lat->jit != j) j = nullptr;
+                if (j) {
+                    run.jit = j;
+                    if (waves_auto) run.waves = default_waves(true, j);
+                    run.grid = default_grid(device, true, run.waves, j);
+                }
+            }
+            t_launch = now_ns();
+            if (e == hipSuccess) e = rb.reserve(groups.size(), (size_t)capacity, tiles + 1);
+            if (e == hipSuccess)
+                e = enqueue_run(stream, rb, groups.data(), (int32_t)groups.size(), true, max_turns, run, d_work);
+            // the chunk's pass count into pinned host memory, the counter cleared for the next
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(h_slots, d_work, kWorkSlots * sizeof(unsigned long long),
+                                   hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipMemsetAsync(d_work, 0, kWorkSlots * sizeof(unsigned long long), stream);
+            if (e == hipSuccess) e = hipEventRecord(ev_chunk, stream);
+            if (e == hipSuccess) in_flight = true;
+        }
+        // downloads done (they overlap the chunk just launched): results out, pool space back
+        if (!copied.empty() && e == hipSuccess) {
+            e = hipStreamSynchronize(copy_stream);
+            if (e == hipSuccess) {
+                std::lock_guard<std::mutex> lk(mu);
+                for (Request *r : copied) finish(r, BMX_OK);
                 active.erase(std::remove_if(active.begin(), active.end(), [](Request *r) { return r->done; }),
                              active.end());
+                copied.clear();
             }
+            cv_done.notify_all();
         }
-        cv_done.notify_all();
+        if (e != hipSuccess) {
+            hipStreamSynchronize(stream);
+            hipStreamSynchronize(copy_stream);
+            std::unique_lock<std::mutex> lk(mu);
+            fail_all(lk);
+            cv_done.notify_all();
+        }
     }
+    hipStreamSynchronize(stream);
+    hipStreamSynchronize(copy_stream);
 }
 
 extern "C" {
@@ -277,6 +332,10 @@ int32_t bmx_server_create(bmx_ctx *ctx, const bmx_server_config *cfg, bmx_server
     hipSetDevice(S->device);
     const size_t nd = (size_t)S->capacity * 8, ni = (size_t)S->capacity * 4;
     hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->copy_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&S->ev_chunk, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&S->ev_up, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_slots, kWorkSlots * sizeof(unsigned long long), 0);
     if (e == hipSuccess) e = hipMalloc(&S->d_work, kWorkSlots * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(S->d_work, 0, kWorkSlots * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&S->pool, 7 * nd + 3 * ni);
@@ -288,6 +347,10 @@ int32_t bmx_server_create(bmx_ctx *ctx, const bmx_server_config *cfg, bmx_server
         S->rb.release();
         hipFree(S->pool);
         hipFree(S->d_work);
+        if (S->h_slots) hipHostFree(S->h_slots);
+        if (S->ev_chunk) hipEventDestroy(S->ev_chunk);
+        if (S->ev_up) hipEventDestroy(S->ev_up);
+        if (S->copy_stream) hipStreamDestroy(S->copy_stream);
         if (S->stream) hipStreamDestroy(S->stream);
         delete S;
         return set_error(BMX_ERR_NOMEM, std::string("bmx_server_create: ") + hipGetErrorString(e));
@@ -320,10 +383,15 @@ int32_t bmx_server_destroy(bmx_server *S)
     if (S->worker.joinable()) S->worker.join();
     hipSetDevice(S->device);
     hipStreamSynchronize(S->stream);
+    hipStreamSynchronize(S->copy_stream);
     for (auto &kv : S->all) delete kv.second;
     S->rb.release();
     hipFree(S->pool);
     hipFree(S->d_work);
+    hipHostFree(S->h_slots);
+    hipEventDestroy(S->ev_chunk);
+    hipEventDestroy(S->ev_up);
+    hipStreamDestroy(S->copy_stream);
     hipStreamDestroy(S->stream);
     delete S;
     return BMX_OK;
@@ -362,6 +430,7 @@ int32_t bmx_server_submit(bmx_server *S, const bmx_lattice *lat, const bmx_beam_
         r->t_submit = S->now_ns();
         S->all[r->id] = r;
         S->pending.push_back(r);
+        S->arrivals = true;
     }
     S->cv_work.notify_all();
     *id = r->id;

@@ -202,6 +202,20 @@ interface
     integer(c_int32_t) :: st
   end function
 
+  function c_track_element(ctx, lat, beam, ix_ele, turn) bind(c, name = 'bmx_track_element') result(st)
+    import :: c_ptr, c_int32_t
+    type(c_ptr), value :: ctx, lat, beam
+    integer(c_int32_t), value :: ix_ele, turn
+    integer(c_int32_t) :: st
+  end function
+
+  function c_track_bunch(ctx, lat, beam, ix_ele1, ix_ele2, turn) bind(c, name = 'bmx_track_bunch') result(st)
+    import :: c_ptr, c_int32_t
+    type(c_ptr), value :: ctx, lat, beam
+    integer(c_int32_t), value :: ix_ele1, ix_ele2, turn
+    integer(c_int32_t) :: st
+  end function
+
   function c_track_orbit(ctx, lat, beam, ix_start, ix_end, turn, orbit) bind(c, name = 'bmx_track_orbit') result(st)
     import :: c_int32_t, c_ptr
     type(c_ptr), value :: ctx, lat, beam
@@ -222,8 +236,17 @@ interface
   end function
 end interface
 
+! A bunch as the hook and span entry points take it: the particles, and their copy on the
+! device, which stays allocated between calls (bmx_bunch_free releases it).
+type, public :: bmx_bunch_struct
+  type(bmx_coord_struct), allocatable :: particle(:)
+  type(c_ptr) :: beam = c_null_ptr
+  integer(c_int64_t) :: n_device = 0
+end type
+
 public :: bmx_lat_init, bmx_lat_free, bmx_track_bunch, bmx_track_all, bmx_last_error_message
 public :: bmx_closed_orbit_calc, bmx_write_beam_file
+public :: bmx_track1_bunch_hook, bmx_track_bunch_span, bmx_bunch_free
 
 contains
 
@@ -513,6 +536,144 @@ subroutine bmx_write_beam_file (file_name, particle, species, charge, ix_ele, er
                          trim(species) // c_null_char, real(charge, c_double), &
                          real(particle(1)%p0c, c_double), int(ix_ele, c_int32_t), 0_c_int32_t)
   err_flag = st /= bmx_ok
+end subroutine
+
+!+
+! Subroutine bmx_track1_bunch_hook (lat, bunch, ix_ele, err, finished, turn, direction)
+!
+! The body of a track1_bunch_hook (bmad_routine_interface.f90:3937, track1_bunch_hook_def;
+! called once per element per bunch at beam_mod.f90:263-266): tracks bunch%particle through
+! element ix_ele (1-based, ele%ix_ele of the hook's ele) with bmx_track_element (the GPU form of
+! track1_bunch_hom, beam_utils.f90:32-90) and sets finished = .true., so Bmad skips its own
+! tracking of that element. A Bmad hook installed in track1_bunch_hook_ptr calls it with its
+! ele%ix_ele and the bunch's particles copied component by component. Particles not alive on
+! entry are skipped; a particle lost in the element gets the loss code in %state, ix_ele and
+! turn in %ix_ele and %ix_turn; the others get %ix_ele = ix_ele. direction must be +1 (the
+! default). err (and finished = .false.) on a library failure, with the message in
+! bmx_last_error_message().
+!-
+subroutine bmx_track1_bunch_hook (lat, bunch, ix_ele, err, finished, turn, direction)
+  type(bmx_lat_struct), intent(in) :: lat
+  type(bmx_bunch_struct), intent(inout) :: bunch
+  integer, intent(in) :: ix_ele
+  logical, intent(out) :: err, finished
+  integer, optional, intent(in) :: turn, direction
+  integer(c_int32_t) :: t
+
+  err = .true.
+  finished = .false.
+  if (present(direction)) then
+    if (direction /= 1) return
+  endif
+  t = 0
+  if (present(turn)) t = turn
+  call bunch_through_device(lat, bunch, err, 1, int(ix_ele, c_int32_t), 0_c_int32_t, t)
+  finished = .not. err
+end subroutine
+
+!+
+! Subroutine bmx_track_bunch_span (lat, bunch, ele1, ele2, err, turn)
+!
+! track_bunch (beam_mod.f90:103) with its argument list (lat, bunch, ele1, ele2, err): the
+! bunch at the end of element ele1 (0..n_ele; ele1 itself is not tracked) to the end of element
+! ele2, i.e. elements ele1+1 .. ele2, or ele1+1 .. n_ele then 1 .. ele2 when ele2 <= ele1
+! (beam_mod.f90:148-166), in one persistent-kernel call (bmx_track_bunch). Loss bookkeeping
+! as bmx_track1_bunch_hook; survivors end with %ix_ele = ele2 (n_ele for ele2 = 0).
+!-
+subroutine bmx_track_bunch_span (lat, bunch, ele1, ele2, err, turn)
+  type(bmx_lat_struct), intent(in) :: lat
+  type(bmx_bunch_struct), intent(inout) :: bunch
+  integer, intent(in) :: ele1, ele2
+  logical, intent(out) :: err
+  integer, optional, intent(in) :: turn
+  integer(c_int32_t) :: t
+
+  t = 0
+  if (present(turn)) t = turn
+  call bunch_through_device(lat, bunch, err, 2, int(ele1, c_int32_t), int(ele2, c_int32_t), t)
+end subroutine
+
+!+
+! Subroutine bmx_bunch_free (bunch)
+! Releases the device copy of the bunch (the particles stay).
+!-
+subroutine bmx_bunch_free (bunch)
+  type(bmx_bunch_struct), intent(inout) :: bunch
+  integer(c_int32_t) :: st
+  if (c_associated(bunch%beam)) st = c_beam_destroy(bunch%beam)
+  bunch%beam = c_null_ptr
+  bunch%n_device = 0
+end subroutine
+
+! The bunch up to the device, one call (mode 1: element a of bmx_track_element; mode 2: span
+! a -> b of bmx_track_bunch) with turn index t, and back.
+subroutine bunch_through_device (lat, bunch, err, mode, a, b, t)
+  type(bmx_lat_struct), intent(in) :: lat
+  type(bmx_bunch_struct), intent(inout) :: bunch
+  logical, intent(out) :: err
+  integer, intent(in) :: mode
+  integer(c_int32_t), intent(in) :: a, b, t
+
+  real(c_double), allocatable, target :: x(:), px(:), y(:), py(:), z(:), pz(:), beta(:)
+  integer(c_int32_t), allocatable, target :: state(:), loss_ele(:), loss_turn(:)
+  type(bmx_beam_view) :: view
+  integer(c_int32_t) :: st
+  integer(c_int64_t) :: n
+  integer :: i, ix_end
+
+  err = .true.
+  n = size(bunch%particle, kind = c_int64_t)
+  if (n == 0) then
+    err = .false.
+    return
+  endif
+  allocate(x(n), px(n), y(n), py(n), z(n), pz(n), beta(n), state(n), loss_ele(n), loss_turn(n))
+  do i = 1, int(n)
+    x(i) = bunch%particle(i)%vec(1); px(i) = bunch%particle(i)%vec(2)
+    y(i) = bunch%particle(i)%vec(3); py(i) = bunch%particle(i)%vec(4)
+    z(i) = bunch%particle(i)%vec(5); pz(i) = bunch%particle(i)%vec(6)
+    beta(i) = bunch%particle(i)%beta
+    state(i) = bunch%particle(i)%state
+    loss_ele(i) = 0
+    loss_turn(i) = -1
+  enddo
+  call fill_view(view, x, px, y, py, z, pz, beta, state, loss_ele, loss_turn)
+  if (bunch%n_device /= n) then
+    call bmx_bunch_free(bunch)
+    st = c_beam_create(lat%ctx, n, bunch%beam)
+    if (st /= bmx_ok) return
+    bunch%n_device = n
+  endif
+  st = c_beam_upload(bunch%beam, view, n)
+  if (st == bmx_ok) then
+    if (mode == 1) then
+      st = c_track_element(lat%ctx, lat%lat, bunch%beam, a, t)
+    else
+      st = c_track_bunch(lat%ctx, lat%lat, bunch%beam, a, b, t)
+    endif
+  endif
+  if (st == bmx_ok) st = c_beam_download(bunch%beam, view, n)
+  if (st /= bmx_ok) return
+
+  ix_end = a
+  if (mode == 2) then
+    ix_end = b
+    if (ix_end == 0) ix_end = lat%n_ele
+  endif
+  do i = 1, int(n)
+    if (bunch%particle(i)%state /= bmx_alive) cycle
+    bunch%particle(i)%vec = [x(i), px(i), y(i), py(i), z(i), pz(i)]
+    bunch%particle(i)%beta = beta(i)
+    bunch%particle(i)%state = state(i)
+    if (state(i) == bmx_alive) then
+      bunch%particle(i)%ix_ele = ix_end
+      bunch%particle(i)%ix_turn = t
+    else
+      bunch%particle(i)%ix_ele = loss_ele(i)
+      bunch%particle(i)%ix_turn = loss_turn(i)
+    endif
+  enddo
+  err = .false.
 end subroutine
 
 subroutine fill_view (view, x, px, y, py, z, pz, beta, state, loss_ele, loss_turn)

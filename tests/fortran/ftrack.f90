@@ -14,6 +14,11 @@
 !   <dir>/beam_out.dat: the tracked bunch as a Bmad beam file (bmx_write_beam_file)
 !   <dir>/co.bin      : int32 err (0/1), 4-D closed orbit vec(6) from zero, one-turn matrix
 !                       (6x6, row-major), with rel_tol 1e-12 and abs_tol 1e-15
+!   <dir>/hook.bin    : the input bunch through one turn four ways -- element by element with
+!                       bmx_track1_bunch_hook, whole with bmx_track_bunch_span(0, n_ele), and
+!                       from element k = n_ele/3 round to k both ways (hooks k+1..n_ele, 1..k;
+!                       bmx_track_bunch_span(k, k)) -- each as x .. beta (n real64 each), state,
+!                       ix_ele, ix_turn (n int32 each), turn index 7
 !-
 program ftrack
 
@@ -22,7 +27,9 @@ use, intrinsic :: iso_c_binding
 implicit none
 
 type(bmx_ele_struct), allocatable :: eles(:)
-type(bmx_coord_struct), allocatable :: p(:), orbit(:)
+type(bmx_coord_struct), allocatable :: p(:), orbit(:), p0(:)
+type(bmx_bunch_struct) :: hb
+logical :: finished
 type(bmx_coord_struct) :: co
 real(c_double) :: mat6(6, 6)
 type(bmx_lat_struct) :: lat
@@ -56,6 +63,7 @@ p(:)%state = ibuf
 close(u)
 p(:)%p0c = eles(1)%p0c
 p(:)%ix_turn = 0
+p0 = p
 
 call bmx_lat_init(lat, eles, mc2, -1, err)
 if (err) then
@@ -109,7 +117,57 @@ write(u) co%vec
 write(u) transpose(mat6)
 close(u)
 
+open(newunit = u, file = trim(dir) // '/hook.bin', access = 'stream', form = 'unformatted', status = 'replace')
+! one turn element by element through the hook
+hb%particle = p0
+do i = 1, n_ele
+  call bmx_track1_bunch_hook(lat, hb, i, err, finished, turn = 7)
+  if (err .or. .not. finished) then
+    print '(a)', 'bmx_track1_bunch_hook failed: ' // bmx_last_error_message()
+    stop 5
+  endif
+enddo
+call write_bunch(u, hb%particle)
+! the same turn in one span call
+hb%particle = p0
+call bmx_track_bunch_span(lat, hb, 0, int(n_ele), err, turn = 7)
+if (err) stop 6
+call write_bunch(u, hb%particle)
+! a turn from the end of element k round to it: hooks, then one span call
+k = int(n_ele) / 3
+hb%particle = p0
+do i = k + 1, n_ele
+  call bmx_track1_bunch_hook(lat, hb, i, err, finished, turn = 7)
+  if (err) stop 7
+enddo
+do i = 1, k
+  call bmx_track1_bunch_hook(lat, hb, i, err, finished, turn = 7)
+  if (err) stop 7
+enddo
+call write_bunch(u, hb%particle)
+hb%particle = p0
+call bmx_track_bunch_span(lat, hb, k, k, err, turn = 7)
+if (err) stop 8
+call write_bunch(u, hb%particle)
+close(u)
+call bmx_bunch_free(hb)
+
 call bmx_lat_free(lat)
 print '(a, i0, a, i0)', 'ftrack: tracked ', n, ' particles; track_state ', track_state
+
+contains
+
+subroutine write_bunch (u, q)
+  integer, intent(in) :: u
+  type(bmx_coord_struct), intent(in) :: q(:)
+  integer :: kk
+  do kk = 1, 6
+    write(u) q(:)%vec(kk)
+  enddo
+  write(u) q(:)%beta
+  write(u) int(q(:)%state, c_int32_t)
+  write(u) int(q(:)%ix_ele, c_int32_t)
+  write(u) int(q(:)%ix_turn, c_int32_t)
+end subroutine
 
 end program

@@ -152,3 +152,49 @@ def test_fortran_closed_orbit_calc(bmx, tmp_path):
     assert st[0] == 0 and abs(vec[0]) > 1e-5
     assert np.array_equal(vec, orb[0])
     assert np.array_equal(mat, M[0])
+
+
+def _read_bunches(path, n, count):
+    raw = open(path, "rb").read()
+    out, off = [], 0
+    for _ in range(count):
+        b = {}
+        for k in P.COORDS + ("beta",):
+            b[k] = np.frombuffer(raw, np.float64, n, off)
+            off += 8 * n
+        for k in ("state", "ix_ele", "ix_turn"):
+            b[k] = np.frombuffer(raw, np.int32, n, off)
+            off += 4 * n
+        out.append(b)
+    return out
+
+
+def test_fortran_hook_and_span(bmx, oracle, tmp_path):
+    """The per-element hook (bmx_track1_bunch_hook, the body of a track1_bunch_hook) called for
+    every element of a turn equals the whole turn in one bmx_track_bunch_span call bit for bit,
+    from element 0 and from element k round to k; both equal the CPU restatement's losses
+    (state, element, turn 7) and its coordinates to 1e-10 after the turn."""
+    L, E = bmx.lattices, bmx.elements
+    arr = E.to_array(L.ring1(n_cells=3))
+    n_ele = len(arr)
+    beam = P.random_beam(bmx, 2000, seed=19, scale=20.0, pz_sigma=4e-3)
+    n = len(beam["x"])
+    _write_inputs(tmp_path, arr, beam, 1, E.M_ELECTRON)
+    _run(tmp_path)
+    hook0, span0, hookk, spank = _read_bunches(os.path.join(tmp_path, "hook.bin"), n, 4)
+    for a, b in ((hook0, span0), (hookk, spank)):
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+    k0 = n_ele // 3
+    for got, first in ((hook0, 0), (hookk, k0)):
+        idx = [(first + j) % n_ele for j in range(n_ele)]
+        ref = P.copy_beam(beam)
+        oracle.track_bunch(arr[idx], ref, 1, turn0=7)
+        lost = ref["state"] != 1
+        assert lost.sum() > 50
+        loss_ele = np.asarray(idx)[ref["loss_ele"][lost] - 1] + 1
+        assert np.array_equal(got["state"], ref["state"])
+        assert np.array_equal(got["ix_ele"][lost], loss_ele)
+        assert np.all(got["ix_turn"][lost] == 7)
+        assert np.all(got["ix_ele"][~lost] == (first if first else n_ele))
+        assert P.rel_err(got, ref, ~lost) <= 1e-10
