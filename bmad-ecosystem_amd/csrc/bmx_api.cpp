@@ -190,6 +190,7 @@ int32_t lattice_upload(bmx_ctx *ctx, HostTable &&table, bmx_lattice **out)
     L->desc.hdr = L->d_hdr;
     L->desc.prm = L->d_prm;
     L->desc.n_ele = (int32_t)nh;
+    L->desc.rare_q = L->host.rare_q ? 1 : 0;
     L->desc.sp = L->host.sp;
     if (e == hipSuccess) e = hipMemcpy(L->d_desc, &L->desc, sizeof(LatDesc), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -388,8 +389,9 @@ int32_t track_group(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t fir
     cfg.fast = ctx->fast_maps;
     cfg.jit = ctx->fast_maps ? L->jit : nullptr;
     cfg.waves = ctx->waves ? ctx->waves : default_waves(ctx->fast_maps, cfg.jit);
-    // a start element or a partial turn runs on the generic span kernel
-    cfg.span = first != 0 || count != L->desc.n_ele;
+    // a start element or a partial turn runs on the generic span kernel, and so does a lattice
+    // with stepped quadrupoles that no cell kernel carries
+    cfg.span = first != 0 || count != L->desc.n_ele || (L->host.rare_q && !jit_rare_q(cfg.jit));
     cfg.grid = ctx->grid > 0 ? ctx->grid
                              : default_grid(ctx->device, ctx->fast_maps, cfg.waves, cfg.jit, cfg.span);
     cfg.timer = ctx->profile ? &ctx->timer : nullptr;

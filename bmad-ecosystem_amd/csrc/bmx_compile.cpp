@@ -173,6 +173,54 @@ int ele_to_ab(const bmx_ele &e, double *a, double *b)
     return -1;
 }
 
+// The highest order below n_max with a nonzero strength, or -1 (max_nonzero).
+int max_order(const double *a, const double *b, int n_max)
+{
+    for (int n = n_max - 1; n >= 0; n--)
+        if (a[n] != 0 || b[n] != 0) return n;
+    return -1;
+}
+
+// Electric a/b strengths (multipole_ele_to_ab.f90:31-236 with pole_type = electric$):
+// a_pole_elec / b_pole_elec, order n scaled by r0_elec^-n unless r0_elec is 0 or 1
+// (convert_this_ab, :519-530). Returns ix_pole_max.
+int ele_to_ab_elec(const bmx_ele &e, double *a, double *b)
+{
+    for (int n = 0; n < BMX_N_POLE; n++) a[n] = b[n] = 0;
+    if (!e.is_on || e.key == BMX_PIPE || e.n_multipole_elec <= 0) return -1;
+    for (int n = 0; n < BMX_N_POLE; n++) {
+        a[n] = e.a_pole_elec[n];
+        b[n] = e.b_pole_elec[n];
+    }
+    const double radius = e.r0_elec;
+    if (radius != 0 && radius != 1) {
+        double factor = radius;
+        for (int n = 0; n < BMX_N_POLE; n++) {
+            factor = factor / radius;
+            a[n] = factor * a[n];
+            b[n] = factor * b[n];
+        }
+    }
+    return max_order(a, b, BMX_N_POLE);
+}
+
+// Series terms for the quadrupole's fast form over a step with b1 s^2/L = w_l: W = w_l/(1+pz),
+// sized for 1+pz >= 0.5 (further off momentum the kernel reduces the argument); the fewest
+// terms whose first omitted term t^M/(4M)! is below 1e-18, and the t range they cover.
+double quad_series_limit(double w_l, int &nterms)
+{
+    const double wmax = std::fabs(w_l) / 0.5;
+    const double tmax = wmax * wmax;
+    double fact[40];
+    fact[0] = 1;
+    for (int k = 1; k < 40; k++) fact[k] = fact[k - 1] * k;
+    nterms = 8;
+    for (int m = 2; m <= 8; m++) {
+        if (std::pow(tmax, m) / fact[4 * m] <= 1e-18) { nterms = m; break; }
+    }
+    return std::pow(1e-18 * fact[4 * nterms], 1.0 / nterms);
+}
+
 // Aperture flags and the 4 aperture values; returns false if unsupported.
 bool aperture_setup(const bmx_ele &e, int32_t &flags, double ap[4], double mom[8], std::string &why)
 {
@@ -257,10 +305,12 @@ void edge_constants(const bmx_ele &e, bool entrance, double *out)
 int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge, HostTable &out,
                   std::string &err)
 {
-    (void)charge;  // the tracked species is the reference species: charge ratios are 1
+    // the tracked species is the reference species: magnetic charge ratios are 1; the charge
+    // enters only the electric multipole strengths
     out.hdr.clear();
     out.prm.clear();
     out.kind_mask = 0;
+    out.rare_q = false;
     if (n_ele <= 0 || !eles) { err = "empty lattice"; return BMX_ERR_ARG; }
     if (!(mc2 > 0)) { err = "mass must be positive"; return BMX_ERR_ARG; }
     const double p0c = eles[0].p0c;
@@ -307,6 +357,10 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             return BMX_ERR_UNSUPPORTED;
         }
         double a[BMX_N_POLE], b[BMX_N_POLE];
+        if (key != BMX_QUADRUPOLE && ele_to_ab_elec(e, a, b) >= 0) {
+            err = where(i, e) + "electric multipoles are supported on quadrupoles only";
+            return BMX_ERR_UNSUPPORTED;
+        }
         switch (key) {
         case BMX_DRIFT:
         case BMX_MARKER:
@@ -351,31 +405,58 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
         case BMX_QUADRUPOLE: {
             int ix = ele_to_ab(e, a, b);
             if (ix == -2) { err = where(i, e) + "multipole scaling"; return BMX_ERR_UNSUPPORTED; }
-            double b1 = b[1];
+            // pull_this_b1 (multipole_ele_to_ab.f90:664-676): k1 L comes out of the multipoles
+            const double b1 = b[1];
             b[1] = 0;
-            for (int n = 0; n < BMX_N_POLE; n++)
-                if (a[n] != 0 || b[n] != 0) { err = where(i, e) + "quadrupole multipoles/kicks are not supported"; return BMX_ERR_UNSUPPORTED; }
+            if (ix <= 1) ix = max_order(a, b, 2);
+            double ae[BMX_N_POLE], be[BMX_N_POLE];
+            const int ixe = ele_to_ab_elec(e, ae, be);
             if (e.fringe_type != BMX_FRINGE_NONE && e.fringe_type != 0) { err = where(i, e) + "quadrupole fringe"; return BMX_ERR_UNSUPPORTED; }
-            if (e.l == 0) {
-                h.kind = K_MARKER;
+            if (ix > 3 || ixe > 3) { err = where(i, e) + "quadrupole multipole orders above 3"; return BMX_ERR_UNSUPPORTED; }
+            if (ix == -1 && ixe == -1) {
+                if (e.l == 0) {
+                    h.kind = K_MARKER;
+                    break;
+                }
+                h.kind = K_QUAD;
+                int nterms = 0;
+                const double t_lim = quad_series_limit(b1 * e.l, nterms);
+                q = {e.l, 1.0 * b1, b1 * e.l, b1 / e.l, t_lim};
+                h.n = nterms;
                 break;
             }
-            h.kind = K_QUAD;
-            q = {e.l, 1.0 * b1, b1 * e.l, b1 / e.l};
-            // Series terms for the fast form: W = b1 L/(1+pz), sized for 1+pz >= 0.5 (further
-            // off momentum the kernel halves the argument); take the fewest terms whose first
-            // omitted term t^M/(4M)! is below 1e-18, and record the t range they cover.
-            const double wmax = std::fabs(b1 * e.l) / 0.5;
-            const double tmax = wmax * wmax;
-            double fact[40];
-            fact[0] = 1;
-            for (int k = 1; k < 40; k++) fact[k] = fact[k - 1] * k;
-            int nterms = 8;
-            for (int m = 2; m <= 8; m++) {
-                if (std::pow(tmax, m) / fact[4 * m] <= 1e-18) { nterms = m; break; }
-            }
-            h.n = nterms;
-            q.push_back(std::pow(1e-18 * fact[4 * nterms], 1.0 / nterms));  // t_lim
+            // multipoles or kicks beside k1, or electric multipoles: stepped
+            // (track_a_quadrupole.f90:46-53,70-130)
+            if (e.l == 0) { err = where(i, e) + "zero-length quadrupole with multipoles"; return BMX_ERR_UNSUPPORTED; }
+            h.kind = K_QMULT;
+            h.flags |= F_RARE_MAP;
+            out.rare_q = true;
+            long ns = f_nint(std::fabs(e.l) / e.ds_step);
+            const int n_step = ns > 1 ? (int)ns : 1;
+            const double r_step = 1.0 / n_step;
+            const double step_len = e.l * r_step;
+            int nterms = 0;
+            const double t_lim = quad_series_limit(b1 * step_len * r_step, nterms);
+            q = {step_len, b1 * r_step, b1 * step_len * r_step, b1 / e.l, t_lim};
+            int mmask = 0, emask = 0;
+            const double half = r_step / 2;
+            for (int n = 0; n < 4; n++) q.push_back(ix >= 0 ? half * a[n] : 0.0);
+            for (int n = 0; n < 4; n++) q.push_back(ix >= 0 ? half * b[n] : 0.0);
+            if (ix >= 0)
+                for (int n = 0; n < 4; n++)
+                    if (a[n] != 0 || b[n] != 0) mmask |= 1 << n;
+            // electric: ab_multipole_kick's f = charge/(beta p0c) without the particle's beta,
+            // times the scale step_len/2 (the element p0c equals the tracked reference p0c)
+            const double fe = charge / e.p0c * (step_len / 2);
+            for (int n = 0; n < 4; n++) q.push_back(ixe >= 0 ? ae[n] * fe : 0.0);
+            for (int n = 0; n < 4; n++) q.push_back(ixe >= 0 ? -be[n] * fe : 0.0);
+            for (int n = 0; n < 4; n++) q.push_back(ixe >= 0 ? charge * ae[n] / (n + 1) : 0.0);
+            for (int n = 0; n < 4; n++) q.push_back(ixe >= 0 ? charge * be[n] / (n + 1) : 0.0);
+            if (ixe >= 0)
+                for (int n = 0; n < 4; n++)
+                    if (ae[n] != 0 || be[n] != 0) emask |= 1 << n;
+            h.n = n_step;
+            h.flags |= (mmask << F_ORDER_SHIFT) | (emask << F_ELEC_SHIFT);
             break;
         }
         case BMX_SBEND: {

@@ -46,6 +46,7 @@ struct HostTable {
     std::vector<double> prm;
     Species sp{};
     uint64_t kind_mask = 0;  // bit = bmx_ele key present
+    bool rare_q = false;     // some element compiled to K_QMULT
 };
 // Returns BMX_OK or an error code with a message in err.
 int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge, HostTable &out,
@@ -87,6 +88,8 @@ constexpr int kTileBlock = 256;
 struct JitKernels;
 int jit_specialize(int device, const HostTable &t, const JitKernels **out, std::string &err);
 hipError_t jit_occupancy(const JitKernels *j, int waves, int *blocks_per_cu);
+// The cell kernels carry the stepped quadrupole (their lattice holds one).
+bool jit_rare_q(const JitKernels *j);
 hipError_t launch_track_tiles_jit(const JitKernels *j, hipStream_t s, int grid, int waves,
                                   const Group *d_groups, const WaveTile *d_tiles,
                                   const uint32_t *d_n_tiles, const uint32_t *d_list_in,
@@ -116,9 +119,10 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
 // Default waves per SIMD of the tracking kernel: the variant measured fastest on the 901-element
 // ring for each kernel (cell-specialised 4: at most 128 VGPRs, so 4 waves per SIMD are resident,
 // without spilling; generic fast maps 5; reference maps 3, whose 4- and 5-wave builds spill).
+// Cell kernels that carry the stepped quadrupole take 3: their 4-wave build spills.
 inline int32_t default_waves(bool fast, const JitKernels *jit = nullptr)
 {
-    return jit ? 4 : (fast ? 5 : 3);
+    return jit ? (jit_rare_q(jit) ? 3 : 4) : (fast ? 5 : 3);
 }
 // Workgroups of the tracking kernel that fit on the device at once (of the cell-specialised
 // kernels when jit is set).

@@ -31,6 +31,7 @@ namespace bmx {
 struct JitKernels {
     hipModule_t module = nullptr;
     hipFunction_t fn[3] = {};  // waves per SIMD 3, 4, 5
+    bool rare_q = false;       // carries the stepped quadrupole (K_QMULT) on its rare path
 };
 
 namespace {
@@ -81,14 +82,15 @@ std::string generate(const HostTable &t, const Period &pd, std::string &sig)
         const EleHdr &h = t.hdr[pd.H + j];
         const int32_t rel = h.off - t.hdr[pd.H].off;
         key << '/' << h.kind << ',' << h.flags << ',' << h.n << ',' << rel;
-        // the rare set only where the element is a cavity or carries offsets or momentum
-        // windows: elsewhere pz cannot change and the exit test for rel_p < 0 drops out, as in
-        // the generic loop
-        const char *maps = (h.kind == K_RF || (h.flags & (F_OFFSET | F_APE_MOMENTUM))) ? "MAPS_RARE" : "MAPS_NO_RF";
+        // the rare set only where the element is a cavity, a quadrupole with multipoles or
+        // carries offsets or momentum windows: elsewhere pz cannot change and the exit test for
+        // rel_p < 0 drops out, as in the generic loop
+        const char *maps = (h.kind == K_RF || (h.flags & (F_OFFSET | F_APE_MOMENTUM | F_RARE_MAP))) ? "kRare" : "MAPS_NO_RF";
         body << "            if (p.state == ST_ALIVE)\n"
              << "                track_element<true, " << maps << ">(p, EleRef{" << h.kind << ", " << h.flags
              << ", " << h.n << ", " << rel << "}, base, L.sp, true, e1 + " << j << ", t_abs);\n";
     }
+    if (t.rare_q) key << "/q";
     sig = key.str();
     std::ostringstream s;
     s << "#include <stdint.h>\n"
@@ -97,17 +99,18 @@ std::string generate(const HostTable &t, const Period &pd, std::string &sig)
       << bmx_jit_src_tiles << '\n'
       << "namespace bmx {\n"
       << "struct CellTurn {\n"
+      << "    static constexpr int kRare = " << (t.rare_q ? "MAPS_RARE_Q" : "MAPS_RARE") << ";\n"
       << "    __device__ __forceinline__ static void turn(Particle &p, const LatDesc &L, int32_t t_abs)\n"
       << "    {\n"
       << "        const CP prm = (CP)L.prm;\n";
-    if (pd.H > 0) s << "        track_range<true>(p, L, 0, " << pd.H << ", t_abs);\n";
+    if (pd.H > 0) s << "        track_range<true, kRare>(p, L, 0, " << pd.H << ", t_abs);\n";
     s << "        for (int32_t c = 0; c < " << pd.N << "; ++c) {\n"
       << "            c = __builtin_amdgcn_readfirstlane(c);\n"
       << "            const CP base = prm + (" << t.hdr[pd.H].off << " + c * " << pd.S << ");\n"
       << "            const int32_t e1 = " << pd.H + 1 << " + c * " << pd.P << ";\n"
       << body.str()
       << "        }\n"
-      << "        track_range<true>(p, L, " << pd.H + pd.N * pd.P << ", L.n_ele, t_abs);\n"
+      << "        track_range<true, kRare>(p, L, " << pd.H + pd.N * pd.P << ", L.n_ele, t_abs);\n"
       << "    }\n"
       << "};\n"
       << "}  // namespace bmx\n";
@@ -166,6 +169,7 @@ int jit_specialize(int device, const HostTable &t, const JitKernels **out, std::
     hiprtcGetCode(prog, code.data());
     hiprtcDestroyProgram(&prog);
     auto k = std::make_unique<JitKernels>();
+    k->rare_q = t.rare_q;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipModuleLoadData(&k->module, code.data());
     for (int w = 3; w <= 5 && e == hipSuccess; w++) {
@@ -180,6 +184,8 @@ int jit_specialize(int device, const HostTable &t, const JitKernels **out, std::
     *out = slot.get();
     return BMX_OK;
 }
+
+bool jit_rare_q(const JitKernels *j) { return j && j->rare_q; }
 
 hipError_t jit_occupancy(const JitKernels *j, int waves, int *blocks_per_cu)
 {

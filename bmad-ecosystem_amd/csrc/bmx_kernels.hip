@@ -17,7 +17,8 @@ namespace bmx {
 // W: waves per SIMD the register allocation targets at least (3: up to 168 VGPRs; 4: up to 128;
 // 5: up to 102, spilling a few values of the rf and bend paths to scratch). SPAN: groups may
 // track element spans (runs with a start element or a partial turn); the full-turn kernels
-// carry no code for them.
+// carry no code for them. The span kernels also carry the stepped quadrupole (K_QMULT) on their
+// rare path and track full turns of lattices that hold one.
 template <bool F, int W, bool SPAN = false>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) k_track_tiles(
     const Group *__restrict__ groups, const WaveTile *__restrict__ tiles,
@@ -25,8 +26,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))
     uint32_t *__restrict__ list_out, uint32_t *__restrict__ cnt_out, uint32_t *tile_ctr,
     int32_t chunk_off, int32_t chunk_turns, unsigned long long *work)
 {
-    track_tiles<F, GenericTurn<F>, SPAN>(groups, tiles, n_tiles_p, list_in, list_out, cnt_out,
-                                         tile_ctr, chunk_off, chunk_turns, work);
+    track_tiles<F, GenericTurn<F, SPAN ? MAPS_RARE_Q : MAPS_RARE>, SPAN>(
+        groups, tiles, n_tiles_p, list_in, list_out, cnt_out, tile_ctr, chunk_off, chunk_turns, work);
 }
 
 // Builds the wave tiles of the next chunk from the alive counts per group (cnt_in; nullptr

@@ -528,16 +528,17 @@ __device__ __forceinline__ void quadrupole(Particle &p, CP q, int32_t n_terms, c
 // orders n in mask with strengths (a_n, b_n): c_multi's signed binomials summed as the real and
 // imaginary parts of (x + i y)^n, kx = -b Re + a Im, ky = a Re + b Im. The table holds the
 // half-step strengths; factor (1 or 2) scales the kick, exactly as scaling the strengths would.
-__device__ __forceinline__ void multipole_kick(Particle &p, CP ah, CP bh, int32_t mask,
-                                               double factor)
+// multipole_sum gives the sums at (x, y); multipole_kick adds them to (px, py).
+__device__ __forceinline__ void multipole_sum(double x, double y, CP ah, CP bh, int32_t mask,
+                                              double &kx, double &ky)
 {
-    double kx = 0, ky = 0;
+    kx = 0;
+    ky = 0;
     if (mask & 1) {
         kx = -bh[0];
         ky = ah[0];
     }
     if (mask >> 1) {
-        const double x = p.x, y = p.y;
         double zr = x, zi = y;
 #pragma unroll
         for (int n = 1; n < 4; n++) {
@@ -552,6 +553,13 @@ __device__ __forceinline__ void multipole_kick(Particle &p, CP ah, CP bh, int32_
             }
         }
     }
+}
+
+__device__ __forceinline__ void multipole_kick(Particle &p, CP ah, CP bh, int32_t mask,
+                                               double factor)
+{
+    double kx, ky;
+    multipole_sum(p.x, p.y, ah, bh, mask, kx, ky);
     p.px = fma(factor, kx, p.px);
     p.py = fma(factor, ky, p.py);
 }
@@ -1174,6 +1182,88 @@ __device__ __forceinline__ void rfcavity(Particle &p, CP r, int32_t n_slice,
     }
 }
 
+// ---- quadrupole with multipoles ----------------------------------------------------------
+
+// bmad/modules/multipole_mod.f90:82-174 ab_multipole_kicks, electric branch (:128-143), with
+// ab_multipole_kick's electric strengths a2 = a f, b2 = -b f, f = charge/(beta p0c)
+// (:343-346): the table holds a f beta and b2 beta at the half step; the kick changes the
+// energy, pz += (1+pz) sqrt_one(alpha), alpha = (kx (2 px + kx) + ky (2 py + ky))/(1+pz)^2
+// (lost_pz for alpha < -1, the coordinates kicked and pz left as it was), beta from the new
+// momentum and z scaled by beta_new/beta_old.
+__device__ __forceinline__ void elec_kick(Particle &p, CP ae, CP be, int32_t mask, double factor,
+                                          const Species &sp)
+{
+    double kx, ky;
+    multipole_sum(p.x, p.y, ae, be, mask, kx, ky);
+    const double fb = factor / p.beta;
+    kx = kx * fb;
+    ky = ky * fb;
+    const double px0 = p.px, py0 = p.py;
+    p.px = px0 + kx;
+    p.py = py0 + ky;
+    const double alpha = (kx * (2 * px0 + kx) + ky * (2 * py0 + ky)) / (p.rel_p * p.rel_p);
+    if (alpha < -1) {
+        p.state = ST_LOST_PZ;
+        return;
+    }
+    p.pz = p.pz + p.rel_p * sqrt_one(alpha);
+    const double rp = 1 + p.pz;
+    const double beta_new = rp / sqrt(rp * rp + sqr(sp.mc2 / sp.p0c));
+    p.z = p.z * beta_new / p.beta;
+    p.beta = beta_new;
+    refresh_energy(p, sp);
+}
+
+// bmad/low_level/apply_element_edge_kick.f90:283-326 electric_longitudinal_fringe (bmad_standard
+// field, not a bend): crossing the hard edge of the electric multipole field changes the
+// energy by dE = sign * charge * sum Re((b_n - i a_n) (x + i y)^(n+1))/(n+1), sign = +1 at
+// the entrance and -1 at the exit (apply_energy_kick.f90:23-75). fa, fb hold charge a_n/(n+1)
+// and charge b_n/(n+1). Returns false if the particle is lost (energy below the rest mass).
+__device__ __forceinline__ bool elec_fringe(Particle &p, CP fa, CP fb, int32_t mask, double sign,
+                                            const Species &sp)
+{
+    const double x = p.x, y = p.y;
+    double zr = x, zi = y, dE = 0;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        if (mask & (1 << n)) dE += fb[n] * zr + fa[n] * zi;
+        if (n < 3 && (mask >> (n + 1))) {
+            const double nr = zr * x - zi * y;
+            zi = zr * y + zi * x;
+            zr = nr;
+        }
+    }
+    if (!energy_kick(p, sign * dE, sp)) return false;
+    refresh_energy(p, sp);
+    return true;
+}
+
+// bmad/low_level/track_a_quadrupole.f90:17-148 with multipoles beside k1 (no fringe): the
+// electric edge at the entrance (init_fringe_info.f90:44-54 gives an element with electric
+// multipoles an edge kick whatever its fringe_type; apply_element_edge_kick.f90:165-167,270-271
+// orders it after the magnetic edge on entry and before it on exit), half kicks, then n_step
+// quadrupole steps with k1 = b1/(L (1+pz)) taken from the momentum at each step, each followed
+// by a full kick (half after the last), then the exit edge. Returns false when an edge loses
+// the particle: the reference then returns before offset_particle's exit transform.
+template <bool F>
+__device__ __forceinline__ bool quad_multipole(Particle &p, CP q, int32_t n_step, int32_t flags,
+                                               const Species &sp)
+{
+    const int32_t mmask = (flags >> F_ORDER_SHIFT) & 0xF;
+    const int32_t emask = (flags >> F_ELEC_SHIFT) & 0xF;
+    if (emask && !elec_fringe(p, q + 21, q + 25, emask, 1.0, sp)) return false;
+    if (mmask) multipole_kick(p, q + 5, q + 9, mmask, 1.0);
+    if (emask) elec_kick(p, q + 13, q + 17, emask, 1.0, sp);
+    for (int i = 1; i <= n_step; i++) {
+        quadrupole<F>(p, q, 0, sp);
+        const double f = i == n_step ? 1.0 : 2.0;
+        if (mmask) multipole_kick(p, q + 5, q + 9, mmask, f);
+        if (emask) elec_kick(p, q + 13, q + 17, emask, f, sp);
+    }
+    if (emask && !elec_fringe(p, q + 21, q + 25, emask, -1.0, sp)) return false;
+    return true;
+}
+
 // One element for one particle: the body of track1 (bmad/code/track1.f90:196-329) after the
 // entrance validity check, i.e. entrance aperture, element map, orbit_too_large (:291),
 // exit aperture (:329). h is the element's header (bmx_table.hpp); prm the lattice's parameter
@@ -1182,12 +1272,14 @@ __device__ __forceinline__ void rfcavity(Particle &p, CP r, int32_t n_slice,
 // MAPS selects the code compiled in: MAPS_ALL (every kind, offsets), or the pair MAPS_NO_RF /
 // MAPS_RARE for loops that keep the rare, register-hungry cases out of the common element
 // path: MAPS_NO_RF has no rf cavity, no offset_particle frame change and no collimator
-// momentum windows; MAPS_RARE has only the kinds that can carry them (no bends) with all three.
+// momentum windows; MAPS_RARE has only the kinds that can carry them (no bends) with all three;
+// MAPS_RARE_Q adds the stepped quadrupole with multipoles (K_QMULT), for loops over lattices
+// that hold one (its code would otherwise shape every rare path's register assignment).
 // from_prev: the particle has just left the element before this one in the same lattice, so
 // an entrance test that repeats that element's exit test (F_APE_ENT_SAME, F_APE_ENT cleared)
 // is skipped; otherwise it is made.
 // A particle lost here gets p.loss_ele = ix1 (the element's 1-based index), p.loss_turn = turn.
-enum MapSet : int { MAPS_ALL = 0, MAPS_NO_RF = 1, MAPS_RARE = 2 };
+enum MapSet : int { MAPS_ALL = 0, MAPS_NO_RF = 1, MAPS_RARE = 2, MAPS_RARE_Q = 3 };
 
 // ---- offset_particle, non-bend branch (bmad/code/offset_particle.f90:183-492) -------------
 // Element orientation +1, particle direction +1; ob is the offset block (bmx_table.hpp).
@@ -1298,12 +1390,13 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
     if constexpr (kOffsets) {
         if (flags & F_OFFSET) offset_set(p, ob, flags);
     }
+    bool unset = true;
     switch (kind) {
     case K_DRIFT: drift<F>(p, q[0]); break;
     case K_QUAD: quadrupole<F>(p, q, n, sp); break;
     case K_MULT: thick_multipole<F>(p, q, n, flags); break;
     case K_BEND:
-        if constexpr (MAPS != MAPS_RARE) {
+        if constexpr (MAPS != MAPS_RARE && MAPS != MAPS_RARE_Q) {
             if ((flags & (F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE | F_BEND_DRIFT)) ==
                 (F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE))
                 sbend<F, true>(p, flags, q, sp);
@@ -1314,12 +1407,16 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
     case K_RF:
         if constexpr (MAPS != MAPS_NO_RF) rfcavity<F>(p, q, n, sp);
         break;
+    case K_QMULT:
+        if constexpr (MAPS == MAPS_ALL || MAPS == MAPS_RARE_Q) unset = quad_multipole<F>(p, q, n, flags, sp);
+        break;
     default: break;
     }
     if constexpr (kOffsets) {
-        // the rf cavity returns before its exit offset when it loses the particle (pz = -1)
+        // the rf cavity returns before its exit offset when it loses the particle (pz = -1),
+        // the quadrupole when an electric edge loses it
         if (flags & F_OFFSET) {
-            if (kind != K_RF || p.state == ST_ALIVE) offset_unset(p, ob, flags);
+            if ((kind != K_RF || p.state == ST_ALIVE) && unset) offset_unset(p, ob, flags);
         }
     }
     // orbit_too_large (overwrites any state), else the exit aperture for live particles; both

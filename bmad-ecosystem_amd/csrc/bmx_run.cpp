@@ -145,6 +145,13 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
     if (e == hipSuccess)
         e = launch_build_tiles(s, rb.d_groups, n_groups, rb.d_cnt[0], 0, rb.d_tiles[0], n_tiles[0],
                                rb.d_cnt[1], tile_ctr, rb.d_goff);
+    // Lattices with stepped quadrupoles (K_QMULT) run on cell kernels generated with them or on
+    // the generic span kernels, which carry them; the other full-turn kernels do not.
+    bool rare_q = false;
+    for (int32_t g = 0; g < n_groups; g++)
+        if (h_groups[g].n_turns > 0 && h_groups[g].lat.rare_q) rare_q = true;
+    const bool use_jit = cfg.jit && !cfg.span && (!rare_q || jit_rare_q(cfg.jit));
+    const bool span = cfg.span || (rare_q && !use_jit);
     int c = 0;
     for (int32_t off = 0; off < max_turns && e == hipSuccess; off += K, c++) {
         const int cur = c & 1, nxt = cur ^ 1;
@@ -155,14 +162,14 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
             if (e == hipSuccess) e = hipEventRecord(ev0, s);
             if (e != hipSuccess) break;
         }
-        if (cfg.jit && !cfg.span)
+        if (use_jit)
             e = launch_track_tiles_jit(cfg.jit, s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur],
                                        n_tiles[cur], rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt],
                                        tile_ctr, off, K, d_work);
         else
             e = launch_track_tiles(s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur], n_tiles[cur],
                                    rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt], tile_ctr, off, K,
-                                   d_work, cfg.fast, cfg.span);
+                                   d_work, cfg.fast, span);
         if (cfg.timer && e == hipSuccess) {
             e = hipEventRecord(ev1, s);
             cfg.timer->pending.emplace_back(ev0, ev1);

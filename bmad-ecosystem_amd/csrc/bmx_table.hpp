@@ -17,7 +17,11 @@ enum Kind : int32_t {
     K_BEND = 3,
     K_RF = 4,
     K_MARKER = 5,
-    K_NKIND = 6,
+    // quadrupole with multipoles or kicks beside k1 and/or electric multipoles: n_step steps of
+    // kick - quadrupole step - kick, electric hard-edge energy kicks at both ends
+    // (track_a_quadrupole.f90:17-148)
+    K_QMULT = 6,
+    K_NKIND = 7,
 };
 
 enum Flags : int32_t {
@@ -51,6 +55,12 @@ enum Flags : int32_t {
     // width and centre); F_APE_MOM_ELLIP: the ecollimator's elliptical pairs
     F_APE_MOMENTUM = 1 << 19,
     F_APE_MOM_ELLIP = 1 << 20,
+    // K_QMULT: bits 21..24 = mask of electric multipole orders 0..3 present (the magnetic
+    // orders use bits 8..11 as for K_MULT)
+    F_ELEC_SHIFT = 21,
+    // An element whose map only the rare element path carries (K_QMULT): the loops send it
+    // there as they do cavities and elements with offsets or momentum windows.
+    F_RARE_MAP = 1 << 25,
 };
 constexpr int kOffsetLen = 24;
 
@@ -74,6 +84,11 @@ struct EleHdr {
 //  K_RF    : [0] L/n_slice, [1] voltage * charge_dir, [2] phi0, [3] rf_frequency,
 //            [4] 0.5/rf_frequency, [5] twopi * rf_frequency, [6] twopi * phi0_multipass
 //  K_MARKER: (none)
+//  K_QMULT : [0..4] one step as a K_QUAD block: step_len s, b1 s/L, b1 s^2/L, b1/L, t_lim;
+//            [5..8] a_n * r_step/2, [9..12] b_n * r_step/2 (magnetic, n = 0..3, b1 removed);
+//            [13..16] a_n * charge/p0c * s/2, [17..20] -b_n * charge/p0c * s/2 (electric
+//            kick strengths without the particle's 1/beta); [21..24] charge a_n/(n+1),
+//            [25..28] charge b_n/(n+1) (electric edge potential); hdr.n = n_step
 // preceded, when F_OFFSET, by the offset block (hdr.off points at it; then the aperture values,
 // then the parameters):
 //  [0] x_offset, [1] y_offset, [2] z_offset, [3] L/2, [4..12] W^-1 (entrance frame change,
@@ -82,7 +97,7 @@ struct EleHdr {
 // when F_APE_MOMENTUM by 8 momentum-window values:
 //  elliptical : x_center, 1/x_width2, y_center, 1/y_width2
 //  rectangular: x1_lim (= -x1_limit), x2_lim, y1_lim, y2_lim (0 = no limit)
-constexpr int kParams[K_NKIND] = {1, 5, 9, 29, 7, 0};
+constexpr int kParams[K_NKIND] = {1, 5, 9, 29, 7, 0, 29};
 
 struct Species {
     double mc2;      // [eV]
@@ -100,7 +115,7 @@ struct LatDesc {
     const EleHdr *hdr;
     const double *prm;
     int32_t n_ele;
-    int32_t pad;
+    int32_t rare_q;  // nonzero when the table holds K_QMULT elements (read on the host only)
     Species sp;
 };
 

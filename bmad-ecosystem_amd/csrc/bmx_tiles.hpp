@@ -47,12 +47,14 @@ __device__ __forceinline__ void add_work(unsigned long long *work, unsigned long
 }
 
 // The generic turn policy: elements [e0, e1) of the lattice read from its table. The rf
-// cavities sit outside the inner element loop: the loop over the other kinds runs until the
-// next cavity, which is tracked on its own, so the cavity's code does not shape the register
+// cavities (and the other rare elements: offsets, momentum windows, quadrupoles with
+// multipoles) sit outside the inner element loop: the loop over the other kinds runs until the
+// next such element, which is tracked on its own, so its code does not shape the register
 // assignment of the common path. The loops are wave-uniform (the element index stays a
 // scalar, so the table is read by scalar loads); a lost particle's lane is masked off for the
-// rest of the turn.
-template <bool F>
+// rest of the turn. RARE: the map set of the rare path (MAPS_RARE, or MAPS_RARE_Q for a
+// lattice with stepped quadrupoles).
+template <bool F, int RARE = MAPS_RARE>
 __device__ __forceinline__ void track_range(Particle &p, const LatDesc &L, int32_t e0, int32_t e1,
                                             int32_t t_abs)
 {
@@ -65,30 +67,31 @@ __device__ __forceinline__ void track_range(Particle &p, const LatDesc &L, int32
             // keeps it (and the table reads) scalar whatever it makes of the loop's control flow
             e = __builtin_amdgcn_readfirstlane(e);
             const CHdr h = hdr + 4 * e;
-            if (h[0] == K_RF || (h[1] & (F_OFFSET | F_APE_MOMENTUM))) break;
+            if (h[0] == K_RF || (h[1] & (F_OFFSET | F_APE_MOMENTUM | F_RARE_MAP))) break;
             if (p.state == ST_ALIVE)
                 track_element<F, MAPS_NO_RF>(p, load_ele(h), prm, L.sp, true, e + 1, t_abs);
         }
         if (e >= e1) break;
         e = __builtin_amdgcn_readfirstlane(e);
         if (p.state == ST_ALIVE)
-            track_element<F, MAPS_RARE>(p, load_ele(hdr + 4 * e), prm, L.sp, true, e + 1, t_abs);
+            track_element<F, RARE>(p, load_ele(hdr + 4 * e), prm, L.sp, true, e + 1, t_abs);
         ++e;
     }
 }
 
-template <bool F>
+template <bool F, int RARE = MAPS_RARE>
 struct GenericTurn {
+    static constexpr int kRare = RARE;
     __device__ __forceinline__ static void turn(Particle &p, const LatDesc &L, int32_t t_abs)
     {
-        track_range<F>(p, L, 0, L.n_ele, t_abs);
+        track_range<F, RARE>(p, L, 0, L.n_ele, t_abs);
     }
 };
 
 // `count` elements from element `first` (0-based), wrapping past the last element to element 0
 // (beam_mod.f90:159-166, track_bunch's forward loops): the turn of a group that does not start
 // at element 0 or does not pass every element.
-template <bool F>
+template <bool F, int RARE>
 __device__ __forceinline__ void track_span(Particle &p, const LatDesc &L, int32_t first,
                                            int32_t count, int32_t t_abs)
 {
@@ -96,7 +99,7 @@ __device__ __forceinline__ void track_span(Particle &p, const LatDesc &L, int32_
     int32_t e0 = first, left = count;
     while (left > 0) {
         const int32_t e1 = left < L.n_ele - e0 ? e0 + left : L.n_ele;
-        track_range<F>(p, L, e0, e1, t_abs);
+        track_range<F, RARE>(p, L, e0, e1, t_abs);
         left -= e1 - e0;
         e0 = 0;
     }
@@ -150,7 +153,7 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
         // a wave stops once all its particles are lost
         for (int32_t turn = 0; turn < n_turns; ++turn) {
             if (!any_lane(p.state == ST_ALIVE)) break;
-            if constexpr (SPAN) track_span<F>(p, L, first, count, turn0 + turn);
+            if constexpr (SPAN) track_span<F, Turn::kRare>(p, L, first, count, turn0 + turn);
             else Turn::turn(p, L, turn0 + turn);
         }
     }

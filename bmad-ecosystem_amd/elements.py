@@ -66,7 +66,10 @@ class BmxEle(ctypes.Structure):
     """ctypes mirror of ``bmx_ele`` (include/bmx.h)."""
     _fields_ = ([(n, ctypes.c_int32) for n in _INT_FIELDS]
                 + [(n, ctypes.c_double) for n in _DBL_FIELDS]
-                + [("a_pole", ctypes.c_double * N_POLE), ("b_pole", ctypes.c_double * N_POLE)])
+                + [("a_pole", ctypes.c_double * N_POLE), ("b_pole", ctypes.c_double * N_POLE)]
+                + [("r0_elec", ctypes.c_double), ("a_pole_elec", ctypes.c_double * N_POLE),
+                   ("b_pole_elec", ctypes.c_double * N_POLE), ("n_multipole_elec", ctypes.c_int32),
+                   ("reserved", ctypes.c_int32)])
 
 
 ELE_DTYPE = np.dtype(BmxEle)
@@ -210,10 +213,12 @@ _DEFAULT_TILT = {QUADRUPOLE: PI / 4.0, SEXTUPOLE: PI / 6.0, OCTUPOLE: PI / 8.0}
 
 def make_element(key: int, p0c: float, e_tot: float, *, num_steps: int = 0, ds_step: float = 0.0,
                  x_limit: float | None = None, y_limit: float | None = None,
-                 default_tilt: bool = False, a_pole=None, b_pole=None, **attrs) -> BmxEle:
+                 default_tilt: bool = False, a_pole=None, b_pole=None, a_pole_elec=None,
+                 b_pole_elec=None, **attrs) -> BmxEle:
     """One element with Bmad defaults applied, then the user attributes, then bookkeeping.
 
-    ``default_tilt=True`` is a bare ``tilt`` in a lattice file."""
+    ``default_tilt=True`` is a bare ``tilt`` in a lattice file. a_pole / b_pole and
+    a_pole_elec / b_pole_elec are {order: value} dicts of the magnetic and electric multipoles."""
     e = BmxEle()
     e.key = key
     e.is_on = 1
@@ -258,6 +263,12 @@ def make_element(key: int, p0c: float, e_tot: float, *, num_steps: int = 0, ds_s
         e.a_pole[:] = ap
         e.b_pole[:] = bp
         e.n_multipole = N_POLE
+    if a_pole_elec is not None or b_pole_elec is not None:
+        for n, v in dict(a_pole_elec or {}).items():
+            e.a_pole_elec[n] = v
+        for n, v in dict(b_pole_elec or {}).items():
+            e.b_pole_elec[n] = v
+        e.n_multipole_elec = N_POLE
     bookkeep_ds_step(e, num_steps, ds_step)
     return e
 

@@ -71,6 +71,10 @@ type, bind(c), public :: bmx_ele_struct
   real(c_double) :: r0_mag = 0
   real(c_double) :: a_pole(0:bmx_n_pole-1) = 0
   real(c_double) :: b_pole(0:bmx_n_pole-1) = 0
+  real(c_double) :: r0_elec = 0
+  real(c_double) :: a_pole_elec(0:bmx_n_pole-1) = 0
+  real(c_double) :: b_pole_elec(0:bmx_n_pole-1) = 0
+  integer(c_int32_t) :: n_multipole_elec = 0, reserved = 0
 end type
 
 ! bmx_beam_view of bmx.h

@@ -124,6 +124,13 @@ typedef struct bmx_ele {
     double r0_mag;
     double a_pole[BMX_N_POLE];
     double b_pole[BMX_N_POLE];
+    /* Electric multipoles: ele%a_pole_elec / %b_pole_elec (bmad_struct.f90:1483-1484) and
+     * value(r0_elec$); n_multipole_elec = 0 when they are not associated. */
+    double r0_elec;
+    double a_pole_elec[BMX_N_POLE];
+    double b_pole_elec[BMX_N_POLE];
+    int32_t n_multipole_elec;
+    int32_t reserved;
 } bmx_ele;
 
 /*

@@ -335,6 +335,32 @@ static int multipole_ele_to_ab(const bmx_ele *e, int include_kicks, double *a, d
     return ix;
 }
 
+/* multipole_ele_to_ab.f90:31-236 for pole_type = electric$ (non-slave element, use_ele_tilt = F):
+ * ele%a_pole_elec / %b_pole_elec scaled by convert_this_ab's electric branch (:519-530: with
+ * r0_elec = r not 0 or 1, order n is multiplied by r^-n); this_ele_non_multipoles adds nothing
+ * for the elements here (only an elseparator has electric kicks, :389-398). Returns
+ * ix_pole_max. */
+static int multipole_ele_to_ab_elec(const bmx_ele *e, double *a, double *b)
+{
+    for (int n = 0; n < BMX_N_POLE; n++) a[n] = b[n] = 0;
+    if (!e->is_on || e->key == BMX_PIPE || e->n_multipole_elec <= 0) return -1;
+    for (int n = 0; n < BMX_N_POLE; n++) {
+        a[n] = e->a_pole_elec[n];
+        b[n] = e->b_pole_elec[n];
+    }
+    if (max_nonzero(a, b, BMX_N_POLE - 1) < 0) return -1;
+    double radius = e->r0_elec;
+    if (radius != 0 && radius != 1) {
+        double factor = radius;
+        for (int n = 0; n < BMX_N_POLE; n++) {
+            factor = factor / radius;
+            a[n] = factor * a[n];
+            b[n] = factor * b[n];
+        }
+    }
+    return max_nonzero(a, b, BMX_N_POLE - 1);
+}
+
 /* ------------------------------------------------------------------------------------------
  * coordinate transforms
  * ------------------------------------------------------------------------------------------ */
@@ -547,15 +573,24 @@ static void track_a_drift(coord_t *o, double length) { track_a_drift_ex(o, lengt
  * multipole kicks
  * ------------------------------------------------------------------------------------------ */
 
-/* bmad/modules/multipole_mod.f90:314-401 ab_multipole_kick, magnetic, ele_orientation = 1,
- * same species as the reference (f = 1). */
-static void ab_multipole_kick(double a, double b, int n, const coord_t *o, double scale,
-                              double *kx, double *ky)
+/* bmad/modules/multipole_mod.f90:314-401 ab_multipole_kick: magnetic (ele_orientation = 1,
+ * same species as the reference, f = 1) or electric (f = charge / (beta p0c), a2 = a f,
+ * b2 = -b f, :343-346). */
+static void ab_multipole_kick_t(double a, double b, int n, const coord_t *o, double scale,
+                                int electric, double *kx, double *ky)
 {
     *kx = 0;
     *ky = 0;
     if (a == 0 && b == 0) return;
-    double a2 = a * 1.0, b2 = b * 1.0;
+    double a2, b2;
+    if (electric) {
+        double f = o->charge / (o->beta * o->p0c);
+        a2 = a * f;
+        b2 = -b * f;
+    } else {
+        a2 = a * 1.0;
+        b2 = b * 1.0;
+    }
     a2 = scale * a2;
     b2 = scale * b2;
     double x = o->vec[0], y = o->vec[2];
@@ -571,6 +606,12 @@ static void ab_multipole_kick(double a, double b, int n, const coord_t *o, doubl
     }
 }
 
+static void ab_multipole_kick(double a, double b, int n, const coord_t *o, double scale,
+                              double *kx, double *ky)
+{
+    ab_multipole_kick_t(a, b, n, o, scale, 0, kx, ky);
+}
+
 /* multipole_mod.f90:82-174 ab_multipole_kicks, magnetic branch. */
 static void ab_multipole_kicks(const double *an, const double *bn, int ix_pole_max,
                                const bmx_ele *e, coord_t *o, double scale)
@@ -584,6 +625,63 @@ static void ab_multipole_kicks(const double *an, const double *bn, int ix_pole_m
     }
     o->vec[1] = o->vec[1] + kx_tot;
     o->vec[3] = o->vec[3] + ky_tot;
+}
+
+/* multipole_mod.f90:82-174 ab_multipole_kicks, electric branch (:128-143): the kicks change the
+ * particle's energy, pz += (1+pz) sqrt_one(alpha) with alpha = (kx (2 px + kx) + ky (2 py + ky))
+ * / (1+pz)^2 (lost_pz if alpha < -1), beta from the new momentum, z scaled by beta_new/beta_old.
+ * scale is the longitudinal length of the field region. */
+static void ab_multipole_kicks_elec(const double *an, const double *bn, int ix_pole_max,
+                                    const bmx_ele *e, coord_t *o, double scale)
+{
+    coord_t o0 = *o;
+    double kx_tot = 0, ky_tot = 0, kx, ky;
+    double r = e->p0c / o->p0c;
+    for (int n = 0; n <= ix_pole_max; n++) {
+        ab_multipole_kick_t(r * an[n], r * bn[n], n, o, scale, 1, &kx, &ky);
+        kx_tot = kx_tot + kx;
+        ky_tot = ky_tot + ky;
+    }
+    o->vec[1] = o->vec[1] + kx_tot;
+    o->vec[3] = o->vec[3] + ky_tot;
+    double alpha = (kx_tot * (2 * o0.vec[1] + kx_tot) + ky_tot * (2 * o0.vec[3] + ky_tot)) /
+                   sq(1 + o0.vec[5]);
+    if (alpha < -1) {
+        o->state = BMX_STATE_LOST_PZ;
+        return;
+    }
+    double dpz = (1 + o0.vec[5]) * bmo_sqrt_one(alpha);
+    o->vec[5] = o0.vec[5] + dpz;
+    o->beta = (1 + o->vec[5]) / sqrt(sq(1 + o->vec[5]) + sq(o->mc2 / o->p0c));
+    o->vec[4] = o0.vec[4] * o->beta / o0.beta;
+}
+
+static void apply_energy_kick(double dE, coord_t *o);
+
+/* apply_element_edge_kick.f90:283-326 electric_longitudinal_fringe, bmad_standard field, not a
+ * bend with exact multipoles: the particle crosses the hard edge of the electric multipole
+ * field and its energy changes by the potential there, phi = -sum_n Re((b_n - i a_n)
+ * (x + i y)^(n+1)) / (n+1), applied as apply_energy_kick(-ff phi) with ff = +charge at the
+ * entrance (at_sign = +1) and -charge at the exit. Complex products written out as Fortran
+ * evaluates them ((a+ib)(c+id) = (ac - bd) + i(ad + bc)). */
+static void electric_longitudinal_fringe(coord_t *o, const bmx_ele *e, const double *a_elec,
+                                         const double *b_elec, int ix_elec_max, double at_sign)
+{
+    if (e->l == 0) return;
+    if (ix_elec_max < 0) return;
+    double ff = at_sign * o->charge;
+    double xr = 1, xi = 0;                 /* xiy */
+    double cr = o->vec[0], ci = o->vec[2]; /* c_vec */
+    double phi = 0;
+    for (int i = 0; i <= ix_elec_max; i++) {
+        double nr = xr * cr - xi * ci, ni = xr * ci + xi * cr;
+        xr = nr;
+        xi = ni;
+        if (a_elec[i] == 0 && b_elec[i] == 0) continue;
+        double abr = b_elec[i], abi = -a_elec[i];
+        phi = phi - (abr * xr - abi * xi) / (i + 1);
+    }
+    apply_energy_kick(-ff * phi, o);
 }
 
 /* ------------------------------------------------------------------------------------------
@@ -632,19 +730,20 @@ static double low_energy_z_correction(const coord_t *o, const bmx_ele *e, double
     return ds * (o->beta - beta0) / beta0;
 }
 
-/* bmad/low_level/track_a_quadrupole.f90:17-148 (magnetic multipoles allowed, no fringe, no
- * electric multipoles). */
+/* bmad/low_level/track_a_quadrupole.f90:17-148 (magnetic and electric multipoles, no fringe). */
 static int track_a_quadrupole(coord_t *o, const bmx_ele *e)
 {
     const double charge_dir = 1.0;
     double length = 1 * e->l;
     double an[BMX_N_POLE], bn[BMX_N_POLE], b1;
+    double an_elec[BMX_N_POLE], bn_elec[BMX_N_POLE];
     int err;
     int ix_mag_max = multipole_ele_to_ab(e, 1, an, bn, &b1, &err);
     if (err) return err;
+    int ix_elec_max = multipole_ele_to_ab_elec(e, an_elec, bn_elec);
     if (e->fringe_type != BMX_FRINGE_NONE && e->fringe_type != 0) return BMX_ERR_UNSUPPORTED;
     int n_step = 1;
-    if (ix_mag_max > -1) {
+    if (ix_mag_max > -1 || ix_elec_max > -1) {
         long ns = f_nint(fabs(length) / e->ds_step);
         n_step = ns > 1 ? (int)ns : 1;
     }
@@ -653,7 +752,16 @@ static int track_a_quadrupole(coord_t *o, const bmx_ele *e)
     if (length == 0) n_step = 0;
 
     if (is_misaligned(e) || e->tilt != 0) offset_particle(e, 1, o, 1, 0, 0, 0);
+    /* init_fringe_info.f90:44-54: electric multipoles give the element an edge kick even with
+     * fringe_type = none; apply_element_edge_kick.f90:165-167,270-271 applies the electric
+     * longitudinal fringe after the (absent) magnetic edge at the entrance and before it at
+     * the exit. */
+    if (ix_elec_max > -1) {
+        electric_longitudinal_fringe(o, e, an_elec, bn_elec, ix_elec_max, 1.0);
+        if (o->state != BMX_STATE_ALIVE) return 0;
+    }
     if (ix_mag_max > -1) ab_multipole_kicks(an, bn, ix_mag_max, e, o, r_step / 2);
+    if (ix_elec_max > -1) ab_multipole_kicks_elec(an_elec, bn_elec, ix_elec_max, e, o, step_len / 2);
 
     for (int i = 1; i <= n_step; i++) {
         double rel_p = 1 + o->vec[5];
@@ -672,6 +780,13 @@ static int track_a_quadrupole(coord_t *o, const bmx_ele *e)
         v[4] = v[4] + low_energy_z_correction(o, e, step_len);
         if (ix_mag_max > -1)
             ab_multipole_kicks(an, bn, ix_mag_max, e, o, i == n_step ? r_step / 2 : r_step);
+        if (ix_elec_max > -1)
+            ab_multipole_kicks_elec(an_elec, bn_elec, ix_elec_max, e, o,
+                                    i == n_step ? step_len / 2 : step_len);
+    }
+    if (ix_elec_max > -1) {
+        electric_longitudinal_fringe(o, e, an_elec, bn_elec, ix_elec_max, -1.0);
+        if (o->state != BMX_STATE_ALIVE) return 0;
     }
     if (is_misaligned(e) || e->tilt != 0) offset_particle(e, 0, o, 1, 0, 0, 0);
     return 0;

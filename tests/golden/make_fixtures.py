@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 TRACKING_ELEMENTS = ["DRIFT1", "ECOLLIMATOR1", "HKICKER1", "INSTRUMENT1", "KICKER1", "MONITOR1",
                      "OCTUPOLE1", "RCOLLIMATOR1", "RFCAVITY1", "RFCAVITY3", "SEXTUPOLE1",
-                     "THICK_MULTIPOLE1", "VKICKER1"]
+                     "THICK_MULTIPOLE1", "VKICKER1", "QUADRUPOLE1", "QUADRUPOLE2", "QUADRUPOLE3",
+                     "QUADRUPOLE4", "QUADRUPOLE5", "SBEND1", "SBEND3", "SBEND5", "SBEND7"]
 
 
 def grab(rel, keep):

@@ -17,6 +17,9 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # particle_start of tracking_method_test.bmad:9-14, e_tot of l1 (:141)
 TM_START = (0.001, 0.002, 0.003, 0.004, 0.005, 0.006)
 TM_E_TOT = 1e6
+# the lattice names no particle, so the reference species is Bmad's default, the positron
+# (charge +1); only the electric multipole kicks and edges depend on its sign
+TM_CHARGE = 1
 
 
 def read_fixture(name):
@@ -52,7 +55,7 @@ def tracking_method_elements(E):
     et = TM_E_TOT
     mk = E.make_element
     off = dict(x_offset=0.05, y_pitch=0.001)
-    return {
+    out = {
         "DRIFT1": mk(E.DRIFT, p0c, et, l=4.5, num_steps=10, tilt=0.1, **off),
         "ECOLLIMATOR1": mk(E.ECOLLIMATOR, p0c, et, l=4.5, x_limit=0.2, y_limit=0.1, num_steps=10,
                            x_offset=0.02, y_pitch=0.001, tilt=0.1, hkick=0.01, vkick=0.002),
@@ -75,6 +78,20 @@ def tracking_method_elements(E):
                                b_pole={3: 20.0}, x_offset=0.02, tilt=0.1),
         "VKICKER1": mk(E.VKICKER, p0c, et, l=4.5, kick=0.005, num_steps=10, tilt=0.1, **off),
     }
+    # q0: quadrupole, a0 = 0.01, a1 = 0.1, vkick = -0.0001, b0_elec = -2e3, a1_elec = 2e4
+    # (tracking_method_test.bmad:62), then quadrupole1-3 (:63-65)
+    q0 = dict(a_pole={0: 0.01, 1: 0.1}, vkick=-0.0001, b_pole_elec={0: -2e3},
+              a_pole_elec={1: 2e4}, l=0.6, default_tilt=True, num_steps=10, **off)
+    out.update({
+        "QUADRUPOLE1": mk(E.QUADRUPOLE, p0c, et, k1=1.0, **q0),
+        "QUADRUPOLE2": mk(E.QUADRUPOLE, p0c, et, k1=-1.0, **q0),
+        "QUADRUPOLE3": mk(E.QUADRUPOLE, p0c, et, k1=1e-20, **q0),
+    })
+    return out
+
+
+# tracking_method_test lines in the fixture whose elements are not restated yet
+NOT_YET = ("QUADRUPOLE4", "QUADRUPOLE5", "SBEND1", "SBEND3", "SBEND5", "SBEND7")
 
 
 def tm_start_beta(E):

@@ -16,8 +16,10 @@ OFF_PATH = {"RCOLLIMATOR1", "RFCAVITY3"}
 def test_tracking_method_known_answers_on_gpu(bmx, maps):
     """tracking_method_test "<ELE>: Bmad_Standard" lines (ABS 2e-14, written es18.10): drift,
     thick multipoles (sextupole, octupole, thick_multipole, kickers, collimator, instrument,
-    monitor) with offsets, pitch and tilt, and the rf cavity, each tracked by the GPU kernel
-    from the fixture's particle_start as a one-element lattice."""
+    monitor) with offsets, pitch and tilt, the rf cavity, and quadrupoles with magnetic and
+    electric multipoles (stepped, electric edge energy kicks), each tracked by the GPU kernel
+    from the fixture's particle_start as a one-element lattice (the lattice's default species,
+    the positron)."""
     E, T = bmx.elements, bmx.tracking
     fx = kat.read_fixture("tracking_method_test.correct")
     eles = kat.tracking_method_elements(E)
@@ -33,7 +35,7 @@ def test_tracking_method_known_answers_on_gpu(bmx, maps):
         beam["state"] = np.ones(1, dtype=np.int32)
         beam["loss_ele"] = np.zeros(1, dtype=np.int32)
         beam["loss_turn"] = np.zeros(1, dtype=np.int32)
-        out = T.track_beam_host(E.to_array([ele]), beam, 1, maps=maps)
+        out = T.track_beam_host(E.to_array([ele]), beam, 1, maps=maps, charge=kat.TM_CHARGE)
         assert out["state"][0] == E.STATE_ALIVE, name
         vec = [float(out[k][0]) for k in P.COORDS]
         ok, worst = kat.abs_match(vec, expect[:6], tol, 10)

@@ -30,7 +30,8 @@ def test_ele_struct_layout_matches_header(bmx, tmp_path):
     """sizeof/offsetof of bmx_ele seen by a C compiler equal the ctypes mirror."""
     E = bmx.elements
     c = tmp_path / "lay.c"
-    fields = ["l", "ds_step", "p0c", "a_pole", "b_pole", "pz_aperture_center", "r0_mag"]
+    fields = ["l", "ds_step", "p0c", "a_pole", "b_pole", "pz_aperture_center", "r0_mag",
+              "r0_elec", "a_pole_elec", "b_pole_elec", "n_multipole_elec"]
     body = "\n".join(f'printf("%zu\\n", offsetof(bmx_ele, {f}));' for f in fields)
     c.write_text(f'#include <stdio.h>\n#include <stddef.h>\n#include "{ROOT}/include/bmx.h"\n'
                  f'int main(void){{printf("%zu\\n", sizeof(bmx_ele));{body}return 0;}}\n')
@@ -76,6 +77,29 @@ def test_outside_path_rejected(bmx, attrs, code):
     rc, _, msg = _validate(bmx, [e])
     assert rc == code, msg
     assert "element 1" in msg
+
+
+def test_electric_multipoles_only_on_quadrupoles(bmx):
+    """Electric multipoles compile on quadrupoles (the stepped map) and are rejected on every
+    other element kind and above order 3; a switched-off element ignores them."""
+    E = bmx.elements
+    p0c, et = bmx.lattices.reference()
+    for key, attrs in ((E.SEXTUPOLE, dict(l=0.2, k2=1.0)), (E.DRIFT, dict(l=1.0)),
+                       (E.RFCAVITY, dict(l=0.5, voltage=1e6, rf_frequency=5e8)),
+                       (E.SBEND, dict(l=1.0, g=0.01))):
+        rc, _, msg = _validate(bmx, [E.make_element(key, p0c, et, b_pole_elec={0: 1e5}, **attrs)])
+        assert rc == 2 and "electric" in msg, (key, msg)
+    off = E.make_element(E.SEXTUPOLE, p0c, et, l=0.2, k2=1.0, b_pole_elec={0: 1e5}, is_on=0)
+    assert _validate(bmx, [off])[0] == 0
+    rc, _, msg = _validate(bmx, [E.make_element(E.QUADRUPOLE, p0c, et, l=0.3, k1=0.5,
+                                                a_pole_elec={4: 1e5})])
+    assert rc == 2 and "above 3" in msg, msg
+    # a stepped quadrupole: 29 values (plus the 24-value offset block when misaligned)
+    for attrs, n in ((dict(b_pole_elec={0: 1e5}), 29), (dict(a_pole={1: 1e-3}), 29),
+                     (dict(vkick=1e-5, x_offset=1e-3), 29 + 24)):
+        rc, nprm, msg = _validate(bmx, [E.make_element(E.QUADRUPOLE, p0c, et, l=0.3, k1=0.5,
+                                                       num_steps=3, **attrs)])
+        assert rc == 0 and nprm == n, (attrs, nprm, msg)
 
 
 def test_p0c_mismatch_rejected(bmx):

@@ -7,7 +7,9 @@ import kat
 def test_tracking_method_known_answers(bmx, oracle):
     """tracking_method_test "<ELE>: Bmad_Standard" lines (ABS 2e-14, written es18.10):
     drift, thick multipoles (sextupole, octupole, thick_multipole, kickers, collimators,
-    instrument, monitor) with offsets/pitch/tilt, and the rf cavity (thick and zero length)."""
+    instrument, monitor) with offsets/pitch/tilt, the rf cavity (thick and zero length), and
+    quadrupoles with magnetic and electric multipoles (stepped body, electric kicks that change
+    the energy, electric hard-edge energy kicks), tracked as a positron."""
     E = bmx.elements
     fx = kat.read_fixture("tracking_method_test.correct")
     eles = kat.tracking_method_elements(E)
@@ -17,13 +19,16 @@ def test_tracking_method_known_answers(bmx, oracle):
         label = f"{name}: Bmad_Standard"
         typ, tol, expect = fx[label]
         assert typ == "ABS"
-        vec, beta, state = oracle.track1(ele, kat.TM_START, beta0)
+        vec, beta, state = oracle.track1(ele, kat.TM_START, beta0, charge=kat.TM_CHARGE)
         assert state == E.STATE_ALIVE, name
         ok, worst = kat.abs_match(vec, expect[:6], tol, 10)
         if not ok:
             failures.append((name, worst, [kat.fortran_es(v, 10) for v in vec], expect[:6]))
     assert not failures, failures
-    assert len(eles) == len(fx)
+    # the fixture's other lines: elements whose restatement needs what this path does not
+    # carry yet (quadrupole fringe, k1 and multipoles in bends, the bend branch of
+    # offset_particle)
+    assert set(f"{n}: Bmad_Standard" for n in eles) | set(f"{n}: Bmad_Standard" for n in kat.NOT_YET) == set(fx)
 
 
 def test_fringe_known_answers(bmx, oracle):
