@@ -411,9 +411,22 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             if (ix <= 1) ix = max_order(a, b, 2);
             double ae[BMX_N_POLE], be[BMX_N_POLE];
             const int ixe = ele_to_ab_elec(e, ae, be);
-            if (e.fringe_type != BMX_FRINGE_NONE && e.fringe_type != 0) { err = where(i, e) + "quadrupole fringe"; return BMX_ERR_UNSUPPORTED; }
+            const int ft = e.fringe_type == 0 ? BMX_FRINGE_NONE : e.fringe_type;
+            if (ft != BMX_FRINGE_NONE && ft != BMX_FRINGE_SOFT_EDGE_ONLY &&
+                ft != BMX_FRINGE_HARD_EDGE_ONLY && ft != BMX_FRINGE_FULL) {
+                err = where(i, e) + "quadrupole fringe_type";
+                return BMX_ERR_UNSUPPORTED;
+            }
             if (ix > 3 || ixe > 3) { err = where(i, e) + "quadrupole multipole orders above 3"; return BMX_ERR_UNSUPPORTED; }
-            if (ix == -1 && ixe == -1) {
+            // edge fields (init_fringe_info.f90:44-54), at the ends fringe_at names
+            const bool hard = ft == BMX_FRINGE_HARD_EDGE_ONLY || ft == BMX_FRINGE_FULL;
+            const bool soft = (ft == BMX_FRINGE_SOFT_EDGE_ONLY || ft == BMX_FRINGE_FULL) &&
+                              (e.fq1 != 0 || e.fq2 != 0) && e.k1 != 0;
+            const bool has_fringe = (ft != BMX_FRINGE_NONE && e.fringe_at != BMX_NO_END) || ixe > -1;
+            const bool edge_ent = has_fringe && (e.fringe_at == BMX_BOTH_ENDS || e.fringe_at == BMX_ENTRANCE_END);
+            const bool edge_exit = has_fringe && (e.fringe_at == BMX_BOTH_ENDS || e.fringe_at == BMX_EXIT_END);
+            const bool edges = (edge_ent || edge_exit) && (hard || soft || ixe > -1);
+            if (ix == -1 && ixe == -1 && !edges) {
                 if (e.l == 0) {
                     h.kind = K_MARKER;
                     break;
@@ -425,13 +438,13 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
                 h.n = nterms;
                 break;
             }
-            // multipoles or kicks beside k1, or electric multipoles: stepped
-            // (track_a_quadrupole.f90:46-53,70-130)
-            if (e.l == 0) { err = where(i, e) + "zero-length quadrupole with multipoles"; return BMX_ERR_UNSUPPORTED; }
+            // multipoles or kicks beside k1, electric multipoles, or edge fields: stepped
+            // (track_a_quadrupole.f90:46-53,70-130; one step when only the edges are present)
+            if (e.l == 0) { err = where(i, e) + "zero-length quadrupole with multipoles or edges"; return BMX_ERR_UNSUPPORTED; }
             h.kind = K_QMULT;
             h.flags |= F_RARE_MAP;
             out.rare_q = true;
-            long ns = f_nint(std::fabs(e.l) / e.ds_step);
+            long ns = (ix > -1 || ixe > -1) ? f_nint(std::fabs(e.l) / e.ds_step) : 1;
             const int n_step = ns > 1 ? (int)ns : 1;
             const double r_step = 1.0 / n_step;
             const double step_len = e.l * r_step;
@@ -455,8 +468,17 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             if (ixe >= 0)
                 for (int n = 0; n < 4; n++)
                     if (ae[n] != 0 || be[n] != 0) emask |= 1 << n;
+            // hard_multipole_edge_kick's cab = k1/(4 (n+2) rel_p) for n = 1; the soft edge's
+            // f1 = k1 fq1/rel_p, f2 = k1 fq2/rel_p (fringe_mod.f90:600-620,760-790)
+            q.push_back(e.k1 / 12.0);
+            q.push_back(e.k1 * e.fq1);
+            q.push_back(e.k1 * e.fq2);
             h.n = n_step;
             h.flags |= (mmask << F_ORDER_SHIFT) | (emask << F_ELEC_SHIFT);
+            if (edge_ent) h.flags |= F_QEDGE_ENT;
+            if (edge_exit) h.flags |= F_QEDGE_EXIT;
+            if (hard) h.flags |= F_QHARD;
+            if (soft) h.flags |= F_QSOFT;
             break;
         }
         case BMX_SBEND: {

@@ -1238,20 +1238,68 @@ __device__ __forceinline__ bool elec_fringe(Particle &p, CP fa, CP fb, int32_t m
     return true;
 }
 
-// bmad/low_level/track_a_quadrupole.f90:17-148 with multipoles beside k1 (no fringe): the
-// electric edge at the entrance (init_fringe_info.f90:44-54 gives an element with electric
-// multipoles an edge kick whatever its fringe_type; apply_element_edge_kick.f90:165-167,270-271
-// orders it after the magnetic edge on entry and before it on exit), half kicks, then n_step
-// quadrupole steps with k1 = b1/(L (1+pz)) taken from the momentum at each step, each followed
-// by a full kick (half after the last), then the exit edge. Returns false when an edge loses
-// the particle: the reference then returns before offset_particle's exit transform.
+// bmad/modules/fringe_mod.f90:692-865 hard_multipole_edge_kick of a quadrupole (n = 1, bp = k1,
+// ap = 0) in closed form: with c = -+k1/(12 (1+pz)) (minus at the entrance), the complex
+// polynomials of the reference reduce to fx = c x (x^2 + 3 y^2), fy = -c y (3 x^2 + y^2),
+// dfx/dx = -dfy/dy = 3 c (x^2 + y^2), dfx/dy = -dfy/dx = 6 c x y; then x -= fx, y -= fy, the
+// momenta from the inverse Jacobian and z += (px fx + py fy)/(1+pz) (:823-830). h12 = k1/12.
+// The division by 1+pz is kept (not the stored reciprocal) so that a particle an electric edge
+// has just taken to pz = -1 ends as the reference's does.
+__device__ __forceinline__ void quad_hard_edge(Particle &p, double h12, bool entering)
+{
+    const double rel_p = 1 + p.pz;
+    const double c = (entering ? -h12 : h12) / rel_p;
+    const double x = p.x, y = p.y, px = p.px, py = p.py;
+    const double xx = x * x, yy = y * y;
+    const double fx = c * x * (xx + 3 * yy);
+    const double fy = -c * y * (3 * xx + yy);
+    const double d = 3 * c * (xx + yy);          // dfx/dx = -dfy/dy
+    const double e = 6 * c * x * y;              // dfx/dy = -dfy/dx
+    const double denom = (1 - d) * (1 + d) + e * e;
+    p.x = x - fx;
+    p.px = px + ((1.0 + d - denom) * px - e * py) / denom;
+    p.y = y - fy;
+    p.py = py + (e * px + (1.0 - d - denom) * py) / denom;
+    p.z = p.z + (p.px * fx + p.py * fy) / rel_p;
+}
+
+// bmad/modules/fringe_mod.f90:556-661 soft_quadrupole_edge_kick (time_dir = +1): f1 = k1 fq1/(1+pz)
+// (negated at the exit), f2 = k1 fq2/(1+pz); kf1 = k1 fq1, kf2 = k1 fq2.
+__device__ __forceinline__ void quad_soft_edge(Particle &p, double kf1, double kf2, bool entering)
+{
+    const double rel_p = 1 + p.pz;
+    double f1 = kf1 / rel_p;
+    const double f2 = kf2 / rel_p;
+    if (!entering) f1 = -f1;
+    const double ef1 = exp(f1);
+    const double x = p.x, px = p.px, y = p.y, py = p.py;
+    const double vx = px / rel_p, vy = py / rel_p;
+    p.z = p.z - (f1 * x + f2 * (1 + f1 / 2) * vx / ef1) * vx + (f1 * y + f2 * (1 - f1 / 2) * vy * ef1) * vy;
+    p.x = x * ef1 + vx * f2;
+    p.px = px / ef1;
+    p.y = y / ef1 - vy * f2;
+    p.py = py * ef1;
+}
+
+// bmad/low_level/track_a_quadrupole.f90:17-148 with multipoles beside k1 and edge fields: the
+// entrance edge (hard multipole edge, soft quadrupole edge, then the electric edge:
+// init_fringe_info.f90:44-54 gives an element with electric multipoles an edge kick whatever
+// its fringe_type; apply_element_edge_kick.f90:165-178,270-271 orders the electric edge after
+// the magnetic ones on entry and before them on exit), half kicks, then n_step quadrupole steps
+// with k1 = b1/(L (1+pz)) taken from the momentum at each step, each followed by a full kick
+// (half after the last), then the exit edge. Returns false when an edge loses the particle:
+// the reference then returns before offset_particle's exit transform.
 template <bool F>
 __device__ __forceinline__ bool quad_multipole(Particle &p, CP q, int32_t n_step, int32_t flags,
                                                const Species &sp)
 {
     const int32_t mmask = (flags >> F_ORDER_SHIFT) & 0xF;
     const int32_t emask = (flags >> F_ELEC_SHIFT) & 0xF;
-    if (emask && !elec_fringe(p, q + 21, q + 25, emask, 1.0, sp)) return false;
+    if (flags & F_QEDGE_ENT) {
+        if (flags & F_QHARD) quad_hard_edge(p, q[29], true);
+        if (flags & F_QSOFT) quad_soft_edge(p, q[30], q[31], true);
+        if (emask && !elec_fringe(p, q + 21, q + 25, emask, 1.0, sp)) return false;
+    }
     if (mmask) multipole_kick(p, q + 5, q + 9, mmask, 1.0);
     if (emask) elec_kick(p, q + 13, q + 17, emask, 1.0, sp);
     for (int i = 1; i <= n_step; i++) {
@@ -1260,7 +1308,12 @@ __device__ __forceinline__ bool quad_multipole(Particle &p, CP q, int32_t n_step
         if (mmask) multipole_kick(p, q + 5, q + 9, mmask, f);
         if (emask) elec_kick(p, q + 13, q + 17, emask, f, sp);
     }
-    if (emask && !elec_fringe(p, q + 21, q + 25, emask, -1.0, sp)) return false;
+    if (flags & F_QEDGE_EXIT) {
+        const bool kept = !emask || elec_fringe(p, q + 21, q + 25, emask, -1.0, sp);
+        if (flags & F_QSOFT) quad_soft_edge(p, q[30], q[31], false);
+        if (flags & F_QHARD) quad_hard_edge(p, q[29], false);
+        if (!kept) return false;
+    }
     return true;
 }
 

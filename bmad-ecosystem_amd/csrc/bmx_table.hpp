@@ -61,6 +61,15 @@ enum Flags : int32_t {
     // An element whose map only the rare element path carries (K_QMULT): the loops send it
     // there as they do cavities and elements with offsets or momentum windows.
     F_RARE_MAP = 1 << 25,
+    // K_QMULT edges (apply_element_edge_kick.f90:107-110,165-178,270-271): F_QEDGE_ENT /
+    // F_QEDGE_EXIT: the element has edge fields (init_fringe_info.f90:44-54) and fringe_at puts
+    // them at that end; F_QHARD: the hard multipole edge (fringe_type hard_edge_only or full);
+    // F_QSOFT: the soft quadrupole edge (soft_edge_only or full). The electric edge comes with
+    // F_QEDGE_* whenever electric orders are present.
+    F_QEDGE_ENT = 1 << 26,
+    F_QEDGE_EXIT = 1 << 27,
+    F_QHARD = 1 << 28,
+    F_QSOFT = 1 << 29,
 };
 constexpr int kOffsetLen = 24;
 
@@ -88,7 +97,8 @@ struct EleHdr {
 //            [5..8] a_n * r_step/2, [9..12] b_n * r_step/2 (magnetic, n = 0..3, b1 removed);
 //            [13..16] a_n * charge/p0c * s/2, [17..20] -b_n * charge/p0c * s/2 (electric
 //            kick strengths without the particle's 1/beta); [21..24] charge a_n/(n+1),
-//            [25..28] charge b_n/(n+1) (electric edge potential); hdr.n = n_step
+//            [25..28] charge b_n/(n+1) (electric edge potential); [29] k1/12 (hard edge,
+//            n = 1), [30] k1 fq1, [31] k1 fq2 (soft edge); hdr.n = n_step
 // preceded, when F_OFFSET, by the offset block (hdr.off points at it; then the aperture values,
 // then the parameters):
 //  [0] x_offset, [1] y_offset, [2] z_offset, [3] L/2, [4..12] W^-1 (entrance frame change,
@@ -97,7 +107,7 @@ struct EleHdr {
 // when F_APE_MOMENTUM by 8 momentum-window values:
 //  elliptical : x_center, 1/x_width2, y_center, 1/y_width2
 //  rectangular: x1_lim (= -x1_limit), x2_lim, y1_lim, y2_lim (0 = no limit)
-constexpr int kParams[K_NKIND] = {1, 5, 9, 29, 7, 0, 29};
+constexpr int kParams[K_NKIND] = {1, 5, 9, 29, 7, 0, 32};
 
 struct Species {
     double mc2;      // [eV]

@@ -69,7 +69,8 @@ class BmxEle(ctypes.Structure):
                 + [("a_pole", ctypes.c_double * N_POLE), ("b_pole", ctypes.c_double * N_POLE)]
                 + [("r0_elec", ctypes.c_double), ("a_pole_elec", ctypes.c_double * N_POLE),
                    ("b_pole_elec", ctypes.c_double * N_POLE), ("n_multipole_elec", ctypes.c_int32),
-                   ("reserved", ctypes.c_int32)])
+                   ("exact_multipoles", ctypes.c_int32), ("fq1", ctypes.c_double),
+                   ("fq2", ctypes.c_double)])
 
 
 ELE_DTYPE = np.dtype(BmxEle)

@@ -74,7 +74,8 @@ type, bind(c), public :: bmx_ele_struct
   real(c_double) :: r0_elec = 0
   real(c_double) :: a_pole_elec(0:bmx_n_pole-1) = 0
   real(c_double) :: b_pole_elec(0:bmx_n_pole-1) = 0
-  integer(c_int32_t) :: n_multipole_elec = 0, reserved = 0
+  integer(c_int32_t) :: n_multipole_elec = 0, exact_multipoles = 0
+  real(c_double) :: fq1 = 0, fq2 = 0
 end type
 
 ! bmx_beam_view of bmx.h

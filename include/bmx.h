@@ -130,7 +130,11 @@ typedef struct bmx_ele {
     double a_pole_elec[BMX_N_POLE];
     double b_pole_elec[BMX_N_POLE];
     int32_t n_multipole_elec;
-    int32_t reserved;
+    /* value(exact_multipoles$) of a bend: 1 off (0 is taken as off), 2 horizontally_pure,
+     * 3 vertically_pure (bmad_struct.f90:256-261) */
+    int32_t exact_multipoles;
+    /* value(fq1$), value(fq2$): soft quadrupole edge coefficients (fringe_mod.f90:556-661) */
+    double fq1, fq2;
 } bmx_ele;
 
 /*

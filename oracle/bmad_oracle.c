@@ -730,7 +730,122 @@ static double low_energy_z_correction(const coord_t *o, const bmx_ele *e, double
     return ds * (o->beta - beta0) / beta0;
 }
 
-/* bmad/low_level/track_a_quadrupole.f90:17-148 (magnetic and electric multipoles, no fringe). */
+/* bmad/modules/fringe_mod.f90:692-865 hard_multipole_edge_kick for a quadrupole (n_max = 1,
+ * ap = 0, bp = k1; orbit only, no derivatives of second order, which only the matrix uses),
+ * fringe_type hard_edge_only or full (:715-722), charge_dir = 1. Complex products are written
+ * out in the order Fortran evaluates them, left to right. */
+static void hard_multipole_edge_kick(coord_t *o, const bmx_ele *e, int particle_at)
+{
+    if (e->fringe_type != BMX_FRINGE_HARD_EDGE_ONLY && e->fringe_type != BMX_FRINGE_FULL) return;
+    const double x = o->vec[0], y = o->vec[2];
+    const double rel_p = 1 + o->vec[5];
+    double fx = 0, dfx_dx = 0, dfx_dy = 0, fy = 0, dfy_dx = 0, dfy_dy = 0;
+    /* poly_n1 = 1, poly = xy, then per n: poly_n1 = poly, poly = poly * xy */
+    double pr = x, pi = y;           /* poly */
+    double n1r = 1, n1i = 0;         /* poly_n1 */
+    const int n_max = 1;
+    for (int n = 1; n <= n_max; n++) {
+        n1r = pr;
+        n1i = pi;
+        double t_r = pr * x - pi * y, t_i = pr * y + pi * x;
+        pr = t_r;
+        pi = t_i;
+        const double ap = 0, bp = e->k1;
+        /* dpoly_dx = (n+1) poly_n1, dpoly_dy = i dpoly_dx */
+        const double dxr = (n + 1) * n1r, dxi = (n + 1) * n1i;
+        const double dyr = -dxi, dyi = dxr;
+        /* cab = charge_dir * cmplx(bp, ap) / (4 (n+2) rel_p), negated at the first edge */
+        const double den = 4 * (n + 2) * rel_p;
+        double cr = 1.0 * bp / den, ci = 1.0 * ap / den;
+        if (particle_at == FIRST_TRACK_EDGE) {
+            cr = -cr;
+            ci = -ci;
+        }
+        const double cn = (double)(n + 3) / (n + 1);
+        /* x part: xny = cmplx(x, -cn y), dxny_dy = cmplx(0, -cn) */
+        double xr = x, xi = -cn * y;
+        double ur, ui, vr, vi;
+        /* fx += Re(cab * poly * xny) */
+        ur = cr * pr - ci * pi;
+        ui = cr * pi + ci * pr;
+        fx = fx + (ur * xr - ui * xi);
+        /* dfx_dx += Re(cab * (dpoly_dx * xny + poly)) */
+        vr = dxr * xr - dxi * xi + pr;
+        vi = dxr * xi + dxi * xr + pi;
+        dfx_dx = dfx_dx + (cr * vr - ci * vi);
+        /* dfx_dy += Re(cab * (dpoly_dy * xny + poly * dxny_dy)) */
+        vr = (dyr * xr - dyi * xi) + (pr * 0.0 - pi * (-cn));
+        vi = (dyr * xi + dyi * xr) + (pr * (-cn) + pi * 0.0);
+        dfx_dy = dfx_dy + (cr * vr - ci * vi);
+        /* y part: xny = cmplx(y, cn x), dxny_dx = cmplx(0, cn) */
+        xr = y;
+        xi = cn * x;
+        fy = fy + (ur * xr - ui * xi);
+        /* dfy_dx += Re(cab * (dpoly_dx * xny + poly * dxny_dx)) */
+        vr = (dxr * xr - dxi * xi) + (pr * 0.0 - pi * cn);
+        vi = (dxr * xi + dxi * xr) + (pr * cn + pi * 0.0);
+        dfy_dx = dfy_dx + (cr * vr - ci * vi);
+        /* dfy_dy += Re(cab * (dpoly_dy * xny + poly)) */
+        vr = dyr * xr - dyi * xi + pr;
+        vi = dyr * xi + dyi * xr + pi;
+        dfy_dy = dfy_dy + (cr * vr - ci * vi);
+    }
+    const double px = o->vec[1], py = o->vec[3];
+    const double denom = (1 - dfx_dx) * (1 - dfy_dy) - dfx_dy * dfy_dx;
+    o->vec[0] = o->vec[0] - fx;
+    o->vec[1] = px + ((1.0 - dfy_dy - denom) * px + dfy_dx * py) / denom;
+    o->vec[2] = o->vec[2] - fy;
+    o->vec[3] = py + (dfx_dy * px + (1.0 - dfx_dx - denom) * py) / denom;
+    o->vec[4] = o->vec[4] + (o->vec[1] * fx + o->vec[3] * fy) / rel_p;
+}
+
+/* bmad/modules/fringe_mod.f90:556-661 soft_quadrupole_edge_kick for a quadrupole, fringe_type
+ * soft_edge_only or full, time_dir = +1, charge_dir = 1 (orbit only). */
+static void soft_quadrupole_edge_kick(coord_t *o, const bmx_ele *e, int particle_at)
+{
+    if (e->fringe_type != BMX_FRINGE_SOFT_EDGE_ONLY && e->fringe_type != BMX_FRINGE_FULL) return;
+    const double rel_p = 1 + o->vec[5];
+    const double k1_rel = 1.0 * e->k1 / rel_p;
+    double f1 = k1_rel * e->fq1;
+    const double f2 = k1_rel * e->fq2;
+    if (f1 == 0 && f2 == 0) return;
+    if (particle_at == SECOND_TRACK_EDGE) f1 = -f1;
+    const double ef1 = exp(f1);
+    const double v0 = o->vec[0], v1 = o->vec[1], v2 = o->vec[2], v3 = o->vec[3];
+    const double vx = v1 / rel_p, vy = v3 / rel_p;
+    o->vec[4] = o->vec[4] - (f1 * v0 + f2 * (1 + f1 / 2) * vx / ef1) * vx +
+                (f1 * v2 + f2 * (1 - f1 / 2) * vy * ef1) * vy;
+    o->vec[0] = v0 * ef1 + vx * f2;
+    o->vec[1] = v1 / ef1;
+    o->vec[2] = v2 / ef1 - vy * f2;
+    o->vec[3] = v3 * ef1;
+}
+
+/* apply_element_edge_kick.f90:64-178,270-271 for a quadrupole with a bmad_standard field: at an
+ * end where fringe_at puts a fringe (at_this_ele_end), the electric longitudinal fringe after
+ * the magnetic edges at the entrance (hard multipole edge, then the soft quadrupole edge) and
+ * before them at the exit (soft, then hard). */
+static void quadrupole_edge(coord_t *o, const bmx_ele *e, int particle_at, const double *an_elec,
+                            const double *bn_elec, int ix_elec_max)
+{
+    const int physical_end = particle_at == FIRST_TRACK_EDGE ? BMX_ENTRANCE_END : BMX_EXIT_END;
+    if (!(e->fringe_at == BMX_BOTH_ENDS || e->fringe_at == physical_end)) return;
+    if (particle_at == FIRST_TRACK_EDGE) {
+        hard_multipole_edge_kick(o, e, particle_at);
+        soft_quadrupole_edge_kick(o, e, particle_at);
+        electric_longitudinal_fringe(o, e, an_elec, bn_elec, ix_elec_max, 1.0);
+    } else {
+        electric_longitudinal_fringe(o, e, an_elec, bn_elec, ix_elec_max, -1.0);
+        soft_quadrupole_edge_kick(o, e, particle_at);
+        hard_multipole_edge_kick(o, e, particle_at);
+    }
+}
+
+/* bmad/low_level/track_a_quadrupole.f90:17-148 (magnetic and electric multipoles, hard and
+ * soft edges). init_fringe_info.f90:44-54: the edges are tracked when fringe_type is not none
+ * and fringe_at is not no_end, or when the element has electric multipoles (whatever its
+ * fringe_type); the track returns at a lost particle after either edge (:64-66,136-137),
+ * before offset_particle's exit transform. */
 static int track_a_quadrupole(coord_t *o, const bmx_ele *e)
 {
     const double charge_dir = 1.0;
@@ -741,7 +856,11 @@ static int track_a_quadrupole(coord_t *o, const bmx_ele *e)
     int ix_mag_max = multipole_ele_to_ab(e, 1, an, bn, &b1, &err);
     if (err) return err;
     int ix_elec_max = multipole_ele_to_ab_elec(e, an_elec, bn_elec);
-    if (e->fringe_type != BMX_FRINGE_NONE && e->fringe_type != 0) return BMX_ERR_UNSUPPORTED;
+    const int ft = e->fringe_type == 0 ? BMX_FRINGE_NONE : e->fringe_type;
+    if (ft != BMX_FRINGE_NONE && ft != BMX_FRINGE_SOFT_EDGE_ONLY && ft != BMX_FRINGE_HARD_EDGE_ONLY &&
+        ft != BMX_FRINGE_FULL)
+        return BMX_ERR_UNSUPPORTED;
+    const int has_fringe = ((ft != BMX_FRINGE_NONE && e->fringe_at != BMX_NO_END) || ix_elec_max > -1);
     int n_step = 1;
     if (ix_mag_max > -1 || ix_elec_max > -1) {
         long ns = f_nint(fabs(length) / e->ds_step);
@@ -752,12 +871,8 @@ static int track_a_quadrupole(coord_t *o, const bmx_ele *e)
     if (length == 0) n_step = 0;
 
     if (is_misaligned(e) || e->tilt != 0) offset_particle(e, 1, o, 1, 0, 0, 0);
-    /* init_fringe_info.f90:44-54: electric multipoles give the element an edge kick even with
-     * fringe_type = none; apply_element_edge_kick.f90:165-167,270-271 applies the electric
-     * longitudinal fringe after the (absent) magnetic edge at the entrance and before it at
-     * the exit. */
-    if (ix_elec_max > -1) {
-        electric_longitudinal_fringe(o, e, an_elec, bn_elec, ix_elec_max, 1.0);
+    if (has_fringe) {
+        quadrupole_edge(o, e, FIRST_TRACK_EDGE, an_elec, bn_elec, ix_elec_max);
         if (o->state != BMX_STATE_ALIVE) return 0;
     }
     if (ix_mag_max > -1) ab_multipole_kicks(an, bn, ix_mag_max, e, o, r_step / 2);
@@ -784,8 +899,8 @@ static int track_a_quadrupole(coord_t *o, const bmx_ele *e)
             ab_multipole_kicks_elec(an_elec, bn_elec, ix_elec_max, e, o,
                                     i == n_step ? step_len / 2 : step_len);
     }
-    if (ix_elec_max > -1) {
-        electric_longitudinal_fringe(o, e, an_elec, bn_elec, ix_elec_max, -1.0);
+    if (has_fringe) {
+        quadrupole_edge(o, e, SECOND_TRACK_EDGE, an_elec, bn_elec, ix_elec_max);
         if (o->state != BMX_STATE_ALIVE) return 0;
     }
     if (is_misaligned(e) || e->tilt != 0) offset_particle(e, 0, o, 1, 0, 0, 0);

@@ -92,6 +92,7 @@ inline cdouble cos(const cdouble &a) { return fn(::cos(a.v), a); }
 inline cdouble asin(const cdouble &a) { return fn(::asin(a.v), a); }
 inline cdouble sinh(const cdouble &a) { return fn(::sinh(a.v), a); }
 inline cdouble cosh(const cdouble &a) { return fn(::cosh(a.v), a); }
+inline cdouble exp(const cdouble &a) { return fn(::exp(a.v), a); }
 inline cdouble fabs(const cdouble &a) { return cdouble(::fabs(a.v), a.dyn); }
 inline cdouble atan2(const cdouble &a, const cdouble &b) { return op(::atan2(a.v, b.v), a, b); }
 inline cdouble fmod(const cdouble &a, const cdouble &b) { return op(::fmod(a.v, b.v), a, b); }

@@ -79,19 +79,21 @@ def tracking_method_elements(E):
         "VKICKER1": mk(E.VKICKER, p0c, et, l=4.5, kick=0.005, num_steps=10, tilt=0.1, **off),
     }
     # q0: quadrupole, a0 = 0.01, a1 = 0.1, vkick = -0.0001, b0_elec = -2e3, a1_elec = 2e4
-    # (tracking_method_test.bmad:62), then quadrupole1-3 (:63-65)
+    # (tracking_method_test.bmad:62), then quadrupole1-5 (:63-67; 4 and 5 with fringe_type = full)
     q0 = dict(a_pole={0: 0.01, 1: 0.1}, vkick=-0.0001, b_pole_elec={0: -2e3},
               a_pole_elec={1: 2e4}, l=0.6, default_tilt=True, num_steps=10, **off)
     out.update({
         "QUADRUPOLE1": mk(E.QUADRUPOLE, p0c, et, k1=1.0, **q0),
         "QUADRUPOLE2": mk(E.QUADRUPOLE, p0c, et, k1=-1.0, **q0),
         "QUADRUPOLE3": mk(E.QUADRUPOLE, p0c, et, k1=1e-20, **q0),
+        "QUADRUPOLE4": mk(E.QUADRUPOLE, p0c, et, k1=1.0, fringe_type=E.FRINGE_FULL, **q0),
+        "QUADRUPOLE5": mk(E.QUADRUPOLE, p0c, et, k1=-1.0, fringe_type=E.FRINGE_FULL, **q0),
     })
     return out
 
 
 # tracking_method_test lines in the fixture whose elements are not restated yet
-NOT_YET = ("QUADRUPOLE4", "QUADRUPOLE5", "SBEND1", "SBEND3", "SBEND5", "SBEND7")
+NOT_YET = ("SBEND1", "SBEND3", "SBEND5", "SBEND7")
 
 
 def tm_start_beta(E):

@@ -1,7 +1,8 @@
-"""Quadrupoles with multipoles beside k1 and with electric multipoles on the GPU (the stepped
-quadrupole map: bmad/low_level/track_a_quadrupole.f90:17-148 with n_step > 1, the electric
-kicks of multipole_mod.f90:128-143 and the electric edge energy kicks of
-apply_element_edge_kick.f90:283-326), against the CPU restatement (needs an MI355X).
+"""Quadrupoles with multipoles beside k1, with electric multipoles and with edge fields on the
+GPU (the stepped quadrupole map: bmad/low_level/track_a_quadrupole.f90:17-148 with n_step > 1,
+the electric kicks of multipole_mod.f90:128-143, the electric edge energy kicks of
+apply_element_edge_kick.f90:283-326, and the hard multipole and soft quadrupole edges of
+fringe_mod.f90:556-865), against the CPU restatement (needs an MI355X).
 
 The tracked species is the electron (charge -1), the opposite sign of the tracking_method_test
 known answers (positron), so both signs of the electric strengths' charge factor are run."""
@@ -43,10 +44,25 @@ def _variants(bmx):
         # k1 = 0 with electric fields: the steps are drifts between the kicks
         "electric_k1_zero": mk(E.QUADRUPOLE, p0c, et, l=0.3, k1=0.0, b_pole_elec={1: 5e8},
                                a_pole_elec={0: 1e6}, num_steps=3),
+        # the hard multipole edge alone (fringe_type = full, fq1 = fq2 = 0), a strong quadrupole
+        "hard_edge": mk(E.QUADRUPOLE, p0c, et, l=0.4, k1=4.0, fringe_type=E.FRINGE_FULL),
+        # the soft quadrupole edge alone
+        "soft_edge": mk(E.QUADRUPOLE, p0c, et, l=0.4, k1=-3.0, fq1=0.02, fq2=0.01,
+                        fringe_type=E.FRINGE_SOFT_EDGE_ONLY),
+        # both edges, multipoles, electric fields and offsets
+        "full_edges_mixed": mk(E.QUADRUPOLE, p0c, et, l=0.5, k1=2.5, fq1=-0.01, fq2=0.03,
+                               fringe_type=E.FRINGE_FULL, a_pole={1: 0.02}, b_pole_elec={0: 1e6},
+                               a_pole_elec={1: 2e8}, num_steps=4, x_offset=2e-4,
+                               default_tilt=True),
+        # edges at the exit only (the entrance keeps no edge field, electric ones included)
+        "exit_edge_only": mk(E.QUADRUPOLE, p0c, et, l=0.5, k1=1.5, fq1=0.02,
+                             fringe_type=E.FRINGE_FULL, fringe_at=E.EXIT_END,
+                             b_pole_elec={0: 1e6}, num_steps=2),
     }
 
 
-@pytest.mark.parametrize("name", ["magnetic", "electric", "mixed_offset", "electric_k1_zero"])
+@pytest.mark.parametrize("name", ["magnetic", "electric", "mixed_offset", "electric_k1_zero",
+                                  "hard_edge", "soft_edge", "full_edges_mixed", "exit_edge_only"])
 def test_stepped_quadrupole_single_element(bmx, oracle, ctx, name):
     """One element, 4096 particles at 10x the base beam sizes and 1e-2 momentum spread: losses
     identical and survivors within 1e-12 of the restatement (the per-element bar)."""
@@ -56,7 +72,7 @@ def test_stepped_quadrupole_single_element(bmx, oracle, ctx, name):
     ref = P.copy_beam(beam)
     oracle.track_bunch(arr, ref, n_turns=1)
     moved = np.abs(ref["pz"] - beam["pz"])
-    if name != "magnetic":
+    if ele.n_multipole_elec:
         assert moved.max() > 1e-11, moved.max()  # the electric fields change the energy
     else:
         assert moved.max() == 0

@@ -31,7 +31,8 @@ def test_ele_struct_layout_matches_header(bmx, tmp_path):
     E = bmx.elements
     c = tmp_path / "lay.c"
     fields = ["l", "ds_step", "p0c", "a_pole", "b_pole", "pz_aperture_center", "r0_mag",
-              "r0_elec", "a_pole_elec", "b_pole_elec", "n_multipole_elec"]
+              "r0_elec", "a_pole_elec", "b_pole_elec", "n_multipole_elec", "exact_multipoles",
+              "fq1", "fq2"]
     body = "\n".join(f'printf("%zu\\n", offsetof(bmx_ele, {f}));' for f in fields)
     c.write_text(f'#include <stdio.h>\n#include <stddef.h>\n#include "{ROOT}/include/bmx.h"\n'
                  f'int main(void){{printf("%zu\\n", sizeof(bmx_ele));{body}return 0;}}\n')
@@ -94,12 +95,19 @@ def test_electric_multipoles_only_on_quadrupoles(bmx):
     rc, _, msg = _validate(bmx, [E.make_element(E.QUADRUPOLE, p0c, et, l=0.3, k1=0.5,
                                                 a_pole_elec={4: 1e5})])
     assert rc == 2 and "above 3" in msg, msg
-    # a stepped quadrupole: 29 values (plus the 24-value offset block when misaligned)
-    for attrs, n in ((dict(b_pole_elec={0: 1e5}), 29), (dict(a_pole={1: 1e-3}), 29),
-                     (dict(vkick=1e-5, x_offset=1e-3), 29 + 24)):
+    # a stepped quadrupole: 32 values (plus the 24-value offset block when misaligned); edge
+    # fields alone also take the stepped map (one step)
+    for attrs, n in ((dict(b_pole_elec={0: 1e5}), 32), (dict(a_pole={1: 1e-3}), 32),
+                     (dict(vkick=1e-5, x_offset=1e-3), 32 + 24),
+                     (dict(fringe_type=E.FRINGE_FULL), 32),
+                     (dict(fringe_type=E.FRINGE_SOFT_EDGE_ONLY, fq1=0.1), 32),
+                     (dict(fringe_type=E.FRINGE_FULL, fringe_at=E.NO_END), 5)):
         rc, nprm, msg = _validate(bmx, [E.make_element(E.QUADRUPOLE, p0c, et, l=0.3, k1=0.5,
                                                        num_steps=3, **attrs)])
         assert rc == 0 and nprm == n, (attrs, nprm, msg)
+    rc, _, msg = _validate(bmx, [E.make_element(E.QUADRUPOLE, p0c, et, l=0.0, k1=0.5,
+                                                fringe_type=E.FRINGE_FULL)])
+    assert rc == 2 and "zero-length" in msg, msg
 
 
 def test_p0c_mismatch_rejected(bmx):

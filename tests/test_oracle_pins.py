@@ -138,3 +138,70 @@ def test_r0_mag_one_is_unscaled(bmx, oracle):
     v0 = np.array([2e-3, -1e-4, 1e-3, 2e-4, 1e-3, 1e-3])
     assert np.array_equal(oracle.track1(a, v0, 0.99999, p0c=p0c)[0],
                           oracle.track1(b, v0, 0.99999, p0c=p0c)[0])
+
+
+# ---- soft quadrupole edge (bmad/modules/fringe_mod.f90:556-661) ----------------------------
+# No reference fixture has fq1 or fq2 set. Pins: with fq2 = 0 the exit kick (f1 -> -f1)
+# undoes the entrance kick exactly (x e^f1 e^-f1, and the z terms -f1 x vx + f1 y vy cancel);
+# the entrance-exit pair of a zero-length quadrupole is symplectic in (x, px, y, py, z, pz)
+# (a dropped or mis-signed z term breaks it); and it reduces to the closed form of the f2
+# terms alone when fq1 = 0 (x += vx f2 at each edge, y -= vy f2, z -= f2 (vx^2 - vy^2)).
+
+def _soft_quad(bmx, fq1, fq2, k1=2.0):
+    E = bmx.elements
+    p0c, et = bmx.lattices.reference()
+    return E.make_element(E.QUADRUPOLE, p0c, et, l=0.0, k1=k1, fq1=fq1, fq2=fq2,
+                          fringe_type=E.FRINGE_SOFT_EDGE_ONLY), p0c
+
+
+def test_soft_quadrupole_edges_cancel_without_fq2(bmx, oracle):
+    ele, p0c = _soft_quad(bmx, 0.3, 0.0)
+    v0 = np.array([1e-3, 2e-4, -3e-3, 5e-4, 1e-3, 2e-3])
+    v, _, st = oracle.track1(ele, v0, 0.99999, p0c=p0c)
+    assert st == 1
+    assert np.allclose(v, v0, rtol=0, atol=1e-18), v - v0
+    # and a nonzero f1 does act on its own: the entrance half alone is not the identity
+    ele.fringe_at = bmx.elements.ENTRANCE_END
+    v1, _, _ = oracle.track1(ele, v0, 0.99999, p0c=p0c)
+    f1 = 2.0 / (1 + v0[5]) * 0.3
+    assert v1[0] == pytest.approx(v0[0] * math.exp(f1), rel=1e-15)
+    assert v1[3] == pytest.approx(v0[3] * math.exp(f1), rel=1e-15)
+
+
+def test_soft_quadrupole_f2_closed_form(bmx, oracle):
+    ele, p0c = _soft_quad(bmx, 0.0, 0.05)
+    v0 = np.array([1e-3, 2e-4, -3e-3, 5e-4, 1e-3, 2e-3])
+    v, _, _ = oracle.track1(ele, v0, 0.99999, p0c=p0c)
+    rp = 1 + v0[5]
+    vx, vy = v0[1] / rp, v0[3] / rp
+    f2 = 2.0 / rp * 0.05
+    assert v[0] == pytest.approx(v0[0] + 2 * vx * f2, rel=1e-14)
+    assert v[2] == pytest.approx(v0[2] - 2 * vy * f2, rel=1e-14)
+    assert v[1] == v0[1] and v[3] == v0[3]
+    assert v[4] == pytest.approx(v0[4] - 2 * f2 * (vx * vx - vy * vy), rel=1e-12)
+
+
+def test_soft_quadrupole_edges_symplectic(bmx, oracle):
+    ele, p0c = _soft_quad(bmx, 0.4, 0.07)
+    v0 = np.array([2e-3, -1e-3, 1e-3, 2e-3, 0.0, 1e-2])
+    h = 1e-7
+    J = np.zeros((6, 6))
+    for j in range(6):
+        a, b = v0.copy(), v0.copy()
+        a[j] += h
+        b[j] -= h
+        va, _, _ = oracle.track1(ele, a, 0.99999, p0c=p0c)
+        vb, _, _ = oracle.track1(ele, b, 0.99999, p0c=p0c)
+        J[:, j] = (va - vb) / (2 * h)
+    S = np.zeros((6, 6))
+    for i in range(3):
+        S[2 * i, 2 * i + 1] = 1
+        S[2 * i + 1, 2 * i] = -1
+    err = np.abs(J.T @ S @ J - S).max()
+    assert err < 1e-8, err
+    # the test has teeth: the same map with its z change negated (z_out -> 2 z_in - z_out) is
+    # not symplectic
+    Jf = J.copy()
+    Jf[4, :] = -J[4, :]
+    Jf[4, 4] += 2
+    assert np.abs(Jf.T @ S @ Jf - S).max() > 1e-3
