@@ -43,6 +43,8 @@ KEY_NAMES = {
     THICK_MULTIPOLE: "thick_multipole",
 }
 FRINGE_NONE, FRINGE_SOFT_EDGE_ONLY, FRINGE_HARD_EDGE_ONLY, FRINGE_FULL, FRINGE_BASIC_BEND = 1, 2, 3, 4, 7
+# exact_multipoles (bmad_struct.f90:256-261)
+EXACT_OFF, EXACT_HORIZONTALLY_PURE, EXACT_VERTICALLY_PURE = 1, 2, 3
 ENTRANCE_END, EXIT_END, BOTH_ENDS, NO_END, CONTINUOUS = 1, 2, 3, 4, 5
 APERTURE_RECTANGULAR, APERTURE_ELLIPTICAL = 2, 3
 

@@ -413,6 +413,86 @@ static void matvec3(double w[3][3], const double *r, double *out)
     for (int i = 0; i < 3; i++) out[i] = t[i];
 }
 
+/* sim_utils/math/rotation_3d_mod.f90:737-835 rotate_vec and rotate_mat (left multiplication by
+ * the rotation about axis 0 = x, 1 = y, 2 = z); nothing for angle 0. */
+static void rotate_vec(double *v, int axis, double angle)
+{
+    if (angle == 0) return;
+    double ca = cos(angle), sa = sin(angle), a0, a1;
+    switch (axis) {
+    case 0: a0 = ca * v[1] - sa * v[2]; a1 = sa * v[1] + ca * v[2]; v[1] = a0; v[2] = a1; break;
+    case 1: a0 = sa * v[2] + ca * v[0]; a1 = ca * v[2] - sa * v[0]; v[0] = a0; v[2] = a1; break;
+    default: a0 = ca * v[0] - sa * v[1]; a1 = sa * v[0] + ca * v[1]; v[0] = a0; v[1] = a1; break;
+    }
+}
+
+static void rotate_mat(double m[3][3], int axis, double angle)
+{
+    if (angle == 0) return;
+    for (int j = 0; j < 3; j++) {
+        double col[3] = {m[0][j], m[1][j], m[2][j]};
+        rotate_vec(col, axis, angle);
+        m[0][j] = col[0];
+        m[1][j] = col[1];
+        m[2][j] = col[2];
+    }
+}
+
+/* A point and frame (floor_position_struct's r and w). */
+typedef struct {
+    double r[3];
+    double w[3][3];
+} bmo_pos_t;
+
+static void mat3_mul(double a[3][3], double b[3][3], double out[3][3])
+{
+    double t[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) t[i][j] = a[i][0] * b[0][j] + a[i][1] * b[1][j] + a[i][2] * b[2][j];
+    memcpy(out, t, sizeof t);
+}
+
+/* bmad/geometry/bend_shift.f90:19-61: the position and frame a distance delta_s further along a
+ * bend of curvature g (with the bend's ref_tilt when has_tilt). */
+static void bend_shift(bmo_pos_t *p, double g, double delta_s, int has_tilt, double ref_tilt)
+{
+    double angle = delta_s * g;
+    if (angle == 0) {
+        p->r[2] = p->r[2] - delta_s;
+        return;
+    }
+    double tlt = has_tilt ? ref_tilt : 0.0;
+    double S[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    if (tlt != 0) {
+        rotate_mat(S, 2, -ref_tilt);
+        rotate_mat(S, 1, angle);
+        rotate_mat(S, 2, ref_tilt);
+    } else {
+        rotate_mat(S, 1, angle);
+    }
+    double cos_one = -2 * sq(sin(angle / 2));
+    double L[3] = {cos_one / g, 0.0 / g, -sin(angle) / g};
+    if (has_tilt) rotate_vec(L, 2, ref_tilt);
+    double r[3];
+    matvec3(S, p->r, r);
+    for (int i = 0; i < 3; i++) p->r[i] = r[i] + L[i];
+    mat3_mul(S, p->w, p->w);
+}
+
+/* bmad/geometry/w_mat_for_tilt.f90:14-38 (not inverse): rotation about z by tilt. */
+static void w_mat_for_tilt(double tilt, double w[3][3])
+{
+    double c = cos(tilt), s = sin(tilt);
+    double m[3][3] = {{c, -s, 0.0}, {s, c, 0.0}, {0.0, 0.0, 1.0}};
+    memcpy(w, m, sizeof m);
+}
+
+static void pos_apply(bmo_pos_t *p, double w[3][3])
+{
+    matvec3(w, p->r, p->r);
+    mat3_mul(w, p->w, p->w);
+}
+
 static void track_a_drift_ex(coord_t *o, double length, int include_ref_motion);
 
 /* bmad/code/offset_particle.f90:66-492, non-bend branch (lines 183-335 / 339-492), for an
@@ -450,9 +530,48 @@ static void offset_particle(const bmx_ele *e, int set, coord_t *o, int set_tilt,
     int misaligned = has_orientation_attributes(e->key) && is_misaligned(e);
     double w[3][3], r[3], ds;
 
+    const int bend_frame = e->key == BMX_SBEND && (e->g != 0 || e->ref_tilt != 0);
     if (set) {
         double s_body;
-        if (misaligned) {
+        if (misaligned && bend_frame) {
+            /* offset_particle.f90:206-232: along the bend to its centre, the misalignment
+             * (ele_misalignment_L_S_calc.f90:16-50 with roll = 0: L = offsets, S =
+             * floor_angles_to_w_mat(x_pitch, y_pitch, 0)), the ref_tilt rotation about the
+             * bend's entrance, back to the entrance face */
+            bmo_pos_t ps = {{o->vec[0], o->vec[2], 0.0}, {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+            const double g = e->g, rt = e->ref_tilt;
+            bend_shift(&ps, g, 1 * ds_center, 1, rt);
+            double Sm[3][3], St[3][3];
+            floor_angles_to_w_mat(e->x_pitch, e->y_pitch, 0.0, Sm, 0);
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) St[i][j] = Sm[j][i];
+            double d[3] = {ps.r[0] - e->x_offset, ps.r[1] - e->y_offset, ps.r[2] - e->z_offset};
+            matvec3(St, d, ps.r);
+            mat3_mul(St, ps.w, ps.w);
+            if (rt != 0) {
+                bend_shift(&ps, g, -L_half, 1, rt);
+                double wt[3][3];
+                w_mat_for_tilt(-rt, wt);
+                pos_apply(&ps, wt);
+                bend_shift(&ps, g, L_half, 0, 0.0);
+            }
+            if (drift_to != 0) {
+                bend_shift(&ps, g, -sign_z_vel * L_half, 0, 0.0);
+                ps.r[2] = ps.r[2] + (1 - sign_z_vel) * L_half;
+            }
+            s_body = ps.r[2];
+            double pz = sq(rel_p) - sq(o->vec[1]) - sq(o->vec[3]);
+            if (pz <= 0) {
+                o->state = BMX_STATE_LOST_PZ;
+            } else {
+                double p0[3] = {o->vec[1], o->vec[3], sign_z_vel * sqrt(pz)}, pv[3];
+                matvec3(ps.w, p0, pv);
+                o->vec[1] = pv[0];
+                o->vec[3] = pv[1];
+                o->vec[0] = ps.r[0];
+                o->vec[2] = ps.r[1];
+            }
+        } else if (misaligned) {
             r[0] = o->vec[0] - e->x_offset;
             r[1] = o->vec[2] - e->y_offset;
             r[2] = 0.0 - (e->z_offset + 1 * ds_center);
@@ -472,6 +591,7 @@ static void offset_particle(const bmx_ele *e, int set, coord_t *o, int set_tilt,
                 o->vec[2] = r[1];
             }
         } else {
+            if (e->key == BMX_SBEND && e->ref_tilt != 0) tilt_coords(e->ref_tilt, o->vec);
             s_body = s_pos0;
         }
         ds = s_target - s_body;
@@ -505,7 +625,43 @@ static void offset_particle(const bmx_ele *e, int set, coord_t *o, int set_tilt,
             o->vec[3] = o->vec[3] + 0.5 * charge_dir * e->vkick;
         }
         double s_lab;
-        if (misaligned) {
+        if (misaligned && bend_frame) {
+            /* offset_particle.f90:404-430, the reverse of the set branch */
+            bmo_pos_t ps = {{o->vec[0], o->vec[2], 0.0}, {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+            const double g = e->g, rt = e->ref_tilt;
+            bend_shift(&ps, g, ds_center, 0, 0.0);
+            if (rt != 0) {
+                bend_shift(&ps, g, -L_half, 0, 0.0);
+                double wt[3][3];
+                w_mat_for_tilt(rt, wt);
+                pos_apply(&ps, wt);
+                bend_shift(&ps, g, L_half, 1, rt);
+            }
+            double Sm[3][3];
+            floor_angles_to_w_mat(e->x_pitch, e->y_pitch, 0.0, Sm, 0);
+            double rr[3];
+            matvec3(Sm, ps.r, rr);
+            ps.r[0] = rr[0] + e->x_offset;
+            ps.r[1] = rr[1] + e->y_offset;
+            ps.r[2] = rr[2] + e->z_offset;
+            mat3_mul(Sm, ps.w, ps.w);
+            if (drift_to != 0) {
+                bend_shift(&ps, g, sign_z_vel * L_half, 1, rt);
+                ps.r[2] = ps.r[2] + (1 + sign_z_vel) * L_half;
+            }
+            s_lab = ps.r[2];
+            double pz = sq(rel_p) - sq(o->vec[1]) - sq(o->vec[3]);
+            if (pz <= 0) {
+                o->state = BMX_STATE_LOST_PZ;
+            } else {
+                double p0[3] = {o->vec[1], o->vec[3], sign_z_vel * sqrt(pz)}, pv[3];
+                matvec3(ps.w, p0, pv);
+                o->vec[1] = pv[0];
+                o->vec[3] = pv[1];
+                o->vec[0] = ps.r[0];
+                o->vec[2] = ps.r[1];
+            }
+        } else if (misaligned) {
             r[0] = o->vec[0];
             r[1] = o->vec[2];
             r[2] = 0.0 - ds_center;
@@ -527,6 +683,7 @@ static void offset_particle(const bmx_ele *e, int set, coord_t *o, int set_tilt,
                 o->vec[2] = r[1];
             }
         } else {
+            if (e->key == BMX_SBEND && e->ref_tilt != 0) tilt_coords(-e->ref_tilt, o->vec);
             s_lab = s_pos0;
         }
         ds = s_target - s_lab;
@@ -730,13 +887,15 @@ static double low_energy_z_correction(const coord_t *o, const bmx_ele *e, double
     return ds * (o->beta - beta0) / beta0;
 }
 
-/* bmad/modules/fringe_mod.f90:692-865 hard_multipole_edge_kick for a quadrupole (n_max = 1,
- * ap = 0, bp = k1; orbit only, no derivatives of second order, which only the matrix uses),
- * fringe_type hard_edge_only or full (:715-722), charge_dir = 1. Complex products are written
+/* bmad/modules/fringe_mod.f90:692-865 hard_multipole_edge_kick for a quadrupole or a bend
+ * (n_max = 1, ap = 0, bp = k1; orbit only, no derivatives of second order, which only the
+ * matrix uses), fringe_type hard_edge_only (quadrupole) or full (:715-722), charge_dir = 1. Complex products are written
  * out in the order Fortran evaluates them, left to right. */
 static void hard_multipole_edge_kick(coord_t *o, const bmx_ele *e, int particle_at)
 {
     if (e->fringe_type != BMX_FRINGE_HARD_EDGE_ONLY && e->fringe_type != BMX_FRINGE_FULL) return;
+    /* a bend's hard edge with k1 is in hwang_bend_edge_kick (:716-717) */
+    if (e->key == BMX_SBEND && e->fringe_type == BMX_FRINGE_HARD_EDGE_ONLY) return;
     const double x = o->vec[0], y = o->vec[2];
     const double rel_p = 1 + o->vec[5];
     double fx = 0, dfx_dx = 0, dfx_dy = 0, fy = 0, dfy_dx = 0, dfy_dy = 0;
@@ -1019,45 +1178,414 @@ void bmo_hwang_bend_edge_kick(const bmx_ele *e, int32_t charge, int32_t first_ed
     vec[4] = v[4] + 1 * dz;
 }
 
-/* fringe_here (bmad/code/fringe_here.f90:15-35) and the sbend branch of
- * apply_element_edge_kick (bmad/low_level/apply_element_edge_kick.f90:107-110,188-189) ->
- * bend_edge_kick (fringe_mod.f90:40-128). hard_multipole_edge_kick is a no-op unless
- * fringe_type = full (fringe_mod.f90:715-722). */
+/* bmad/modules/fringe_mod.f90:1713-1771 bmad_rot_xz (orbit only): the frame rotated by a
+ * about the y axis, no field. Returns nonzero on err_flag. */
+static int bmad_rot_xz(double a, double *vec)
+{
+    double x = vec[0], px = vec[1], py = vec[3], delta = vec[5];
+    double pp = 1 + delta;
+    double ps = sq(pp) - sq(px) - sq(py);
+    if (ps <= 0) return 1;
+    ps = sqrt(ps);
+    double ca = cos(a), sa = sin(a);
+    double den = ps * ca - px * sa;
+    double L = x * sa / den;
+    vec[0] = x * ps / den;
+    vec[1] = px * ca + ps * sa;
+    vec[2] = vec[2] + py * L;
+    vec[4] = vec[4] - pp * L;
+    return 0;
+}
+
+/* bmad/modules/fringe_mod.f90:1588-1690 bmad_wedger (orbit only): a wedge of angle a of the
+ * dipole field g_tot (bmad_rot_xz when g_tot = 0). Returns nonzero on err_flag. */
+static int bmad_wedger(double a, double g_tot, double *vec)
+{
+    double b1 = g_tot;
+    if (b1 == 0) return bmad_rot_xz(a, vec);
+    double x = vec[0], px = vec[1], py = vec[3], delta = vec[5];
+    double pp = 1 + delta;
+    double ps = sq(pp) - sq(px) - sq(py);
+    if (ps <= 0) return 1;
+    ps = sqrt(ps);
+    double pt = sq(pp) - sq(py);
+    if (pt < 1e-10) return 1;
+    pt = sqrt(pt);
+    double ca = cos(a), sa = sin(a), s2a = sin(2.0 * a), s2 = sq(sa);
+    double pxn = px * ca + (ps - b1 * x) * sa;
+    double psn = sq(pt) - sq(pxn);
+    if (psn < 1e-10) return 1;
+    psn = sqrt(psn);
+    double den = psn + ps * ca - px * sa;
+    double num = x * px * s2a + s2 * (2 * x * ps - b1 * sq(x));
+    double L = (a + asin(px / pt) - asin(pxn / pt)) / b1;
+    vec[0] = x * ca + num / den;
+    vec[1] = pxn;
+    vec[2] = vec[2] + py * L;
+    vec[4] = vec[4] - pp * L;
+    return 0;
+}
+
+/* bmad/modules/fringe_mod.f90:1797-1933 bmad_fringe_dipoler (orbit only): the hard-edge dipole
+ * fringe with fint, hgap in the frame of the edge. Returns nonzero on err_flag. */
+static int bmad_fringe_dipoler(double *vec, double g_tot, double beta0, double fint, double hgap,
+                               int particle_at)
+{
+    double b = particle_at == SECOND_TRACK_EDGE ? -g_tot : g_tot;
+    double x = vec[0], px = vec[1], y = vec[2], py = vec[3], z = vec[4], delta = vec[5];
+    double pp = 1 + delta;
+    double ps = sq(pp) - sq(px) - sq(py);
+    if (ps <= 0) return 1;
+    ps = sqrt(ps);
+    double ee = sqrt(sq(pp) + 1 / sq(beta0) - 1);
+    double beta = pp / ee;
+    double xp = px / ps, yp = py / ps;
+    double kap = b * fint * hgap;
+    double fac2 = 1 + sq(yp);
+    double d[3][3];
+    d[0][0] = (1 + sq(xp)) / ps;  d[1][0] = xp * yp / ps;  d[2][0] = -xp;
+    d[0][1] = xp * yp / ps;       d[1][1] = fac2 / ps;     d[2][1] = -yp;
+    d[0][2] = -ee * xp / sq(ps);  d[1][2] = -ee * yp / sq(ps); d[2][2] = ee / ps;
+    double fi0 = atan(xp / fac2) - 2 * kap * (1 + sq(xp) * (2 + sq(yp))) * ps;
+    double co2 = b / sq(cos(fi0));
+    double co1 = co2 / (1 + sq(xp / fac2));
+    double fi[3];
+    fi[0] = co1 / fac2 - 4 * kap * co2 * xp * (2 + sq(yp)) * ps;
+    fi[1] = -2 * co1 * xp * yp / sq(fac2) - 4 * kap * co2 * sq(xp) * yp * ps;
+    fi[2] = -2 * kap * co2 * (1 + sq(xp) * (2 + sq(yp)));
+    double fi0t = b * tan(fi0);
+    double bb2 = fi[0] * d[0][1] + fi[1] * d[1][1] + fi[2] * d[2][1];
+    double sqv = 1 - 2 * bb2 * y;
+    if (sqv < 0) return 1;
+    sqv = sqrt(sqv);
+    double ynew = 2 * y / (1 + sqv);
+    double bb1 = fi[0] * d[0][0] + fi[1] * d[1][0] + fi[2] * d[2][0];
+    double bb3 = fi[0] * d[0][2] + fi[1] * d[1][2] + fi[2] * d[2][2];
+    vec[0] = x + 0.5 * bb1 * sq(ynew);
+    vec[2] = ynew;
+    vec[3] = py - fi0t * ynew;
+    vec[4] = z + 0.5 * beta * bb3 * sq(ynew);
+    return 0;
+}
+
+/* bmad/modules/fringe_mod.f90:1474-1562 exact_bend_edge_kick (orbit only, time_dir = +1):
+ * at the entrance the wedge of the edge angle without field, the fringe kick, then the wedge
+ * back through the field; at the exit the reverse. g_tot here is g + the element's dg
+ * attribute (0 when switched off). lost_pz, coordinates unchanged, on any err_flag. */
+static void exact_bend_edge_kick(coord_t *o, const bmx_ele *e, int particle_at)
+{
+    double beta0 = e->p0c / e->e_tot;
+    double g_tot = e->is_on ? e->g + e->dg : 0;
+    double edge_angle, fint, hgap;
+    if (particle_at == FIRST_TRACK_EDGE) {
+        edge_angle = 1 * e->e1;
+        fint = e->fint;
+        hgap = e->hgap;
+    } else {
+        edge_angle = 1 * e->e2;
+        fint = e->fintx;
+        hgap = e->hgapx;
+    }
+    double vec[6];
+    for (int i = 0; i < 6; i++) vec[i] = o->vec[i];
+    int err;
+    if (particle_at == FIRST_TRACK_EDGE) {
+        err = bmad_wedger(edge_angle, 0.0, vec);
+        if (!err) err = bmad_fringe_dipoler(vec, g_tot, beta0, 1 * fint, hgap, particle_at);
+        if (!err) err = bmad_wedger(-edge_angle, g_tot, vec);
+    } else {
+        err = bmad_wedger(-edge_angle, g_tot, vec);
+        if (!err) err = bmad_fringe_dipoler(vec, g_tot, beta0, 1 * fint, hgap, particle_at);
+        if (!err) err = bmad_wedger(edge_angle, 0.0, vec);
+    }
+    if (err) {
+        o->state = BMX_STATE_LOST_PZ;
+        return;
+    }
+    for (int i = 0; i < 6; i++) o->vec[i] = vec[i];
+}
+
+/* fringe_here (bmad/code/fringe_here.f90:15-35) and bend_edge_kick (fringe_mod.f90:40-128): the
+ * hard multipole edge before the bend's edge kick at the entrance and after it at the exit,
+ * the edge kick by fringe_type (full: exact_bend_edge_kick; basic_bend, hard_edge_only:
+ * hwang_bend_edge_kick; none: nothing). Other fringe types are outside this restatement. */
 static int bend_edge_kick(coord_t *o, const bmx_ele *e, int particle_at)
 {
     int fringe_at = e->fringe_at;
     int physical_end = particle_at == FIRST_TRACK_EDGE ? BMX_ENTRANCE_END : BMX_EXIT_END;
     int here = (fringe_at == BMX_BOTH_ENDS) || (fringe_at == physical_end);
-    if (!here) return 0;
+    if (!here || e->fringe_type == BMX_FRINGE_NONE) return 0;
+    if (particle_at == FIRST_TRACK_EDGE) hard_multipole_edge_kick(o, e, particle_at);
     switch (e->fringe_type) {
-    case BMX_FRINGE_NONE: return 0;
+    case BMX_FRINGE_FULL:
+        exact_bend_edge_kick(o, e, particle_at);
+        break;
     case BMX_FRINGE_BASIC_BEND: case BMX_FRINGE_HARD_EDGE_ONLY:
         bmo_hwang_bend_edge_kick(e, o->charge, particle_at == FIRST_TRACK_EDGE, o->vec);
-        return 0;
+        break;
     default: return BMX_ERR_UNSUPPORTED;
     }
+    if (particle_at == SECOND_TRACK_EDGE) hard_multipole_edge_kick(o, e, particle_at);
+    return 0;
+}
+
+/* apply_element_edge_kick.f90:64-189,270-271 for a bend with a bmad_standard field and
+ * exact_multipoles off or g = 0: at an end fringe_at names, the electric longitudinal fringe
+ * before the magnetic edge at the exit and after it at the entrance. */
+static int bend_element_edge(coord_t *o, const bmx_ele *e, int particle_at, const double *an_elec,
+                             const double *bn_elec, int ix_elec_max)
+{
+    const int physical_end = particle_at == FIRST_TRACK_EDGE ? BMX_ENTRANCE_END : BMX_EXIT_END;
+    if (!(e->fringe_at == BMX_BOTH_ENDS || e->fringe_at == physical_end)) return 0;
+    if (particle_at == SECOND_TRACK_EDGE)
+        electric_longitudinal_fringe(o, e, an_elec, bn_elec, ix_elec_max, -1.0);
+    int err = bend_edge_kick(o, e, particle_at);
+    if (err) return err;
+    if (particle_at == FIRST_TRACK_EDGE)
+        electric_longitudinal_fringe(o, e, an_elec, bn_elec, ix_elec_max, 1.0);
+    return 0;
 }
 
 static int orbit_too_large(coord_t *o, double *unstable);
 
-/* bmad/low_level/track_a_bend.f90:18-333 for exact_multipoles = off, no body multipoles
- * (ix_mag_max = ix_elec_max = -1, so n_step = 1), b1 = 0, no misalignment. */
+/* bmad/code/elec_multipole_field.f90:1-84 (field only): Ex, Ey of the electric multipole of
+ * order n with strengths a, b at the particle. */
+static void elec_multipole_field(double a, double b, int n, const coord_t *o, double *Ex,
+                                 double *Ey)
+{
+    *Ex = 0;
+    *Ey = 0;
+    if (a == 0 && b == 0) return;
+    double x = o->vec[0], y = o->vec[2];
+    for (int m = 0; m <= n; m += 2) {
+        double f = bmo_c_multi(n, m) * mexp(x, n - m) * mexp(y, m);
+        *Ex = *Ex - b * f;
+        *Ey = *Ey - a * f;
+    }
+    for (int m = 1; m <= n; m += 2) {
+        double f = bmo_c_multi(n, m) * mexp(x, n - m) * mexp(y, m);
+        *Ex = *Ex + a * f;
+        *Ey = *Ey - b * f;
+    }
+}
+
+/* track_a_bend.f90:336-467 apply_multipole_kicks, not exact (exact_multipoles off or g = 0):
+ * the magnetic multipoles as a field B (from ab_multipole_kick with ele_orientation = 0, scaled
+ * by r_step p0c / (c charge_ref)) and the kick coef c_dir (1 + g x) c / p0c x B; the electric
+ * multipoles as a field E and the kick step_len coef (1 + g x) charge / (ps beta p0c) x E with
+ * the energy change of a transverse kick. */
+static void bend_multipole_kicks(coord_t *o, const bmx_ele *e, double coef, double step_len,
+                                 double r_step, int ix_mag_max, const double *an,
+                                 const double *bn, int ix_elec_max, const double *an_elec,
+                                 const double *bn_elec)
+{
+    const double charge = o->charge;
+    const double c_dir = 1.0 * charge;
+    double B1 = 0, B2 = 0, E1 = 0, E2 = 0;
+    double f_p0c = r_step * e->p0c / (C_LIGHT * o->charge);
+    for (int i = 0; i <= ix_mag_max; i++) {
+        if (an[i] == 0 && bn[i] == 0) continue;
+        double kx, ky;
+        ab_multipole_kick(an[i], bn[i], i, o, 1.0, &kx, &ky);
+        B1 = B1 + f_p0c * ky;
+        B2 = B2 - f_p0c * kx;
+    }
+    for (int i = 0; i <= ix_elec_max; i++) {
+        if (an_elec[i] == 0 && bn_elec[i] == 0) continue;
+        double Ex, Ey;
+        elec_multipole_field(an_elec[i], bn_elec[i], i, o, &Ex, &Ey);
+        E1 = E1 + Ex;
+        E2 = E2 + Ey;
+    }
+    if (ix_mag_max > -1) {
+        double f_coef = coef * c_dir * (1 + e->g * o->vec[0]) * C_LIGHT / o->p0c;
+        o->vec[1] = o->vec[1] - f_coef * B2;
+        o->vec[3] = o->vec[3] + f_coef * B1;
+    }
+    if (ix_elec_max > -1) {
+        coord_t o0 = *o;
+        double rel_p0 = 1 + o0.vec[5];
+        double ps2 = sq(rel_p0) - sq(o0.vec[1]) - sq(o0.vec[3]);
+        double ps = sqrt(ps2) / rel_p0;
+        double f_coef = step_len * coef * (1 + e->g * o0.vec[0]) * charge / (ps * o0.beta * o0.p0c);
+        double Kx = f_coef * E1, Ky = f_coef * E2;
+        double alpha = (Kx * (2 * o0.vec[1] + Kx) + Ky * (2 * o0.vec[3] + Ky)) / sq(rel_p0);
+        o->vec[1] = o0.vec[1] + Kx;
+        o->vec[3] = o0.vec[3] + Ky;
+        o->vec[5] = o0.vec[5] + rel_p0 * bmo_sqrt_one(alpha);
+        o->beta = (1 + o->vec[5]) /
+                  sqrt(ps2 + sq(o->vec[1]) + sq(o->vec[3]) + sq(o->mc2 / o->p0c));
+        o->vec[4] = o0.vec[4] * o->beta / o0.beta;
+    }
+}
+
+/* track_a_bend.f90:500-659 sbend_body_with_k1_map (orbit only, time_dir = direction =
+ * orientation = +1): one step of a bend with a quadrupole gradient, b1 = k1 L. */
+static void sbend_body_with_k1_map(coord_t *o, const bmx_ele *e, double dg, double b1, int n_step)
+{
+    const double rel_charge_dir = 1.0;
+    double g = e->g;
+    double length = 1 * e->l / n_step;
+    double k1 = b1 / e->l;
+    double g_tot = (g + dg) * rel_charge_dir;
+    double rel_p = 1 + o->vec[5];
+    double rel_p2 = sq(rel_p);
+    double k_x = k1 + g * g_tot;
+    double x_c = (g * rel_p - g_tot) / k_x;
+    double om_x = sqrt(fabs(k_x) / rel_p);
+    double om_y = sqrt(fabs(k1) / rel_p);
+    double tau_x = -copysign(1.0, k_x);
+    double tau_y = copysign(1.0, k1);
+    double s_x, c_x, z2, s_y, c_y;
+    double arg = om_x * length;
+    if (arg < 1e-6) {
+        s_x = (1 + tau_x * sq(arg) / 6) * length;
+        c_x = 1 + tau_x * sq(arg) / 2;
+        z2 = g * sq(length) / (2 * rel_p);
+    } else if (k_x > 0) {
+        s_x = sin(arg) / om_x;
+        c_x = cos(arg);
+        z2 = tau_x * g * (1 - c_x) / (rel_p * sq(om_x));
+    } else {
+        s_x = sinh(arg) / om_x;
+        c_x = cosh(arg);
+        z2 = tau_x * g * (1 - c_x) / (rel_p * sq(om_x));
+    }
+    arg = om_y * length;
+    if (arg < 1e-6) {
+        s_y = (1 + tau_y * sq(arg) / 6) * length;
+        c_y = 1 + tau_y * sq(arg) / 2;
+    } else if (k1 < 0) {
+        s_y = sin(om_y * length) / om_y;
+        c_y = cos(om_y * length);
+    } else {
+        s_y = sinh(om_y * length) / om_y;
+        c_y = cosh(om_y * length);
+    }
+    double r[6];
+    for (int i = 0; i < 6; i++) r[i] = o->vec[i];
+    r[0] = r[0] - x_c;
+    double z0 = -g * x_c * length;
+    double z1 = -g * s_x;
+    double z11 = tau_x * sq(om_x) * (length - c_x * s_x) / 4;
+    double z12 = -tau_x * sq(om_x) * sq(s_x) / (2 * rel_p);
+    double z22 = -(length + c_x * s_x) / (4 * rel_p2);
+    double z33 = tau_y * sq(om_y) * (length - c_y * s_y) / 4;
+    double z34 = -tau_y * sq(om_y) * sq(s_y) / (2 * rel_p);
+    double z44 = -(length + c_y * s_y) / (4 * rel_p2);
+    o->vec[0] = c_x * r[0] + s_x * r[1] / rel_p + x_c;
+    o->vec[1] = tau_x * sq(om_x) * rel_p * s_x * r[0] + c_x * r[1];
+    o->vec[2] = c_y * r[2] + s_y * r[3] / rel_p;
+    o->vec[3] = tau_y * sq(om_y) * rel_p * s_y * r[2] + c_y * r[3];
+    o->vec[4] = r[4] + 1 * z0 + z1 * r[0] + z2 * r[1] + z11 * sq(r[0]) + z12 * r[0] * r[1] +
+                z22 * sq(r[1]) + z33 * sq(r[2]) + z34 * r[2] * r[3] + z44 * sq(r[3]);
+}
+
+/* One step of the bend body without a gradient (track_a_bend.f90:120-209): the semi-linear map
+ * near the zero orbit, else the exact circle. Returns nonzero when the body loses the particle
+ * (the reference returns from track_a_bend there). */
+static int bend_body_step(coord_t *o, const bmx_ele *e, double g, double dg, double g_tot,
+                          double angle, double step_len)
+{
+    double *v = o->vec;
+    double x = v[0], px = v[1], y = v[2], py = v[3], z = v[4], pz = v[5];
+    double rel_p = 1 + pz;
+    double rel_p2 = sq(rel_p);
+    if (dg == 0 && fabs(x * g) < 1e-9 && fabs(px) < 1e-9 && fabs(py) < 1e-9 && fabs(pz) < 1e-9) {
+        /* semi-linear map near the zero orbit, track_a_bend.f90:136-159 */
+        double ll = step_len;
+        double cos_a = cos(angle), sin_a = sin(angle);
+        double sinc_a = bmo_sinc(angle), cosc_a = bmo_cosc(angle);
+        double gam2 = sq(o->mc2) / (sq(rel_p * o->p0c) + sq(o->mc2));
+        double m56 = ll * (gam2 - sq(g * ll) * bmo_sincc(angle));
+        v[0] = cos_a * x + ll * sinc_a * px + g * sq(ll) * cosc_a * pz;
+        v[1] = -g * sin_a * x + cos_a * px + g * ll * sinc_a * pz;
+        v[2] = y + ll * py;
+        v[4] = -g * ll * sinc_a * x - g * sq(ll) * cosc_a * px + z + m56 * pz;
+        return 0;
+    }
+    /* exact body, track_a_bend.f90:163-209 */
+    double beta_ref = e->p0c / e->e_tot;
+    double sinc_a = bmo_sinc(angle);
+    double pt = sqrt(rel_p2 - sq(py));
+    if (fabs(px) > pt) {
+        o->state = BMX_STATE_LOST_PZ;
+        return 1;
+    }
+    double g_p = g_tot / pt;
+    double phi_1 = asin(px / pt);
+    double cos_a = cos(angle), sin_a = sin(angle);
+    double cosc_a = bmo_cosc(angle);
+    double cos_plus = cos(angle + phi_1);
+    double sin_plus = sin(angle + phi_1);
+    double alpha = 2 * (1 + g * x) * sin_plus * step_len * sinc_a -
+                   g_p * sq((1 + g * x) * step_len * sinc_a);
+    double r = sq(cos_plus) + g_p * alpha;
+    if (r < 0 || (fabs(g_p) < 1e-5 && fabs(cos_plus) < 1e-5)) {
+        o->state = BMX_STATE_LOST;
+        v[0] = 2 * MAX_APERTURE_LIMIT;
+        return 1;
+    }
+    double rad = sqrt(r), denom, xi;
+    if (cos_plus > 0) {
+        denom = rad + cos_plus;
+        xi = alpha / denom;
+    } else {
+        denom = rad - cos_plus;
+        xi = denom / g_p;
+    }
+    v[0] = x * cos_a - sq(step_len) * g * cosc_a + xi;
+    if (fabs(v[0]) > MAX_APERTURE_LIMIT) {
+        o->state = BMX_STATE_LOST;
+        return 1;
+    }
+    double L_u = xi;
+    double L_v = -1 * (step_len * sinc_a + x * sin_a);
+    double L_c = sqrt(sq(L_v) + sq(L_u));
+    double angle_p = 2 * (angle + phi_1 - 1 * atan2(L_u, -L_v));
+    double L_p = 1 * L_c / bmo_sinc(angle_p / 2);
+    v[1] = pt * sin(phi_1 + angle - angle_p);
+    v[2] = y + py * L_p / pt;
+    v[4] = z + o->beta * step_len / beta_ref - rel_p * L_p / pt;
+    return 0;
+}
+
+/* bmad/low_level/track_a_bend.f90:18-333 (time_dir = direction = orientation = +1): the
+ * offset_particle frame change (bend branch), the entrance edge, the multipoles (magnetic, with
+ * b1 pulled out for the k1 map unless exact_multipoles is set, and the b0 part as the field
+ * error dg; electric) as kicks around n_step body steps, orbit_too_large, the exit edge.
+ * exact_multipoles with g /= 0 (bend_exact_multipole_field) is outside this restatement. */
 static int track_a_bend(coord_t *o, const bmx_ele *e)
 {
-    if (is_misaligned(e) || e->ref_tilt != 0) return BMX_ERR_UNSUPPORTED;
+    const int exact = e->exact_multipoles > 1;
+    if (exact && e->g != 0) return BMX_ERR_UNSUPPORTED;
+    if (e->fringe_type != BMX_FRINGE_NONE && e->fringe_type != BMX_FRINGE_BASIC_BEND &&
+        e->fringe_type != BMX_FRINGE_HARD_EDGE_ONLY && e->fringe_type != BMX_FRINGE_FULL)
+        return BMX_ERR_UNSUPPORTED;
+    const int has_misalign = is_misaligned(e) || e->ref_tilt != 0;
+    if (has_misalign) offset_particle(e, 1, o, 1, 0, 0, 0);
     const double rel_charge_dir = 1.0;
-    int err = bend_edge_kick(o, e, FIRST_TRACK_EDGE);
+
+    double an_elec[BMX_N_POLE], bn_elec[BMX_N_POLE];
+    int ix_elec_max = multipole_ele_to_ab_elec(e, an_elec, bn_elec);
+    int err = bend_element_edge(o, e, FIRST_TRACK_EDGE, an_elec, bn_elec, ix_elec_max);
     if (err) return err;
     if (orbit_too_large(o, NULL)) return 0;
 
     double an[BMX_N_POLE], bn[BMX_N_POLE], b1;
-    int ix_mag_max = multipole_ele_to_ab(e, 1, an, bn, &b1, &err);
-    if (err) return err;
-    b1 = b1 * rel_charge_dir;
-    if (fabs(b1) < 1e-10) {
-        bn[1] = b1;
+    int ix_mag_max;
+    if (exact) {
+        ix_mag_max = multipole_ele_to_ab(e, 1, an, bn, NULL, &err);
         b1 = 0;
+    } else {
+        ix_mag_max = multipole_ele_to_ab(e, 1, an, bn, &b1, &err);
+        b1 = b1 * rel_charge_dir;
+        if (fabs(b1) < 1e-10) {
+            bn[1] = b1;
+            b1 = 0;
+        }
     }
+    if (err) return err;
     double length = 1 * e->l;
     double g = e->g, dg;
     if (length == 0) {
@@ -1072,80 +1600,35 @@ static int track_a_bend(coord_t *o, const bmx_ele *e)
         g_tot = 0;
         b1 = 0;
     }
-    /* Body multipoles (anything left in an/bn) and a k1 map are outside this restatement. */
-    if (b1 != 0) return BMX_ERR_UNSUPPORTED;
-    if (max_nonzero(an, bn, BMX_N_POLE - 1) > -1 && ix_mag_max > -1) return BMX_ERR_UNSUPPORTED;
-
-    double step_len = length / 1;
+    int n_step = 1;
+    if (ix_mag_max > -1 || ix_elec_max > -1) {
+        long ns = f_nint(fabs(length) / e->ds_step);
+        n_step = ns > 1 ? (int)ns : 1;
+    }
+    double r_step = 1.0 / n_step;
+    double step_len = length / n_step;
     double angle = g * step_len;
-    double *v = o->vec;
-
-    if ((g == 0 && dg == 0) || step_len == 0) {
-        track_a_drift(o, step_len);
-    } else {
-        double x = v[0], px = v[1], y = v[2], py = v[3], z = v[4], pz = v[5];
-        double rel_p = 1 + pz;
-        double rel_p2 = sq(rel_p);
-        if (dg == 0 && fabs(x * g) < 1e-9 && fabs(px) < 1e-9 && fabs(py) < 1e-9 &&
-            fabs(pz) < 1e-9) {
-            /* semi-linear map near the zero orbit, track_a_bend.f90:136-159 */
-            double ll = step_len;
-            double cos_a = cos(angle), sin_a = sin(angle);
-            double sinc_a = bmo_sinc(angle), cosc_a = bmo_cosc(angle);
-            double gam2 = sq(o->mc2) / (sq(rel_p * o->p0c) + sq(o->mc2));
-            double m56 = ll * (gam2 - sq(g * ll) * bmo_sincc(angle));
-            v[0] = cos_a * x + ll * sinc_a * px + g * sq(ll) * cosc_a * pz;
-            v[1] = -g * sin_a * x + cos_a * px + g * ll * sinc_a * pz;
-            v[2] = y + ll * py;
-            v[4] = -g * ll * sinc_a * x - g * sq(ll) * cosc_a * px + z + m56 * pz;
+    if (ix_mag_max > -1 || ix_elec_max > -1)
+        bend_multipole_kicks(o, e, 0.5, step_len, r_step, ix_mag_max, an, bn, ix_elec_max,
+                             an_elec, bn_elec);
+    for (int n = 1; n <= n_step; n++) {
+        if (b1 != 0) {
+            sbend_body_with_k1_map(o, e, dg, b1, n_step);
+            o->vec[4] = o->vec[4] + low_energy_z_correction(o, e, step_len);
+        } else if ((g == 0 && dg == 0) || step_len == 0) {
+            track_a_drift(o, step_len);
         } else {
-            /* exact body, track_a_bend.f90:163-209 */
-            double beta_ref = e->p0c / e->e_tot;
-            double sinc_a = bmo_sinc(angle);
-            double pt = sqrt(rel_p2 - sq(py));
-            if (fabs(px) > pt) {
-                o->state = BMX_STATE_LOST_PZ;
-                return 0;
-            }
-            double g_p = g_tot / pt;
-            double phi_1 = asin(px / pt);
-            double cos_a = cos(angle), sin_a = sin(angle);
-            double cosc_a = bmo_cosc(angle);
-            double cos_plus = cos(angle + phi_1);
-            double sin_plus = sin(angle + phi_1);
-            double alpha = 2 * (1 + g * x) * sin_plus * step_len * sinc_a -
-                           g_p * sq((1 + g * x) * step_len * sinc_a);
-            double r = sq(cos_plus) + g_p * alpha;
-            if (r < 0 || (fabs(g_p) < 1e-5 && fabs(cos_plus) < 1e-5)) {
-                o->state = BMX_STATE_LOST;
-                v[0] = 2 * MAX_APERTURE_LIMIT;
-                return 0;
-            }
-            double rad = sqrt(r), denom, xi;
-            if (cos_plus > 0) {
-                denom = rad + cos_plus;
-                xi = alpha / denom;
-            } else {
-                denom = rad - cos_plus;
-                xi = denom / g_p;
-            }
-            v[0] = x * cos_a - sq(step_len) * g * cosc_a + xi;
-            if (fabs(v[0]) > MAX_APERTURE_LIMIT) {
-                o->state = BMX_STATE_LOST;
-                return 0;
-            }
-            double L_u = xi;
-            double L_v = -1 * (step_len * sinc_a + x * sin_a);
-            double L_c = sqrt(sq(L_v) + sq(L_u));
-            double angle_p = 2 * (angle + phi_1 - 1 * atan2(L_u, -L_v));
-            double L_p = 1 * L_c / bmo_sinc(angle_p / 2);
-            v[1] = pt * sin(phi_1 + angle - angle_p);
-            v[2] = y + py * L_p / pt;
-            v[4] = z + o->beta * step_len / beta_ref - rel_p * L_p / pt;
+            if (bend_body_step(o, e, g, dg, g_tot, angle, step_len)) return 0;
         }
+        if (ix_mag_max > -1 || ix_elec_max > -1)
+            bend_multipole_kicks(o, e, n == n_step ? 0.5 : 1.0, step_len, r_step, ix_mag_max, an,
+                                 bn, ix_elec_max, an_elec, bn_elec);
     }
     if (orbit_too_large(o, NULL)) return 0;
-    return bend_edge_kick(o, e, SECOND_TRACK_EDGE);
+    err = bend_element_edge(o, e, SECOND_TRACK_EDGE, an_elec, bn_elec, ix_elec_max);
+    if (err) return err;
+    if (has_misalign) offset_particle(e, 0, o, 1, 0, 0, 0);
+    return 0;
 }
 
 /* ------------------------------------------------------------------------------------------

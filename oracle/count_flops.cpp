@@ -93,6 +93,9 @@ inline cdouble asin(const cdouble &a) { return fn(::asin(a.v), a); }
 inline cdouble sinh(const cdouble &a) { return fn(::sinh(a.v), a); }
 inline cdouble cosh(const cdouble &a) { return fn(::cosh(a.v), a); }
 inline cdouble exp(const cdouble &a) { return fn(::exp(a.v), a); }
+inline cdouble atan(const cdouble &a) { return fn(::atan(a.v), a); }
+inline cdouble tan(const cdouble &a) { return fn(::tan(a.v), a); }
+inline cdouble copysign(double m, const cdouble &s) { return cdouble(::copysign(m, s.v), false); }
 inline cdouble fabs(const cdouble &a) { return cdouble(::fabs(a.v), a.dyn); }
 inline cdouble atan2(const cdouble &a, const cdouble &b) { return op(::atan2(a.v, b.v), a, b); }
 inline cdouble fmod(const cdouble &a, const cdouble &b) { return op(::fmod(a.v, b.v), a, b); }

@@ -89,11 +89,23 @@ def tracking_method_elements(E):
         "QUADRUPOLE4": mk(E.QUADRUPOLE, p0c, et, k1=1.0, fringe_type=E.FRINGE_FULL, **q0),
         "QUADRUPOLE5": mk(E.QUADRUPOLE, p0c, et, k1=-1.0, fringe_type=E.FRINGE_FULL, **q0),
     })
+    # sb0: sbend, a0 = 0.01, a1 = 0.1, vkick = -0.0001, b0_elec = -2e3, a1_elec = 2e4,
+    # r0_mag = 2, r0_elec = 3 (tracking_method_test.bmad:79), then sbend1/3/5/7 (:82-91)
+    sb0 = dict(a_pole={0: 0.01, 1: 0.1}, vkick=-0.0001, b_pole_elec={0: -2e3},
+               a_pole_elec={1: 2e4}, r0_mag=2.0, r0_elec=3.0, l=0.6, num_steps=10,
+               ref_tilt=0.1, **off)
+    out.update({
+        "SBEND1": mk(E.SBEND, p0c, et, k1=0.3, exact_multipoles=E.EXACT_VERTICALLY_PURE, **sb0),
+        "SBEND3": mk(E.SBEND, p0c, et, k1=0.3, k2=3.0, **sb0),
+        "SBEND5": mk(E.SBEND, p0c, et, g=0.01, e1=0.1, e2=0.2, fringe_type=E.FRINGE_FULL,
+                     fint=0.7, hgap=2.0, **sb0),
+        "SBEND7": mk(E.SBEND, p0c, et, g=0.01, e1=0.1, e2=0.2, **sb0),
+    })
     return out
 
 
 # tracking_method_test lines in the fixture whose elements are not restated yet
-NOT_YET = ("SBEND1", "SBEND3", "SBEND5", "SBEND7")
+NOT_YET = ()
 
 
 def tm_start_beta(E):

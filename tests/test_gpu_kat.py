@@ -8,8 +8,9 @@ import parity as P
 
 pytestmark = pytest.mark.gpu
 
-# not on the GPU path: apertures that move with the offsets, rf cavity multipoles
-OFF_PATH = {"RCOLLIMATOR1", "RFCAVITY3"}
+# not on the GPU path: apertures that move with the offsets, rf cavity multipoles; bends with
+# offsets, multipoles, k1 or a full fringe
+OFF_PATH = {"RCOLLIMATOR1", "RFCAVITY3", "SBEND1", "SBEND3", "SBEND5", "SBEND7"}
 
 
 @pytest.mark.parametrize("maps", ["fast", "reference"])

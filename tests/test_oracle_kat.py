@@ -7,9 +7,12 @@ import kat
 def test_tracking_method_known_answers(bmx, oracle):
     """tracking_method_test "<ELE>: Bmad_Standard" lines (ABS 2e-14, written es18.10):
     drift, thick multipoles (sextupole, octupole, thick_multipole, kickers, collimators,
-    instrument, monitor) with offsets/pitch/tilt, the rf cavity (thick and zero length), and
+    instrument, monitor) with offsets/pitch/tilt, the rf cavity (thick and zero length),
     quadrupoles with magnetic and electric multipoles (stepped body, electric kicks that change
-    the energy, electric hard-edge energy kicks), tracked as a positron."""
+    the energy, electric hard-edge energy kicks, hard and soft edges), and sector bends with
+    offsets, pitch and ref_tilt (the bend branch of offset_particle), magnetic and electric
+    multipoles, k1 (the k1 body map), k2, exact_multipoles with g = 0 and the full (exact) edge,
+    tracked as a positron."""
     E = bmx.elements
     fx = kat.read_fixture("tracking_method_test.correct")
     eles = kat.tracking_method_elements(E)
@@ -25,9 +28,7 @@ def test_tracking_method_known_answers(bmx, oracle):
         if not ok:
             failures.append((name, worst, [kat.fortran_es(v, 10) for v in vec], expect[:6]))
     assert not failures, failures
-    # the fixture's other lines: elements whose restatement needs what this path does not
-    # carry yet (quadrupole fringe, k1 and multipoles in bends, the bend branch of
-    # offset_particle)
+    # every line of the fixture is restated
     assert set(f"{n}: Bmad_Standard" for n in eles) | set(f"{n}: Bmad_Standard" for n in kat.NOT_YET) == set(fx)
 
 
