@@ -389,11 +389,13 @@ int32_t track_group(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t fir
     cfg.fast = ctx->fast_maps;
     cfg.jit = ctx->fast_maps ? L->jit : nullptr;
     cfg.waves = ctx->waves ? ctx->waves : default_waves(ctx->fast_maps, cfg.jit);
-    // a start element or a partial turn runs on the generic span kernel, and so does a lattice
-    // with stepped quadrupoles that no cell kernel carries
-    cfg.span = first != 0 || count != L->desc.n_ele || (L->host.rare_q && !jit_rare_q(cfg.jit));
+    // a start element or a partial turn runs on the generic span kernel; a lattice with stepped
+    // quadrupoles or general bends that no cell kernel carries on the generic kernels built
+    // with them
+    cfg.span = first != 0 || count != L->desc.n_ele;
+    const bool ext = L->host.rare_q && (cfg.span || !jit_rare_q(cfg.jit));
     cfg.grid = ctx->grid > 0 ? ctx->grid
-                             : default_grid(ctx->device, ctx->fast_maps, cfg.waves, cfg.jit, cfg.span);
+                             : default_grid(ctx->device, ctx->fast_maps, cfg.waves, cfg.jit, cfg.span, ext);
     cfg.timer = ctx->profile ? &ctx->timer : nullptr;
     HIP_TRY(enqueue_run(ctx->stream, *rb, &g, 1, true, n_turns, cfg, ctx->d_work), "k_track_tiles");
     return BMX_OK;

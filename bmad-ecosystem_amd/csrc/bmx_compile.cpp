@@ -300,6 +300,152 @@ void edge_constants(const bmx_ele &e, bool entrance, double *out)
     out[5] = tan_e * tan_e;
 }
 
+// The body block of a sector bend for one step of length L (the K_BEND parameter layout):
+// g_tot = g + dg of the body; the edge constants are those of the element.
+void bend_body_block(const bmx_ele &e, double L, double g, double g_tot, double *q)
+{
+    const double angle = g * L;
+    q[0] = L;
+    q[1] = g;
+    q[2] = g_tot * 1.0;
+    q[3] = angle;
+    q[4] = std::cos(angle);
+    q[5] = std::sin(angle);
+    q[6] = h_sinc(angle);
+    q[7] = h_cosc(angle);
+    q[8] = e.p0c / e.e_tot;
+    q[9] = h_sincc(angle);
+    edge_constants(e, true, &q[10]);
+    edge_constants(e, false, &q[16]);
+    q[22] = L * L * g * q[7];
+    q[23] = L / q[8];
+    q[24] = L * q[6];
+    q[25] = 0.5 * q[12];
+    q[26] = 0.5 * q[18];
+    q[27] = angle * angle * q[7];  // 1 - cos(angle) without cancellation
+    q[28] = g != 0 ? 1.0 / g : 0.0;
+}
+
+// ---- the bend branch of offset_particle (offset_particle.f90:206-232, 404-430) ----
+// A point and frame; every step below maps r -> S r + t and w -> S w, so the whole frame change
+// of a particle at (x, y, 0) with the unit frame is r = W (x, y, 0) + c, w = W: W and c follow
+// from composing the steps once, here.
+struct Frame {
+    double r[3];
+    double w[3][3];
+};
+
+void frame_apply(Frame &f, const double S[3][3], const double *t)
+{
+    double r[3], w[3][3];
+    for (int i = 0; i < 3; i++) {
+        r[i] = S[i][0] * f.r[0] + S[i][1] * f.r[1] + S[i][2] * f.r[2] + (t ? t[i] : 0.0);
+        for (int j = 0; j < 3; j++) w[i][j] = S[i][0] * f.w[0][j] + S[i][1] * f.w[1][j] + S[i][2] * f.w[2][j];
+    }
+    std::memcpy(f.r, r, sizeof r);
+    std::memcpy(f.w, w, sizeof w);
+}
+
+// Rotation by angle about axis (0 = x, 1 = y, 2 = z), as applied by rotate_vec / rotate_mat
+// (sim_utils/math/rotation_3d_mod.f90:737-835).
+void axis_rotation(int axis, double angle, double R[3][3])
+{
+    const double c = std::cos(angle), s = std::sin(angle);
+    const double I[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    std::memcpy(R, I, sizeof I);
+    if (axis == 0) { R[1][1] = c; R[1][2] = -s; R[2][1] = s; R[2][2] = c; }
+    else if (axis == 1) { R[0][0] = c; R[0][2] = s; R[2][0] = -s; R[2][2] = c; }
+    else { R[0][0] = c; R[0][1] = -s; R[1][0] = s; R[1][1] = c; }
+}
+
+void mat3_mul(const double A[3][3], const double B[3][3], double C[3][3])
+{
+    double T[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) T[i][j] = A[i][0] * B[0][j] + A[i][1] * B[1][j] + A[i][2] * B[2][j];
+    std::memcpy(C, T, sizeof T);
+}
+
+// bmad/geometry/bend_shift.f90:19-61: along an arc of curvature g by delta_s (with the bend's
+// ref_tilt when with_tilt).
+void bend_shift_frame(Frame &f, double g, double delta_s, bool with_tilt, double ref_tilt)
+{
+    const double angle = delta_s * g;
+    if (angle == 0) {
+        f.r[2] = f.r[2] - delta_s;
+        return;
+    }
+    double S[3][3], R[3][3];
+    axis_rotation(1, angle, S);
+    if (with_tilt && ref_tilt != 0) {
+        axis_rotation(2, -ref_tilt, R);
+        double T[3][3];
+        mat3_mul(S, R, T);
+        axis_rotation(2, ref_tilt, R);
+        mat3_mul(R, T, S);
+    }
+    double L[3] = {-2 * std::pow(std::sin(angle / 2), 2) / g, 0.0, -std::sin(angle) / g};
+    if (with_tilt && ref_tilt != 0) {
+        const double c = std::cos(ref_tilt), sn = std::sin(ref_tilt);
+        const double l0 = c * L[0] - sn * L[1], l1 = sn * L[0] + c * L[1];
+        L[0] = l0;
+        L[1] = l1;
+    }
+    frame_apply(f, S, L);
+}
+
+// out[0..8] W and [9..11] d of the entering frame change (r = W ((x, y, 0) - d)), [12..20] W and
+// [21..23] c of the leaving one (r = W (x, y, 0) + c, including the move to the exit face), for
+// a misaligned bend (orientation +1, drift to the edges).
+void bend_frame_blocks(const bmx_ele &e, double *out)
+{
+    const double g = e.g, rt = e.ref_tilt, L_half = 0.5 * e.l;
+    double Sm[3][3], St[3][3], wm[9];
+    pitch_matrix(e.x_pitch, e.y_pitch, false, wm);  // floor_angles_to_w_mat(x_pitch, y_pitch, 0)
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            Sm[i][j] = wm[3 * i + j];
+            St[j][i] = wm[3 * i + j];
+        }
+    const double offs[3] = {e.x_offset, e.y_offset, e.z_offset};
+    // entering: ds_center = L/2
+    Frame f = {{0, 0, 0}, {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+    bend_shift_frame(f, g, L_half, true, rt);
+    {
+        double t[3];
+        for (int i = 0; i < 3; i++) t[i] = -(St[i][0] * offs[0] + St[i][1] * offs[1] + St[i][2] * offs[2]);
+        frame_apply(f, St, t);
+    }
+    if (rt != 0) {
+        bend_shift_frame(f, g, -L_half, true, rt);
+        double R[3][3];
+        axis_rotation(2, -rt, R);
+        frame_apply(f, R, nullptr);
+        bend_shift_frame(f, g, L_half, false, 0.0);
+    }
+    bend_shift_frame(f, g, -L_half, false, 0.0);
+    // r = W (x, y, 0) + c with W = f.w, c = f.r; d = -W^T c
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) out[3 * i + j] = f.w[i][j];
+    for (int j = 0; j < 3; j++) out[9 + j] = -(f.w[0][j] * f.r[0] + f.w[1][j] * f.r[1] + f.w[2][j] * f.r[2]);
+    // leaving: ds_center = -L/2
+    Frame u = {{0, 0, 0}, {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+    bend_shift_frame(u, g, -L_half, false, 0.0);
+    if (rt != 0) {
+        bend_shift_frame(u, g, -L_half, false, 0.0);
+        double R[3][3];
+        axis_rotation(2, rt, R);
+        frame_apply(u, R, nullptr);
+        bend_shift_frame(u, g, L_half, true, rt);
+    }
+    frame_apply(u, Sm, offs);
+    bend_shift_frame(u, g, L_half, true, rt);
+    u.r[2] = u.r[2] + 2 * L_half;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) out[12 + 3 * i + j] = u.w[i][j];
+    for (int i = 0; i < 3; i++) out[21 + i] = u.r[i];
+}
+
 }  // namespace
 
 int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge, HostTable &out,
@@ -344,21 +490,21 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             key = BMX_DRIFT;  // track1_bmad.f90:51-60
         }
         // Misalignments: drifts and markers ignore them in the reference (track1_bmad.f90:93-94);
-        // the other non-bend kinds get offset_particle's frame change around their map; the
-        // bend branch of offset_particle is not implemented here.
-        const bool offset = key != BMX_DRIFT && key != BMX_MARKER && misaligned(e);
-        if (offset && (key == BMX_SBEND || e.ref_tilt != 0)) {
-            err = where(i, e) + "bend offsets, pitches and tilts are not supported on the GPU path";
+        // the other non-bend kinds get offset_particle's frame change around their map; bends
+        // take the bend branch inside their own map (K_BMULT).
+        const bool offset = key != BMX_DRIFT && key != BMX_MARKER && key != BMX_SBEND && misaligned(e);
+        if (offset && e.ref_tilt != 0) {
+            err = where(i, e) + "ref_tilt on an element that is not a bend";
             return BMX_ERR_UNSUPPORTED;
         }
-        if (offset && e.offset_moves_aperture &&
+        if ((offset || (key == BMX_SBEND && misaligned(e))) && e.offset_moves_aperture &&
             (e.x1_limit != 0 || e.x2_limit != 0 || e.y1_limit != 0 || e.y2_limit != 0)) {
             err = where(i, e) + "apertures that move with the element offsets are not supported";
             return BMX_ERR_UNSUPPORTED;
         }
         double a[BMX_N_POLE], b[BMX_N_POLE];
-        if (key != BMX_QUADRUPOLE && ele_to_ab_elec(e, a, b) >= 0) {
-            err = where(i, e) + "electric multipoles are supported on quadrupoles only";
+        if (key != BMX_QUADRUPOLE && key != BMX_SBEND && ele_to_ab_elec(e, a, b) >= 0) {
+            err = where(i, e) + "electric multipoles are supported on quadrupoles and bends only";
             return BMX_ERR_UNSUPPORTED;
         }
         switch (key) {
@@ -484,41 +630,108 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
         case BMX_SBEND: {
             int ix = ele_to_ab(e, a, b);
             if (ix == -2) { err = where(i, e) + "multipole scaling"; return BMX_ERR_UNSUPPORTED; }
-            for (int n = 0; n < BMX_N_POLE; n++)
-                if (a[n] != 0 || b[n] != 0) { err = where(i, e) + "bend k1/k2/dg/multipoles/kicks are not supported"; return BMX_ERR_UNSUPPORTED; }
-            int ft = e.fringe_type;
-            bool kick_type = ft == BMX_FRINGE_BASIC_BEND || ft == BMX_FRINGE_HARD_EDGE_ONLY;
-            if (!kick_type && ft != BMX_FRINGE_NONE) { err = where(i, e) + "bend fringe_type"; return BMX_ERR_UNSUPPORTED; }
-            h.kind = K_BEND;
-            if (kick_type && (e.fringe_at == BMX_ENTRANCE_END || e.fringe_at == BMX_BOTH_ENDS)) h.flags |= F_EDGE_ENT;
-            if (kick_type && (e.fringe_at == BMX_EXIT_END || e.fringe_at == BMX_BOTH_ENDS)) h.flags |= F_EDGE_EXIT;
+            double ae[BMX_N_POLE], be[BMX_N_POLE];
+            const int ixe = ele_to_ab_elec(e, ae, be);
+            const int ft = e.fringe_type == 0 ? BMX_FRINGE_NONE : e.fringe_type;
+            const bool kick_type = ft == BMX_FRINGE_BASIC_BEND || ft == BMX_FRINGE_HARD_EDGE_ONLY;
+            if (!kick_type && ft != BMX_FRINGE_NONE && ft != BMX_FRINGE_FULL) { err = where(i, e) + "bend fringe_type"; return BMX_ERR_UNSUPPORTED; }
+            const bool plain = ix == -1 && ixe == -1 && ft != BMX_FRINGE_FULL && !misaligned(e);
+            const bool at_ent = e.fringe_at == BMX_ENTRANCE_END || e.fringe_at == BMX_BOTH_ENDS;
+            const bool at_exit = e.fringe_at == BMX_EXIT_END || e.fringe_at == BMX_BOTH_ENDS;
             double fg_ent = e.fint * e.hgap, fg_exit = e.fintx * e.hgapx;
             if (ft == BMX_FRINGE_HARD_EDGE_ONLY) fg_ent = fg_exit = 0;
+            if (ft != BMX_FRINGE_NONE && at_ent) h.flags |= F_EDGE_ENT;
+            if (ft != BMX_FRINGE_NONE && at_exit) h.flags |= F_EDGE_EXIT;
             if (e.e1 == 0 && e.e2 == 0 && fg_ent == 0 && fg_exit == 0 && e.k1 == 0) h.flags |= F_EDGE_SIMPLE;
-            const double g = e.g, dg = 0.0, L = e.l;
-            if ((g == 0 && dg == 0) || L == 0) h.flags |= F_BEND_DRIFT;
-            const double angle = g * L;
-            q.resize(kParams[K_BEND]);
-            q[0] = L;
-            q[1] = g;
-            q[2] = (g + dg) * 1.0;
-            q[3] = angle;
-            q[4] = std::cos(angle);
-            q[5] = std::sin(angle);
-            q[6] = h_sinc(angle);
-            q[7] = h_cosc(angle);
-            q[8] = e.p0c / e.e_tot;
-            q[9] = h_sincc(angle);
-            edge_constants(e, true, &q[10]);
-            edge_constants(e, false, &q[16]);
-            q[22] = L * L * g * q[7];
-            q[23] = L / q[8];
-            q[24] = L * q[6];
-            q[25] = 0.5 * q[12];
-            q[26] = 0.5 * q[18];
-            q[27] = angle * angle * q[7];  // 1 - cos(angle) without cancellation
-            q[28] = g != 0 ? 1.0 / g : 0.0;
-            if (g != 0 && q[2] == g && std::fabs(angle) <= 0.1) h.flags |= F_BEND_SERIES;
+            if (plain) {
+                h.kind = K_BEND;
+                const double g = e.g, L = e.l;
+                if (g == 0 || L == 0) h.flags |= F_BEND_DRIFT;
+                q.resize(kParams[K_BEND]);
+                bend_body_block(e, L, g, g, q.data());
+                if (g != 0 && q[2] == g && std::fabs(g * L) <= 0.1) h.flags |= F_BEND_SERIES;
+                break;
+            }
+            // the general bend (track_a_bend.f90:18-333)
+            const bool exact = e.exact_multipoles > 1;
+            if (exact && e.g != 0 && (ix > -1 || ixe > -1)) {
+                err = where(i, e) + "exact multipoles in a curved bend";
+                return BMX_ERR_UNSUPPORTED;
+            }
+            if (e.l == 0) { err = where(i, e) + "zero-length bend with multipoles, offsets or edges"; return BMX_ERR_UNSUPPORTED; }
+            // b1 pulled out for the k1 map unless exact_multipoles is set (:55-70)
+            double b1 = 0;
+            if (!exact) {
+                b1 = b[1];
+                b[1] = 0;
+                if (ix <= 1) ix = max_order(a, b, 2);
+                if (std::fabs(b1) < 1e-10) {
+                    b[1] = b1;
+                    b1 = 0;
+                }
+            }
+            const bool stepped = ix > -1 || ixe > -1;
+            const double L = e.l, g = e.g;
+            const double dg = b[0] / L;  // the b0 part is the field error (:74-80)
+            b[0] = 0;
+            if (max_order(a, b, BMX_N_POLE) > 3 || max_order(ae, be, BMX_N_POLE) > 3) {
+                err = where(i, e) + "bend multipole orders above 3";
+                return BMX_ERR_UNSUPPORTED;
+            }
+            long ns = stepped ? f_nint(std::fabs(L) / e.ds_step) : 1;
+            const int n_step = ns > 1 ? (int)ns : 1;
+            const double r_step = 1.0 / n_step;
+            const double step_len = L / n_step;
+            h.kind = K_BMULT;
+            h.flags |= F_RARE_MAP;
+            out.rare_q = true;
+            h.n = n_step;
+            q.resize(kParams[K_BMULT]);
+            bend_body_block(e, step_len, g, g + dg, q.data());
+            if (g != 0 && q[2] == g && std::fabs(g * step_len) <= 0.1) h.flags |= F_BEND_SERIES;
+            if (b1 != 0) h.flags |= F_BK1;
+            else if (g == 0 && dg == 0) h.flags |= F_BEND_DRIFT;
+            if (dg == 0) h.flags |= F_BSEMI;
+            q[29] = e.g + e.dg;
+            q[30] = b1 / L;
+            int mmask = 0, emask = 0;
+            const double fe = charge / e.p0c * (step_len / 2);
+            for (int n = 0; n < 4; n++) {
+                if (ix > -1 && (a[n] != 0 || b[n] != 0)) mmask |= 1 << n;
+                if (ixe > -1 && (ae[n] != 0 || be[n] != 0)) emask |= 1 << n;
+                q[31 + n] = ix > -1 ? a[n] * (r_step / 2) : 0.0;
+                q[35 + n] = ix > -1 ? b[n] * (r_step / 2) : 0.0;
+                q[39 + n] = ixe > -1 ? ae[n] * fe : 0.0;
+                q[43 + n] = ixe > -1 ? -be[n] * fe : 0.0;
+                q[47 + n] = ixe > -1 ? charge * ae[n] / (n + 1) : 0.0;
+                q[51 + n] = ixe > -1 ? charge * be[n] / (n + 1) : 0.0;
+            }
+            h.flags |= (mmask << F_ORDER_SHIFT) | (emask << F_ELEC_SHIFT);
+            if (at_ent) h.flags |= F_QEDGE_ENT;
+            if (at_exit) h.flags |= F_QEDGE_EXIT;
+            if (ft == BMX_FRINGE_FULL) h.flags |= F_BEXACT;
+            q[55] = e.k1 / 12.0;
+            q[56] = std::cos(e.e1);
+            q[57] = std::sin(e.e1);
+            q[58] = std::sin(2.0 * e.e1);
+            q[59] = e.fint * e.hgap;
+            q[60] = std::cos(e.e2);
+            q[61] = std::sin(e.e2);
+            q[62] = std::sin(2.0 * e.e2);
+            q[63] = e.fintx * e.hgapx;
+            const double beta0 = e.p0c / e.e_tot;
+            q[64] = 1 / (beta0 * beta0);
+            q[65] = e.e1;
+            q[66] = e.e2;
+            if (e.x_offset != 0 || e.y_offset != 0 || e.z_offset != 0 || e.x_pitch != 0 || e.y_pitch != 0) {
+                h.flags |= F_MISALIGN;
+                bend_frame_blocks(e, &q[67]);
+            } else if (e.ref_tilt != 0) {
+                h.flags |= F_TILT;
+            }
+            q[91] = std::cos(e.ref_tilt);
+            q[92] = std::sin(e.ref_tilt);
+            q[93] = L;
             break;
         }
         case BMX_RFCAVITY: {

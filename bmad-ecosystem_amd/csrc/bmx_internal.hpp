@@ -15,7 +15,7 @@ hipError_t launch_track_tiles(hipStream_t s, int grid, int waves, const Group *d
                               const WaveTile *d_tiles, const uint32_t *d_n_tiles,
                               const uint32_t *d_list_in, uint32_t *d_list_out, uint32_t *d_cnt_out,
                               uint32_t *d_tile_ctr, int32_t chunk_off, int32_t chunk_turns,
-                              unsigned long long *d_work, bool fast, bool span);
+                              unsigned long long *d_work, bool fast, bool span, bool ext = false);
 hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_groups,
                               const uint32_t *d_cnt_in, int32_t next_off, WaveTile *d_tiles,
                               uint32_t *d_n_tiles, uint32_t *d_cnt_zero, uint32_t *d_tile_ctr,
@@ -23,10 +23,14 @@ hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_gr
 hipError_t launch_collect_alive(hipStream_t s, int grid, const Group *d_groups,
                                 const WaveTile *d_tiles, const uint32_t *d_n_tiles,
                                 uint32_t *d_list_out, uint32_t *d_cnt_out);
-hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu, bool span = false);
+hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu, bool span = false,
+                                 bool ext = false);
 // Waves per SIMD of the kernel for runs with element spans (Group first/count other than the
 // full turn from element 0).
 constexpr int kSpanWaves = 3;
+// Waves per SIMD of the generic kernels for lattices with stepped quadrupoles or general bends
+// (the register budget of 2 waves, so that their long maps do not spill).
+constexpr int kExtWaves = 2;
 hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, int32_t turn,
                                 const BeamPtrs &b, int64_t n, unsigned long long *d_work,
                                 bool fast);
@@ -119,7 +123,8 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
 // Default waves per SIMD of the tracking kernel: the variant measured fastest on the 901-element
 // ring for each kernel (cell-specialised 4: at most 128 VGPRs, so 4 waves per SIMD are resident,
 // without spilling; generic fast maps 5; reference maps 3, whose 4- and 5-wave builds spill).
-// Cell kernels that carry the stepped quadrupole take 3: their 4-wave build spills.
+// Cell kernels that carry the stepped quadrupole or the general bend are all built for 2 waves per
+// SIMD (their longer maps would spill otherwise), whichever variant is asked for.
 inline int32_t default_waves(bool fast, const JitKernels *jit = nullptr)
 {
     return jit ? (jit_rare_q(jit) ? 3 : 4) : (fast ? 5 : 3);
@@ -127,7 +132,7 @@ inline int32_t default_waves(bool fast, const JitKernels *jit = nullptr)
 // Workgroups of the tracking kernel that fit on the device at once (of the cell-specialised
 // kernels when jit is set).
 int default_grid(int device, bool fast, int waves, const JitKernels *jit = nullptr,
-                 bool span = false);
+                 bool span = false, bool ext = false);
 
 }  // namespace bmx
 

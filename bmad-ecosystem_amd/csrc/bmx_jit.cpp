@@ -31,7 +31,7 @@ namespace bmx {
 struct JitKernels {
     hipModule_t module = nullptr;
     hipFunction_t fn[3] = {};  // waves per SIMD 3, 4, 5
-    bool rare_q = false;       // carries the stepped quadrupole (K_QMULT) on its rare path
+    bool rare_q = false;       // carries K_QMULT / K_BMULT on its rare path (all built for 2 waves)
 };
 
 namespace {
@@ -120,7 +120,7 @@ std::string generate(const HostTable &t, const Period &pd, std::string &sig)
         "uint32_t *__restrict__ co, uint32_t *tc, int32_t off, int32_t K, unsigned long long *w)";
     for (int w = 3; w <= 5; w++)
         s << "extern \"C\" __global__ void __launch_bounds__(bmx::kBlock) "
-          << "__attribute__((amdgpu_waves_per_eu(" << w << "))) bmx_cell_tiles_" << w << args
+          << "__attribute__((amdgpu_waves_per_eu(" << (t.rare_q ? 2 : w) << "))) bmx_cell_tiles_" << w << args
           << "\n{\n    bmx::track_tiles<true, bmx::CellTurn>(g, t, nt, li, lo, co, tc, off, K, w);\n}\n";
     return s.str();
 }

@@ -17,16 +17,16 @@ namespace bmx {
 // W: waves per SIMD the register allocation targets at least (3: up to 168 VGPRs; 4: up to 128;
 // 5: up to 102, spilling a few values of the rf and bend paths to scratch). SPAN: groups may
 // track element spans (runs with a start element or a partial turn); the full-turn kernels
-// carry no code for them. The span kernels also carry the stepped quadrupole (K_QMULT) on their
-// rare path and track full turns of lattices that hold one.
-template <bool F, int W, bool SPAN = false>
+// carry no code for them. RARE: the map set of the rare path; MAPS_RARE_Q (with W = 2) for
+// lattices that hold a stepped quadrupole or a general bend.
+template <bool F, int W, bool SPAN = false, int RARE = MAPS_RARE>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) k_track_tiles(
     const Group *__restrict__ groups, const WaveTile *__restrict__ tiles,
     const uint32_t *__restrict__ n_tiles_p, const uint32_t *__restrict__ list_in,
     uint32_t *__restrict__ list_out, uint32_t *__restrict__ cnt_out, uint32_t *tile_ctr,
     int32_t chunk_off, int32_t chunk_turns, unsigned long long *work)
 {
-    track_tiles<F, GenericTurn<F, SPAN ? MAPS_RARE_Q : MAPS_RARE>, SPAN>(
+    track_tiles<F, GenericTurn<F, RARE>, SPAN>(
         groups, tiles, n_tiles_p, list_in, list_out, cnt_out, tile_ctr, chunk_off, chunk_turns, work);
 }
 
@@ -121,8 +121,9 @@ __global__ void __launch_bounds__(kBlock) k_collect_alive(
     }
 }
 
-// One element, whole bunch: track1 (track1.f90:44-343) per live particle.
-template <bool F>
+// One element, whole bunch: track1 (track1.f90:44-343) per live particle. Q: the kernel carries
+// the stepped quadrupole and the general bend (built for lattices that hold one).
+template <bool F, bool Q>
 __device__ __forceinline__ void track_element_pass(const LatDesc &L, int32_t ie, int32_t turn,
                                                    const BeamPtrs &b, int64_t n,
                                                    unsigned long long *work)
@@ -134,7 +135,8 @@ __device__ __forceinline__ void track_element_pass(const LatDesc &L, int32_t ie,
         load_particle(p, b, i);
         refresh_energy(p, L.sp);
         if (!orbit_too_large(p))
-            track_element<F>(p, load_ele((CHdr)L.hdr + 4 * ie), (CP)L.prm, L.sp, false, ie + 1, turn);
+            track_element<F, Q ? MAPS_ALL_Q : MAPS_ALL>(p, load_ele((CHdr)L.hdr + 4 * ie), (CP)L.prm, L.sp,
+                                                      false, ie + 1, turn);
         store_particle(p, b, i);
         if (p.state != ST_ALIVE) {
             b.state[i] = p.state;
@@ -157,23 +159,23 @@ __device__ __forceinline__ void track_element_pass(const LatDesc &L, int32_t ie,
     }
 }
 
-template <bool F>
+template <bool F, bool Q>
 __global__ void __launch_bounds__(kBlock) k_track_element(LatDesc L, int32_t ie, int32_t turn,
                                                           BeamPtrs b, int64_t n,
                                                           unsigned long long *work)
 {
-    track_element_pass<F>(L, ie, turn, b, n, work);
+    track_element_pass<F, Q>(L, ie, turn, b, n, work);
 }
 
 // The same with the turn index read from device memory: the launches of one turn captured once
 // in a graph serve every turn, the turn advanced by k_step_turn at the end of each replay.
-template <bool F>
+template <bool F, bool Q>
 __global__ void __launch_bounds__(kBlock) k_track_element_at(LatDesc L, int32_t ie,
                                                              const int32_t *__restrict__ turn_p,
                                                              BeamPtrs b, int64_t n,
                                                              unsigned long long *work)
 {
-    track_element_pass<F>(L, ie, *turn_p, b, n, work);
+    track_element_pass<F, Q>(L, ie, *turn_p, b, n, work);
 }
 
 // turn_p = v (set_v != 0) or turn_p += 1: one thread.
@@ -200,7 +202,7 @@ __device__ __forceinline__ void record_orbit(double *r, const Particle &p, bool 
     r[7] = (double)p.state;
 }
 
-template <bool F>
+template <bool F, bool Q>
 __global__ void __launch_bounds__(kBlock) k_track_range(LatDesc L, int32_t e0, int32_t e1,
                                                         int32_t turn, BeamPtrs b, int64_t n,
                                                         double *__restrict__ rec,
@@ -221,7 +223,7 @@ __global__ void __launch_bounds__(kBlock) k_track_range(LatDesc L, int32_t e0, i
         } else {
             for (int32_t e = e0; e <= e1; ++e) {
                 const bool at_entrance =
-                    track_element<F>(p, load_ele((CHdr)L.hdr + 4 * e), (CP)L.prm, L.sp, e > e0, e + 1,
+                    track_element<F, Q ? MAPS_ALL_Q : MAPS_ALL>(p, load_ele((CHdr)L.hdr + 4 * e), (CP)L.prm, L.sp, e > e0, e + 1,
                                      turn);
                 passes++;
                 record_orbit(rec + (i * ne + (e - e0)) * 8, p, at_entrance);
@@ -257,12 +259,12 @@ __global__ void __launch_bounds__(kBlock) k_count_alive(const int32_t *__restric
     }
 }
 
-template <bool F, int W, bool SPAN = false>
+template <bool F, int W, bool SPAN = false, int RARE = MAPS_RARE>
 static void launch_tiles(hipStream_t s, int grid, const Group *g, const WaveTile *t, const uint32_t *nt,
                          const uint32_t *li, uint32_t *lo, uint32_t *co, uint32_t *tc, int32_t off,
                          int32_t K, unsigned long long *w)
 {
-    hipLaunchKernelGGL((k_track_tiles<F, W, SPAN>), dim3((unsigned)grid), dim3(kBlock), 0, s, g, t, nt,
+    hipLaunchKernelGGL((k_track_tiles<F, W, SPAN, RARE>), dim3((unsigned)grid), dim3(kBlock), 0, s, g, t, nt,
                        li, lo, co, tc, off, K, w);
 }
 
@@ -270,14 +272,22 @@ hipError_t launch_track_tiles(hipStream_t s, int grid, int waves, const Group *d
                               const WaveTile *d_tiles, const uint32_t *d_n_tiles,
                               const uint32_t *d_list_in, uint32_t *d_list_out, uint32_t *d_cnt_out,
                               uint32_t *d_tile_ctr, int32_t chunk_off, int32_t chunk_turns,
-                              unsigned long long *d_work, bool fast, bool span)
+                              unsigned long long *d_work, bool fast, bool span, bool ext)
 {
     if (grid <= 0) return hipSuccess;
 #define BMX_TILES(F, W, ...)                                                                     \
     launch_tiles<F, W, ##__VA_ARGS__>(s, grid, d_groups, d_tiles, d_n_tiles, d_list_in,          \
                                       d_list_out, d_cnt_out, d_tile_ctr, chunk_off, chunk_turns, \
                                       d_work)
-    if (span) {
+    if (ext) {
+        if (span) {
+            if (fast) BMX_TILES(true, kExtWaves, true, MAPS_RARE_Q);
+            else BMX_TILES(false, kExtWaves, true, MAPS_RARE_Q);
+        } else {
+            if (fast) BMX_TILES(true, kExtWaves, false, MAPS_RARE_Q);
+            else BMX_TILES(false, kExtWaves, false, MAPS_RARE_Q);
+        }
+    } else if (span) {
         if (fast) BMX_TILES(true, kSpanWaves, true);
         else BMX_TILES(false, kSpanWaves, true);
     } else if (fast) {
@@ -313,10 +323,16 @@ hipError_t launch_collect_alive(hipStream_t s, int grid, const Group *d_groups,
     return hipGetLastError();
 }
 
-hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu, bool span)
+hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu, bool span, bool ext)
 {
     const void *k;
-    if (span)
+    if (ext && span)
+        k = fast ? (const void *)k_track_tiles<true, kExtWaves, true, MAPS_RARE_Q>
+                 : (const void *)k_track_tiles<false, kExtWaves, true, MAPS_RARE_Q>;
+    else if (ext)
+        k = fast ? (const void *)k_track_tiles<true, kExtWaves, false, MAPS_RARE_Q>
+                 : (const void *)k_track_tiles<false, kExtWaves, false, MAPS_RARE_Q>;
+    else if (span)
         k = fast ? (const void *)k_track_tiles<true, kSpanWaves, true>
                  : (const void *)k_track_tiles<false, kSpanWaves, true>;
     else if (fast) k = waves == 5 ? (const void *)k_track_tiles<true, 5> : waves == 4 ? (const void *)k_track_tiles<true, 4> : (const void *)k_track_tiles<true, 3>;
@@ -330,12 +346,16 @@ hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, i
 {
     int64_t grid = (n + kBlock - 1) / kBlock;
     if (grid <= 0) return hipSuccess;
-    if (fast)
-        hipLaunchKernelGGL(k_track_element<true>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat, ie,
-                           turn, b, n, d_work);
-    else
-        hipLaunchKernelGGL(k_track_element<false>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat, ie,
-                           turn, b, n, d_work);
+#define BMX_ELEMENT(F, Q) \
+    hipLaunchKernelGGL((k_track_element<F, Q>), dim3((unsigned)grid), dim3(kBlock), 0, s, lat, ie, turn, b, n, d_work)
+    if (lat.rare_q) {
+        if (fast) BMX_ELEMENT(true, true);
+        else BMX_ELEMENT(false, true);
+    } else {
+        if (fast) BMX_ELEMENT(true, false);
+        else BMX_ELEMENT(false, false);
+    }
+#undef BMX_ELEMENT
     return hipGetLastError();
 }
 
@@ -345,12 +365,16 @@ hipError_t launch_track_element_at(hipStream_t s, const LatDesc &lat, int32_t ie
 {
     int64_t grid = (n + kBlock - 1) / kBlock;
     if (grid <= 0) return hipSuccess;
-    if (fast)
-        hipLaunchKernelGGL(k_track_element_at<true>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat,
-                           ie, d_turn, b, n, d_work);
-    else
-        hipLaunchKernelGGL(k_track_element_at<false>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat,
-                           ie, d_turn, b, n, d_work);
+#define BMX_ELEMENT(F, Q) \
+    hipLaunchKernelGGL((k_track_element_at<F, Q>), dim3((unsigned)grid), dim3(kBlock), 0, s, lat, ie, d_turn, b, n, d_work)
+    if (lat.rare_q) {
+        if (fast) BMX_ELEMENT(true, true);
+        else BMX_ELEMENT(false, true);
+    } else {
+        if (fast) BMX_ELEMENT(true, false);
+        else BMX_ELEMENT(false, false);
+    }
+#undef BMX_ELEMENT
     return hipGetLastError();
 }
 
@@ -366,12 +390,16 @@ hipError_t launch_track_range(hipStream_t s, const LatDesc &lat, int32_t e0, int
 {
     int64_t grid = (n + kBlock - 1) / kBlock;
     if (grid <= 0) return hipSuccess;
-    if (fast)
-        hipLaunchKernelGGL(k_track_range<true>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat, e0,
-                           e1, turn, b, n, d_rec, d_work);
-    else
-        hipLaunchKernelGGL(k_track_range<false>, dim3((unsigned)grid), dim3(kBlock), 0, s, lat, e0,
-                           e1, turn, b, n, d_rec, d_work);
+#define BMX_RANGE(F, Q) \
+    hipLaunchKernelGGL((k_track_range<F, Q>), dim3((unsigned)grid), dim3(kBlock), 0, s, lat, e0, e1, turn, b, n, d_rec, d_work)
+    if (lat.rare_q) {
+        if (fast) BMX_RANGE(true, true);
+        else BMX_RANGE(false, true);
+    } else {
+        if (fast) BMX_RANGE(true, false);
+        else BMX_RANGE(false, false);
+    }
+#undef BMX_RANGE
     return hipGetLastError();
 }
 

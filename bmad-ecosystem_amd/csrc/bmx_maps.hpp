@@ -1326,13 +1326,14 @@ __device__ __forceinline__ bool quad_multipole(Particle &p, CP q, int32_t n_step
 // MAPS_RARE for loops that keep the rare, register-hungry cases out of the common element
 // path: MAPS_NO_RF has no rf cavity, no offset_particle frame change and no collimator
 // momentum windows; MAPS_RARE has only the kinds that can carry them (no bends) with all three;
-// MAPS_RARE_Q adds the stepped quadrupole with multipoles (K_QMULT), for loops over lattices
-// that hold one (its code would otherwise shape every rare path's register assignment).
+// MAPS_ALL_Q and MAPS_RARE_Q add the stepped quadrupole (K_QMULT) and the general bend
+// (K_BMULT) to MAPS_ALL and MAPS_RARE, for kernels built for lattices that hold one (their
+// code would otherwise shape the register assignment of every kernel).
 // from_prev: the particle has just left the element before this one in the same lattice, so
 // an entrance test that repeats that element's exit test (F_APE_ENT_SAME, F_APE_ENT cleared)
 // is skipped; otherwise it is made.
 // A particle lost here gets p.loss_ele = ix1 (the element's 1-based index), p.loss_turn = turn.
-enum MapSet : int { MAPS_ALL = 0, MAPS_NO_RF = 1, MAPS_RARE = 2, MAPS_RARE_Q = 3 };
+enum MapSet : int { MAPS_ALL = 0, MAPS_NO_RF = 1, MAPS_RARE = 2, MAPS_RARE_Q = 3, MAPS_ALL_Q = 4 };
 
 // ---- offset_particle, non-bend branch (bmad/code/offset_particle.f90:183-492) -------------
 // Element orientation +1, particle direction +1; ob is the offset block (bmx_table.hpp).
@@ -1414,6 +1415,313 @@ __device__ __forceinline__ void offset_unset(Particle &p, CP ob, int32_t flags)
     }
 }
 
+// ---- the general sector bend -------------------------------------------------------------
+
+// bmad/modules/fringe_mod.f90:1713-1771 bmad_rot_xz on (x, px, y, py, z) with pp = 1 + pz: the
+// frame rotated by a (ca = cos a, sa = sin a) about y. False when ps^2 <= 0.
+__device__ __forceinline__ bool edge_rot_xz(double *v, double pp, double ca, double sa)
+{
+    const double x = v[0], px = v[1], py = v[3];
+    double ps = pp * pp - px * px - py * py;
+    if (ps <= 0) return false;
+    ps = sqrt(ps);
+    const double den = ps * ca - px * sa;
+    const double L = x * sa / den;
+    v[0] = x * ps / den;
+    v[1] = px * ca + ps * sa;
+    v[2] = v[2] + py * L;
+    v[4] = v[4] - pp * L;
+    return true;
+}
+
+// fringe_mod.f90:1588-1690 bmad_wedger with g_tot /= 0: a wedge of angle a (ca, sa, s2a =
+// sin 2a) through the dipole field b1 = g_tot. False on its error exits.
+__device__ __forceinline__ bool edge_wedger(double *v, double pp, double a, double ca, double sa,
+                                            double s2a, double b1)
+{
+    const double x = v[0], px = v[1], py = v[3];
+    double ps = pp * pp - px * px - py * py;
+    if (ps <= 0) return false;
+    ps = sqrt(ps);
+    double pt = pp * pp - py * py;
+    if (pt < 1e-10) return false;
+    pt = sqrt(pt);
+    const double s2 = sa * sa;
+    const double pxn = px * ca + (ps - b1 * x) * sa;
+    double psn = pt * pt - pxn * pxn;
+    if (psn < 1e-10) return false;
+    psn = sqrt(psn);
+    const double den = psn + ps * ca - px * sa;
+    const double num = x * px * s2a + s2 * (2 * x * ps - b1 * x * x);
+    const double L = (a + asin(px / pt) - asin(pxn / pt)) / b1;
+    v[0] = x * ca + num / den;
+    v[1] = pxn;
+    v[2] = v[2] + py * L;
+    v[4] = v[4] - pp * L;
+    return true;
+}
+
+// fringe_mod.f90:1797-1933 bmad_fringe_dipoler: the hard-edge dipole fringe with b = +-g_tot,
+// fint hgap = fh, inv_b02 = 1/beta_ref^2. False on its error exits.
+__device__ __forceinline__ bool edge_dipoler(double *v, double pp, double b, double fh,
+                                             double inv_b02)
+{
+    const double x = v[0], px = v[1], y = v[2], py = v[3], z = v[4];
+    double ps = pp * pp - px * px - py * py;
+    if (ps <= 0) return false;
+    ps = sqrt(ps);
+    const double ee = sqrt(pp * pp + inv_b02 - 1);
+    const double beta = pp / ee;
+    const double xp = px / ps, yp = py / ps;
+    const double kap = b * fh;
+    const double fac2 = 1 + yp * yp;
+    const double d11 = (1 + xp * xp) / ps, d21 = xp * yp / ps, d31 = -xp;
+    const double d12 = xp * yp / ps, d22 = fac2 / ps, d32 = -yp;
+    const double d13 = -ee * xp / (ps * ps), d23 = -ee * yp / (ps * ps), d33 = ee / ps;
+    const double w2 = 1 + xp * xp * (2 + yp * yp);
+    const double fi0 = atan(xp / fac2) - 2 * kap * w2 * ps;
+    const double cf = cos(fi0);
+    const double co2 = b / (cf * cf);
+    const double co1 = co2 / (1 + (xp / fac2) * (xp / fac2));
+    const double fi1 = co1 / fac2 - 4 * kap * co2 * xp * (2 + yp * yp) * ps;
+    const double fi2 = -2 * co1 * xp * yp / (fac2 * fac2) - 4 * kap * co2 * xp * xp * yp * ps;
+    const double fi3 = -2 * kap * co2 * w2;
+    const double fi0t = b * tan(fi0);
+    const double bb2 = fi1 * d12 + fi2 * d22 + fi3 * d32;
+    double sq = 1 - 2 * bb2 * y;
+    if (sq < 0) return false;
+    sq = sqrt(sq);
+    const double ynew = 2 * y / (1 + sq);
+    const double bb1 = fi1 * d11 + fi2 * d21 + fi3 * d31;
+    const double bb3 = fi1 * d13 + fi2 * d23 + fi3 * d33;
+    v[0] = x + 0.5 * bb1 * ynew * ynew;
+    v[2] = ynew;
+    v[3] = py - fi0t * ynew;
+    v[4] = z + 0.5 * beta * bb3 * ynew * ynew;
+    return true;
+}
+
+// fringe_mod.f90:1474-1562 exact_bend_edge_kick: at the entrance the wedge of the edge angle
+// without field, the fringe, the wedge back through the field g_tot; at the exit the reverse.
+// On any error lost_pz with the coordinates unchanged.
+__device__ __noinline__ void exact_bend_edge(Particle &p, CP q, bool entering)
+{
+    const double g_tot = q[29];
+    const double pp = 1 + p.pz;
+    double v[6] = {p.x, p.px, p.y, p.py, p.z, p.pz};
+    const CP e = q + (entering ? 56 : 60);  // cos, sin, sin 2a, fint hgap
+    const double a = entering ? q[65] : q[66];
+    bool ok;
+    if (entering) {
+        ok = edge_rot_xz(v, pp, e[0], e[1]);  // bmad_wedger without field
+        ok = ok && edge_dipoler(v, pp, g_tot, e[3], q[64]);
+        ok = ok && (g_tot == 0 ? edge_rot_xz(v, pp, e[0], -e[1]) : edge_wedger(v, pp, -a, e[0], -e[1], -e[2], g_tot));
+    } else {
+        ok = g_tot == 0 ? edge_rot_xz(v, pp, e[0], -e[1]) : edge_wedger(v, pp, -a, e[0], -e[1], -e[2], g_tot);
+        ok = ok && edge_dipoler(v, pp, -g_tot, e[3], q[64]);
+        ok = ok && edge_rot_xz(v, pp, e[0], e[1]);
+    }
+    if (!ok) {
+        p.state = ST_LOST_PZ;
+        return;
+    }
+    p.x = v[0];
+    p.px = v[1];
+    p.y = v[2];
+    p.py = v[3];
+    p.z = v[4];
+}
+
+// The bend's edge at one end (apply_element_edge_kick.f90:64-189,270-271 with bend_edge_kick,
+// fringe_mod.f90:40-128): the electric edge before the magnetic one at the exit and after it at
+// the entrance; the magnetic edge is the hard multipole edge then exact_bend_edge_kick (fringe
+// full; the hard multipole edge after it at the exit) or hwang_bend_edge_kick.
+__device__ __forceinline__ void bend_general_edge(Particle &p, CP q, int32_t flags, int32_t emask,
+                                                  bool entering, const Species &sp)
+{
+    if (!entering && emask) elec_fringe(p, q + 47, q + 51, emask, -1.0, sp);
+    if (flags & (entering ? F_EDGE_ENT : F_EDGE_EXIT)) {
+        if (flags & F_BEXACT) {
+            if (entering) quad_hard_edge(p, q[55], true);
+            exact_bend_edge(p, q, entering);
+            if (!entering) quad_hard_edge(p, q[55], false);
+        } else {
+            hwang_edge(p, q + (entering ? 10 : 16), q[entering ? 25 : 26], q[29], entering,
+                       flags & F_EDGE_SIMPLE);
+        }
+    }
+    if (entering && emask) elec_fringe(p, q + 47, q + 51, emask, 1.0, sp);
+}
+
+// The bend's multipole kicks (track_a_bend.f90:336-467, not exact): the magnetic kick
+// coef r_step (1 + g x) (kx, ky) of the multipoles, then the electric one, step_len coef
+// (1 + g x) charge / (ps beta p0c) E with ps = sqrt(1+pz)^2 - px^2 - py^2) / (1+pz), which
+// changes the energy; factor = 2 coef (the table holds the half-step strengths).
+__device__ __forceinline__ void bend_general_kicks(Particle &p, CP q, int32_t mmask, int32_t emask,
+                                                   double factor, const Species &sp)
+{
+    const double g = q[1];
+    double kx, ky;
+    if (mmask) {
+        multipole_sum(p.x, p.y, q + 31, q + 35, mmask, kx, ky);
+        const double f = factor * fma(g, p.x, 1.0);
+        p.px = fma(f, kx, p.px);
+        p.py = fma(f, ky, p.py);
+    }
+    if (emask) {
+        multipole_sum(p.x, p.y, q + 39, q + 43, emask, kx, ky);
+        const double rel_p0 = p.rel_p, px0 = p.px, py0 = p.py;
+        const double ps2 = rel_p0 * rel_p0 - px0 * px0 - py0 * py0;
+        const double ps = sqrt(ps2) / rel_p0;
+        const double f = factor * fma(g, p.x, 1.0) / (ps * p.beta);
+        kx = kx * f;
+        ky = ky * f;
+        const double alpha = (kx * (2 * px0 + kx) + ky * (2 * py0 + ky)) / (rel_p0 * rel_p0);
+        p.px = px0 + kx;
+        p.py = py0 + ky;
+        p.pz = p.pz + rel_p0 * sqrt_one(alpha);
+        const double beta_new =
+            (1 + p.pz) / sqrt(ps2 + p.px * p.px + p.py * p.py + sqr(sp.mc2 / sp.p0c));
+        p.z = p.z * beta_new / p.beta;
+        p.beta = beta_new;
+        refresh_energy(p, sp);
+    }
+}
+
+// track_a_bend.f90:500-659 sbend_body_with_k1_map: one step of a bend with a gradient k1 = b1/L
+// (q[30]), g and g_tot = g + dg of the body block; then the low-energy z correction.
+__device__ __noinline__ void sbend_k1_step(Particle &p, CP q)
+{
+    const double length = q[0], g = q[1], g_tot = q[2], k1 = q[30];
+    const double rel_p = p.rel_p, rel_p2 = rel_p * rel_p;
+    const double k_x = k1 + g * g_tot;
+    const double x_c = (g * rel_p - g_tot) / k_x;
+    const double om_x = sqrt(fabs(k_x) / rel_p);
+    const double om_y = sqrt(fabs(k1) / rel_p);
+    const double tau_x = -copysign(1.0, k_x);
+    const double tau_y = copysign(1.0, k1);
+    double s_x, c_x, z2, s_y, c_y;
+    double arg = om_x * length;
+    if (arg < 1e-6) {
+        s_x = (1 + tau_x * arg * arg / 6) * length;
+        c_x = 1 + tau_x * arg * arg / 2;
+        z2 = g * length * length / (2 * rel_p);
+    } else {
+        if (k_x > 0) {
+            s_x = sin(arg) / om_x;
+            c_x = cos(arg);
+        } else {
+            s_x = sinh(arg) / om_x;
+            c_x = cosh(arg);
+        }
+        z2 = tau_x * g * (1 - c_x) / (rel_p * om_x * om_x);
+    }
+    arg = om_y * length;
+    if (arg < 1e-6) {
+        s_y = (1 + tau_y * arg * arg / 6) * length;
+        c_y = 1 + tau_y * arg * arg / 2;
+    } else if (k1 < 0) {
+        s_y = sin(arg) / om_y;
+        c_y = cos(arg);
+    } else {
+        s_y = sinh(arg) / om_y;
+        c_y = cosh(arg);
+    }
+    const double r0 = p.x - x_c, r1 = p.px, r2 = p.y, r3 = p.py;
+    const double ox2 = om_x * om_x, oy2 = om_y * om_y;
+    const double z0 = -g * x_c * length;
+    const double z1 = -g * s_x;
+    const double z11 = tau_x * ox2 * (length - c_x * s_x) / 4;
+    const double z12 = -tau_x * ox2 * s_x * s_x / (2 * rel_p);
+    const double z22 = -(length + c_x * s_x) / (4 * rel_p2);
+    const double z33 = tau_y * oy2 * (length - c_y * s_y) / 4;
+    const double z34 = -tau_y * oy2 * s_y * s_y / (2 * rel_p);
+    const double z44 = -(length + c_y * s_y) / (4 * rel_p2);
+    p.x = c_x * r0 + s_x * r1 / rel_p + x_c;
+    p.px = tau_x * ox2 * rel_p * s_x * r0 + c_x * r1;
+    p.y = c_y * r2 + s_y * r3 / rel_p;
+    p.py = tau_y * oy2 * rel_p * s_y * r2 + c_y * r3;
+    p.z = p.z + z0 + z1 * r0 + z2 * r1 + z11 * r0 * r0 + z12 * r0 * r1 + z22 * r1 * r1 +
+          z33 * r2 * r2 + z34 * r2 * r3 + z44 * r3 * r3;
+    p.z = p.z + length * p.lez;
+}
+
+// orbit_too_large inside the general bend, where pz may have changed (electric fields).
+__device__ __forceinline__ bool bend_orbit_flagged(Particle &p)
+{
+    if (any_lane(orbit_bad(p))) {
+        if (orbit_bad(p)) {
+            orbit_too_large(p);
+            return true;
+        }
+    }
+    return false;
+}
+
+// bmad/low_level/track_a_bend.f90:18-333, the general sector bend: the frame change of
+// offset_particle's bend branch (or the ref_tilt rotation alone), the entrance edge,
+// orbit_too_large, half kicks, n_step body steps (the k1 map, a drift for g = dg = 0, else the
+// semi-linear or exact body of K_BEND with g_tot = g + dg) each followed by a kick (half after
+// the last), orbit_too_large, the exit edge, the frame change back. A particle the body loses,
+// or orbit_too_large flags, leaves without the exit edge and frame change, as in the reference.
+template <bool F>
+__device__ __forceinline__ void bend_general(Particle &p, CP q, int32_t n_step, int32_t flags,
+                                             const Species &sp)
+{
+    const int32_t mmask = (flags >> F_ORDER_SHIFT) & 0xF;
+    const int32_t emask = (flags >> F_ELEC_SHIFT) & 0xF;
+    if (flags & F_MISALIGN) {
+        double s_body;
+        offset_frame(p, q + 67, p.x - q[76], p.y - q[77], 0.0 - q[78], 0.0, 0.0, s_body);
+        const double ds = 0.0 - s_body;
+        if (p.state == ST_ALIVE && fabs(ds) > 1e-14) drift_edge(p, ds);
+    } else if (flags & F_TILT) {
+        tilt_rotate(p, q[91], q[92]);
+    }
+    if (flags & F_QEDGE_ENT) bend_general_edge(p, q, flags, emask, true, sp);
+    if (bend_orbit_flagged(p)) return;
+    if (mmask | emask) bend_general_kicks(p, q, mmask, emask, 1.0, sp);
+    for (int i = 1; i <= n_step; i++) {
+        if (flags & F_BK1) {
+            sbend_k1_step(p, q);
+        } else if (flags & F_BEND_DRIFT) {
+            if (q[0] != 0) drift_exact(p, q[0]);
+        } else {
+            const int32_t st0 = p.state;
+            const double g = q[1];
+            bool semi = false;
+            if (flags & F_BSEMI) {
+                semi = (fabs(p.x * g) < 1e-9) & (fabs(p.px) < 1e-9) & (fabs(p.py) < 1e-9) &
+                       (fabs(p.pz) < 1e-9);
+                if (semi) sbend_semilinear(p, q, sp);
+            }
+            if (!semi) {
+                if (F && (flags & F_BEND_SERIES)) {
+                    const bool done = sbend_momentum_form(p, q);
+                    if (!done) sbend_exact_fast(p, q);
+                } else if (F) {
+                    sbend_exact_fast(p, q);
+                } else {
+                    sbend_exact_ref(p, q);
+                }
+            }
+            if (p.state != st0) return;
+        }
+        if (mmask | emask) bend_general_kicks(p, q, mmask, emask, i == n_step ? 1.0 : 2.0, sp);
+    }
+    if (bend_orbit_flagged(p)) return;
+    if (flags & F_QEDGE_EXIT) bend_general_edge(p, q, flags, emask, false, sp);
+    if (flags & F_MISALIGN) {
+        double s_lab;
+        offset_frame(p, q + 79, p.x, p.y, 0.0, q[88], q[89], s_lab);
+        s_lab = s_lab + q[90];
+        const double ds = q[93] - s_lab;
+        if (p.state == ST_ALIVE && fabs(ds) > 1e-14) drift_edge(p, ds);
+    } else if (flags & F_TILT) {
+        tilt_rotate(p, q[91], -q[92]);
+    }
+}
+
 template <bool F, int MAPS = MAPS_ALL>
 __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, const Species &sp,
                                               bool from_prev, int32_t ix1, int32_t turn)
@@ -1461,7 +1769,10 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
         if constexpr (MAPS != MAPS_NO_RF) rfcavity<F>(p, q, n, sp);
         break;
     case K_QMULT:
-        if constexpr (MAPS == MAPS_ALL || MAPS == MAPS_RARE_Q) unset = quad_multipole<F>(p, q, n, flags, sp);
+        if constexpr (MAPS == MAPS_ALL_Q || MAPS == MAPS_RARE_Q) unset = quad_multipole<F>(p, q, n, flags, sp);
+        break;
+    case K_BMULT:
+        if constexpr (MAPS == MAPS_ALL_Q || MAPS == MAPS_RARE_Q) bend_general<F>(p, q, n, flags, sp);
         break;
     default: break;
     }

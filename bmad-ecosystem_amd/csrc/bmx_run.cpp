@@ -104,12 +104,13 @@ void KernelTimer::release()
     pool.clear();
 }
 
-int default_grid(int device, bool fast, int waves, const JitKernels *jit, bool span)
+int default_grid(int device, bool fast, int waves, const JitKernels *jit, bool span, bool ext)
 {
     int blocks = 0, cus = 0;
-    const hipError_t e = span  ? track_tiles_occupancy(fast, kSpanWaves, &blocks, true)
-                         : jit ? jit_occupancy(jit, waves, &blocks)
-                               : track_tiles_occupancy(fast, waves, &blocks);
+    const hipError_t e = ext   ? track_tiles_occupancy(fast, kExtWaves, &blocks, span, true)
+                         : span ? track_tiles_occupancy(fast, kSpanWaves, &blocks, true)
+                         : jit  ? jit_occupancy(jit, waves, &blocks)
+                                : track_tiles_occupancy(fast, waves, &blocks);
     if (e != hipSuccess || blocks <= 0) blocks = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
         cus = 1;
@@ -145,13 +146,14 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
     if (e == hipSuccess)
         e = launch_build_tiles(s, rb.d_groups, n_groups, rb.d_cnt[0], 0, rb.d_tiles[0], n_tiles[0],
                                rb.d_cnt[1], tile_ctr, rb.d_goff);
-    // Lattices with stepped quadrupoles (K_QMULT) run on cell kernels generated with them or on
-    // the generic span kernels, which carry them; the other full-turn kernels do not.
+    // Lattices with stepped quadrupoles or general bends (K_QMULT, K_BMULT) run on cell kernels
+    // generated with them or on the generic kernels built with them (ext); the other kernels
+    // do not carry them.
     bool rare_q = false;
     for (int32_t g = 0; g < n_groups; g++)
         if (h_groups[g].n_turns > 0 && h_groups[g].lat.rare_q) rare_q = true;
     const bool use_jit = cfg.jit && !cfg.span && (!rare_q || jit_rare_q(cfg.jit));
-    const bool span = cfg.span || (rare_q && !use_jit);
+    const bool ext = rare_q && !use_jit;
     int c = 0;
     for (int32_t off = 0; off < max_turns && e == hipSuccess; off += K, c++) {
         const int cur = c & 1, nxt = cur ^ 1;
@@ -169,7 +171,7 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
         else
             e = launch_track_tiles(s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur], n_tiles[cur],
                                    rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt], tile_ctr, off, K,
-                                   d_work, cfg.fast, span);
+                                   d_work, cfg.fast, cfg.span, ext);
         if (cfg.timer && e == hipSuccess) {
             e = hipEventRecord(ev1, s);
             cfg.timer->pending.emplace_back(ev0, ev1);

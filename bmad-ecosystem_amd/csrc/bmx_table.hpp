@@ -21,7 +21,12 @@ enum Kind : int32_t {
     // kick - quadrupole step - kick, electric hard-edge energy kicks at both ends
     // (track_a_quadrupole.f90:17-148)
     K_QMULT = 6,
-    K_NKIND = 7,
+    // sector bend with what K_BEND leaves out: offsets, pitches and ref_tilt (offset_particle's
+    // bend branch), multipoles and kicks (dg from their b0 part), k1 (the k1 body map), electric
+    // multipoles, the full edge (exact_bend_edge_kick); n_step steps of kick - body - kick
+    // (track_a_bend.f90:18-333)
+    K_BMULT = 7,
+    K_NKIND = 8,
 };
 
 enum Flags : int32_t {
@@ -70,6 +75,12 @@ enum Flags : int32_t {
     F_QEDGE_EXIT = 1 << 27,
     F_QHARD = 1 << 28,
     F_QSOFT = 1 << 29,
+    // K_BMULT (with F_QEDGE_* for the ends fringe_at names, F_EDGE_ENT / F_EDGE_EXIT where
+    // bend_edge_kick acts, F_EDGE_SIMPLE, F_BEND_DRIFT, F_BEND_SERIES as for K_BEND, the order
+    // masks as for K_QMULT, F_MISALIGN: the bend frame change, F_TILT: ref_tilt alone):
+    F_BSEMI = F_MULT_NORMAL,  // dg = 0: the semi-linear body near the zero orbit applies
+    F_BK1 = F_QSOFT,          // b1 /= 0: the k1 body map
+    F_BEXACT = F_QHARD,       // fringe_type full: hard multipole edge and exact_bend_edge_kick
 };
 constexpr int kOffsetLen = 24;
 
@@ -99,6 +110,17 @@ struct EleHdr {
 //            kick strengths without the particle's 1/beta); [21..24] charge a_n/(n+1),
 //            [25..28] charge b_n/(n+1) (electric edge potential); [29] k1/12 (hard edge,
 //            n = 1), [30] k1 fq1, [31] k1 fq2 (soft edge); hdr.n = n_step
+//  K_BMULT : [0..28] one body step as a K_BEND block (L = step_len, g_tot = g + dg with dg
+//            from the multipoles' b0, edge constants of the element); [29] g + the element's dg
+//            (the edges' g_tot); [30] k1 = b1/L; [31..34] a_n r_step/2, [35..38] b_n r_step/2
+//            (magnetic); [39..42] a_n charge/p0c step_len/2, [43..46] -b_n charge/p0c
+//            step_len/2 (electric field strengths); [47..50] charge a_n/(n+1), [51..54]
+//            charge b_n/(n+1) (electric edge potential); [55] k1/12 (hard multipole edge);
+//            entrance [56] cos e1, [57] sin e1, [58] sin 2 e1, [59] fint hgap, exit [60..63]
+//            likewise with e2, fintx hgapx; [64] 1/beta_ref^2; [65] e1, [66] e2; frame change
+//            [67..75] W (entering, row-major), [76..78] d (r = W ((x, y, 0) - d)), [79..87] W
+//            (leaving), [88..90] c (r = W (x, y, 0) + c); [91] cos ref_tilt, [92] sin
+//            ref_tilt, [93] L; hdr.n = n_step
 // preceded, when F_OFFSET, by the offset block (hdr.off points at it; then the aperture values,
 // then the parameters):
 //  [0] x_offset, [1] y_offset, [2] z_offset, [3] L/2, [4..12] W^-1 (entrance frame change,
@@ -107,7 +129,7 @@ struct EleHdr {
 // when F_APE_MOMENTUM by 8 momentum-window values:
 //  elliptical : x_center, 1/x_width2, y_center, 1/y_width2
 //  rectangular: x1_lim (= -x1_limit), x2_lim, y1_lim, y2_lim (0 = no limit)
-constexpr int kParams[K_NKIND] = {1, 5, 9, 29, 7, 0, 32};
+constexpr int kParams[K_NKIND] = {1, 5, 9, 29, 7, 0, 32, 94};
 
 struct Species {
     double mc2;      // [eV]

@@ -8,19 +8,18 @@ import parity as P
 
 pytestmark = pytest.mark.gpu
 
-# not on the GPU path: apertures that move with the offsets, rf cavity multipoles; bends with
-# offsets, multipoles, k1 or a full fringe
-OFF_PATH = {"RCOLLIMATOR1", "RFCAVITY3", "SBEND1", "SBEND3", "SBEND5", "SBEND7"}
+# not on the GPU path: apertures that move with the offsets, rf cavity multipoles
+OFF_PATH = {"RCOLLIMATOR1", "RFCAVITY3"}
 
 
 @pytest.mark.parametrize("maps", ["fast", "reference"])
 def test_tracking_method_known_answers_on_gpu(bmx, maps):
     """tracking_method_test "<ELE>: Bmad_Standard" lines (ABS 2e-14, written es18.10): drift,
     thick multipoles (sextupole, octupole, thick_multipole, kickers, collimator, instrument,
-    monitor) with offsets, pitch and tilt, the rf cavity, and quadrupoles with magnetic and
-    electric multipoles (stepped, electric edge energy kicks), each tracked by the GPU kernel
-    from the fixture's particle_start as a one-element lattice (the lattice's default species,
-    the positron)."""
+    monitor) with offsets, pitch and tilt, the rf cavity, quadrupoles with magnetic and
+    electric multipoles and edges, and sector bends with offsets, ref_tilt, multipoles, k1, k2
+    and the full edge, each tracked by the GPU kernel from the fixture's particle_start as a
+    one-element lattice (the lattice's default species, the positron)."""
     E, T = bmx.elements, bmx.tracking
     fx = kat.read_fixture("tracking_method_test.correct")
     eles = kat.tracking_method_elements(E)

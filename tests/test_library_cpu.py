@@ -61,11 +61,12 @@ def test_benchmark_rings_compile(bmx):
 
 
 @pytest.mark.parametrize("attrs,code", [
-    (dict(key=2, l=1.0, g=0.01, x_offset=1e-3), 2),       # misaligned bend
-    (dict(key=2, l=1.0, g=0.01, ref_tilt=0.1), 2),        # tilted bend
+    (dict(key=2, l=1.0, g=0.01, k1=0.1, exact_multipoles=2), 2),  # exact multipoles, curved bend
+    (dict(key=2, l=1.0, g=0.01, b_pole={5: 1.0}), 2),     # bend multipole above order 3
+    (dict(key=2, l=0.0, g=0.01, fringe_type=4), 2),       # zero-length bend with a full edge
+    (dict(key=2, l=1.0, g=0.01, fringe_type=2), 2),       # soft-edge-only bend fringe
+    (dict(key=2, l=1.0, g=0.01, x_offset=1e-3, x_limit=0.01, offset_moves_aperture=1), 2),
     (dict(key=3, l=0.3, k1=0.5, x_offset=1e-3, x_limit=0.01, offset_moves_aperture=1), 2),
-    (dict(key=2, l=1.0, g=0.01, k1=0.1), 2),              # bend with k1
-    (dict(key=2, l=1.0, g=0.01, fringe_type=4), 2),       # full bend fringe
     (dict(key=9, l=0.5, voltage=1e6, rf_frequency=0.0), 2),
     (dict(key=23, l=1.0), 2),                             # solenoid: not on this path
     (dict(key=1, l=1.0, aperture_at=5, x_limit=0.01), 2),  # continuous aperture
@@ -80,14 +81,14 @@ def test_outside_path_rejected(bmx, attrs, code):
     assert "element 1" in msg
 
 
-def test_electric_multipoles_only_on_quadrupoles(bmx):
-    """Electric multipoles compile on quadrupoles (the stepped map) and are rejected on every
-    other element kind and above order 3; a switched-off element ignores them."""
+def test_electric_multipoles_only_on_quadrupoles_and_bends(bmx):
+    """Electric multipoles compile on quadrupoles (the stepped map) and bends (the general bend)
+    and are rejected on every other element kind and above order 3; a switched-off element
+    ignores them."""
     E = bmx.elements
     p0c, et = bmx.lattices.reference()
     for key, attrs in ((E.SEXTUPOLE, dict(l=0.2, k2=1.0)), (E.DRIFT, dict(l=1.0)),
-                       (E.RFCAVITY, dict(l=0.5, voltage=1e6, rf_frequency=5e8)),
-                       (E.SBEND, dict(l=1.0, g=0.01))):
+                       (E.RFCAVITY, dict(l=0.5, voltage=1e6, rf_frequency=5e8))):
         rc, _, msg = _validate(bmx, [E.make_element(key, p0c, et, b_pole_elec={0: 1e5}, **attrs)])
         assert rc == 2 and "electric" in msg, (key, msg)
     off = E.make_element(E.SEXTUPOLE, p0c, et, l=0.2, k2=1.0, b_pole_elec={0: 1e5}, is_on=0)
@@ -108,6 +109,22 @@ def test_electric_multipoles_only_on_quadrupoles(bmx):
     rc, _, msg = _validate(bmx, [E.make_element(E.QUADRUPOLE, p0c, et, l=0.0, k1=0.5,
                                                 fringe_type=E.FRINGE_FULL)])
     assert rc == 2 and "zero-length" in msg, msg
+
+
+def test_general_bends_accepted(bmx):
+    """Bends with offsets, ref_tilt, k1, k2, multipoles, electric multipoles or the full edge
+    take the general bend's 94 values; a plain bend keeps the 29 of the common path."""
+    E = bmx.elements
+    p0c, et = bmx.lattices.reference()
+    rc, nprm, msg = _validate(bmx, [E.make_element(E.SBEND, p0c, et, l=1.0, g=0.01)])
+    assert rc == 0 and nprm == 29, msg
+    for attrs in (dict(x_offset=1e-3), dict(ref_tilt=0.1), dict(k1=0.1), dict(k2=1.0),
+                  dict(a_pole={1: 1e-3}), dict(b_pole_elec={0: 1e5}), dict(fringe_type=4, e1=0.1),
+                  dict(vkick=1e-5), dict(dg=1e-4), dict(k1=0.1, exact_multipoles=3, g=0.0)):
+        kw = dict(l=1.0, g=0.01)
+        kw.update(attrs)
+        rc, nprm, msg = _validate(bmx, [E.make_element(E.SBEND, p0c, et, **kw)])
+        assert rc == 0 and nprm == 94, (attrs, nprm, msg)
 
 
 def test_p0c_mismatch_rejected(bmx):
