@@ -294,7 +294,9 @@ def test_edge_sizes(bmx, oracle, ctx):
 def test_unsupported_element_raises(bmx, ctx):
     E = bmx.elements
     p0c, et = bmx.lattices.reference()
-    bad = [E.make_element(E.SBEND, p0c, et, l=1.0, g=0.01, x_offset=1e-3)]
+    # exact multipoles in a curved bend (bend_exact_multipole_field) are outside the GPU path
+    bad = [E.make_element(E.SBEND, p0c, et, l=1.0, g=0.01, k1=0.1,
+                          exact_multipoles=E.EXACT_HORIZONTALLY_PURE)]
     with pytest.raises(bmx.native.BmxError) as ei:
         bmx.tracking.Lattice(ctx, bad)
     assert ei.value.code == 2
