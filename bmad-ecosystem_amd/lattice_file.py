@@ -7,9 +7,11 @@ expressions over numbers, constants, variables and ``ele[attr]`` references. Ele
 those of this path (drift, quadrupole, sbend, rbend, sextupole, octupole, marker, rfcavity,
 kickers, monitor, instrument, collimators, pipe, thick_multipole), matched by unique prefix as
 the reference parser does. An rbend becomes an sbend as the reference's bookkeeping makes it
-(arc length from the chord, edge angles increased by half the bend angle). Superposition,
-overlays, groups, reversed elements (``--``) and attributes outside this path raise
-``LatticeFileError``.
+(arc length from the chord, edge angles increased by half the bend angle). Overlays set the
+attributes they control to the sum of their expressions at the overlay variables' values
+(slave names may use the wildcards % and *); groups, which only change attributes when their
+variables change after the lattice is set up, change nothing at load. Superposition, reversed
+elements (``--``) and attributes outside this path raise ``LatticeFileError``.
 The result goes through ``elements.make_element`` (Bmad defaults and step bookkeeping).
 """
 from __future__ import annotations
@@ -51,17 +53,24 @@ _FUNCS = {"sin": math.sin, "cos": math.cos, "tan": math.tan, "asin": math.asin,
           "tanh": math.tanh}
 
 # lattice attribute -> make_element keyword (same name unless listed)
-_ATTRS = {"l", "k1", "k2", "k3", "g", "e1", "e2", "fint", "fintx", "hgap", "hgapx", "hkick",
-          "vkick", "kick", "tilt", "x_offset", "y_offset", "z_offset", "x_pitch", "y_pitch",
-          "voltage", "rf_frequency", "phi0", "phi0_multipass", "x_limit", "y_limit", "x1_limit",
-          "x2_limit", "y1_limit", "y2_limit", "num_steps", "ds_step", "angle", "rho",
-          "aperture_type", "aperture_at", "fringe_type", "harmon"}
+_ATTRS = {"l", "k1", "k2", "k3", "g", "dg", "e1", "e2", "fint", "fintx", "hgap", "hgapx", "hkick",
+          "vkick", "kick", "tilt", "ref_tilt", "x_offset", "y_offset", "z_offset", "x_pitch",
+          "y_pitch", "voltage", "rf_frequency", "phi0", "phi0_multipass", "x_limit", "y_limit",
+          "x1_limit", "x2_limit", "y1_limit", "y2_limit", "num_steps", "ds_step", "angle", "rho",
+          "aperture_type", "aperture_at", "fringe_type", "fringe_at", "harmon", "r0_mag",
+          "r0_elec", "fq1", "fq2", "exact_multipoles"}
+# multipoles: a0..a21, b0..b21 (magnetic) and a0_elec..b21_elec (electric)
+_MULTIPOLE = re.compile(r"^([ab])(\d+)(_elec)?$")
 _ENUMS = {"aperture_type": {"rectangular": E.APERTURE_RECTANGULAR, "elliptical": E.APERTURE_ELLIPTICAL},
           "aperture_at": {"entrance_end": E.ENTRANCE_END, "exit_end": E.EXIT_END,
                           "both_ends": E.BOTH_ENDS, "no_end": E.NO_END},
           "fringe_type": {"none": E.FRINGE_NONE, "hard_edge_only": E.FRINGE_HARD_EDGE_ONLY,
                           "basic_bend": E.FRINGE_BASIC_BEND, "full": E.FRINGE_FULL,
-                          "soft_edge_only": E.FRINGE_SOFT_EDGE_ONLY}}
+                          "soft_edge_only": E.FRINGE_SOFT_EDGE_ONLY},
+          "fringe_at": {"entrance_end": E.ENTRANCE_END, "exit_end": E.EXIT_END,
+                        "both_ends": E.BOTH_ENDS, "no_end": E.NO_END},
+          "exact_multipoles": {"off": E.EXACT_OFF, "horizontally_pure": E.EXACT_HORIZONTALLY_PURE,
+                               "vertically_pure": E.EXACT_VERTICALLY_PURE}}
 _IGNORED = ("beginning", "bmad_com", "particle_start", "space_charge_com", "no_digested",
             "title", "expand_lattice", "end_file", "debug_marker")
 
@@ -81,6 +90,9 @@ class _Parser:
         self.eles = {}    # name -> (type name, {attr: value})
         self.lines = {}   # name -> list of items
         self.use = None
+        self.start_at = None  # start_branch_at: the element the branch starts with
+        # name -> {"kind": "overlay" | "group", "slaves": [(pattern, attr, expr)], "vars": {}}
+        self.controllers = {}
 
     # -- expressions ----------------------------------------------------------------------
     def value(self, text: str):
@@ -133,7 +145,7 @@ class _Parser:
             if not line.strip():
                 continue
             cur = (cur + " " + line.strip()) if cur else line.strip()
-            depth = cur.count("(") - cur.count(")")
+            depth = cur.count("(") + cur.count("{") - cur.count(")") - cur.count("}")
             if cur.endswith("&"):
                 cur = cur[:-1]
                 continue
@@ -158,6 +170,15 @@ class _Parser:
             if low.startswith("use"):
                 self.use = re.split(r"\s*,\s*", st, maxsplit=1)[1].strip().lower()
                 continue
+            m = re.match(r"start_branch_at\s*,?\s*([\w.]+)\s*$", st, re.I)
+            if m:
+                self.start_at = m.group(1).lower()
+                continue
+            m = re.match(r"beginning\s*\[\s*(e_tot|p0c)\s*\]\s*=\s*(.+)$", st, re.I)
+            if m:
+                # the reference energy of an open lattice, as parameter[e_tot] / [p0c]
+                self.params.setdefault(m.group(1).lower(), self.value(m.group(2)))
+                continue
             if any(low.startswith(k) for k in _IGNORED):
                 continue
             m = re.match(r"parameter\s*\[\s*(\w+)\s*\]\s*=\s*(.+)$", st, re.I)
@@ -168,6 +189,12 @@ class _Parser:
             m = re.match(r"([\w.]+)\s*\[\s*(\w+)\s*\]\s*=\s*(.+)$", st)
             if m:
                 name, attr = m.group(1).lower(), m.group(2).lower()
+                if name in self.controllers:
+                    ctl = self.controllers[name]
+                    if attr not in ctl["vars"]:
+                        raise LatticeFileError(f"{name} has no variable {attr!r}: {st!r}")
+                    ctl["vars"][attr] = self.value(m.group(3))
+                    continue
                 if name not in self.eles:
                     raise LatticeFileError(f"attribute of unknown element in {st!r}")
                 self._set(self.eles[name][1], attr, m.group(3))
@@ -189,6 +216,10 @@ class _Parser:
         if head.lower().startswith("line") and m:
             self.lines[name] = _split_top(m.group(1))
             return
+        mc = re.match(r"(overlay|group)\s*=\s*\{(.*)\}\s*$", head, re.I | re.S)
+        if mc:
+            self._controller(name, mc.group(1).lower(), mc.group(2), parts[1:], st)
+            return
         if head.lower() in ("list", "overlay", "group", "girder") or "superimpose" in rest.lower():
             raise LatticeFileError(f"{head} elements are not supported: {st!r}")
         tname = _lookup_type(head)
@@ -209,7 +240,72 @@ class _Parser:
                 raise LatticeFileError(f"attribute {p.strip()!r} not supported: {st!r}")
         self.eles[name] = (tname, attrs)
 
+    def _controller(self, name, kind, body, rest, st):
+        """overlay / group: slaves 'name[attr]: expression' (an omitted expression is the first
+        variable), 'var = {...}' and initial variable values."""
+        variables = {}
+        others = []
+        for p in rest:
+            m = re.match(r"var\s*=\s*\{(.*)\}$", p.strip(), re.I | re.S)
+            if m:
+                for v in _split_top(m.group(1)):
+                    if v:
+                        variables[v.lower()] = 0.0
+            elif p.strip():
+                others.append(p)
+        if not variables:
+            raise LatticeFileError(f"{kind} without var = {{...}}: {st!r}")
+        for p in others:
+            k, _, v = p.partition("=")
+            k = k.strip().lower()
+            if k == "type":
+                continue
+            if k not in variables or not v:
+                raise LatticeFileError(f"{kind} attribute {p.strip()!r} not supported: {st!r}")
+            variables[k] = self.value(v)
+        slaves = []
+        for item in _split_top(body):
+            if not item:
+                continue
+            m = re.match(r"([\w.%*]+)\s*\[\s*(\w+)\s*\]\s*(?::\s*(.+))?$", item.strip(), re.S)
+            if not m:
+                raise LatticeFileError(f"{kind} slave {item!r} not supported: {st!r}")
+            attr = m.group(2).lower()
+            if attr not in _ATTRS and not _MULTIPOLE.match(attr):
+                raise LatticeFileError(f"{kind} controls attribute {attr!r}, not on this path")
+            expr = m.group(3) if m.group(3) is not None else next(iter(variables))
+            slaves.append((m.group(1).lower(), attr, expr))
+        self.controllers[name] = {"kind": kind, "slaves": slaves, "vars": variables}
+
+    def apply_overlays(self):
+        """Each attribute an overlay controls becomes the sum of its overlays' expressions at
+        their variables' values (bookkeeper_mod.f90:1779-1831, overlay_change_this); groups
+        change attributes only when their variables change later, so they change nothing
+        here."""
+        sums = {}
+        for ctl in self.controllers.values():
+            if ctl["kind"] != "overlay":
+                continue
+            saved = dict(self.vars)
+            self.vars.update(ctl["vars"])
+            try:
+                for pattern, attr, expr in ctl["slaves"]:
+                    rx = re.compile("^" + re.escape(pattern).replace("%", ".").replace("\\*", ".*") + "$")
+                    hits = [n for n in self.eles if rx.match(n)]
+                    if not hits:
+                        raise LatticeFileError(f"overlay slave {pattern!r} matches no element")
+                    val = self.value(expr)
+                    for n in hits:
+                        sums[(n, attr)] = sums.get((n, attr), 0.0) + val
+            finally:
+                self.vars = saved
+        for (n, attr), val in sums.items():
+            self.eles[n][1][attr] = val
+
     def _set(self, attrs, k, v):
+        if _MULTIPOLE.match(k):
+            attrs[k] = self.value(v)
+            return
         if k not in _ATTRS:
             raise LatticeFileError(f"attribute {k!r} is not supported")
         if k in _ENUMS:
@@ -251,12 +347,12 @@ class _Parser:
 
 
 def _split_top(s: str):
-    """Split at commas outside parentheses."""
+    """Split at commas outside parentheses and braces."""
     out, depth, cur = [], 0, ""
     for ch in s:
-        if ch == "(":
+        if ch in "({":
             depth += 1
-        elif ch == ")":
+        elif ch in ")}":
             depth -= 1
         if ch == "," and depth == 0:
             out.append(cur)
@@ -287,6 +383,13 @@ def load(path: str):
     if use is None:
         raise LatticeFileError("no line to use")
     names = P.expand(use)
+    if P.start_at is not None:
+        # start_branch_at: the elements before the named one move to the end of the branch
+        if P.start_at not in names:
+            raise LatticeFileError(f"start_branch_at {P.start_at!r}: no such element in the line")
+        i = names.index(P.start_at)
+        names = names[i:] + names[:i]
+    P.apply_overlays()
     eles = []
     # total length, for rf cavities given by harmonic number (f = harmon beta0 c / L_total)
     def arc_length(a, tname):
@@ -324,6 +427,14 @@ def load(path: str):
             a["rf_frequency"] = harmon * (p0c / e_tot) * E.C_LIGHT / l_total
         num_steps = int(a.pop("num_steps", 0))
         ds_step = a.pop("ds_step", 0.0)
+        poles = {"a_pole": {}, "b_pole": {}, "a_pole_elec": {}, "b_pole_elec": {}}
+        for k in [k for k in a if _MULTIPOLE.match(k)]:
+            ab, n, elec = _MULTIPOLE.match(k).groups()
+            n = int(n)
+            if n >= E.N_POLE:
+                raise LatticeFileError(f"multipole order {n} above {E.N_POLE - 1}")
+            poles[f"{ab}_pole{'_elec' if elec else ''}"][n] = a.pop(k)
+        poles = {k: v for k, v in poles.items() if v}
         eles.append(E.make_element(_TYPES[tname], p0c, e_tot, num_steps=num_steps, ds_step=ds_step,
-                                   default_tilt=default_tilt, **a))
+                                   default_tilt=default_tilt, **poles, **a))
     return eles, mc2, charge

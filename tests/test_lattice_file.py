@@ -1,6 +1,6 @@
 """The lattice-file loader (lattice_file.py, a subset of bmad/parsing/bmad_parser.f90): a test
-ring written in the Bmad language, and the reference's tutorial rings when the reference tree
-is present."""
+ring written in the Bmad language, two of the reference's tutorial lattices kept as fixtures,
+and overlays, groups, start_branch_at and the attributes of the general element maps."""
 import math
 import os
 
@@ -9,7 +9,7 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 RING = os.path.join(HERE, "golden", "lattices", "ring_small.bmad")
-TUTORIAL = "/root/reference/bmad-doc/tutorial_ring_design/lattices"
+TUTORIAL = os.path.join(HERE, "golden", "lattices", "tutorial")
 
 
 @pytest.fixture(scope="module")
@@ -54,6 +54,7 @@ def test_small_ring(bmx, lf):
 @pytest.mark.parametrize("text", [
     "parameter[e_tot] = 1e9\nd: drift, l = 1\nr: line = (d, --d)\nuse, r\n",
     "parameter[e_tot] = 1e9\nd: drift, l = 1\no: overlay = {d[l]: x}\nr: line = (d)\nuse, r\n",
+    "parameter[e_tot] = 1e9\nd: drift, l = 1\ng: girder = {d}\nr: line = (d)\nuse, r\n",
     "parameter[e_tot] = 1e9\nd: drift, l = 1, superimpose, ref = d\nr: line = (d)\nuse, r\n",
     "parameter[e_tot] = 1e9\ns: solenoid, l = 1, ks = 0.1\nr: line = (s)\nuse, r\n",
     "d: drift, l = 1\nr: line = (d)\nuse, r\n",
@@ -75,20 +76,57 @@ def test_reflection_and_repetition(lf, tmp_path):
     assert [e.l for e in eles] == [1, 2, 4, 3, 1, 3, 2, 1, 3, 2]
 
 
-@pytest.mark.skipif(not os.path.isdir(TUTORIAL), reason="reference tree not present")
 def test_reference_tutorial_rings(bmx, lf):
-    """The reference's ring-design tutorial lattices that stay inside this path's element set
-    load, and the closed rings close: 01_FODO's arc cell (pi/66 of bend per cell, both files of
-    the call) and 13_LatticeTypes/ring.bmad (rf frequency from its harmonic number)."""
+    """Two of the reference's ring-design tutorial lattices, kept as fixtures
+    (bmad-doc/tutorial_ring_design/lattices): 01_FODO's arc cell (pi/66 of bend per cell,
+    both files of the call) and 13_LatticeTypes/ring.bmad (rf frequency from its harmonic
+    number); the closed ring closes and compiles for the GPU path."""
     E = bmx.elements
-    eles, mc2, charge = lf.load(os.path.join(TUTORIAL, "01_FODO/a_Arc/fodoAR0.bmad"))
+    eles, mc2, charge = lf.load(os.path.join(TUTORIAL, "fodoAR0.bmad"))
     arr = E.to_array(eles)
     assert len(arr) == 8
     assert abs(float((arr["g"] * arr["l"]).sum()) - 2 * math.pi / 132) < 1e-15
-    eles, mc2, charge = lf.load(os.path.join(TUTORIAL, "13_LatticeTypes/ring.bmad"))
+    eles, mc2, charge = lf.load(os.path.join(TUTORIAL, "ring.bmad"))
     arr = E.to_array(eles)
     assert abs(float((arr["g"] * arr["l"]).sum()) - 2 * math.pi) < 1e-9
     rf = arr[arr["key"] == E.RFCAVITY]
     assert len(rf) >= 1 and np.all(rf["rf_frequency"] > 1e8)
     rc, msg = _validate(bmx, arr, mc2, charge)
+    assert rc == 0, msg
+
+
+def test_overlays_groups_and_branch_start(bmx, lf, tmp_path):
+    """Overlays set each attribute they control to the sum of their expressions (wildcard
+    slaves, variables set in the definition or later, unset variables 0); a switched-on group
+    changes nothing at load; start_branch_at rotates the line; beginning[e_tot] gives the
+    reference energy; multipole attributes (a1, b2, b0_elec) land in the element record."""
+    E = bmx.elements
+    p = tmp_path / "lat.bmad"
+    p.write_text(
+        "beginning[e_tot] = 2e9\n"
+        "sf_1: sextupole, l = 0.2, k2 = 5\nsf_2: sextupole, l = 0.2, k2 = 6\nsd_1: sextupole, l = 0.2, k2 = -4\n"
+        "q: quadrupole, l = 0.5, k1 = 0.3, a1 = 1e-3, b2 = 0.5, b0_elec = 2e5, r0_elec = 0.1\n"
+        "b: sbend, l = 1, g = 0.01, dg = 1e-4, ref_tilt = 0.1\n"
+        "bz: sbend, l = 0.5, k1 = 0.1, exact_multipoles = vertically_pure\n"
+        "d: drift, l = 1\nip: marker\n"
+        "osf: overlay = {sf_*[k2]: 2 + x}, var = {x}, x = 0.5\n"
+        "o2: overlay = {sf_1[k2]: 3*y, q[k1]: y}, var = {y}\n"
+        "o2[y] = 0.1\n"
+        "gr: group = {sd_1[k2]: 10*z}, var = {z}, z = 3\n"
+        "r: line = (d, sf_1, q, ip, sf_2, b, sd_1, bz)\nuse, r\nstart_branch_at ip\n")
+    eles, mc2, charge = lf.load(str(p))
+    names = [e.key for e in eles]
+    assert names == [E.MARKER, E.SEXTUPOLE, E.SBEND, E.SEXTUPOLE, E.SBEND, E.DRIFT, E.SEXTUPOLE,
+                     E.QUADRUPOLE]
+    sext = [e for e in eles if e.key == E.SEXTUPOLE]
+    # sf_2 (after ip): 2 + 0.5; sd_1: group leaves it; sf_1: (2 + 0.5) + 3 * 0.1
+    assert sext[0].k2 == pytest.approx(2.5) and sext[1].k2 == -4
+    assert sext[2].k2 == pytest.approx(2.8)
+    q = [e for e in eles if e.key == E.QUADRUPOLE][0]
+    assert q.k1 == pytest.approx(0.1) and q.a_pole[1] == 1e-3 and q.b_pole[2] == 0.5
+    assert q.b_pole_elec[0] == 2e5 and q.r0_elec == 0.1 and q.n_multipole_elec == E.N_POLE
+    b, bz = [e for e in eles if e.key == E.SBEND]
+    assert b.dg == 1e-4 and b.ref_tilt == 0.1 and bz.exact_multipoles == E.EXACT_VERTICALLY_PURE
+    assert eles[0].e_tot == 2e9
+    rc, msg = _validate(bmx, E.to_array(eles), mc2, charge)
     assert rc == 0, msg
