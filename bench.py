@@ -84,8 +84,9 @@ def traffic_file(special: bool) -> str:
 
 
 def profiled_traffic(special: bool):
-    """HBM bytes per tracking-kernel launch from the committed rocprofv3 summary (FETCH_SIZE and
-    WRITE_SIZE runs scaled by a copy-kernel calibration), or None if it is absent."""
+    """HBM bytes per tracking-kernel launch, averaged over the timed launches of the default
+    run, from the committed rocprofv3 summary (FETCH_SIZE and WRITE_SIZE runs scaled by a
+    copy-kernel calibration), or None if it is absent."""
     import re
     path = os.path.join(ROOT, traffic_file(special))
     try:
@@ -193,8 +194,8 @@ def main():
         except bmx.native.BmxError as e:  # reported in the JSON; the generic kernel runs
             special_err = str(e)
     t_special = time.perf_counter() - t0
-    waves = args.waves or (4 if special else (5 if args.maps == "fast" else 3))
-    kernel_name = f"bmx_cell_tiles_{waves}" if special else f"k_track_tiles<{str(args.maps == 'fast').lower()}, {waves}>"
+    def kernel_of(w):
+        return f"bmx_cell_tiles_{w}" if special else f"k_track_tiles<{str(args.maps == 'fast').lower()}, {w}>"
     # the turns of each step: the run's turns split evenly (the first steps take the remainder)
     step_turns = [args.turns // args.steps + (1 if s < args.turns % args.steps else 0)
                   for s in range(args.steps)]
@@ -212,10 +213,12 @@ def main():
     ctx.synchronize()
     t_wall0 = time.perf_counter()
     ctx.record(0)
+    shapes = []
     for s in range(args.steps):
         ctx.record(1 + 2 * s)
         T.track_beam(ctx, lat, b, step_turns[s], step_turn0[s])
         ctx.record(2 + 2 * s)
+        shapes.append(ctx.last_run())  # the run shape this step used (host-side record)
     ctx.record(63)
     ctx.synchronize()
     t_wall = time.perf_counter() - t_wall0
@@ -227,9 +230,26 @@ def main():
     alive = b.count_alive()
 
     value = passes / (total_ms * 1e-3)
+    # the kernel of each step: with waves chosen automatically the library picks the 5-wave
+    # build while the live particles fill more than 1.45 rounds of the 4-wave build, then the
+    # 4-wave one (the run phase)
+    step_waves = [int(w) for _, _, w in shapes]
+    kernels = {}
+    for w in step_waves:
+        kernels[kernel_of(w)] = kernels.get(kernel_of(w), 0) + 1
     # roofline of the tracking kernel: algorithmic FLOPs per launch / average launch duration
     kernel_s = kernel_ms * 1e-3 / max(launches, 1)
     achieved = flops * (passes / max(launches, 1)) / kernel_s / 1e12
+    # algorithmic HBM bytes per launch: 60 B read per particle tracked (seven coordinates and
+    # its index in the packed buffer), 60 B written per particle (to the other packed buffer or
+    # the beam); particles tracked per launch from the passes per launch over the chunk's
+    # element passes per particle
+    chunk = args.chunk_turns if args.chunk_turns > 0 else step_turns[0]
+    bytes_alg = 120.0 * (passes / max(launches, 1)) / (len(arr) * chunk)
+    # the profiled traffic is that of the default workload only
+    default_run = (args.particles == 1_000_000 and args.turns == 1000 and args.steps == 10
+                   and args.chunk_turns == 5 and args.waves == 0 and args.maps == "fast"
+                   and args.seed == 1)
     out = {
         "metric": METRIC,
         "value": value,
@@ -249,19 +269,24 @@ def main():
                         f"{args.turns} turns in {args.steps} steps",
             "particles": args.particles, "elements": len(arr), "turns": args.turns,
             "turns_per_step": step_turns[0], "alive_at_end": alive, "wall_s": t_wall,
-            "maps": args.maps, "cell_specialized": bool(special), "waves_per_simd": waves,
+            "maps": args.maps, "cell_specialized": bool(special),
+            "waves_per_simd": step_waves if len(set(step_waves)) > 1 else step_waves[0],
             "specialize_s": t_special, "specialize_error": special_err,
         },
         "roofline": {
             "bound": "valu", "compute": "fp64 vector ALU (no MFMA: the maps are not contractions)",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": profiled_traffic(special),
-            "traffic_note": "HBM bytes of the first 5-turn launch over 1e6 particles, "
-                            + traffic_file(special),
+            "traffic": profiled_traffic(special) if default_run else None,
+            "traffic_note": "HBM bytes per launch averaged over the 200 timed launches of the "
+                            "default run, " + traffic_file(special),
+            "traffic_algorithmic": bytes_alg,
             "flop_per_particle_element": flops,
-            "kernel": kernel_name, "kernel_ms_avg": kernel_ms / max(launches, 1),
+            "kernel": " then ".join(kernels), "kernel_steps": kernels,
+            "kernel_ms_avg": kernel_ms / max(launches, 1),
             "launches": launches, "step_ms": [float(x) for x in launch_ms],
+            "step_run_shape": [{"chunk_turns": int(c), "grid": int(g), "waves": int(w)}
+                               for c, g, w in shapes],
         },
     }
     if args.cpu_seconds > 0:

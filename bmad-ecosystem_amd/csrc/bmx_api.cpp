@@ -394,9 +394,25 @@ int32_t track_group(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t fir
     // with them
     cfg.span = first != 0 || count != L->desc.n_ele;
     const bool ext = L->host.rare_q && (cfg.span || !jit_rare_q(cfg.jit));
+    // Run phase (waves chosen automatically, full turns on the cell kernels): while the live
+    // particles fill more than kPhaseRounds rounds of the 4-wave kernel's resident waves, the
+    // 5-wave build (more resident waves, a few spilled registers) is faster; below that, with
+    // the beam thinned by losses, the 4-wave build is. Measured per 100-turn step of the
+    // 901-element ring: 1e6, 5.7e5 and 4.2e5 live particles (3.8, 2.2, 1.6 rounds) run 3-6%
+    // faster on 5 waves, 3.4e5 and fewer (1.3 rounds) 1-4% slower.
+    if (!ctx->waves && cfg.jit && !cfg.span && !L->host.rare_q) {
+        HIP_TRY(hipMemsetAsync(ctx->d_counter, 0, sizeof(unsigned long long), ctx->stream), "memset");
+        HIP_TRY(launch_count_alive(ctx->stream, B->d.state, B->n, ctx->d_counter), "k_count_alive");
+        unsigned long long alive = 0;
+        HIP_TRY(hipMemcpyAsync(&alive, ctx->d_counter, sizeof alive, hipMemcpyDeviceToHost, ctx->stream), "copy");
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        const double resident4 = (double)default_grid(ctx->device, true, 4, cfg.jit) * (kTileBlock / 64);
+        cfg.waves = (double)((alive + 63) / 64) >= kPhaseRounds * resident4 ? 5 : 4;
+    }
     cfg.grid = ctx->grid > 0 ? ctx->grid
                              : default_grid(ctx->device, ctx->fast_maps, cfg.waves, cfg.jit, cfg.span, ext);
     cfg.timer = ctx->profile ? &ctx->timer : nullptr;
+    ctx->last = {cfg.chunk_turns, cfg.grid, cfg.waves};
     HIP_TRY(enqueue_run(ctx->stream, *rb, &g, 1, true, n_turns, cfg, ctx->d_work), "k_track_tiles");
     return BMX_OK;
 }
@@ -466,6 +482,15 @@ int32_t bmx_ctx_set_run(bmx_ctx *ctx, int32_t chunk_turns, int32_t grid, int32_t
     ctx->chunk_turns = chunk_turns;
     ctx->grid = grid;
     ctx->waves = waves;
+    return BMX_OK;
+}
+
+int32_t bmx_ctx_last_run(const bmx_ctx *ctx, int32_t *chunk_turns, int32_t *grid, int32_t *waves)
+{
+    if (!ctx) return set_error(BMX_ERR_ARG, "bmx_ctx_last_run: null context");
+    if (chunk_turns) *chunk_turns = ctx->last.chunk_turns;
+    if (grid) *grid = ctx->last.grid;
+    if (waves) *waves = ctx->last.waves;
     return BMX_OK;
 }
 

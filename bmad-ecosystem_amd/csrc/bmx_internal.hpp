@@ -13,16 +13,17 @@ namespace bmx {
 // ---- kernel launchers (bmx_kernels.hip) ----
 hipError_t launch_track_tiles(hipStream_t s, int grid, int waves, const Group *d_groups,
                               const WaveTile *d_tiles, const uint32_t *d_n_tiles,
-                              const uint32_t *d_list_in, uint32_t *d_list_out, uint32_t *d_cnt_out,
-                              uint32_t *d_tile_ctr, int32_t chunk_off, int32_t chunk_turns,
-                              unsigned long long *d_work, bool fast, bool span, bool ext = false);
+                              const double *d_pk_in, double *d_pk_out, uint32_t stride,
+                              uint32_t *d_cnt_out, uint32_t *d_tile_ctr, int32_t chunk_off,
+                              int32_t chunk_turns, unsigned long long *d_work, bool fast, bool span,
+                              bool ext = false);
 hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_groups,
                               const uint32_t *d_cnt_in, int32_t next_off, WaveTile *d_tiles,
                               uint32_t *d_n_tiles, uint32_t *d_cnt_zero, uint32_t *d_tile_ctr,
                               uint32_t *d_goff);
-hipError_t launch_collect_alive(hipStream_t s, int grid, const Group *d_groups,
-                                const WaveTile *d_tiles, const uint32_t *d_n_tiles,
-                                uint32_t *d_list_out, uint32_t *d_cnt_out);
+hipError_t launch_pack_alive(hipStream_t s, int grid, const Group *d_groups,
+                             const WaveTile *d_tiles, const uint32_t *d_n_tiles, double *d_pk_out,
+                             uint32_t stride, uint32_t *d_cnt_out);
 hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu, bool span = false,
                                  bool ext = false);
 // Waves per SIMD of the kernel for runs with element spans (Group first/count other than the
@@ -60,13 +61,14 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
 int set_error(int code, const std::string &msg);
 
 // ---- chunked tile runs (bmx_run.cpp) ----
-// Device buffers of a tile run over a set of groups: the groups, two alive lists, two tile
-// lists with their counts, per-group alive counters for the two lists and the tile counter.
+// Device buffers of a tile run over a set of groups: the groups, two packed particle buffers
+// (cap_particles slots each, the Packed layout of bmx_table.hpp), two tile lists with their
+// counts, per-group alive counters for the two packed buffers and the tile counter.
 struct RunBuffers {
     int device = -1;
     size_t cap_groups = 0, cap_particles = 0, cap_tiles = 0;
     Group *d_groups = nullptr;
-    uint32_t *d_list[2] = {nullptr, nullptr};
+    double *d_packed[2] = {nullptr, nullptr};
     WaveTile *d_tiles[2] = {nullptr, nullptr};
     uint32_t *d_cnt[2] = {nullptr, nullptr};
     uint32_t *d_goff = nullptr;   // first tile of each group (n_groups + 1), tile builder scratch
@@ -96,9 +98,9 @@ hipError_t jit_occupancy(const JitKernels *j, int waves, int *blocks_per_cu);
 bool jit_rare_q(const JitKernels *j);
 hipError_t launch_track_tiles_jit(const JitKernels *j, hipStream_t s, int grid, int waves,
                                   const Group *d_groups, const WaveTile *d_tiles,
-                                  const uint32_t *d_n_tiles, const uint32_t *d_list_in,
-                                  uint32_t *d_list_out, uint32_t *d_cnt_out, uint32_t *d_tile_ctr,
-                                  int32_t chunk_off, int32_t chunk_turns,
+                                  const uint32_t *d_n_tiles, const double *d_pk_in,
+                                  double *d_pk_out, uint32_t stride, uint32_t *d_cnt_out,
+                                  uint32_t *d_tile_ctr, int32_t chunk_off, int32_t chunk_turns,
                                   unsigned long long *d_work);
 
 // Parameters of one run.
@@ -125,6 +127,8 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
 // without spilling; generic fast maps 5; reference maps 3, whose 4- and 5-wave builds spill).
 // Cell kernels that carry the stepped quadrupole or the general bend are all built for 2 waves per
 // SIMD (their longer maps would spill otherwise), whichever variant is asked for.
+// The live-particle rounds of the 4-wave cell kernel above which the 5-wave build runs.
+constexpr double kPhaseRounds = 1.45;
 inline int32_t default_waves(bool fast, const JitKernels *jit = nullptr)
 {
     return jit ? (jit_rare_q(jit) ? 3 : 4) : (fast ? 5 : 3);
@@ -146,6 +150,9 @@ struct bmx_ctx {
     int32_t chunk_turns = 5;
     int32_t grid = 0;       // 0: default_grid
     int32_t waves = 0;      // waves per SIMD of the tracking kernel variant (0: default_waves)
+    struct LastRun {
+        int32_t chunk_turns = 0, grid = 0, waves = 0;
+    } last;                 // the run shape the last bmx_track_turns* call used
     bmx::RunBuffers run;    // buffers of bmx_track_turns
     bmx::Group group{};     // host copy of the single group uploaded by bmx_track_turns
     bool profile = false;   // time every tracking launch of bmx_track_turns

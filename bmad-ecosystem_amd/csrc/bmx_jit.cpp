@@ -116,12 +116,13 @@ std::string generate(const HostTable &t, const Period &pd, std::string &sig)
       << "}  // namespace bmx\n";
     const char *args =
         "(const bmx::Group *__restrict__ g, const bmx::WaveTile *__restrict__ t, "
-        "const uint32_t *__restrict__ nt, const uint32_t *__restrict__ li, uint32_t *__restrict__ lo, "
-        "uint32_t *__restrict__ co, uint32_t *tc, int32_t off, int32_t K, unsigned long long *w)";
+        "const uint32_t *__restrict__ nt, const double *__restrict__ pi, double *__restrict__ po, "
+        "uint32_t ps, uint32_t *__restrict__ co, uint32_t *tc, int32_t off, int32_t K, "
+        "unsigned long long *w)";
     for (int w = 3; w <= 5; w++)
         s << "extern \"C\" __global__ void __launch_bounds__(bmx::kBlock) "
           << "__attribute__((amdgpu_waves_per_eu(" << (t.rare_q ? 2 : w) << "))) bmx_cell_tiles_" << w << args
-          << "\n{\n    bmx::track_tiles<true, bmx::CellTurn>(g, t, nt, li, lo, co, tc, off, K, w);\n}\n";
+          << "\n{\n    bmx::track_tiles<true, bmx::CellTurn>(g, t, nt, pi, po, ps, co, tc, off, K, w);\n}\n";
     return s.str();
 }
 
@@ -195,15 +196,15 @@ hipError_t jit_occupancy(const JitKernels *j, int waves, int *blocks_per_cu)
 
 hipError_t launch_track_tiles_jit(const JitKernels *j, hipStream_t s, int grid, int waves,
                                   const Group *d_groups, const WaveTile *d_tiles,
-                                  const uint32_t *d_n_tiles, const uint32_t *d_list_in,
-                                  uint32_t *d_list_out, uint32_t *d_cnt_out, uint32_t *d_tile_ctr,
-                                  int32_t chunk_off, int32_t chunk_turns,
+                                  const uint32_t *d_n_tiles, const double *d_pk_in,
+                                  double *d_pk_out, uint32_t stride, uint32_t *d_cnt_out,
+                                  uint32_t *d_tile_ctr, int32_t chunk_off, int32_t chunk_turns,
                                   unsigned long long *d_work)
 {
     if (grid <= 0) return hipSuccess;
     const int w = waves < 3 ? 3 : (waves > 5 ? 5 : waves);
-    void *args[] = {&d_groups, &d_tiles, &d_n_tiles, &d_list_in, &d_list_out, &d_cnt_out,
-                    &d_tile_ctr, &chunk_off, &chunk_turns, &d_work};
+    void *args[] = {&d_groups, &d_tiles,   &d_n_tiles,  &d_pk_in,     &d_pk_out, &stride,
+                    &d_cnt_out, &d_tile_ctr, &chunk_off, &chunk_turns, &d_work};
     return hipModuleLaunchKernel(j->fn[w - 3], (unsigned)grid, 1, 1, kTileBlock, 1, 1, 0, s, args,
                                  nullptr);
 }

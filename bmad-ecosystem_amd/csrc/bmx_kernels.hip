@@ -4,6 +4,7 @@
 //                  registers for all elements and turns of the chunk; work comes as wave tiles
 //                  of one group each, so single-lattice, batched and served runs share it.
 //  k_build_tiles   next chunk's tiles from the alive counts (lost particles drop out).
+//  k_pack_alive    a run's live particles packed for its first chunk.
 //  k_track_element one element for every particle, state round-trips HBM: the GPU form of
 //                  track1_bunch_hom (bmad/multiparticle/beam_utils.f90:32-90).
 //  k_count_alive   number of live particles (lt_tracking_mod.f90:1672 bunch%n_live).
@@ -22,12 +23,12 @@ namespace bmx {
 template <bool F, int W, bool SPAN = false, int RARE = MAPS_RARE>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) k_track_tiles(
     const Group *__restrict__ groups, const WaveTile *__restrict__ tiles,
-    const uint32_t *__restrict__ n_tiles_p, const uint32_t *__restrict__ list_in,
-    uint32_t *__restrict__ list_out, uint32_t *__restrict__ cnt_out, uint32_t *tile_ctr,
-    int32_t chunk_off, int32_t chunk_turns, unsigned long long *work)
+    const uint32_t *__restrict__ n_tiles_p, const double *__restrict__ pk_in,
+    double *__restrict__ pk_out, uint32_t stride, uint32_t *__restrict__ cnt_out,
+    uint32_t *tile_ctr, int32_t chunk_off, int32_t chunk_turns, unsigned long long *work)
 {
-    track_tiles<F, GenericTurn<F, RARE>, SPAN>(
-        groups, tiles, n_tiles_p, list_in, list_out, cnt_out, tile_ctr, chunk_off, chunk_turns, work);
+    track_tiles<F, GenericTurn<F, RARE>, SPAN>(groups, tiles, n_tiles_p, pk_in, pk_out, stride,
+                                               cnt_out, tile_ctr, chunk_off, chunk_turns, work);
 }
 
 // Builds the wave tiles of the next chunk from the alive counts per group (cnt_in; nullptr
@@ -94,14 +95,15 @@ __global__ void __launch_bounds__(kBuildThreads) k_build_tiles(
     }
 }
 
-// The alive list of a run's first chunk: one wave per tile of every group's particles (tiles
-// built from the group sizes), a ballot of state == alive and one atomic per wave appending
-// the live indices to the group's segment of list_out (counts in cnt_out, zeroed by the tile
-// build). Dead particles of the caller's beam thus take no lane in the first chunk, as
-// track1_bunch_hom skips them (beam_utils.f90:83). Grid-stride over the tiles.
-__global__ void __launch_bounds__(kBlock) k_collect_alive(
+// The live particles of a run packed for its first chunk: one wave per tile of every group's
+// particles (tiles built from the group sizes), a ballot of state == alive and one atomic per
+// wave appending the live particles' coordinates and indices to the group's slots of the packed
+// buffer (counts in cnt_out, zeroed by the tile build). Dead particles of the caller's beam thus
+// take no lane in the run, as track1_bunch_hom skips them (beam_utils.f90:83). Grid-stride over
+// the tiles.
+__global__ void __launch_bounds__(kBlock) k_pack_alive(
     const Group *__restrict__ groups, const WaveTile *__restrict__ tiles,
-    const uint32_t *__restrict__ n_tiles_p, uint32_t *__restrict__ list_out,
+    const uint32_t *__restrict__ n_tiles_p, double *__restrict__ pk_out, uint32_t stride,
     uint32_t *__restrict__ cnt_out)
 {
     const uint32_t nt = *n_tiles_p;
@@ -117,7 +119,12 @@ __global__ void __launch_bounds__(kBlock) k_collect_alive(
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(&cnt_out[T.group], (uint32_t)__popcll(m));
         base = __builtin_amdgcn_readfirstlane(base);
-        if (alive) list_out[G.list_off + base + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = idx;
+        if (alive) {
+            Particle p;
+            load_particle(p, G.b, idx);
+            store_packed(p, pk_out, stride,
+                         G.list_off + base + (uint32_t)__popcll(m & ((1ull << lane) - 1)), idx);
+        }
     }
 }
 
@@ -261,24 +268,24 @@ __global__ void __launch_bounds__(kBlock) k_count_alive(const int32_t *__restric
 
 template <bool F, int W, bool SPAN = false, int RARE = MAPS_RARE>
 static void launch_tiles(hipStream_t s, int grid, const Group *g, const WaveTile *t, const uint32_t *nt,
-                         const uint32_t *li, uint32_t *lo, uint32_t *co, uint32_t *tc, int32_t off,
-                         int32_t K, unsigned long long *w)
+                         const double *pi, double *po, uint32_t stride, uint32_t *co, uint32_t *tc,
+                         int32_t off, int32_t K, unsigned long long *w)
 {
     hipLaunchKernelGGL((k_track_tiles<F, W, SPAN, RARE>), dim3((unsigned)grid), dim3(kBlock), 0, s, g, t, nt,
-                       li, lo, co, tc, off, K, w);
+                       pi, po, stride, co, tc, off, K, w);
 }
 
 hipError_t launch_track_tiles(hipStream_t s, int grid, int waves, const Group *d_groups,
                               const WaveTile *d_tiles, const uint32_t *d_n_tiles,
-                              const uint32_t *d_list_in, uint32_t *d_list_out, uint32_t *d_cnt_out,
-                              uint32_t *d_tile_ctr, int32_t chunk_off, int32_t chunk_turns,
-                              unsigned long long *d_work, bool fast, bool span, bool ext)
+                              const double *d_pk_in, double *d_pk_out, uint32_t stride,
+                              uint32_t *d_cnt_out, uint32_t *d_tile_ctr, int32_t chunk_off,
+                              int32_t chunk_turns, unsigned long long *d_work, bool fast, bool span,
+                              bool ext)
 {
     if (grid <= 0) return hipSuccess;
-#define BMX_TILES(F, W, ...)                                                                     \
-    launch_tiles<F, W, ##__VA_ARGS__>(s, grid, d_groups, d_tiles, d_n_tiles, d_list_in,          \
-                                      d_list_out, d_cnt_out, d_tile_ctr, chunk_off, chunk_turns, \
-                                      d_work)
+#define BMX_TILES(F, W, ...)                                                                      \
+    launch_tiles<F, W, ##__VA_ARGS__>(s, grid, d_groups, d_tiles, d_n_tiles, d_pk_in, d_pk_out,   \
+                                      stride, d_cnt_out, d_tile_ctr, chunk_off, chunk_turns, d_work)
     if (ext) {
         if (span) {
             if (fast) BMX_TILES(true, kExtWaves, true, MAPS_RARE_Q);
@@ -313,13 +320,13 @@ hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_gr
     return hipGetLastError();
 }
 
-hipError_t launch_collect_alive(hipStream_t s, int grid, const Group *d_groups,
-                                const WaveTile *d_tiles, const uint32_t *d_n_tiles,
-                                uint32_t *d_list_out, uint32_t *d_cnt_out)
+hipError_t launch_pack_alive(hipStream_t s, int grid, const Group *d_groups,
+                             const WaveTile *d_tiles, const uint32_t *d_n_tiles, double *d_pk_out,
+                             uint32_t stride, uint32_t *d_cnt_out)
 {
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_collect_alive, dim3((unsigned)grid), dim3(kBlock), 0, s, d_groups, d_tiles,
-                       d_n_tiles, d_list_out, d_cnt_out);
+    hipLaunchKernelGGL(k_pack_alive, dim3((unsigned)grid), dim3(kBlock), 0, s, d_groups, d_tiles,
+                       d_n_tiles, d_pk_out, stride, d_cnt_out);
     return hipGetLastError();
 }
 

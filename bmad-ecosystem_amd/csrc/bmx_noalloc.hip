@@ -21,9 +21,10 @@
 #include "bmx_internal.hpp"
 
 namespace bmx {
-// Beams of at most 2^20 particles: two alive lists (4 B per particle each), two tile lists
-// (one 16-byte tile per 64 particles each), the group and its counters -- about 8.5 MiB.
-constexpr int64_t kNoAllocMaxParticles = int64_t(1) << 20;
+// Beams of at most 2^18 particles: two packed particle buffers (60 B per particle each: seven
+// coordinates and the particle's index), two tile lists (one 16-byte tile per 64 particles
+// each), the group and its counters -- about 30 MiB.
+constexpr int64_t kNoAllocMaxParticles = int64_t(1) << 18;
 
 namespace {
 
@@ -35,9 +36,9 @@ constexpr size_t kOffGoff = kOffGroups + align256(sizeof(Group));
 constexpr size_t kOffCnt0 = kOffGoff + 256;   // goff: n_groups + 1 = 2 uint32
 constexpr size_t kOffCnt1 = kOffCnt0 + 256;
 constexpr size_t kOffSmall = kOffCnt1 + 256;  // 4 uint32
-constexpr size_t kOffList0 = kOffSmall + 256;
-constexpr size_t kOffList1 = kOffList0 + align256(kNoAllocMaxParticles * sizeof(uint32_t));
-constexpr size_t kOffTiles0 = kOffList1 + align256(kNoAllocMaxParticles * sizeof(uint32_t));
+constexpr size_t kOffPacked0 = kOffSmall + 256;
+constexpr size_t kOffPacked1 = kOffPacked0 + align256(kNoAllocMaxParticles * kPackedBytes);
+constexpr size_t kOffTiles0 = kOffPacked1 + align256(kNoAllocMaxParticles * kPackedBytes);
 constexpr size_t kOffTiles1 = kOffTiles0 + align256(kMaxTiles * sizeof(WaveTile));
 constexpr size_t kScratchBytes = kOffTiles1 + align256(kMaxTiles * sizeof(WaveTile));
 
@@ -55,8 +56,8 @@ RunBuffers scratch_buffers(int device, unsigned char *base)
     rb.d_cnt[0] = reinterpret_cast<uint32_t *>(base + kOffCnt0);
     rb.d_cnt[1] = reinterpret_cast<uint32_t *>(base + kOffCnt1);
     rb.d_small = reinterpret_cast<uint32_t *>(base + kOffSmall);
-    rb.d_list[0] = reinterpret_cast<uint32_t *>(base + kOffList0);
-    rb.d_list[1] = reinterpret_cast<uint32_t *>(base + kOffList1);
+    rb.d_packed[0] = reinterpret_cast<double *>(base + kOffPacked0);
+    rb.d_packed[1] = reinterpret_cast<double *>(base + kOffPacked1);
     rb.d_tiles[0] = reinterpret_cast<WaveTile *>(base + kOffTiles0);
     rb.d_tiles[1] = reinterpret_cast<WaveTile *>(base + kOffTiles1);
     rb.cap_groups = 1;

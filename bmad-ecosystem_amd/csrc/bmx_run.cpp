@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 
 #include "bmx_internal.hpp"
 
@@ -18,10 +19,10 @@ void RunBuffers::release()
     if (device >= 0) hipSetDevice(device);
     hipFree(d_groups);
     for (int k = 0; k < 2; k++) {
-        hipFree(d_list[k]);
+        hipFree(d_packed[k]);
         hipFree(d_tiles[k]);
         hipFree(d_cnt[k]);
-        d_list[k] = nullptr;
+        d_packed[k] = nullptr;
         d_tiles[k] = nullptr;
         d_cnt[k] = nullptr;
     }
@@ -56,13 +57,17 @@ hipError_t RunBuffers::reserve(size_t n_groups, size_t n_particles, size_t n_til
         cap_groups = c;
     }
     if (n_particles > cap_particles) {
-        const size_t c = std::max(n_particles, cap_particles + cap_particles / 2);
+        // slots are indexed by uint32_t
+        if (n_particles > UINT32_MAX) return hipErrorInvalidValue;
+        const size_t c = std::min<size_t>(std::max(n_particles, cap_particles + cap_particles / 2),
+                                          UINT32_MAX);
         for (int k = 0; k < 2; k++) {
-            hipFree(d_list[k]);
-            d_list[k] = nullptr;
+            hipFree(d_packed[k]);
+            d_packed[k] = nullptr;
         }
         cap_particles = 0;
-        for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMalloc(&d_list[k], c * sizeof(uint32_t));
+        for (int k = 0; k < 2 && e == hipSuccess; k++)
+            e = hipMalloc(reinterpret_cast<void **>(&d_packed[k]), c * kPackedBytes);
         if (e != hipSuccess) return e;
         cap_particles = c;
     }
@@ -131,9 +136,11 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
     const int32_t K = cfg.chunk_turns > 0 ? cfg.chunk_turns : max_turns;
     uint32_t *n_tiles[2] = {rb.d_small, rb.d_small + 1};
     uint32_t *tile_ctr = rb.d_small + 2;
-    // Chunk 0's alive list (list/cnt 0) from the particles' states: tiles over every particle
-    // of every group, a ballot pass over them, then the chunk's tiles from the live counts; the
-    // chunk appends to list/cnt 1.
+    // Chunk 0's particles (packed buffer / count 0) from the beams' states: tiles over every
+    // particle of every group, a ballot pass over them packing the live ones, then the chunk's
+    // tiles from the live counts; chunk c reads packed buffer c % 2 and appends its survivors to
+    // the other one.
+    const uint32_t stride = (uint32_t)rb.cap_particles;
     size_t all_tiles = 0;
     for (int32_t g = 0; g < n_groups; g++)
         if (h_groups[g].n_turns > 0) all_tiles += (h_groups[g].n + 63) / 64;
@@ -141,8 +148,8 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
     e = launch_build_tiles(s, rb.d_groups, n_groups, nullptr, 0, rb.d_tiles[1], n_tiles[1],
                            rb.d_cnt[0], tile_ctr, rb.d_goff);
     if (e == hipSuccess)
-        e = launch_collect_alive(s, collect_grid, rb.d_groups, rb.d_tiles[1], n_tiles[1],
-                                 rb.d_list[0], rb.d_cnt[0]);
+        e = launch_pack_alive(s, collect_grid, rb.d_groups, rb.d_tiles[1], n_tiles[1],
+                              rb.d_packed[0], stride, rb.d_cnt[0]);
     if (e == hipSuccess)
         e = launch_build_tiles(s, rb.d_groups, n_groups, rb.d_cnt[0], 0, rb.d_tiles[0], n_tiles[0],
                                rb.d_cnt[1], tile_ctr, rb.d_goff);
@@ -166,12 +173,12 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
         }
         if (use_jit)
             e = launch_track_tiles_jit(cfg.jit, s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur],
-                                       n_tiles[cur], rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt],
-                                       tile_ctr, off, K, d_work);
+                                       n_tiles[cur], rb.d_packed[cur], rb.d_packed[nxt], stride,
+                                       rb.d_cnt[nxt], tile_ctr, off, K, d_work);
         else
             e = launch_track_tiles(s, cfg.grid, cfg.waves, rb.d_groups, rb.d_tiles[cur], n_tiles[cur],
-                                   rb.d_list[cur], rb.d_list[nxt], rb.d_cnt[nxt], tile_ctr, off, K,
-                                   d_work, cfg.fast, cfg.span, ext);
+                                   rb.d_packed[cur], rb.d_packed[nxt], stride, rb.d_cnt[nxt],
+                                   tile_ctr, off, K, d_work, cfg.fast, cfg.span, ext);
         if (cfg.timer && e == hipSuccess) {
             e = hipEventRecord(ev1, s);
             cfg.timer->pending.emplace_back(ev0, ev1);

@@ -30,7 +30,8 @@ using namespace bmx;
 
 namespace {
 
-constexpr int64_t kBytesPerParticle = 7 * 8 + 3 * 4 + 2 * 4;  // SoA state + two alive lists
+// SoA state + the run's two packed particle buffers
+constexpr int64_t kBytesPerParticle = 7 * 8 + 3 * 4 + 2 * (int64_t)kPackedBytes;
 
 struct Request {
     int64_t id = 0;

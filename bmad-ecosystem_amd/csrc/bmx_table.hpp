@@ -159,7 +159,8 @@ struct BeamPtrs {
 
 // A group: particles [0, n) of one beam view tracked through one lattice for turns
 // turn0 .. turn0 + n_turns - 1. A single-lattice call is one group; batched variants and served
-// requests are one group each. list_off locates the group's segment in the alive lists.
+// requests are one group each. list_off locates the group's segment in the packed particle
+// buffers of the run (Packed below).
 // Each turn passes through `count` elements starting at element `first` (0-based) and wrapping
 // past the last element to element 0: first = 0, count = n_ele is the full turn from the start
 // of the lattice; a turn that starts after element s (LTT's track_bunch(ele_start, ele_start),
@@ -180,8 +181,18 @@ struct Group {
 // host reads their sum.
 constexpr int kWorkSlots = 16;
 
+// The live particles of a run between two chunks, packed: slot j of the run holds coordinate k
+// (0..6: x, px, y, py, z, pz, beta) at d[k * stride + j] and the particle's index in its beam
+// at ((uint32_t *)(d + kPackedDoubles * stride))[j]. Each group owns slots list_off ..
+// list_off + n - 1; a chunk reads the live particles from one such buffer and appends the ones
+// that survive it to the other, so every chunk reads and writes whole cache lines however many
+// of the beam's particles are lost.
+constexpr int kPackedDoubles = 7;
+constexpr uint64_t kPackedBytes = kPackedDoubles * sizeof(double) + sizeof(uint32_t);
+
 // One wave's unit of work: up to 64 particles of one group for one chunk of turns. start
-// indexes the group's alive-list segment (or the particles themselves in the first chunk).
+// indexes the group's slots in the packed buffer (or the particles of its beam when the run
+// packs its live particles before the first chunk).
 struct WaveTile {
     int32_t group;
     uint32_t start;

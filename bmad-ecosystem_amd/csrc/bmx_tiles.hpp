@@ -33,6 +33,43 @@ __device__ __forceinline__ void store_particle(const Particle &p, const BeamPtrs
     b.beta[i] = p.beta;
 }
 
+// A lost particle goes back to its beam with its loss record.
+__device__ __forceinline__ void store_lost(const Particle &p, const BeamPtrs &b, int64_t i)
+{
+    store_particle(p, b, i);
+    b.state[i] = p.state;
+    b.loss_ele[i] = p.loss_ele;
+    b.loss_turn[i] = p.loss_turn;
+}
+
+// Slot j of a packed particle buffer (bmx_table.hpp, Packed layout).
+__device__ __forceinline__ uint32_t load_packed(Particle &p, const double *__restrict__ d,
+                                                uint32_t stride, uint32_t j)
+{
+    p.x = d[j];
+    p.px = d[(size_t)stride + j];
+    p.y = d[2 * (size_t)stride + j];
+    p.py = d[3 * (size_t)stride + j];
+    p.z = d[4 * (size_t)stride + j];
+    p.pz = d[5 * (size_t)stride + j];
+    p.beta = d[6 * (size_t)stride + j];
+    p.state = ST_ALIVE;
+    return reinterpret_cast<const uint32_t *>(d + kPackedDoubles * (size_t)stride)[j];
+}
+
+__device__ __forceinline__ void store_packed(const Particle &p, double *__restrict__ d,
+                                             uint32_t stride, uint32_t j, uint32_t id)
+{
+    d[j] = p.x;
+    d[(size_t)stride + j] = p.px;
+    d[2 * (size_t)stride + j] = p.y;
+    d[3 * (size_t)stride + j] = p.py;
+    d[4 * (size_t)stride + j] = p.z;
+    d[5 * (size_t)stride + j] = p.pz;
+    d[6 * (size_t)stride + j] = p.beta;
+    reinterpret_cast<uint32_t *>(d + kPackedDoubles * (size_t)stride)[j] = id;
+}
+
 // Adds the number of particle-element passes tracked by this wave to the pass counter (if
 // non-null): element passes of live particles only, as the reference's loop skips dead ones
 // (beam_utils.f90:83). The counter is kWorkSlots slots that the reader sums (bmx_table.hpp), so
@@ -130,13 +167,10 @@ __device__ __forceinline__ void entrance_retest(Particle &p, const LatDesc &L, i
 // n_ele) by the turn policy Turn; SPAN true: turns of the group's element span (first, count)
 // through the generic element loop. Returns the number of element passes made.
 template <bool F, class Turn, bool SPAN>
-__device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, const BeamPtrs &b,
-                                                             uint32_t i, int32_t n_turns,
-                                                             int32_t turn0, int32_t first,
-                                                             int32_t count)
+__device__ __forceinline__ unsigned long long track_particle(Particle &p, const LatDesc &L,
+                                                             int32_t n_turns, int32_t turn0,
+                                                             int32_t first, int32_t count)
 {
-    Particle p;
-    load_particle(p, b, i);
     refresh_energy(p, L.sp);
     p.loss_ele = 0;
     p.loss_turn = 0;
@@ -157,11 +191,7 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
             else Turn::turn(p, L, turn0 + turn);
         }
     }
-    store_particle(p, b, i);
     if (p.state != ST_ALIVE) {
-        b.state[i] = p.state;
-        b.loss_ele[i] = p.loss_ele;
-        b.loss_turn[i] = p.loss_turn;
         // passes up to and including the loss element, counted from `first`
         const int32_t pos = p.loss_ele - 1 - first;
         return (unsigned long long)(p.loss_turn - turn0) * count + (pos < 0 ? pos + L.n_ele : pos) + 1;
@@ -171,11 +201,13 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
 
 // Persistent tracking of one chunk of turns over a list of wave tiles.
 //
-// Each wave repeatedly takes the next tile (atomic counter) and tracks its particles for the
-// chunk's turns with the state in registers. Particles still alive at the end are appended to
-// their group's segment of the next alive list (one atomic per wave), so the next chunk's
-// tiles hold live particles only and waves stay full as the beam is lost. The order inside a
-// list depends on scheduling; each particle's result does not.
+// Each wave repeatedly takes the next tile (atomic counter), reads its particles from the packed
+// buffer pk_in and tracks them for the chunk's turns with the state in registers. A particle
+// lost in the chunk goes back to its beam with its loss record at once; one that survives is
+// appended to its group's slots of pk_out (one atomic per wave), so the next chunk's tiles hold
+// live particles only, waves stay full as the beam is lost, and every tile reads whole cache
+// lines. In its group's last chunk the survivors go back to the beam instead. The order of the
+// packed slots depends on scheduling; each particle's result does not.
 //
 // Every resident wave takes tiles (as many waves per SIMD as the register budget allows hide
 // each other's latencies better than fewer waves with evenly divided tiles).
@@ -185,9 +217,9 @@ __device__ __forceinline__ unsigned long long track_particle(const LatDesc &L, c
 template <bool F, class Turn, bool SPAN = false>
 __device__ __forceinline__ void track_tiles(
     const Group *__restrict__ groups, const WaveTile *__restrict__ tiles,
-    const uint32_t *__restrict__ n_tiles_p, const uint32_t *__restrict__ list_in,
-    uint32_t *__restrict__ list_out, uint32_t *__restrict__ cnt_out, uint32_t *tile_ctr,
-    int32_t chunk_off, int32_t chunk_turns, unsigned long long *work)
+    const uint32_t *__restrict__ n_tiles_p, const double *__restrict__ pk_in,
+    double *__restrict__ pk_out, uint32_t stride, uint32_t *__restrict__ cnt_out,
+    uint32_t *tile_ctr, int32_t chunk_off, int32_t chunk_turns, unsigned long long *work)
 {
     const uint32_t nt = *n_tiles_p;
     const int lane = threadIdx.x & 63;
@@ -202,27 +234,29 @@ __device__ __forceinline__ void track_tiles(
         const WaveTile T = tiles[t];
         const Group &G = groups[T.group];
         const int32_t rem = G.n_turns - chunk_off;
-        const int32_t n_turns = rem < chunk_turns ? rem : chunk_turns;
+        const bool last = rem <= chunk_turns;  // the group's last chunk of this run
+        const int32_t n_turns = last ? rem : chunk_turns;
         const int32_t turn0 = G.turn0 + chunk_off;
-        bool alive_end = false;
-        uint32_t idx = 0;
+        bool keep = false;
+        uint32_t id = 0;
+        Particle p;
         if ((uint32_t)lane < T.count) {
-            idx = list_in[G.list_off + T.start + lane];
-            if (G.b.state[idx] == ST_ALIVE) {
-                passes += track_particle<F, Turn, SPAN>(G.lat, G.b, idx, n_turns, turn0,
-                                                        SPAN ? G.first : 0,
-                                                        SPAN ? G.count : G.lat.n_ele);
-                alive_end = G.b.state[idx] == ST_ALIVE;
-            }
+            id = load_packed(p, pk_in, stride, G.list_off + T.start + lane);
+            passes += track_particle<F, Turn, SPAN>(p, G.lat, n_turns, turn0,
+                                                    SPAN ? G.first : 0,
+                                                    SPAN ? G.count : G.lat.n_ele);
+            if (p.state != ST_ALIVE) store_lost(p, G.b, id);
+            else if (last) store_particle(p, G.b, id);
+            else keep = true;
         }
-        const unsigned long long m = __ballot(alive_end);
+        const unsigned long long m = __ballot(keep);
         if (m) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(&cnt_out[T.group], (uint32_t)__popcll(m));
             base = __builtin_amdgcn_readfirstlane(base);
-            if (alive_end) {
+            if (keep) {
                 const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1));
-                list_out[G.list_off + base + rank] = idx;
+                store_packed(p, pk_out, stride, G.list_off + base + rank, id);
             }
         }
     }

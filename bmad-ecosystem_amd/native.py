@@ -65,6 +65,7 @@ _SIGS = {
     "bmx_ctx_record": (_I32, [_P, _I32]),
     "bmx_ctx_set_maps": (_I32, [_P, _I32]),
     "bmx_ctx_set_run": (_I32, [_P, _I32, _I32, _I32]),
+    "bmx_ctx_last_run": (_I32, [_P, _P, _P, _P]),
     "bmx_ctx_profile": (_I32, [_P, _I32]),
     "bmx_ctx_kernel_time": (_I32, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]),
     "bmx_ctx_elapsed_ms": (_I32, [_P, _I32, _I32, ctypes.POINTER(ctypes.c_float)]),

@@ -52,9 +52,16 @@ class Context:
 
     def set_run(self, chunk_turns: int = 5, grid: int = 0, waves: int = 0) -> None:
         """Turns per chunk of bmx_track_turns (0 = one chunk), persistent grid size (0 = device
-        occupancy limit) and waves per SIMD of the kernel variant (3, 4 or 5; 0 = default: 5 with
-        the fast maps, 3 with the reference maps)."""
+        occupancy limit) and waves per SIMD of the kernel variant (3, 4 or 5; 0 = chosen per
+        call: on a specialised lattice 5 or 4 by the live particles' count, else 5 with the fast
+        maps, 3 with the reference maps)."""
         native.call("bmx_ctx_set_run", self.h, int(chunk_turns), int(grid), int(waves))
+
+    def last_run(self):
+        """(turns per chunk, workgroups, waves per SIMD) of the last persistent tracking call."""
+        c, g, w = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        native.call("bmx_ctx_last_run", self.h, ctypes.byref(c), ctypes.byref(g), ctypes.byref(w))
+        return c.value, g.value, w.value
 
     def profile(self, on: bool = True) -> None:
         """Bracket every tracking-kernel launch of track_beam with HIP events."""

@@ -181,10 +181,16 @@ int32_t bmx_ctx_set_maps(bmx_ctx *ctx, int32_t form);
 /* Run shape of this context's bmx_track_turns calls: turns per chunk (alive particles are
  * compacted between chunks; 0 = the whole call is one chunk; default 5), workgroups of the
  * persistent tracking grid (0 = as many as fit on the device at once, the default) and the
- * waves per SIMD the tracking kernel variant is compiled for at least (3, 4 or 5; 0 = the
- * default: 3 for a cell-specialised lattice, else 5 with the fast maps, 3 with the reference
+ * waves per SIMD the tracking kernel variant is compiled for at least (3, 4 or 5; 0 = chosen
+ * per call: for a cell-specialised lattice 5 while the live particles fill more than 1.45
+ * rounds of the 4-wave kernel's resident waves and 4 below (the run phase, from a count of the
+ * live particles at the start of the call), else 5 with the fast maps, 3 with the reference
  * maps). */
 int32_t bmx_ctx_set_run(bmx_ctx *ctx, int32_t chunk_turns, int32_t grid, int32_t waves);
+/* The run shape (turns per chunk, workgroups, waves per SIMD) the context's last
+ * bmx_track_turns / bmx_track_turns_from / bmx_track_bunch call used. No counterpart in the
+ * reference (its tracking has no run shape); for reports and autotuning. */
+int32_t bmx_ctx_last_run(const bmx_ctx *ctx, int32_t *chunk_turns, int32_t *grid, int32_t *waves);
 /* Kernel timing of bmx_track_turns: while on, every launch of the tracking kernel is bracketed
  * by HIP events on the context's stream; bmx_ctx_kernel_time synchronises and returns the sum
  * of those launch durations and their number since the last call. */
@@ -278,11 +284,11 @@ int32_t bmx_track_bunch(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam, in
                         int32_t ix_ele2, int32_t turn);
 /* bmx_track_turns without any device allocation. No counterpart in the reference: for callers
  * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
- * harnesses that forbid hipMalloc). Its run buffers (alive lists, wave tiles, counters) are a
- * fixed device array of about 8.5 MiB per GPU, held until the process ends; beams of more than
- * 2^20 particles are refused with BMX_ERR_ARG. All calls on one GPU share that array, so calls
- * that overlap on one GPU (other contexts, other host threads) race: the caller serialises
- * them. Same results as bmx_track_turns. Asynchronous. */
+ * harnesses that forbid hipMalloc). Its run buffers (packed particle buffers, wave tiles,
+ * counters) are a fixed device array of about 30 MiB per GPU, held until the process ends;
+ * beams of more than 2^18 particles are refused with BMX_ERR_ARG. All calls on one GPU share
+ * that array, so calls that overlap on one GPU (other contexts, other host threads) race: the
+ * caller serialises them. Same results as bmx_track_turns. Asynchronous. */
 int32_t bmx_track_turns_noalloc(bmx_ctx *ctx, const bmx_lattice *lat, bmx_beam *beam,
                                 int32_t n_turns, int32_t turn0);
 /* One element for the whole bunch: track1_bunch_hom (beam_utils.f90:32-90) on the GPU, state
@@ -347,7 +353,7 @@ int32_t bmx_closed_orbit(bmx_ctx *ctx, bmx_lattice *const *lats, int32_t n_lat, 
  * are copied to its host_out arrays and its pool space is reused. Each particle's result is
  * the same as tracking its request alone with bmx_track_turns. */
 typedef struct bmx_server_config {
-    int64_t hbm_budget;     /* bytes of device memory for particle state (76 B per particle) */
+    int64_t hbm_budget;     /* bytes of device memory for particle state (188 B per particle) */
     int32_t chunk_turns;    /* turns per scheduling chunk (0 = 100) */
     int32_t compact_turns;  /* turns between alive-list compactions inside a chunk (0 = 5) */
 } bmx_server_config;

@@ -49,7 +49,7 @@ def test_budget_admission_and_late_arrivals(bmx, ctx):
     T, L = bmx.tracking, bmx.lattices
     arr = bmx.elements.to_array(L.ring1(n_cells=10))
     lat = T.Lattice(ctx, arr)
-    per = 76  # bytes of pool per particle
+    per = 188  # bytes of pool per particle
     srv = T.Server(ctx, hbm_budget=per * 9000, chunk_turns=20, compact_turns=5)
     with pytest.raises(bmx.native.BmxError) as ei:
         srv.submit(lat, P.random_beam(bmx, 10000, seed=1), 10)

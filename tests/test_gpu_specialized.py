@@ -216,3 +216,36 @@ def test_specialized_agrees_with_generic(bmx, ctx):
     b = _track(bmx, ctx, arr1, beam, 20, False)
     assert (b["state"] != 1).sum() > 300
     assert P.loss_mismatch(a, b).size == 0
+
+
+def test_run_phase_picks_the_kernel_build(bmx, ctx):
+    """With waves chosen automatically, a beam that fills more than 1.45 rounds of the 4-wave
+    cell kernel's resident waves runs on the 5-wave build and a thinner one on the 4-wave build
+    (the run phase); the two builds of the same maps give bit-identical results."""
+    L, E, T = bmx.lattices, bmx.elements, bmx.tracking
+    lat = T.Lattice(ctx, E.to_array(L.ring1(n_cells=10)))
+    assert lat.specialize()
+    beam = L.gaussian_beam(600_000, seed=2, sigma=3 * L.SIGMA0)
+    outs = {}
+    for waves in (0, 4):
+        ctx.set_run(5, 0, waves)
+        try:
+            b = T.Beam.from_host(ctx, beam)
+            print(f"run phase: 6e5 particles, waves {waves}", flush=True)
+            T.track_beam(ctx, lat, b, 3, 0)
+            ctx.synchronize()
+            outs[waves] = (b.download(), ctx.last_run())
+            b.close()
+        finally:
+            ctx.set_run(5, 0, 0)
+    (auto, shape_auto), (four, shape_four) = outs[0], outs[4]
+    assert shape_auto[2] == 5 and shape_four[2] == 4, (shape_auto, shape_four)
+    for k in P.KEYS:
+        assert np.array_equal(auto[k], four[k]), k
+    small = L.gaussian_beam(20_000, seed=3, sigma=3 * L.SIGMA0)
+    b = T.Beam.from_host(ctx, small)
+    T.track_beam(ctx, lat, b, 2, 0)
+    ctx.synchronize()
+    assert ctx.last_run()[2] == 4
+    b.close()
+    lat.close()

@@ -46,7 +46,7 @@ def test_noalloc_refuses_beams_beyond_its_scratch(bmx):
     L, E, T = bmx.lattices, bmx.elements, bmx.tracking
     ctx = T.Context(0)
     lat = T.Lattice(ctx, E.to_array(L.ring1(n_cells=2)))
-    b = T.Beam(ctx, (1 << 20) + 1)
+    b = T.Beam(ctx, (1 << 18) + 1)
     with pytest.raises(bmx.native.BmxError, match="exceed the fixed scratch"):
         bmx.native.call("bmx_track_turns_noalloc", ctx.h, lat.h, b.h, 1, 0)
     b.close()
