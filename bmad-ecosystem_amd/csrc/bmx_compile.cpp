@@ -181,6 +181,73 @@ int max_order(const double *a, const double *b, int n_max)
     return -1;
 }
 
+// Thin multipole strengths as a/b values (track1_bmad.f90:142-168). multipole$: knl and the
+// order tilts from multipole_ele_to_kt (multipole_ele_to_kt.f90:46-76): knl = a_pole, tilt =
+// b_pole + the element tilt, or, with skew strengths knsl (a_pole_elec), knl = sqrt(knl^2 +
+// knsl^2) and tilt = tn - atan2(knsl, knl) / (n + 1) folded into |tilt| <= pi / (2 (n + 1)),
+// where the element tilt is not added. The kick of an order (multipole_kick, multipole_mod.f90:
+// 203-286) is then the a/b kick with b_n = knl cos((n+1) tilt) / n!, a_n = -knl sin((n+1) tilt)
+// / n! (multipole1_kt_to_ab.f90), n! by repeated multiplication as c_multi's 1/n! factor;
+// k0 / t0 return the dipole order's knl and tilt (its reference-orbit form). ab_multipole$:
+// a_pole / b_pole unscaled (convert_this_ab.f90:575) and rotated by the element tilt
+// (tilt_this_multipole, multipole_ele_to_ab.f90:441-467). Returns the highest order present
+// (-1 none).
+int thin_multipole_ab(const bmx_ele &e, double *a, double *b, double &k0, double &t0)
+{
+    const double pi = 3.141592653589793238462643383279;
+    for (int n = 0; n < BMX_N_POLE; n++) a[n] = b[n] = 0;
+    k0 = t0 = 0;
+    if (e.key == BMX_MULTIPOLE) {
+        bool skew = false;
+        if (e.n_multipole_elec > 0)
+            for (int n = 0; n < BMX_N_POLE; n++) skew |= e.a_pole_elec[n] != 0;
+        double fact = 1;
+        int ix = -1;
+        for (int n = 0; n < BMX_N_POLE; n++) {
+            if (n > 0) fact = n * fact;
+            double knl = e.n_multipole > 0 ? e.a_pole[n] : 0;
+            double t = e.n_multipole > 0 ? e.b_pole[n] : 0;
+            if (!skew) {
+                t = t + e.tilt;
+            } else {
+                const double ks = e.a_pole_elec[n];
+                const double k = knl;
+                knl = std::sqrt(k * k + ks * ks);
+                t = t - std::atan2(ks, k) / (n + 1);
+                if (2 * (n + 1) * std::fabs(t) > pi) {
+                    knl = -knl;
+                    t = (t < 0 ? -1.0 : 1.0) * (std::fabs(t) - pi / (n + 1));
+                }
+            }
+            if (knl == 0) continue;
+            ix = n;
+            if (n == 0) {
+                k0 = knl;
+                t0 = t;
+            }
+            const double angle = -t * (n + 1);
+            b[n] = knl * std::cos(angle) / fact;
+            a[n] = knl * std::sin(angle) / fact;
+        }
+        return ix;
+    }
+    for (int n = 0; n < BMX_N_POLE; n++) {
+        a[n] = e.n_multipole > 0 ? e.a_pole[n] : 0;
+        b[n] = e.n_multipole > 0 ? e.b_pole[n] : 0;
+    }
+    int ix = max_order(a, b, BMX_N_POLE);
+    if (e.tilt != 0) {
+        for (int n = 0; n <= ix; n++) {
+            if (a[n] == 0 && b[n] == 0) continue;
+            const double an = a[n], bn = b[n];
+            const double c = std::cos((n + 1) * e.tilt), sn = std::sin((n + 1) * e.tilt);
+            b[n] = bn * c + an * sn;
+            a[n] = -bn * sn + an * c;
+        }
+    }
+    return ix;
+}
+
 // Electric a/b strengths (multipole_ele_to_ab.f90:31-236 with pole_type = electric$):
 // a_pole_elec / b_pole_elec, order n scaled by r0_elec^-n unless r0_elec is 0 or 1
 // (convert_this_ab, :519-530). Returns ix_pole_max.
@@ -492,7 +559,12 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
         // Misalignments: drifts and markers ignore them in the reference (track1_bmad.f90:93-94);
         // the other non-bend kinds get offset_particle's frame change around their map; bends
         // take the bend branch inside their own map (K_BMULT).
-        const bool offset = key != BMX_DRIFT && key != BMX_MARKER && key != BMX_SBEND && misaligned(e);
+        // a thin multipole's frame change leaves out its tilt (offset_particle with set_tilt =
+        // F, track1_bmad.f90:144): the tilt is in its kicks
+        const bool thin = key == BMX_MULTIPOLE || key == BMX_AB_MULTIPOLE;
+        bmx_ele e_frame = e;
+        if (thin) e_frame.tilt = 0;
+        const bool offset = key != BMX_DRIFT && key != BMX_MARKER && key != BMX_SBEND && misaligned(e_frame);
         if (offset && e.ref_tilt != 0) {
             err = where(i, e) + "ref_tilt on an element that is not a bend";
             return BMX_ERR_UNSUPPORTED;
@@ -503,7 +575,9 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             return BMX_ERR_UNSUPPORTED;
         }
         double a[BMX_N_POLE], b[BMX_N_POLE];
-        if (key != BMX_QUADRUPOLE && key != BMX_SBEND && ele_to_ab_elec(e, a, b) >= 0) {
+        // (a multipole$ keeps its skew strengths in a_pole_elec)
+        if (key != BMX_QUADRUPOLE && key != BMX_SBEND && key != BMX_MULTIPOLE &&
+            ele_to_ab_elec(e, a, b) >= 0) {
             err = where(i, e) + "electric multipoles are supported on quadrupoles and bends only";
             return BMX_ERR_UNSUPPORTED;
         }
@@ -751,6 +825,25 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
                  twopi * e.phi0_multipass};
             break;
         }
+        case BMX_MULTIPOLE:
+        case BMX_AB_MULTIPOLE: {
+            double k0, t0;
+            thin_multipole_ab(e, a, b, k0, t0);
+            h.kind = K_THIN;
+            h.flags |= F_RARE_MAP;
+            const bool ref0 = key == BMX_MULTIPOLE && k0 != 0;
+            if (ref0) {
+                h.flags |= F_THIN_REF0;
+                a[0] = b[0] = 0;
+            }
+            const int n_ord = max_order(a, b, BMX_N_POLE) + 1;
+            h.n = n_ord;
+            q.push_back(ref0 ? k0 * std::cos(t0) : 0.0);
+            q.push_back(ref0 ? k0 * std::sin(t0) : 0.0);
+            for (int n = 0; n < n_ord; n++) q.push_back(a[n]);
+            for (int n = 0; n < n_ord; n++) q.push_back(b[n]);
+            break;
+        }
         default:
             err = where(i, e) + "element kind is not implemented on the GPU path";
             return BMX_ERR_UNSUPPORTED;
@@ -760,11 +853,12 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             err = where(i, e) + why + " are not supported";
             return BMX_ERR_UNSUPPORTED;
         }
-        if ((int)q.size() != kParams[h.kind]) { err = where(i, e) + "internal parameter count"; return BMX_ERR_ARG; }
+        const int n_prm = h.kind == K_THIN ? kParams[K_THIN] + 2 * h.n : kParams[h.kind];
+        if ((int)q.size() != n_prm) { err = where(i, e) + "internal parameter count"; return BMX_ERR_ARG; }
         h.off = (int32_t)out.prm.size();
         if (offset) {
             double ob[kOffsetLen];
-            offset_block(e, h.flags, ob);
+            offset_block(e_frame, h.flags, ob);
             out.prm.insert(out.prm.end(), ob, ob + kOffsetLen);
         }
         if (h.flags & (F_APE_ENT | F_APE_EXIT)) out.prm.insert(out.prm.end(), ap, ap + 4);

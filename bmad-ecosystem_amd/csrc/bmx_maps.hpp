@@ -1722,6 +1722,34 @@ __device__ __forceinline__ void bend_general(Particle &p, CP q, int32_t n_step, 
     }
 }
 
+// Thin multipole (multipole$, ab_multipole$; track1_bmad.f90:142-168 with multipole_kicks,
+// multipole_mod.f90:32-51, 203-286): the kicks of orders 0 .. n-1 summed over (x + i y)^k as
+// a/b strengths (the order tilts folded in on the host), and for a multipole$ the dipole order in
+// the reference-orbit form (:250-254): px += k0l cos t0 pz, py += k0l sin t0 pz, z -= k0l (cos t0
+// x + sin t0 y). The loop over the orders is wave-uniform.
+__device__ __forceinline__ void thin_multipole(Particle &p, CP m, int32_t n, int32_t flags)
+{
+    const double x = p.x, y = p.y;
+    double kx = 0, ky = 0, zr = 1, zi = 0;
+    for (int32_t k = 0; k < n; k++) {
+        k = __builtin_amdgcn_readfirstlane(k);
+        const double a = m[2 + k], b = m[2 + n + k];
+        kx += -b * zr + a * zi;
+        ky += a * zr + b * zi;
+        const double nr = zr * x - zi * y;
+        zi = zr * y + zi * x;
+        zr = nr;
+    }
+    if (flags & F_THIN_REF0) {
+        const double c0 = m[0], s0 = m[1];
+        p.px = p.px + c0 * p.pz;
+        p.py = p.py + s0 * p.pz;
+        p.z = p.z - (c0 * x + s0 * y);
+    }
+    p.px = p.px + kx;
+    p.py = p.py + ky;
+}
+
 template <bool F, int MAPS = MAPS_ALL>
 __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, const Species &sp,
                                               bool from_prev, int32_t ix1, int32_t turn)
@@ -1773,6 +1801,9 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
         break;
     case K_BMULT:
         if constexpr (MAPS == MAPS_ALL_Q || MAPS == MAPS_RARE_Q) bend_general<F>(p, q, n, flags, sp);
+        break;
+    case K_THIN:
+        if constexpr (MAPS != MAPS_NO_RF) thin_multipole(p, q, n, flags);
         break;
     default: break;
     }

@@ -26,7 +26,10 @@ enum Kind : int32_t {
     // multipoles, the full edge (exact_bend_edge_kick); n_step steps of kick - body - kick
     // (track_a_bend.f90:18-333)
     K_BMULT = 7,
-    K_NKIND = 8,
+    // thin multipole (multipole$, ab_multipole$): kicks of any order 0..21 at zero length,
+    // offset_particle without the tilt around them (track1_bmad.f90:142-168)
+    K_THIN = 8,
+    K_NKIND = 9,
 };
 
 enum Flags : int32_t {
@@ -81,6 +84,9 @@ enum Flags : int32_t {
     F_BSEMI = F_MULT_NORMAL,  // dg = 0: the semi-linear body near the zero orbit applies
     F_BK1 = F_QSOFT,          // b1 /= 0: the k1 body map
     F_BEXACT = F_QHARD,       // fringe_type full: hard multipole edge and exact_bend_edge_kick
+    // K_THIN: the dipole order of a multipole$ in the reference-orbit form (multipole_kick's
+    // ref_orb_offset, multipole_mod.f90:250-254)
+    F_THIN_REF0 = F_QHARD,
 };
 constexpr int kOffsetLen = 24;
 
@@ -121,6 +127,9 @@ struct EleHdr {
 //            [67..75] W (entering, row-major), [76..78] d (r = W ((x, y, 0) - d)), [79..87] W
 //            (leaving), [88..90] c (r = W (x, y, 0) + c); [91] cos ref_tilt, [92] sin
 //            ref_tilt, [93] L; hdr.n = n_step
+//  K_THIN  : [0] k0l cos t0, [1] k0l sin t0 (the reference-orbit dipole, F_THIN_REF0),
+//            [2 .. 2+N) a_n, [2+N .. 2+2N) b_n (orders 0 .. N-1 as a/b strengths, the order
+//            tilts folded in); hdr.n = N (0..22), so the block holds 2 + 2N values
 // preceded, when F_OFFSET, by the offset block (hdr.off points at it; then the aperture values,
 // then the parameters):
 //  [0] x_offset, [1] y_offset, [2] z_offset, [3] L/2, [4..12] W^-1 (entrance frame change,
@@ -129,7 +138,8 @@ struct EleHdr {
 // when F_APE_MOMENTUM by 8 momentum-window values:
 //  elliptical : x_center, 1/x_width2, y_center, 1/y_width2
 //  rectangular: x1_lim (= -x1_limit), x2_lim, y1_lim, y2_lim (0 = no limit)
-constexpr int kParams[K_NKIND] = {1, 5, 9, 29, 7, 0, 32, 94};
+// Parameter count per kind (K_THIN: 2 + 2 hdr.n, here its minimum)
+constexpr int kParams[K_NKIND] = {1, 5, 9, 29, 7, 0, 32, 94, 2};
 
 struct Species {
     double mc2;      // [eV]

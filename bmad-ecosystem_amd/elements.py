@@ -35,12 +35,15 @@ BMAD_DEFAULT_DS_STEP = 0.2  # bmad_struct.f90:2316
 DRIFT, SBEND, QUADRUPOLE, SEXTUPOLE, RFCAVITY, MARKER, KICKER, OCTUPOLE = 1, 2, 3, 5, 9, 14, 15, 17
 MONITOR, INSTRUMENT, HKICKER, VKICKER, RCOLLIMATOR, ECOLLIMATOR, PIPE = 31, 32, 33, 34, 35, 36, 44
 THICK_MULTIPOLE = 67
+# thin multipoles: multipole$ holds knl in a_pole, tn in b_pole and the skew knsl in
+# a_pole_elec (pointer_to_ele_multipole.f90:30-34); ab_multipole$ holds a/b in a_pole/b_pole
+MULTIPOLE, AB_MULTIPOLE = 19, 22
 KEY_NAMES = {
     DRIFT: "drift", SBEND: "sbend", QUADRUPOLE: "quadrupole", SEXTUPOLE: "sextupole",
     RFCAVITY: "rfcavity", MARKER: "marker", KICKER: "kicker", OCTUPOLE: "octupole",
     MONITOR: "monitor", INSTRUMENT: "instrument", HKICKER: "hkicker", VKICKER: "vkicker",
     RCOLLIMATOR: "rcollimator", ECOLLIMATOR: "ecollimator", PIPE: "pipe",
-    THICK_MULTIPOLE: "thick_multipole",
+    THICK_MULTIPOLE: "thick_multipole", MULTIPOLE: "multipole", AB_MULTIPOLE: "ab_multipole",
 }
 FRINGE_NONE, FRINGE_SOFT_EDGE_ONLY, FRINGE_HARD_EDGE_ONLY, FRINGE_FULL, FRINGE_BASIC_BEND = 1, 2, 3, 4, 7
 # exact_multipoles (bmad_struct.f90:256-261)
@@ -217,11 +220,19 @@ _DEFAULT_TILT = {QUADRUPOLE: PI / 4.0, SEXTUPOLE: PI / 6.0, OCTUPOLE: PI / 8.0}
 def make_element(key: int, p0c: float, e_tot: float, *, num_steps: int = 0, ds_step: float = 0.0,
                  x_limit: float | None = None, y_limit: float | None = None,
                  default_tilt: bool = False, a_pole=None, b_pole=None, a_pole_elec=None,
-                 b_pole_elec=None, **attrs) -> BmxEle:
+                 b_pole_elec=None, knl=None, tn=None, ksl=None, **attrs) -> BmxEle:
     """One element with Bmad defaults applied, then the user attributes, then bookkeeping.
 
     ``default_tilt=True`` is a bare ``tilt`` in a lattice file. a_pole / b_pole and
-    a_pole_elec / b_pole_elec are {order: value} dicts of the magnetic and electric multipoles."""
+    a_pole_elec / b_pole_elec are {order: value} dicts of the magnetic and electric multipoles.
+    knl / tn / ksl: a thin multipole's integrated strengths, tilts and skew strengths ({order:
+    value}), stored where ele_struct keeps them (a_pole, b_pole, a_pole_elec)."""
+    if knl is not None or tn is not None or ksl is not None:
+        if key != MULTIPOLE:
+            raise AttributeError("knl / tn / ksl belong to a multipole element")
+        a_pole, b_pole = dict(knl or {}), dict(tn or {})
+        if ksl:
+            a_pole_elec = dict(ksl)
     e = BmxEle()
     e.key = key
     e.is_on = 1

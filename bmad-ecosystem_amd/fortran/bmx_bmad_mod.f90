@@ -45,6 +45,7 @@ integer, parameter, public :: bmx_moving_forward = -1
 ! Element keys (bmad_struct.f90:1696-1710)
 integer(c_int32_t), parameter, public :: bmx_drift = 1, bmx_sbend = 2, bmx_quadrupole = 3, &
     bmx_sextupole = 5, bmx_rfcavity = 9, bmx_marker = 14, bmx_kicker = 15, bmx_octupole = 17, &
+    bmx_multipole = 19, bmx_ab_multipole = 22, &
     bmx_monitor = 31, bmx_instrument = 32, bmx_hkicker = 33, bmx_vkicker = 34, &
     bmx_rcollimator = 35, bmx_ecollimator = 36, bmx_pipe = 44, bmx_thick_multipole = 67
 

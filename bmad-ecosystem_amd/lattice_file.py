@@ -34,7 +34,8 @@ _TYPES = {
     "sextupole": E.SEXTUPOLE, "octupole": E.OCTUPOLE, "marker": E.MARKER, "rfcavity": E.RFCAVITY,
     "hkicker": E.HKICKER, "vkicker": E.VKICKER, "kicker": E.KICKER, "monitor": E.MONITOR,
     "instrument": E.INSTRUMENT, "ecollimator": E.ECOLLIMATOR, "rcollimator": E.RCOLLIMATOR,
-    "pipe": E.PIPE, "thick_multipole": E.THICK_MULTIPOLE,
+    "pipe": E.PIPE, "thick_multipole": E.THICK_MULTIPOLE, "multipole": E.MULTIPOLE,
+    "ab_multipole": E.AB_MULTIPOLE,
 }
 
 # particle_species_mod.f90 masses [eV] and charges
@@ -61,6 +62,9 @@ _ATTRS = {"l", "k1", "k2", "k3", "g", "dg", "e1", "e2", "fint", "fintx", "hgap",
           "r0_elec", "fq1", "fq2", "exact_multipoles"}
 # multipoles: a0..a21, b0..b21 (magnetic) and a0_elec..b21_elec (electric)
 _MULTIPOLE = re.compile(r"^([ab])(\d+)(_elec)?$")
+# a thin multipole's integrated strengths k0l..k21l, skew strengths k0sl..k21sl and order tilts
+# t0..t21
+_THIN = re.compile(r"^(?:k(\d+)(s?)l|t(\d+))$")
 _ENUMS = {"aperture_type": {"rectangular": E.APERTURE_RECTANGULAR, "elliptical": E.APERTURE_ELLIPTICAL},
           "aperture_at": {"entrance_end": E.ENTRANCE_END, "exit_end": E.EXIT_END,
                           "both_ends": E.BOTH_ENDS, "no_end": E.NO_END},
@@ -99,6 +103,7 @@ class _Parser:
         t = text.strip()
         t = re.sub(r"([A-Za-z_][\w.]*)\[([A-Za-z_]\w*)\]", lambda m: f"__ref__('{m.group(1)}','{m.group(2)}')", t)
         t = t.replace("^", "**")
+        t = re.sub(r"\.(true|false)\.", lambda m: "1" if m.group(1).lower() == "true" else "0", t, flags=re.I)
         try:
             tree = ast.parse(t, mode="eval")
         except SyntaxError as e:
@@ -186,7 +191,22 @@ class _Parser:
                 k, v = m.group(1).lower(), m.group(2).strip()
                 self.params[k] = v.strip("'\"").lower() if k in ("particle", "geometry") else self.value(v)
                 continue
+            m = re.match(r"(\w+)::([\w.%*]+)\s*\[\s*(\w+)\s*\]\s*=\s*(.+)$", st)
+            if m:
+                # type::name-pattern[attr] = value: every element of that type whose name
+                # matches (wildcards % and *); %[attr] in the value is the element's own value
+                tname = _lookup_type(m.group(1))
+                if tname is None:
+                    raise LatticeFileError(f"unknown element type in {st!r}")
+                rx = re.compile("^" + re.escape(m.group(2).lower()).replace("%", ".").replace("\\*", ".*") + "$")
+                self._set_class(tname, rx, m.group(3).lower(), m.group(4), st)
+                continue
             m = re.match(r"([\w.]+)\s*\[\s*(\w+)\s*\]\s*=\s*(.+)$", st)
+            if m and m.group(1).lower() not in self.eles and m.group(1).lower() not in self.controllers \
+                    and _lookup_type(m.group(1)) is not None:
+                # type[attr] = value: every element of that type defined so far
+                self._set_class(_lookup_type(m.group(1)), None, m.group(2).lower(), m.group(3), st)
+                continue
             if m:
                 name, attr = m.group(1).lower(), m.group(2).lower()
                 if name in self.controllers:
@@ -222,12 +242,14 @@ class _Parser:
             return
         if head.lower() in ("list", "overlay", "group", "girder") or "superimpose" in rest.lower():
             raise LatticeFileError(f"{head} elements are not supported: {st!r}")
-        tname = _lookup_type(head)
-        if tname is not None:
-            attrs = {}
-        elif head.lower() in self.eles:
+        # an element defined earlier takes precedence over a type name it abbreviates
+        # ("rfcav" after "rfcav: rfcavity, ...")
+        if head.lower() in self.eles:
             tname, parent = self.eles[head.lower()]
             attrs = dict(parent)
+        elif _lookup_type(head) is not None:
+            tname = _lookup_type(head)
+            attrs = {}
         else:
             raise LatticeFileError(f"element type {head!r} is not on this path: {st!r}")
         for p in parts[1:]:
@@ -271,7 +293,7 @@ class _Parser:
             if not m:
                 raise LatticeFileError(f"{kind} slave {item!r} not supported: {st!r}")
             attr = m.group(2).lower()
-            if attr not in _ATTRS and not _MULTIPOLE.match(attr):
+            if attr not in _ATTRS and not _MULTIPOLE.match(attr) and not _THIN.match(attr):
                 raise LatticeFileError(f"{kind} controls attribute {attr!r}, not on this path")
             expr = m.group(3) if m.group(3) is not None else next(iter(variables))
             slaves.append((m.group(1).lower(), attr, expr))
@@ -302,8 +324,15 @@ class _Parser:
         for (n, attr), val in sums.items():
             self.eles[n][1][attr] = val
 
+    def _set_class(self, tname, rx, attr, text, st):
+        hits = [n for n, (t, _) in self.eles.items() if t == tname and (rx is None or rx.match(n))]
+        if rx is not None and not hits:
+            raise LatticeFileError(f"no element matches {st!r}")
+        for n in hits:
+            self._set(self.eles[n][1], attr, text.replace("%[", f"{n}["))
+
     def _set(self, attrs, k, v):
-        if _MULTIPOLE.match(k):
+        if _MULTIPOLE.match(k) or _THIN.match(k):
             attrs[k] = self.value(v)
             return
         if k not in _ATTRS:
@@ -435,6 +464,21 @@ def load(path: str):
                 raise LatticeFileError(f"multipole order {n} above {E.N_POLE - 1}")
             poles[f"{ab}_pole{'_elec' if elec else ''}"][n] = a.pop(k)
         poles = {k: v for k, v in poles.items() if v}
+        thin = {"knl": {}, "tn": {}, "ksl": {}}
+        for k in [k for k in a if _THIN.match(k)]:
+            kn, skew, tn = _THIN.match(k).groups()
+            n = int(kn if kn is not None else tn)
+            if tname != "multipole":
+                raise LatticeFileError(f"{k} on a {tname} (only a multipole has it)")
+            if n >= E.N_POLE:
+                raise LatticeFileError(f"multipole order {n} above {E.N_POLE - 1}")
+            thin["tn" if tn is not None else ("ksl" if skew else "knl")][n] = a.pop(k)
+        if tname == "multipole":
+            if poles:
+                raise LatticeFileError("a multipole takes KnL, KnSL and Tn, not a/b values")
+            poles = {k: v for k, v in thin.items() if v}
+            if "knl" not in poles and poles:
+                poles["knl"] = {}
         eles.append(E.make_element(_TYPES[tname], p0c, e_tot, num_steps=num_steps, ds_step=ds_step,
                                    default_tilt=default_tilt, **poles, **a))
     return eles, mc2, charge

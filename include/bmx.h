@@ -66,6 +66,8 @@ extern "C" {
 #define BMX_MARKER          14
 #define BMX_KICKER          15
 #define BMX_OCTUPOLE        17
+#define BMX_MULTIPOLE       19  /* thin: a_pole = knl, b_pole = tn, a_pole_elec = ksl */
+#define BMX_AB_MULTIPOLE    22  /* thin: a_pole, b_pole */
 #define BMX_MONITOR         31
 #define BMX_INSTRUMENT      32
 #define BMX_HKICKER         33

@@ -1886,6 +1886,182 @@ void bmo_check_aperture_limit(const bmx_ele *ele, int32_t second_edge, double *v
  * ------------------------------------------------------------------------------------------ */
 
 /* bmad/low_level/track1_bmad.f90:22-283 (charged particles). */
+/* ------------------------------------------------------------------------------------------
+ * thin multipoles (multipole$, ab_multipole$)
+ * ------------------------------------------------------------------------------------------ */
+
+/* sim_utils/special_functions/factorial.f90:13-48: n! from the reference's table of 13-digit
+ * decimal literals (exact up to 18!, rounded from 19! on). */
+static double bmo_factorial(int n)
+{
+    static const double f[22] = {
+        1.000000000000E+00, 1.000000000000E+00, 2.000000000000E+00, 6.000000000000E+00,
+        2.400000000000E+01, 1.200000000000E+02, 7.200000000000E+02, 5.040000000000E+03,
+        4.032000000000E+04, 3.628800000000E+05, 3.628800000000E+06, 3.991680000000E+07,
+        4.790016000000E+08, 6.227020800000E+09, 8.717829120000E+10, 1.307674368000E+12,
+        2.092278988800E+13, 3.556874280960E+14, 6.402373705728E+15, 1.216451004088E+17,
+        2.432902008177E+18, 5.109094217171E+19};
+    return f[n];
+}
+
+/* sim_utils/special_functions/sign_of_mod.f90:41-61 with zero_is_zero = F. */
+static double sign_of_nz(double x) { return x < 0 ? -1.0 : 1.0; }
+
+/* bmad/low_level/multipole1_kt_to_ab.f90:18-41. */
+static void multipole1_kt_to_ab(double knl, double knsl, double tn, int n, double *an, double *bn)
+{
+    if (knl == 0 && knsl == 0) {
+        *bn = 0;
+        *an = 0;
+    } else {
+        double angle = -tn * (n + 1);
+        *bn = (knl * cos(angle) - knsl * sin(angle)) / bmo_factorial(n);
+        *an = (knl * sin(angle) + knsl * cos(angle)) / bmo_factorial(n);
+    }
+}
+
+/* bmad/low_level/multipole1_ab_to_kt.f90:17-47. */
+static void multipole1_ab_to_kt(double an, double bn, int n, double *knl, double *tn)
+{
+    if (an == 0 && bn == 0) {
+        *knl = 0;
+        *tn = 0;
+    } else {
+        double a = an, b = bn;
+        *knl = bmo_factorial(n) * sqrt(a * a + b * b);
+        *tn = -atan2(a, b) / (n + 1);
+        if (2 * (n + 1) * fabs(*tn) > PI) {
+            *knl = -*knl;
+            *tn = sign_of_nz(*tn) * (fabs(*tn) - PI / (n + 1));
+        }
+    }
+}
+
+/* bmad/code/multipole_kt_to_ab.f90:17-34 over orders 0 .. 21. */
+void bmo_multipole_kt_to_ab(const double *knl, const double *knsl, const double *tn, double *an,
+                            double *bn)
+{
+    for (int n = 0; n < BMX_N_POLE; n++) multipole1_kt_to_ab(knl[n], knsl[n], tn[n], n, &an[n], &bn[n]);
+}
+
+/* bmad/code/multipole_ele_to_kt.f90:29-95 (magnetic, field_master = F). multipole$: the
+ * element's knl (a_pole) and tn (b_pole) plus its tilt, or, with skew strengths ksl
+ * (a_pole_elec, pointer_to_ele_multipole.f90:30-34), knl = sqrt(knl^2 + ksl^2) and tilt = tn -
+ * atan2(ksl, knl) / (n + 1) folded into |tilt| <= pi / (2 (n + 1)), where the element tilt is not
+ * added (:57-66). ab_multipole$: multipole_ele_to_ab with use_ele_tilt = T and no kicks (the a/b
+ * values unscaled, convert_this_ab.f90:575, rotated by tilt_this_multipole, :441-467), then
+ * multipole_ab_to_kt. Returns ix_pole_max (-1 if no order is set). */
+static int thin_multipole_ele_to_kt(const bmx_ele *e, double *knl, double *tilt)
+{
+    for (int n = 0; n < BMX_N_POLE; n++) knl[n] = tilt[n] = 0;
+    if (e->key == BMX_MULTIPOLE) {
+        int skew = 0;
+        if (e->n_multipole_elec > 0)
+            for (int n = 0; n < BMX_N_POLE; n++) skew |= e->a_pole_elec[n] != 0;
+        for (int n = 0; n < BMX_N_POLE; n++) {
+            double k = e->n_multipole > 0 ? e->a_pole[n] : 0;
+            double t = e->n_multipole > 0 ? e->b_pole[n] : 0;
+            if (!skew) {
+                knl[n] = k;
+                tilt[n] = t + e->tilt;
+            } else {
+                double s = e->a_pole_elec[n];
+                knl[n] = sqrt(k * k + s * s);
+                tilt[n] = t - atan2(s, k) / (n + 1);
+                if (2 * (n + 1) * fabs(tilt[n]) > PI) {
+                    knl[n] = -knl[n];
+                    tilt[n] = sign_of_nz(tilt[n]) * (fabs(tilt[n]) - PI / (n + 1));
+                }
+            }
+        }
+        for (int n = BMX_N_POLE - 1; n >= 0; n--)
+            if (knl[n] != 0) return n;
+        return -1;
+    }
+    double a[BMX_N_POLE], b[BMX_N_POLE];
+    for (int n = 0; n < BMX_N_POLE; n++) {
+        a[n] = e->n_multipole > 0 ? e->a_pole[n] : 0;
+        b[n] = e->n_multipole > 0 ? e->b_pole[n] : 0;
+    }
+    int ix = max_nonzero(a, b, BMX_N_POLE - 1);
+    if (ix < 0) return -1;
+    double t = e->tilt;
+    if (t != 0) {
+        for (int n = 0; n <= ix; n++) {
+            if (a[n] == 0 && b[n] == 0) continue;
+            double an = a[n], bn = b[n];
+            double cos_t = cos((n + 1) * t), sin_t = sin((n + 1) * t);
+            b[n] = bn * cos_t + an * sin_t;
+            a[n] = -bn * sin_t + an * cos_t;
+        }
+    }
+    for (int n = 0; n < BMX_N_POLE; n++) multipole1_ab_to_kt(a[n], b[n], n, &knl[n], &tilt[n]);
+    return ix;
+}
+
+/* bmad/modules/multipole_mod.f90:203-286 multipole_kick, magnetic, for the tracked species equal
+ * to the reference species (charge = 1). ref_orb_offset: the dipole order of a multipole$ bends
+ * the reference orbit (:250-254). */
+static void multipole_kick_kt(double knl, double tilt, int n, coord_t *o, int ref_orb_offset)
+{
+    if (knl == 0) return;
+    const double charge = 1.0;
+    double t = tilt, sin_ang, cos_ang, x, y;
+    if (t == 0) {
+        sin_ang = 0;
+        cos_ang = 1;
+        x = o->vec[0];
+        y = o->vec[2];
+    } else {
+        sin_ang = sin(t);
+        cos_ang = cos(t);
+        x = o->vec[0] * cos_ang + o->vec[2] * sin_ang;
+        y = -o->vec[0] * sin_ang + o->vec[2] * cos_ang;
+    }
+    if (n == 0 && ref_orb_offset) {
+        o->vec[1] = o->vec[1] + charge * knl * cos_ang * o->vec[5];
+        o->vec[3] = o->vec[3] + charge * knl * sin_ang * o->vec[5];
+        o->vec[4] = o->vec[4] - charge * knl * (cos_ang * o->vec[0] + sin_ang * o->vec[2]);
+        return;
+    }
+    double x_terms[BMX_N_POLE], y_terms[BMX_N_POLE];
+    x_terms[n] = 1.0;
+    y_terms[0] = 1.0;
+    for (int m = 1; m <= n; m++) {
+        x_terms[n - m] = x_terms[n - m + 1] * x;
+        y_terms[m] = y_terms[m - 1] * y;
+    }
+    /* cc = c_multi's table with the 1/n! factor (c_full) */
+    double x_value = 0, y_value = 0;
+    for (int m = 0; m <= n; m += 2) x_value = x_value + c_multi_tab[n][m] * x_terms[m] * y_terms[m];
+    for (int m = 1; m <= n; m += 2) y_value = y_value + c_multi_tab[n][m] * x_terms[m] * y_terms[m];
+    double x_vel = charge * knl * x_value;
+    double y_vel = charge * knl * y_value;
+    if (t == 0) {
+        o->vec[1] = o->vec[1] + x_vel;
+        o->vec[3] = o->vec[3] + y_vel;
+    } else {
+        o->vec[1] = o->vec[1] + x_vel * cos_ang - y_vel * sin_ang;
+        o->vec[3] = o->vec[3] + x_vel * sin_ang + y_vel * cos_ang;
+    }
+}
+
+/* bmad/low_level/track1_bmad.f90:142-168: offset_particle without the tilt (the tilt is in the
+ * kicks' angles), multipole_kicks (multipole_mod.f90:32-51) over every order with knl /= 0,
+ * offset_particle unset. */
+static int track_a_thin_multipole(coord_t *o, const bmx_ele *e)
+{
+    double knl[BMX_N_POLE], tilt[BMX_N_POLE];
+    int ix = thin_multipole_ele_to_kt(e, knl, tilt);
+    int mis = is_misaligned(e);
+    if (mis) offset_particle(e, 1, o, 0, 0, 0, 0);
+    if (ix > -1)
+        for (int n = 0; n < BMX_N_POLE; n++)
+            multipole_kick_kt(knl[n], tilt[n], n, o, e->key == BMX_MULTIPOLE);
+    if (mis) offset_particle(e, 0, o, 0, 0, 0, 0);
+    return 0;
+}
+
 static int track1_bmad(coord_t *o, const bmx_ele *e)
 {
     int key = e->key;
@@ -1901,6 +2077,7 @@ static int track1_bmad(coord_t *o, const bmx_ele *e)
     case BMX_QUADRUPOLE: return track_a_quadrupole(o, e);
     case BMX_RFCAVITY: return track_a_rfcavity(o, e);
     case BMX_SBEND: return track_a_bend(o, e);
+    case BMX_MULTIPOLE: case BMX_AB_MULTIPOLE: return track_a_thin_multipole(o, e);
     default: return BMX_ERR_UNSUPPORTED;
     }
 }

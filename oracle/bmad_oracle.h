@@ -26,6 +26,9 @@ double bmo_cosc(double x);
 double bmo_sincc(double x);
 double bmo_modulo2(double x, double amp);
 double bmo_c_multi(int32_t n, int32_t m);
+/* multipole_kt_to_ab (bmad/code/multipole_kt_to_ab.f90) over orders 0 .. 21. */
+void bmo_multipole_kt_to_ab(const double *knl, const double *knsl, const double *tn, double *an,
+                            double *bn);
 
 /* One particle through one element: track1 (bmad/code/track1.f90:44-343) with
  * tracking_method = bmad_standard. vec[6], beta and state are updated in place.

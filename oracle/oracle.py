@@ -65,6 +65,8 @@ def lib(fma: bool = False, path: str | None = None):
         L.bmo_modulo2.restype = ctypes.c_double
         L.bmo_c_multi.argtypes = [ctypes.c_int32, ctypes.c_int32]
         L.bmo_c_multi.restype = ctypes.c_double
+        L.bmo_multipole_kt_to_ab.argtypes = [_D, _D, _D, _D, _D]
+        L.bmo_multipole_kt_to_ab.restype = None
         _libs[path] = L
     return _libs[path]
 
@@ -150,3 +152,16 @@ def check_aperture(ele, vec, second_edge=True, state=1):
     lib().bmo_check_aperture_limit(_ele_ptr(ele), 1 if second_edge else 0, _ptr(v, _D),
                                    ctypes.byref(s), ctypes.byref(u))
     return s.value, u.value
+
+
+def multipole_kt_to_ab(knl, knsl, tn):
+    """multipole_kt_to_ab (bmad/code/multipole_kt_to_ab.f90): integrated strengths knl, skew
+    knsl and tilts tn of orders 0..21 -> (a, b)."""
+    k = [np.zeros(22) for _ in range(3)]
+    for arr, src in zip(k, (knl, knsl, tn)):
+        for n, v in dict(src).items():
+            arr[n] = v
+    a = np.zeros(22)
+    b = np.zeros(22)
+    lib().bmo_multipole_kt_to_ab(*[_ptr(x, _D) for x in (*k, a, b)])
+    return a, b

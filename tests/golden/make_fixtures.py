@@ -8,6 +8,8 @@ Sources:
   regression_tests/tracking_method_test/output.correct  -- "<ELE>: Bmad_Standard" lines
   regression_tests/fringe_test/output.correct           -- Up-Orb / Dn-Orb (Hwang edge kick)
   regression_tests/aperture_test/output.correct         -- EC_*/RC_* state and unstable lines
+  regression_tests/multipole_test/output.correct        -- the thin multipole MM's lines
+  regression_tests/closed_orbit_test/ad05f_{pos,neg}_RF.lat -- lattices (lattices/closed_orbit)
 """
 import os
 import sys
@@ -40,6 +42,14 @@ def main():
                  lambda ln: ln.startswith('"EC_') or ln.startswith('"RC_'))
     with open(os.path.join(HERE, "aperture_test.correct"), "w") as f:
         f.write("\n".join(lines) + "\n")
+    lines = grab("regression_tests/multipole_test/output.correct", lambda ln: '-MM"' in ln)
+    with open(os.path.join(HERE, "multipole_test_mm.correct"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    os.makedirs(os.path.join(HERE, "lattices", "closed_orbit"), exist_ok=True)
+    for name in ("ad05f_pos_RF.lat", "ad05f_neg_RF.lat"):
+        lines = grab("regression_tests/closed_orbit_test/" + name, lambda ln: True)
+        with open(os.path.join(HERE, "lattices", "closed_orbit", name), "w") as f:
+            f.write("\n".join(lines) + "\n")
 
 
 if __name__ == "__main__":

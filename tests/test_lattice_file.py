@@ -130,3 +130,66 @@ def test_overlays_groups_and_branch_start(bmx, lf, tmp_path):
     assert eles[0].e_tot == 2e9
     rc, msg = _validate(bmx, E.to_array(eles), mc2, charge)
     assert rc == 0, msg
+
+
+def test_thin_multipoles_and_class_settings(bmx, lf, tmp_path):
+    """multipole (KnL, KnSL, Tn, tilt) and ab_multipole (an, bn) elements land in the element
+    record where ele_struct keeps them (knl -> a_pole, tn -> b_pole, knsl -> a_pole_elec);
+    'type[attr] = v' sets every element of that type, 'type::pattern[attr] = f(%[attr])' the
+    matching ones from their own value; an element name shadows a type name it abbreviates."""
+    E = bmx.elements
+    p = tmp_path / "lat.bmad"
+    p.write_text(
+        "parameter[e_tot] = 3e9\nparameter[ptc_exact_model] = .false.\n"
+        "m1: multipole, k1l = 0.2, t1 = 0.1, k3l = 40, k2sl = 5, tilt = 0.05\n"
+        "ab: ab_multipole, a2 = 1.5, b4 = -3, tilt = 0.3\n"
+        "sx1: sextupole, l = 0.2, k2 = 10\nsx2: sextupole, l = 0.2, k2 = -8\n"
+        "quad1: quadrupole, l = 0.5, k1 = 0.4\n"
+        "rfc: rfcavity, l = 0.5, voltage = 1e6, rf_frequency = 5e8\nrfx: rfc\n"
+        "sextupole[num_steps] = 4\nsext::sx1[k2] = 0.5 * %[k2]\nquad::*[ds_step] = 0.05\n"
+        "r: line = (m1, ab, sx1, sx2, quad1, rfx)\nuse, r\n")
+    eles, mc2, charge = lf.load(str(p))
+    m1, ab, sx1, sx2, q, rfx = eles
+    assert m1.key == E.MULTIPOLE and ab.key == E.AB_MULTIPOLE
+    assert (m1.a_pole[1], m1.b_pole[1], m1.a_pole[3], m1.a_pole_elec[2], m1.tilt) == (0.2, 0.1, 40, 5, 0.05)
+    assert (ab.a_pole[2], ab.b_pole[4], ab.tilt) == (1.5, -3, 0.3)
+    assert (sx1.k2, sx2.k2) == (5.0, -8.0)
+    assert sx1.ds_step == pytest.approx(0.05) and sx2.ds_step == pytest.approx(0.05)
+    assert q.ds_step == pytest.approx(0.05)
+    assert rfx.key == E.RFCAVITY and rfx.voltage == 1e6 and rfx.l == 0.5
+    rc, msg = _validate(bmx, E.to_array(eles), mc2, charge)
+    assert rc == 0, msg
+    p.write_text("parameter[e_tot] = 3e9\nq: quadrupole, l = 0.5, k1l = 0.2\nr: line = (q)\nuse, r\n")
+    with pytest.raises(lf.LatticeFileError):
+        lf.load(str(p))
+
+
+def test_closed_orbit_test_lattices(bmx, lf):
+    """The SLS-2 lattices of the reference's closed-orbit regression test (closed_orbit_test/
+    ad05f_{pos,neg}_RF.lat, kept as fixtures): 3120 elements each, 96 thin octupole
+    multipoles, 12 rf cavities at harmonic 96 of the 288 m ring (99.93 MHz), the pos_RF file's
+    sextupoles halved by 'sextupole::*[k2] = 0.5 * %[k2]', bends at ds_step 0.005; both compile
+    for the GPU path."""
+    E = bmx.elements
+    d = os.path.join(HERE, "golden", "lattices", "closed_orbit")
+    pos, _, _ = lf.load(os.path.join(d, "ad05f_pos_RF.lat"))
+    neg, mc2, charge = lf.load(os.path.join(d, "ad05f_neg_RF.lat"))
+    for eles in (pos, neg):
+        arr = E.to_array(eles)
+        assert len(arr) == 3120
+        assert (arr["key"] == E.MULTIPOLE).sum() == 96
+        rf = arr[arr["key"] == E.RFCAVITY]
+        assert len(rf) == 12
+        L = float(arr["l"].sum())
+        assert L == pytest.approx(288.00264, rel=1e-12)
+        beta0 = float(arr[0]["p0c"] / arr[0]["e_tot"])
+        assert np.allclose(rf["rf_frequency"], 96 * beta0 * E.C_LIGHT / L, rtol=1e-14)
+        assert np.all(arr["ds_step"][arr["key"] == E.SBEND] == 0.005)
+        rc, msg = _validate(bmx, arr, mc2, charge)
+        assert rc == 0, msg
+    kp = np.array([e.k2 for e in pos if e.key == E.SEXTUPOLE])
+    kn = np.array([e.k2 for e in neg if e.key == E.SEXTUPOLE])
+    assert np.array_equal(kp, 0.5 * kn) and np.all(kn != 0)
+    vp = [e.voltage for e in pos if e.key == E.RFCAVITY]
+    vn = [e.voltage for e in neg if e.key == E.RFCAVITY]
+    assert vp == [-v for v in vn] and vp[0] > 0

@@ -202,3 +202,23 @@ def test_gaussian_beam_is_seeded(bmx):
         assert np.array_equal(a[k], b[k])
     assert abs(np.std(a["x"]) - 100e-6) < 10e-6
     assert a["beta"].dtype == np.float64 and np.all(a["beta"] < 1)
+
+
+def test_thin_multipoles_compile(bmx):
+    """multipole / ab_multipole compile to the thin-multipole block, 2 + 2 N values for orders
+    0 .. N-1 (a multipole's dipole order in the reference-orbit form takes the first two), plus
+    the 24-value offset block when offset or pitched (the tilt alone adds none: it is in the
+    kicks); electric multipoles on an ab_multipole are rejected."""
+    E = bmx.elements
+    p0c, et = bmx.lattices.reference()
+    mk = E.make_element
+    for ele, n in ((mk(E.MULTIPOLE, p0c, et, knl={0: 1e-3, 1: 0.2, 3: 40}), 2 + 2 * 4),
+                   (mk(E.MULTIPOLE, p0c, et, knl={0: 1e-3}), 2),
+                   (mk(E.MULTIPOLE, p0c, et, knl={21: 1.0}, tilt=0.2), 2 + 2 * 22),
+                   (mk(E.MULTIPOLE, p0c, et, knl={2: 3.0}, ksl={1: 0.1}, x_offset=1e-4), 2 + 6 + 24),
+                   (mk(E.AB_MULTIPOLE, p0c, et, a_pole={2: 1.0}, tilt=0.3), 2 + 6),
+                   (mk(E.AB_MULTIPOLE, p0c, et, b_pole={0: 1e-5}, y_pitch=1e-4), 2 + 2 + 24)):
+        rc, nprm, msg = _validate(bmx, [ele])
+        assert rc == 0 and nprm == n, (rc, nprm, n, msg)
+    rc, _, msg = _validate(bmx, [mk(E.AB_MULTIPOLE, p0c, et, a_pole={1: 1.0}, b_pole_elec={0: 1e5})])
+    assert rc == 2 and "electric" in msg, msg
