@@ -15,6 +15,7 @@ LIB_PATH = os.environ.get("BMX_LIBRARY") or os.path.join(HERE, "libbmx.so")
 _lib = None
 
 ERR_UNSUPPORTED = 2  # BMX_ERR_UNSUPPORTED
+ERR_UNSTABLE = 9  # BMX_ERR_UNSTABLE
 
 
 class BmxError(RuntimeError):
@@ -82,6 +83,11 @@ _SIGS = {
     "bmx_closed_orbit": (_I32, [_P, _PP, _I32, _I32, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double), _I32, ctypes.c_double,
                                 ctypes.c_double, ctypes.POINTER(_I32)]),
+    "bmx_normal_mode_tunes": (_I32, [ctypes.POINTER(ctypes.c_double), _I32,
+                                     ctypes.POINTER(ctypes.c_double)]),
+    "bmx_chromaticity": (_I32, [_P, _PP, _I32, ctypes.c_double, ctypes.c_double,
+                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(_I32)]),
     "bmx_lattice_info": (_I32, [_P, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_uint64)]),
     "bmx_beam_create": (_I32, [_P, _I64, _PP]),
     "bmx_beam_wrap": (_I32, [_P, ctypes.POINTER(BeamView), _I64, _PP]),

@@ -273,6 +273,34 @@ def closed_orbit(ctx: Context, lats, n_dim: int = 4, guess=None, max_iter: int =
     return orb, mat, status
 
 
+def normal_mode_tunes(mat, n_dim: int = 6):
+    """Normal-mode tunes of a 6x6 one-turn matrix in radians, [0, 2 pi), ordered by plane
+    dominance (bmx_normal_mode_tunes; make_N of mode3_mod.f90). n_dim = 4: the two transverse
+    modes of its x-y block. Host only."""
+    m = np.ascontiguousarray(np.asarray(mat, dtype=np.float64).reshape(6, 6))
+    out = np.zeros(n_dim // 2)
+    dp = ctypes.POINTER(ctypes.c_double)
+    native.call("bmx_normal_mode_tunes", m.ctypes.data_as(dp), int(n_dim), out.ctypes.data_as(dp))
+    return out
+
+
+def chromaticity(ctx: Context, lats, delta_e: float = 1e-4, pz: float = 0.0):
+    """Chromaticities and fractional tunes of several lattices at once (bmx_chromaticity;
+    chrom_calc.f90). Returns (chrom (n, 2), tune (n, 2), status (n,)): a and b modes; status 0
+    ok, 1..3 a closed orbit failed (bmx_closed_orbit's codes), 4 unstable."""
+    lats = list(lats)
+    n = len(lats)
+    chrom = np.zeros((n, 2))
+    tune = np.zeros((n, 2))
+    status = np.zeros(n, dtype=np.int32)
+    handles = (ctypes.c_void_p * max(n, 1))(*[lt.h.value for lt in lats])
+    dp = ctypes.POINTER(ctypes.c_double)
+    native.call("bmx_chromaticity", ctx.h, handles, n, float(delta_e), float(pz),
+                chrom.ctypes.data_as(dp), tune.ctypes.data_as(dp),
+                status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return chrom, tune, status
+
+
 def track_beam_host(eles, beam: dict, n_turns: int, device: int = 0, turn0: int = 0,
                     mc2: float = E.M_ELECTRON, charge: int = -1, maps: str = "fast",
                     specialize: bool = False) -> dict:

@@ -46,6 +46,7 @@ extern "C" {
 #define BMX_ERR_BUDGET      6  /* request larger than the server's HBM budget */
 #define BMX_ERR_STATE       7  /* call not valid in the handle's current state */
 #define BMX_ERR_NOT_FOUND   8  /* unknown request id */
+#define BMX_ERR_UNSTABLE    9  /* one-turn matrix with an eigenvalue off the unit circle: no tune */
 
 /* ---- coord_struct%state codes (bmad_struct.f90:527-532) -------------------------------- */
 #define BMX_STATE_ALIVE      1
@@ -333,8 +334,8 @@ int32_t bmx_batch_tune(bmx_batch *batch, int32_t sample_turns, int32_t *chunk_tu
 
 /* ---- closed orbit and one-turn matrix (bmad/code/closed_orbit_calc.f90:192) ----------------- *
  * For each of n_lat lattices: the fixed point of the one-turn map, found by Newton steps whose
- * Jacobian is the one-turn matrix from central differences of tracked probes (13 particles per
- * lattice per iteration, all lattices in one batch). n_dim = 4 closes x, px, y, py at the pz of
+ * Jacobian is the one-turn matrix from five-point central differences of tracked probes (25
+ * particles per lattice per iteration, all lattices in one batch). n_dim = 4 closes x, px, y, py at the pz of
  * the guess with the rf cavities off (as closed_orbit_calc's i_dim = 4); n_dim = 6 closes all
  * six coordinates with the rf on. orbit[6 i .. 6 i + 5] holds lattice i's guess on entry and its
  * closed orbit on return; mat (optional, 36 per lattice, row-major) receives the one-turn matrix
@@ -344,6 +345,29 @@ int32_t bmx_batch_tune(bmx_batch *batch, int32_t sample_turns, int32_t *chunk_tu
 int32_t bmx_closed_orbit(bmx_ctx *ctx, bmx_lattice *const *lats, int32_t n_lat, int32_t n_dim,
                          double *orbit, double *mat, int32_t max_iter, double rel_tol,
                          double abs_tol, int32_t *status);
+
+/* Normal-mode tunes of a one-turn matrix, as make_N computes them (bmad/modules/mode3_mod.f90:
+ * eigen_decomp_6mat and order_evecs_by_plane_dominance): for each conjugate pair of eigenvalues
+ * the member whose eigenvector v has Im(v^T S conj(v)) >= 0 is conjugated to lambda, and the
+ * tune is the angle of conj(lambda) in [0, 2 pi); the modes are ordered by plane dominance (the
+ * largest |Re x| of the eigenvectors, normalised to unit length with their largest component
+ * real, first; then the largest |Re y|). mat: 6x6 row-major. n_dim = 6: the three modes of mat;
+ * n_dim = 4: the two transverse modes of its x-y block (the one-turn matrix with the rf off).
+ * tunes: n_dim / 2 values in radians. Host only (no device). BMX_ERR_UNSTABLE when an eigenvalue
+ * is real or off the unit circle by more than 1e-6. */
+int32_t bmx_normal_mode_tunes(const double *mat, int32_t n_dim, double *tunes);
+
+/* Chromaticities of n_lat lattices at once, as chrom_calc does for a closed ring
+ * (bmad/code/chrom_calc.f90:60-190): closed orbits with the rf off (bmx_closed_orbit, n_dim = 4)
+ * at pz - dE_low and pz + delta_e, the normal-mode tunes of their one-turn matrices
+ * (bmx_normal_mode_tunes, n_dim = 4), and chrom = (tune_high - tune_low) / (delta_e + dE_low) per
+ * mode, tunes in units of 2 pi; dE_low = delta_e unless pz - delta_e < -0.99 (then pz + 0.99).
+ * delta_e <= 0 means 1e-4. chrom: 2 per lattice (a, b modes); tune (optional, 2 per lattice):
+ * the fractional tunes at pz itself, in [0, 1). status[i]: 0 ok, 1..3 the failing closed
+ * orbit's bmx_closed_orbit status, 4 no tune (unstable). All 3 n_lat closed orbits are found
+ * in one batch. */
+int32_t bmx_chromaticity(bmx_ctx *ctx, bmx_lattice *const *lats, int32_t n_lat, double delta_e,
+                         double pz, double *chrom, double *tune, int32_t *status);
 
 /* ---- server: continuous batching of tracking requests ---------------------------------------- *
  * A server owns a worker thread, a HIP stream and a device pool of hbm_budget bytes for

@@ -10,6 +10,7 @@ Sources:
   regression_tests/aperture_test/output.correct         -- EC_*/RC_* state and unstable lines
   regression_tests/multipole_test/output.correct        -- the thin multipole MM's lines
   regression_tests/closed_orbit_test/ad05f_{pos,neg}_RF.lat -- lattices (lattices/closed_orbit)
+  regression_tests/mode3_test/t6.dat and output.correct   -- a one-turn matrix and make_N lines
 """
 import os
 import sys
@@ -44,6 +45,13 @@ def main():
         f.write("\n".join(lines) + "\n")
     lines = grab("regression_tests/multipole_test/output.correct", lambda ln: '-MM"' in ln)
     with open(os.path.join(HERE, "multipole_test_mm.correct"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    os.makedirs(os.path.join(HERE, "mode3"), exist_ok=True)
+    lines = grab("regression_tests/mode3_test/t6.dat", lambda ln: True)
+    with open(os.path.join(HERE, "mode3", "t6.dat"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    lines = grab("regression_tests/mode3_test/output.correct", lambda ln: ln.startswith('"make_N'))
+    with open(os.path.join(HERE, "mode3", "make_n.correct"), "w") as f:
         f.write("\n".join(lines) + "\n")
     os.makedirs(os.path.join(HERE, "lattices", "closed_orbit"), exist_ok=True)
     for name in ("ad05f_pos_RF.lat", "ad05f_neg_RF.lat"):
