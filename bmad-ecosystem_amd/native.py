@@ -83,6 +83,13 @@ _SIGS = {
     "bmx_closed_orbit": (_I32, [_P, _PP, _I32, _I32, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double), _I32, ctypes.c_double,
                                 ctypes.c_double, ctypes.POINTER(_I32)]),
+    "bmx_beam_h5_info": (_I32, [ctypes.c_char_p, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_int64),
+                                ctypes.c_char_p, _I32]),
+    "bmx_beam_h5_read": (_I32, [ctypes.c_char_p, _I32, _P, ctypes.c_int64, ctypes.c_double,
+                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double)]),
+    "bmx_beam_h5_write": (_I32, [ctypes.c_char_p, _P, ctypes.c_int64, ctypes.c_char_p, ctypes.c_double,
+                                 ctypes.c_double, _I32, _I32]),
     "bmx_normal_mode_tunes": (_I32, [ctypes.POINTER(ctypes.c_double), _I32,
                                      ctypes.POINTER(ctypes.c_double)]),
     "bmx_chromaticity": (_I32, [_P, _PP, _I32, ctypes.c_double, ctypes.c_double,

@@ -253,6 +253,29 @@ def write_beam_file(path: str, beam: dict, p0c: float, species: str = "Electron"
                 species.encode(), float(charge), float(p0c), int(ix_ele), int(bool(append)))
 
 
+def read_beam_h5(path: str, mc2: float = E.M_ELECTRON, ix_bunch: int = 1):
+    """openPMD HDF5 beam file (bmx_beam_h5_read) -> (host SoA dict, p0c, charge_tot, species)."""
+    nb, npart = ctypes.c_int32(), ctypes.c_int64()
+    sp = ctypes.create_string_buffer(64)
+    native.call("bmx_beam_h5_info", path.encode(), ctypes.byref(nb), ctypes.byref(npart), sp, 64)
+    cap = int(npart.value) if ix_bunch == 1 else 1 << 26
+    out = {k: np.zeros(cap, dtype=np.float64) for k in FLOAT_KEYS}
+    out.update({k: np.zeros(cap, dtype=np.int32) for k in INT_KEYS})
+    v = _host_view(out)
+    n, p0c, q = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+    native.call("bmx_beam_h5_read", path.encode(), int(ix_bunch), ctypes.byref(v), cap, float(mc2),
+                ctypes.byref(n), ctypes.byref(p0c), ctypes.byref(q))
+    return {k: a[:n.value].copy() for k, a in out.items()}, p0c.value, q.value, sp.value.decode()
+
+
+def write_beam_h5(path: str, beam: dict, p0c: float, species: str = "electron",
+                  charge: float = 0.0, ix_ele: int = 0, append: bool = False) -> None:
+    """Host SoA dict -> openPMD HDF5 beam file, one bunch (bmx_beam_h5_write)."""
+    v = _host_view(beam)
+    native.call("bmx_beam_h5_write", path.encode(), ctypes.byref(v), int(beam["x"].shape[0]),
+                species.encode(), float(charge), float(p0c), int(ix_ele), int(bool(append)))
+
+
 def closed_orbit(ctx: Context, lats, n_dim: int = 4, guess=None, max_iter: int = 20,
                  rel_tol: float = 1e-12, abs_tol: float = 1e-15):
     """Closed orbits and one-turn matrices of several lattices at once (bmx_closed_orbit;

@@ -260,6 +260,30 @@ int32_t bmx_beam_file_read(const char *path, int32_t ix_bunch, const bmx_beam_vi
  * tracking checkpoint. append != 0 adds the bunch to an existing file. */
 int32_t bmx_beam_file_write(const char *path, const bmx_beam_view *host, int64_t n, const char *species,
                             double charge, double p0c, int32_t ix_ele, int32_t append);
+/* openPMD HDF5 beam files of charged particles (bmad/hdf5/hdf5_read_beam.f90,
+ * hdf5_write_beam.f90), host memory, no device needed. libhdf5 (1.10) is loaded at the first
+ * call: BMX_HDF5_LIB if set, else libhdf5.so.103 / libhdf5.so, else /opt/conda/lib;
+ * BMX_ERR_UNSUPPORTED when none loads. bmx_beam_h5_info: bunch count, the first bunch's particle
+ * count and species name (speciesType; species may be null). bmx_beam_h5_read: bunch ix_bunch
+ * (1-based), the newer layout (a species group under particlesPath) or the older one
+ * (particlesPath holds the records); positions and positionOffset, momenta in eV/c and their
+ * offsets, time, totalMomentum and totalMomentumOffset (p0c, which must be one value), state,
+ * elementIndex (loss_ele of lost particles), as hdf5_read_beam converts them without a lattice
+ * element: px = Px / p0c, pz = totalMomentum / p0c, beta from (1 + pz) p0c and mc2,
+ * z = -beta c (t - t_ref); a file not written by Bmad (or this library) has its non-alive
+ * states read as lost. charge_tot = totalCharge * chargeUnitSI. Photon bunches are refused
+ * (BMX_ERR_UNSUPPORTED). bmx_beam_h5_write: n particles as one bunch /data/NNNNN/particles/
+ * <species>/ in hdf5_write_beam's layout (equal values as constant records), the absolute time
+ * (not carried here) as t_ref = -(t - t_ref); append != 0 adds the bunch to an existing file.
+ * Reading a written file back gives x and y exactly and the other coordinates to within a few
+ * ulp (px, py and pz pass through px p0c, py p0c and pz p0c, z through -z / (beta c)); the ASCII
+ * writer is the bit-identical checkpoint. */
+int32_t bmx_beam_h5_info(const char *path, int32_t *n_bunch, int64_t *n_particle, char *species,
+                         int32_t species_len);
+int32_t bmx_beam_h5_read(const char *path, int32_t ix_bunch, const bmx_beam_view *host, int64_t cap,
+                         double mc2, int64_t *n_out, double *p0c, double *charge_tot);
+int32_t bmx_beam_h5_write(const char *path, const bmx_beam_view *host, int64_t n, const char *species,
+                          double charge, double p0c, int32_t ix_ele, int32_t append);
 /* Count of particles with state == alive (lt_tracking_mod.f90:1672, bunch%n_live). */
 int32_t bmx_beam_count_alive(bmx_beam *beam, int64_t *n_alive);
 

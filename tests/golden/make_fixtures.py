@@ -11,6 +11,8 @@ Sources:
   regression_tests/multipole_test/output.correct        -- the thin multipole MM's lines
   regression_tests/closed_orbit_test/ad05f_{pos,neg}_RF.lat -- lattices (lattices/closed_orbit)
   regression_tests/mode3_test/t6.dat and output.correct   -- a one-turn matrix and make_N lines
+  regression_tests/hdf5_test/bunch_old.h5 and output.correct -- an openPMD beam file and the
+                                                               particle values the test prints
 """
 import os
 import sys
@@ -45,6 +47,13 @@ def main():
         f.write("\n".join(lines) + "\n")
     lines = grab("regression_tests/multipole_test/output.correct", lambda ln: '-MM"' in ln)
     with open(os.path.join(HERE, "multipole_test_mm.correct"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(os.path.join(REF, "regression_tests/hdf5_test/bunch_old.h5"), "rb") as f:
+        data = f.read()
+    with open(os.path.join(HERE, "bunch_old.h5"), "wb") as f:
+        f.write(data)
+    lines = grab("regression_tests/hdf5_test/output.correct", lambda ln: ln.startswith('"Bunch1-'))
+    with open(os.path.join(HERE, "hdf5_test_bunch.correct"), "w") as f:
         f.write("\n".join(lines) + "\n")
     os.makedirs(os.path.join(HERE, "mode3"), exist_ok=True)
     lines = grab("regression_tests/mode3_test/t6.dat", lambda ln: True)
