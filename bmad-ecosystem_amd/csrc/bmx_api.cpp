@@ -1,6 +1,7 @@
 // C-ABI of the tracking library (declared in include/bmx.h).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -34,7 +35,9 @@ static bool same_launch(const LatDesc &a, const LatDesc &b)
 {
     return a.hdr == b.hdr && a.prm == b.prm && a.n_ele == b.n_ele && a.sp.mc2 == b.sp.mc2 &&
            a.sp.p0c == b.sp.p0c && a.sp.mc2_sq == b.sp.mc2_sq && a.sp.beta0 == b.sp.beta0 &&
-           a.sp.me2 == b.sp.me2 && a.sp.lez_f == b.sp.lez_f && a.sp.lez_thr == b.sp.lez_thr;
+           a.sp.me2 == b.sp.me2 && a.sp.lez_f == b.sp.lez_f && a.sp.lez_thr == b.sp.lez_thr &&
+           a.sp.rad.scale == b.sp.rad.scale && a.sp.rad.flags == b.sp.rad.flags &&
+           a.sp.rad.key0 == b.sp.rad.key0 && a.sp.rad.key1 == b.sp.rad.key1;
 }
 
 static bool same_launch(const BeamPtrs &a, const BeamPtrs &b)
@@ -166,6 +169,34 @@ int32_t bmx_lattice_create(bmx_ctx *ctx, const bmx_ele *eles, int32_t n_ele, dou
     int rc = compile_table(eles, n_ele, mc2, charge, t, err);
     if (rc != BMX_OK) return set_error(rc, "bmx_lattice_create: " + err);
     return lattice_upload(ctx, std::move(t), out);
+}
+
+int32_t bmx_lattice_create_rad(bmx_ctx *ctx, const bmx_ele *eles, int32_t n_ele, double mc2,
+                               int32_t charge, const bmx_rad_map *maps,
+                               const bmx_rad_config *config, bmx_lattice **out)
+{
+    if (!ctx || !out || !maps || !config)
+        return set_error(BMX_ERR_ARG, "bmx_lattice_create_rad: null argument");
+    *out = nullptr;
+    HostTable t;
+    std::string err;
+    int rc = compile_table(eles, n_ele, mc2, charge, t, err, maps, config);
+    if (rc != BMX_OK) return set_error(rc, "bmx_lattice_create_rad: " + err);
+    return lattice_upload(ctx, std::move(t), out);
+}
+
+int32_t bmx_lattice_set_radiation(bmx_lattice *L, const bmx_rad_config *config)
+{
+    if (!L || !config || config->reserved != 0 || !std::isfinite(config->synch_rad_scale))
+        return set_error(BMX_ERR_ARG, "bmx_lattice_set_radiation: bad argument");
+    L->host.sp.rad = rad_settings(*config);
+    L->desc.sp.rad = L->host.sp.rad;
+    hipSetDevice(L->device);
+    // ordered after the context's queued work that reads the descriptor
+    HIP_TRY(hipMemcpyAsync(L->d_desc, &L->desc, sizeof(LatDesc), hipMemcpyHostToDevice, L->ctx->stream),
+            "bmx_lattice_set_radiation: copy");
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream), "bmx_lattice_set_radiation: sync");
+    return BMX_OK;
 }
 
 }  // extern "C"

@@ -515,8 +515,66 @@ void bend_frame_blocks(const bmx_ele &e, double *out)
 
 }  // namespace
 
+RadCfg rad_settings(const bmx_rad_config &c)
+{
+    RadCfg r{};
+    r.scale = c.synch_rad_scale;
+    r.flags = (c.damping_on ? RAD_DAMP : 0u) | (c.fluctuations_on ? RAD_FLUCT : 0u) |
+              (c.zero_average ? RAD_ZERO_AVERAGE : 0u);
+    r.key0 = (uint32_t)c.seed;
+    r.key1 = (uint32_t)(c.seed >> 32);
+    return r;
+}
+
+namespace {
+
+// Kinds that radiate in track1_radiation (radiation_mod.f90:86-93): all with a length but
+// drifts, pipes, markers and thin multipoles (the other kinds it leaves out are not on this
+// path).
+bool radiates(const bmx_ele &e)
+{
+    if (!e.is_on || e.l == 0) return false;
+    switch (e.key) {
+    case BMX_DRIFT: case BMX_PIPE: case BMX_MARKER: case BMX_MULTIPOLE: case BMX_AB_MULTIPOLE:
+        return false;
+    default: return true;
+    }
+}
+
+bool map_is_zero(const bmx_rad_map &m)
+{
+    for (int i = 0; i < 6; i++) {
+        if (m.xfer_damp_vec[i] != 0) return false;
+        for (int j = 0; j < 6; j++)
+            if (m.damp_dmat[i][j] != 0 || (j <= i && m.stoc_mat[i][j] != 0)) return false;
+    }
+    return true;
+}
+
+// One half's map in the radiation block layout (bmx_table.hpp).
+void rad_edge_block(const bmx_rad_map &m, double *out)
+{
+    for (int i = 0; i < 6; i++) out[i] = m.ref_orb[i];
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) out[6 + 6 * i + j] = m.damp_dmat[i][j];
+    for (int i = 0; i < 6; i++) out[42 + i] = m.xfer_damp_vec[i];
+    int k = 48;
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j <= i; j++) out[k++] = m.stoc_mat[i][j];
+}
+
+bool finite_map(const bmx_rad_map &m)
+{
+    const double *v = &m.ref_orb[0];
+    for (size_t i = 0; i < sizeof(bmx_rad_map) / sizeof(double); i++)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+}  // namespace
+
 int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge, HostTable &out,
-                  std::string &err)
+                  std::string &err, const bmx_rad_map *rad, const bmx_rad_config *rad_cfg)
 {
     // the tracked species is the reference species: magnetic charge ratios are 1; the charge
     // enters only the electric multipole strengths
@@ -540,6 +598,15 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
         out.sp.lez_f = out.sp.beta0 * out.sp.beta0 * (2 * (out.sp.beta0 * out.sp.beta0) - (me * me) / 2);
     }
     out.sp.lez_thr = 3e-7 * e_tot;
+    out.sp.rad = RadCfg{};
+    if (rad) {
+        if (!rad_cfg) { err = "radiation maps without settings"; return BMX_ERR_ARG; }
+        if (rad_cfg->reserved != 0 || !std::isfinite(rad_cfg->synch_rad_scale)) {
+            err = "bad radiation settings";
+            return BMX_ERR_ARG;
+        }
+        out.sp.rad = rad_settings(*rad_cfg);
+    }
 
     for (int i = 0; i < n_ele; i++) {
         const bmx_ele &e = eles[i];
@@ -561,6 +628,15 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
         // take the bend branch inside their own map (K_BMULT).
         // a thin multipole's frame change leaves out its tilt (offset_particle with set_tilt =
         // F, track1_bmad.f90:144): the tilt is in its kicks
+        // radiation kicks at the element's ends (track1_radiation), when its maps are not zero
+        bool rad_here = false;
+        if (rad && radiates(e)) {
+            if (!finite_map(rad[2 * i]) || !finite_map(rad[2 * i + 1])) {
+                err = where(i, e) + "radiation map with a value that is not finite";
+                return BMX_ERR_ARG;
+            }
+            rad_here = !map_is_zero(rad[2 * i]) || !map_is_zero(rad[2 * i + 1]);
+        }
         const bool thin = key == BMX_MULTIPOLE || key == BMX_AB_MULTIPOLE;
         bmx_ele e_frame = e;
         if (thin) e_frame.tilt = 0;
@@ -709,7 +785,8 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             const int ft = e.fringe_type == 0 ? BMX_FRINGE_NONE : e.fringe_type;
             const bool kick_type = ft == BMX_FRINGE_BASIC_BEND || ft == BMX_FRINGE_HARD_EDGE_ONLY;
             if (!kick_type && ft != BMX_FRINGE_NONE && ft != BMX_FRINGE_FULL) { err = where(i, e) + "bend fringe_type"; return BMX_ERR_UNSUPPORTED; }
-            const bool plain = ix == -1 && ixe == -1 && ft != BMX_FRINGE_FULL && !misaligned(e);
+            // (a bend that radiates takes the general bend, whose rare path carries the kicks)
+            const bool plain = ix == -1 && ixe == -1 && ft != BMX_FRINGE_FULL && !misaligned(e) && !rad_here;
             const bool at_ent = e.fringe_at == BMX_ENTRANCE_END || e.fringe_at == BMX_BOTH_ENDS;
             const bool at_exit = e.fringe_at == BMX_EXIT_END || e.fringe_at == BMX_BOTH_ENDS;
             double fg_ent = e.fint * e.hgap, fg_exit = e.fintx * e.hgapx;
@@ -864,6 +941,14 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
         if (h.flags & (F_APE_ENT | F_APE_EXIT)) out.prm.insert(out.prm.end(), ap, ap + 4);
         if (h.flags & F_APE_MOMENTUM) out.prm.insert(out.prm.end(), mom, mom + 8);
         out.prm.insert(out.prm.end(), q.begin(), q.end());
+        if (rad_here) {
+            h.flags |= F_RAD | F_RARE_MAP;
+            out.rare_q = true;  // the kernels that carry the kicks
+            double rb[kRadLen];
+            rad_edge_block(rad[2 * i], rb);
+            rad_edge_block(rad[2 * i + 1], rb + kRadEdge);
+            out.prm.insert(out.prm.end(), rb, rb + kRadLen);
+        }
         out.hdr.push_back(h);
     }
     // Entrance tests equal to the exit test just before (a particle does not move between

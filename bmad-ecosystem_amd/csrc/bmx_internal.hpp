@@ -54,8 +54,13 @@ struct HostTable {
     bool rare_q = false;     // some element compiled to K_QMULT
 };
 // Returns BMX_OK or an error code with a message in err.
+// rad: null, or rm0 and rm1 of every element (2 n_ele maps) for radiation kicks; rad_cfg the
+// lattice's radiation settings (with rad).
 int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge, HostTable &out,
-                  std::string &err);
+                  std::string &err, const bmx_rad_map *rad = nullptr,
+                  const bmx_rad_config *rad_cfg = nullptr);
+// The device form of radiation settings.
+RadCfg rad_settings(const bmx_rad_config &c);
 
 // ---- error reporting (bmx_api.cpp) ----
 int set_error(int code, const std::string &msg);

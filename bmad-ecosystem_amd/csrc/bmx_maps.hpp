@@ -63,6 +63,8 @@ struct Particle {
     double lez;     // low_energy_z_correction per metre (low_energy_z_correction.f90:44-57)
     // Where the particle was lost (set with the loss state): element index (1-based), turn.
     int32_t loss_ele, loss_turn;
+    // The particle's index in its beam: the counter of its radiation fluctuation stream.
+    uint32_t id;
 };
 
 __device__ __forceinline__ double sqr(double v) { return v * v; }
@@ -1750,6 +1752,112 @@ __device__ __forceinline__ void thin_multipole(Particle &p, CP m, int32_t n, int
     p.py = p.py + ky;
 }
 
+// ---- radiation ----------------------------------------------------------------------------
+
+// Parameter count of an element kind (kParams; K_THIN, which has no radiation block, at its
+// minimum).
+__device__ __forceinline__ constexpr int param_len(int kind)
+{
+    return kind == K_DRIFT ? 1 : kind == K_QUAD ? 5 : kind == K_MULT ? 9 : kind == K_BEND ? 29
+         : kind == K_RF ? 7 : kind == K_MARKER ? 0 : kind == K_QMULT ? 32 : kind == K_BMULT ? 94 : 2;
+}
+static_assert(param_len(K_BMULT) == kParams[K_BMULT] && param_len(K_QMULT) == kParams[K_QMULT] &&
+              param_len(K_BEND) == kParams[K_BEND] && param_len(K_RF) == kParams[K_RF] &&
+              param_len(K_MULT) == kParams[K_MULT] && param_len(K_QUAD) == kParams[K_QUAD] &&
+              param_len(K_DRIFT) == kParams[K_DRIFT] && param_len(K_MARKER) == kParams[K_MARKER],
+              "param_len disagrees with kParams");
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC11): 10 rounds of two 32x32 -> 64-bit products on a 128-bit counter under a 64-bit key.
+struct Ctr4 {
+    uint32_t c0, c1, c2, c3;
+};
+__device__ __forceinline__ Ctr4 philox4x32_10(Ctr4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.c2;
+        c = Ctr4{(uint32_t)(p1 >> 32) ^ c.c1 ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.c3 ^ k1,
+                 (uint32_t)p0};
+    }
+    return c;
+}
+
+// A uniform double in (0, 1] from 53 bits of two words (the sum rounds to nearest even).
+__device__ __forceinline__ double uniform53(uint32_t hi, uint32_t lo)
+{
+    const uint64_t m = ((uint64_t)hi << 21) | (lo >> 11);
+    return ((double)m + 0.5) * 0x1p-53;
+}
+
+// Six standard normal deviates for one particle at one edge of one element (ran_gauss's role in
+// track1_radiation, radiation_mod.f90:116): three Philox blocks with the counter (particle,
+// turn, element, 4 edge + block), each giving two deviates by the Box-Muller transform
+// (g = sqrt(-2 ln u1) (cos, sin)(2 pi u2)).
+__device__ __forceinline__ void gauss6(double *g, uint32_t id, int32_t turn, int32_t ix1, int32_t edge,
+                                    uint32_t k0, uint32_t k1)
+{
+    for (int b = 0; b < 3; b++) {
+        const Ctr4 r = philox4x32_10(Ctr4{id, (uint32_t)turn, (uint32_t)ix1, (uint32_t)(4 * edge + b)},
+                                     k0, k1);
+        const double u1 = uniform53(r.c0, r.c1), u2 = uniform53(r.c2, r.c3);
+        const double rad = sqrt(-2 * log(u1));
+        double sn, cs;
+        sincos(kTwoPi * u2, &sn, &cs);
+        g[2 * b] = rad * cs;
+        g[2 * b + 1] = rad * sn;
+    }
+}
+
+// track1_radiation (radiation_mod.f90:61-147) at one edge (0 entrance, 1 exit) with the
+// element's map m (radiation block layout in bmx_table.hpp): damping
+//   vec += rr damp_dmat (vec - ref_orb) [+ rr xfer_damp_vec unless zero_average],
+// then fluctuations vec += scale stoc_mat g, with rr = scale = synch_rad_scale (time_dir = 1);
+// lost_pz if pz < -1 afterwards. beta is left as it was, as the reference leaves it (the
+// caller refreshes the functions of the energy). The settings come as values: flags and
+// scale of RadCfg, key the fluctuation key.
+__device__ __noinline__ void rad_kick(Particle &p, CP m, uint32_t flags, double scale, uint32_t key0,
+                                      uint32_t key1, int32_t edge, int32_t ix1, int32_t turn)
+{
+    double v[6] = {p.x, p.px, p.y, p.py, p.z, p.pz};
+    if (flags & RAD_DAMP) {
+        const double rr = scale;
+        double d[6], w[6];
+        for (int j = 0; j < 6; j++) d[j] = v[j] - m[j];
+        for (int i = 0; i < 6; i++) {
+            double s = 0;
+            for (int j = 0; j < 6; j++) s = s + m[6 + 6 * i + j] * d[j];
+            w[i] = v[i] + rr * s;
+        }
+        for (int i = 0; i < 6; i++) v[i] = w[i];
+        if (!(flags & RAD_ZERO_AVERAGE))
+            for (int i = 0; i < 6; i++) v[i] = v[i] + rr * m[42 + i];
+    }
+    if (flags & RAD_FLUCT) {
+        double g[6];
+        gauss6(g, p.id, turn, ix1, edge, key0, key1);
+        CP L = m + 48;
+        int k = 0;
+        for (int i = 0; i < 6; i++) {
+            double s = 0;
+            for (int j = 0; j <= i; j++) s = s + L[k++] * g[j];
+            v[i] = v[i] + scale * s;
+        }
+    }
+    p.x = v[0];
+    p.px = v[1];
+    p.y = v[2];
+    p.py = v[3];
+    p.z = v[4];
+    p.pz = v[5];
+    if (p.pz < -1) p.state = ST_LOST_PZ;
+}
+
 template <bool F, int MAPS = MAPS_ALL>
 __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, const Species &sp,
                                               bool from_prev, int32_t ix1, int32_t turn)
@@ -1758,7 +1866,7 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
     // The common element path (MAPS_NO_RF) never meets offsets or momentum windows: the loops
     // send those elements to the rare path, so here their bits are known to be clear and every
     // test and table offset that depends on them folds away.
-    const int32_t flags = MAPS != MAPS_NO_RF ? h.flags : (h.flags & ~(F_OFFSET | F_APE_MOMENTUM));
+    const int32_t flags = MAPS != MAPS_NO_RF ? h.flags : (h.flags & ~(F_OFFSET | F_APE_MOMENTUM | F_RAD));
     CP ob = prm + off;
     CP ap = (flags & F_OFFSET) ? ob + kOffsetLen : ob;
     CP q = (flags & (F_APE_ENT | F_APE_EXIT | F_APE_ENT_SAME)) ? ap + 4 : ap;
@@ -1776,7 +1884,23 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
         }
     }
     constexpr bool kOffsets = MAPS != MAPS_NO_RF;
+    // the radiation block follows the parameters (track1.f90:199-202: the upstream half's kick
+    // after the entrance aperture test, in laboratory coordinates; a particle it loses is lost
+    // at the entrance). Only the map sets of the stepped elements carry the kicks (a lattice
+    // with radiation is compiled for them), so the kernels of the other sets keep their
+    // register budget.
+    constexpr bool kRad = MAPS == MAPS_ALL_Q || MAPS == MAPS_RARE_Q;
+    CP rb = q + param_len(kind);
     if constexpr (kOffsets) {
+        if (kRad && (flags & F_RAD)) {
+            rad_kick(p, rb, sp.rad.flags, sp.rad.scale, sp.rad.key0, sp.rad.key1, 0, ix1, turn);
+            refresh_energy(p, sp);
+            if (p.state != ST_ALIVE) {
+                p.loss_ele = uniform_here(ix1);
+                p.loss_turn = uniform_here(turn);
+                return true;
+            }
+        }
         if (flags & F_OFFSET) offset_set(p, ob, flags);
     }
     bool unset = true;
@@ -1812,6 +1936,30 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
         // the quadrupole when an electric edge loses it
         if (flags & F_OFFSET) {
             if ((kind != K_RF || p.state == ST_ALIVE) && unset) offset_unset(p, ob, flags);
+        }
+    }
+    if constexpr (kRad) {
+        if (flags & F_RAD) {
+            // track1.f90:291-326: orbit_too_large, the state test, the downstream half's kick,
+            // then check_aperture_limit, whose own orbit_too_large (check_aperture_limit.f90:61)
+            // now tests the kicked coordinates and overrides the kick's lost_pz
+            bool done = p.state != ST_ALIVE;
+            if (orbit_bad(p)) {
+                orbit_too_large(p);
+                done = true;
+            }
+            if (!done) {
+                rad_kick(p, rb + kRadEdge, sp.rad.flags, sp.rad.scale, sp.rad.key0, sp.rad.key1, 1,
+                         ix1, turn);
+                refresh_energy(p, sp);
+                if (orbit_bad(p)) orbit_too_large(p);
+                else if ((flags & F_APE_EXIT) && aperture_fails(p, aflags, ap)) aperture(p, aflags, ap);
+            }
+            if (p.state != ST_ALIVE) {
+                p.loss_ele = uniform_here(ix1);
+                p.loss_turn = uniform_here(turn);
+            }
+            return false;
         }
     }
     // orbit_too_large (overwrites any state), else the exit aperture for live particles; both

@@ -238,27 +238,34 @@ int32_t bmx_closed_orbit(bmx_ctx *ctx, bmx_lattice *const *lats, int32_t n_lat, 
     for (int32_t i = 0; i < n_lat; i++)
         if (!lats[i] || lats[i]->device != ctx->device)
             return set_error(BMX_ERR_ARG, "bmx_closed_orbit: lattice missing or on another device");
-    // the tracked lattices: the rf cavities switched off for n_dim = 4
+    // the tracked lattices: the rf cavities switched off for n_dim = 4; radiation
+    // fluctuations off, and damping too for n_dim = 4 (closed_orbit_calc.f90:264, 322)
     Lattices held;
     std::vector<bmx_lattice *> use(lats, lats + n_lat);
-    if (n_dim == 4) {
-        for (int32_t i = 0; i < n_lat; i++) {
-            HostTable t = lats[i]->host;
-            bool has_rf = false;
+    for (int32_t i = 0; i < n_lat; i++) {
+        HostTable t = lats[i]->host;
+        bool changed = false;
+        const uint32_t rad_keep = n_dim == 4 ? ~(RAD_FLUCT | RAD_DAMP) : ~RAD_FLUCT;
+        if (t.sp.rad.flags & ~rad_keep) {
+            t.sp.rad.flags &= rad_keep;
+            changed = true;
+        }
+        if (n_dim == 4) {
             for (const EleHdr &h : t.hdr)
                 if (h.kind == K_RF) {
                     const int32_t q = h.off + ((h.flags & F_OFFSET) ? kOffsetLen : 0) +
-                                      ((h.flags & (F_APE_ENT | F_APE_EXIT | F_APE_ENT_SAME)) ? 4 : 0);
+                                      ((h.flags & (F_APE_ENT | F_APE_EXIT | F_APE_ENT_SAME)) ? 4 : 0) +
+                                      ((h.flags & F_APE_MOMENTUM) ? 8 : 0);
                     t.prm[q + 1] = 0;  // voltage
-                    has_rf = true;
+                    changed = true;
                 }
-            if (!has_rf) continue;
-            bmx_lattice *L = nullptr;
-            const int32_t rc = lattice_upload(ctx, std::move(t), &L);
-            if (rc != BMX_OK) return rc;
-            held.owned.push_back(L);
-            use[i] = L;
         }
+        if (!changed) continue;
+        bmx_lattice *L = nullptr;
+        const int32_t rc = lattice_upload(ctx, std::move(t), &L);
+        if (rc != BMX_OK) return rc;
+        held.owned.push_back(L);
+        use[i] = L;
     }
     const int64_t n = (int64_t)n_lat * kProbe;
     bmx_beam *beam = nullptr;

@@ -87,6 +87,9 @@ enum Flags : int32_t {
     // K_THIN: the dipole order of a multipole$ in the reference-orbit form (multipole_kick's
     // ref_orb_offset, multipole_mod.f90:250-254)
     F_THIN_REF0 = F_QHARD,
+    // radiation damping and/or fluctuation kicks at both ends of the element (track1_radiation,
+    // radiation_mod.f90:61-147): the element's radiation block follows its parameters
+    F_RAD = 1 << 30,
 };
 constexpr int kOffsetLen = 24;
 
@@ -140,6 +143,22 @@ struct EleHdr {
 //  rectangular: x1_lim (= -x1_limit), x2_lim, y1_lim, y2_lim (0 = no limit)
 // Parameter count per kind (K_THIN: 2 + 2 hdr.n, here its minimum)
 constexpr int kParams[K_NKIND] = {1, 5, 9, 29, 7, 0, 32, 94, 2};
+// With F_RAD, the element's parameters are followed by its radiation block: the maps of the
+// upstream and the downstream half (rad_map_ele_struct rm0, rm1, bmad_struct.f90:988-1000), each
+// kRadEdge values: [0..5] ref_orb, [6..41] damp_dmat (row-major), [42..47] xfer_damp_vec,
+// [48..68] stoc_mat's lower triangle row by row (row i holds columns 0..i).
+constexpr int kRadEdge = 69;
+constexpr int kRadLen = 2 * kRadEdge;
+
+// Radiation settings of a lattice (bmad_com%radiation_damping_on, %radiation_fluctuations_on,
+// %radiation_zero_average, %synch_rad_scale), read by the kicks of F_RAD elements.
+enum : uint32_t { RAD_DAMP = 1, RAD_FLUCT = 2, RAD_ZERO_AVERAGE = 4 };
+struct RadCfg {
+    double scale;       // synch_rad_scale
+    uint32_t flags;     // RAD_*
+    uint32_t key0, key1;  // the fluctuation stream's key (the seed)
+    uint32_t pad;
+};
 
 struct Species {
     double mc2;      // [eV]
@@ -150,6 +169,7 @@ struct Species {
     double me2;      // (mc2 / E_tot)^2
     double lez_f;    // beta0^2 (2 beta0^2 - (mc2/E_tot)^2 / 2)
     double lez_thr;  // 3e-7 E_tot
+    RadCfg rad;      // radiation settings of the lattice (only F_RAD elements read them)
 };
 
 // Per-lattice device descriptor.

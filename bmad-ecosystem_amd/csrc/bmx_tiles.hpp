@@ -20,6 +20,7 @@ __device__ __forceinline__ void load_particle(Particle &p, const BeamPtrs &b, in
     p.pz = b.pz[i];
     p.beta = b.beta[i];
     p.state = ST_ALIVE;
+    p.id = (uint32_t)i;
 }
 
 __device__ __forceinline__ void store_particle(const Particle &p, const BeamPtrs &b, int64_t i)
@@ -54,7 +55,8 @@ __device__ __forceinline__ uint32_t load_packed(Particle &p, const double *__res
     p.pz = d[5 * (size_t)stride + j];
     p.beta = d[6 * (size_t)stride + j];
     p.state = ST_ALIVE;
-    return reinterpret_cast<const uint32_t *>(d + kPackedDoubles * (size_t)stride)[j];
+    p.id = reinterpret_cast<const uint32_t *>(d + kPackedDoubles * (size_t)stride)[j];
+    return p.id;
 }
 
 __device__ __forceinline__ void store_packed(const Particle &p, double *__restrict__ d,

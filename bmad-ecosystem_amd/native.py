@@ -8,6 +8,8 @@ from __future__ import annotations
 import ctypes
 import os
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 # BMX_LIBRARY names another build of the library to load (same C ABI), else the in-tree one.
 LIB_PATH = os.environ.get("BMX_LIBRARY") or os.path.join(HERE, "libbmx.so")
@@ -54,6 +56,18 @@ class RequestStats(ctypes.Structure):
                 ("done_ns", ctypes.c_int64), ("status", ctypes.c_int32), ("chunks", ctypes.c_int32)]
 
 
+class RadConfig(ctypes.Structure):
+    """ctypes mirror of bmx_rad_config."""
+    _fields_ = [("damping_on", ctypes.c_int32), ("fluctuations_on", ctypes.c_int32),
+                ("zero_average", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("synch_rad_scale", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+# numpy mirror of bmx_rad_map (rad_map_struct without xfer_damp_mat, row-major)
+RAD_MAP_DTYPE = np.dtype([("ref_orb", "<f8", (6,)), ("damp_dmat", "<f8", (6, 6)),
+                          ("xfer_damp_vec", "<f8", (6,)), ("stoc_mat", "<f8", (6, 6))])
+
+
 # name: (restype, argtypes)
 _SIGS = {
     "bmx_version": (_I32, []),
@@ -72,6 +86,8 @@ _SIGS = {
     "bmx_ctx_elapsed_ms": (_I32, [_P, _I32, _I32, ctypes.POINTER(ctypes.c_float)]),
     "bmx_lattice_create": (_I32, [_P, _P, _I32, ctypes.c_double, _I32, _PP]),
     "bmx_lattice_destroy": (_I32, [_P]),
+    "bmx_lattice_create_rad": (_I32, [_P, _P, _I32, ctypes.c_double, _I32, _P, _P, _PP]),
+    "bmx_lattice_set_radiation": (_I32, [_P, _P]),
     "bmx_lattice_validate": (_I32, [_P, _I32, ctypes.c_double, _I32, ctypes.POINTER(_I32)]),
     "bmx_lattice_specialize": (_I32, [_P]),
     "bmx_beam_file_info": (_I32, [ctypes.c_char_p, ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),

@@ -100,14 +100,41 @@ class Context:
             pass
 
 
+def rad_config(damping: bool = True, fluctuations: bool = False, zero_average: bool = False,
+               scale: float = 1.0, seed: int = 0) -> native.RadConfig:
+    """bmx_rad_config: bmad_com%radiation_damping_on, %radiation_fluctuations_on,
+    %radiation_zero_average, %synch_rad_scale and the fluctuation stream's seed."""
+    return native.RadConfig(int(bool(damping)), int(bool(fluctuations)), int(bool(zero_average)), 0,
+                            float(scale), int(seed) & (2**64 - 1))
+
+
+def rad_maps(n_ele: int) -> np.ndarray:
+    """Zeroed radiation maps for n_ele elements, shape (n_ele, 2): [i, 0] = rm0 (upstream half),
+    [i, 1] = rm1 (downstream half) of element i, fields as rad_map_struct."""
+    return np.zeros((n_ele, 2), dtype=native.RAD_MAP_DTYPE)
+
+
 class Lattice:
-    def __init__(self, ctx: Context, eles, mc2: float = E.M_ELECTRON, charge: int = -1):
+    """A lattice on the device (bmx_lattice_create). With radiation maps (rad_maps(), filled
+    from each element's rad_map%rm0 / %rm1) and settings (rad_config()), the elements that radiate
+    take track1_radiation's kicks at both ends (bmx_lattice_create_rad)."""
+
+    def __init__(self, ctx: Context, eles, mc2: float = E.M_ELECTRON, charge: int = -1,
+                 rad_maps: np.ndarray | None = None, rad: native.RadConfig | None = None):
         arr = eles if isinstance(eles, np.ndarray) else E.to_array(eles)
         self.arr = np.ascontiguousarray(arr)
         self.ctx = ctx
         h = ctypes.c_void_p()
-        native.call("bmx_lattice_create", ctx.h, self.arr.ctypes.data, len(self.arr), mc2, charge,
-                    ctypes.byref(h))
+        if rad_maps is None:
+            native.call("bmx_lattice_create", ctx.h, self.arr.ctypes.data, len(self.arr), mc2,
+                        charge, ctypes.byref(h))
+        else:
+            maps = np.ascontiguousarray(rad_maps, dtype=native.RAD_MAP_DTYPE)
+            if maps.shape != (len(self.arr), 2):
+                raise ValueError(f"rad_maps must have shape ({len(self.arr)}, 2), not {maps.shape}")
+            cfg = rad if rad is not None else rad_config()
+            native.call("bmx_lattice_create_rad", ctx.h, self.arr.ctypes.data, len(self.arr), mc2,
+                        charge, maps.ctypes.data, ctypes.byref(cfg), ctypes.byref(h))
         self.h = h
         n = ctypes.c_int32()
         mask = ctypes.c_uint64()
@@ -115,6 +142,10 @@ class Lattice:
         self.n_ele = n.value
         self.kind_mask = mask.value
         self.specialized = False
+
+    def set_radiation(self, rad: native.RadConfig) -> None:
+        """New radiation settings for later tracking calls (bmx_lattice_set_radiation)."""
+        native.call("bmx_lattice_set_radiation", self.h, ctypes.byref(rad))
 
     def specialize(self) -> bool:
         """Compile tile kernels specialised to this lattice's repeated cell (bmx_lattice_specialize).
@@ -326,12 +357,14 @@ def chromaticity(ctx: Context, lats, delta_e: float = 1e-4, pz: float = 0.0):
 
 def track_beam_host(eles, beam: dict, n_turns: int, device: int = 0, turn0: int = 0,
                     mc2: float = E.M_ELECTRON, charge: int = -1, maps: str = "fast",
-                    specialize: bool = False) -> dict:
+                    specialize: bool = False, rad_maps: np.ndarray | None = None,
+                    rad: native.RadConfig | None = None) -> dict:
     """Convenience: host SoA dict in, host SoA dict out (tracks a copy). specialize: use the
-    lattice's cell-specialised kernels when it has a repeated cell."""
+    lattice's cell-specialised kernels when it has a repeated cell; rad_maps, rad: radiation
+    kicks (Lattice)."""
     ctx = Context(device)
     ctx.set_maps(maps)
-    lat = Lattice(ctx, eles, mc2, charge)
+    lat = Lattice(ctx, eles, mc2, charge, rad_maps=rad_maps, rad=rad)
     if specialize:
         lat.specialize()
     b = Beam.from_host(ctx, beam)

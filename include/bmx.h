@@ -232,6 +232,54 @@ int32_t bmx_lattice_specialize(bmx_lattice *lat);
 /* Number of elements and bit set of element kinds present (bit = key). */
 int32_t bmx_lattice_info(const bmx_lattice *lat, int32_t *n_ele, uint64_t *kind_mask);
 
+/* ---- radiation: damping and fluctuation kicks at the element ends -------------------------- *
+ * bmx_rad_map -- one half of an element's radiation map, rad_map_struct (bmad_struct.f90:988-994)
+ * as radiation_map_setup leaves it in ele%rad_map%rm0 (upstream half) and %rm1 (downstream
+ * half), row-major here: damp_dmat[i][j] = damp_dmat(i+1, j+1). stoc_mat is the lower
+ * triangular kick matrix (tracking_rad_map_setup.f90:71-82); its upper triangle is not read.
+ * xfer_damp_mat is not used by the kick and has no field. */
+typedef struct bmx_rad_map {
+    double ref_orb[6];
+    double damp_dmat[6][6];
+    double xfer_damp_vec[6];
+    double stoc_mat[6][6];
+} bmx_rad_map;
+
+/* bmad_com%radiation_damping_on, %radiation_fluctuations_on, %radiation_zero_average and
+ * %synch_rad_scale (bmad_struct.f90), and the seed of the fluctuation stream. The fluctuation
+ * kicks of a particle are a function of (seed, the particle's index, turn, element, edge), the
+ * index counted from the first particle of what one call tracks (the beam; a bmx_batch group's
+ * range; a bmx_server request): a call repeats the kicks of an earlier call over the same turns,
+ * so a run split over several calls continues the turn count (bmx_track_turns_from) or changes
+ * the seed. The deviates are Philox4x32-10 blocks under the seed with the counter (index, turn,
+ * element 1-based, 4 edge + block), turned into normal deviates by Box-Muller. The reference
+ * draws them from one sequential generator (ran_gauss), so its kicks are not reproduced number
+ * for number; their distribution is the same. */
+typedef struct bmx_rad_config {
+    int32_t damping_on;
+    int32_t fluctuations_on;
+    int32_t zero_average;
+    int32_t reserved;        /* 0 */
+    double synch_rad_scale;  /* 1 in the reference's default */
+    uint64_t seed;
+} bmx_rad_config;
+
+/* bmx_lattice_create with radiation: maps[2 i] and maps[2 i + 1] are rm0 and rm1 of element i.
+ * track1_radiation (radiation_mod.f90:61-147) is applied at both ends of every element that
+ * radiates in the reference -- is_on, l /= 0 and not a drift, pipe, marker, multipole or
+ * ab_multipole (radiation_mod.f90:88-93) -- and whose maps are not all zero, in track1's order
+ * (track1.f90:197-202, 291-326): the upstream half's kick after the entrance aperture test, the
+ * downstream half's after the body's orbit test and before the exit aperture test. A particle
+ * whose pz falls below -1 is lost (lost_pz). Elements that radiate take the generic kernel's
+ * rare path (a cell-specialised kernel writes them out like any other element). */
+int32_t bmx_lattice_create_rad(bmx_ctx *ctx, const bmx_ele *eles, int32_t n_ele, double mc2,
+                               int32_t charge, const bmx_rad_map *maps,
+                               const bmx_rad_config *config, bmx_lattice **out);
+/* Change the radiation settings (switches, scale, seed) of a lattice without recompiling it:
+ * tracking calls made after it use them (a bmx_batch or bmx_server request keeps the settings
+ * its lattice had when it was created or submitted). */
+int32_t bmx_lattice_set_radiation(bmx_lattice *lat, const bmx_rad_config *config);
+
 /* ---- beam: particle state resident in HBM --------------------------------------------------- */
 /* Library-owned device storage for n particles. */
 int32_t bmx_beam_create(bmx_ctx *ctx, int64_t n, bmx_beam **out);

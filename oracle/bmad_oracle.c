@@ -2082,13 +2082,112 @@ static int track1_bmad(coord_t *o, const bmx_ele *e)
     }
 }
 
-/* bmad/code/track1.f90:44-343, bmad_standard, no radiation/spin/space charge/hooks. */
-static int track1(coord_t *o, const bmx_ele *e)
+/* ------------------------------------------------------------------------------------------
+ * radiation
+ * ------------------------------------------------------------------------------------------ */
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+ * SC11): ten rounds of (hi, lo) = M0 c0, M1 c2; c = (hi1 ^ c1 ^ k0, lo1, hi0 ^ c3 ^ k1, lo0),
+ * the key bumped by the Weyl constants between rounds. */
+void bmo_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out)
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* The fluctuation deviates of one particle at one edge of one element (ran_gauss (ran6) in
+ * radiation_mod.f90:116): three Philox blocks under the key (seed) with the counter (particle
+ * index, turn, element index 1-based, 4 edge + block); from each block two uniforms in (0, 1]
+ * of 53 bits, ((w0 << 21 | w1 >> 11) + 1/2) 2^-53 and likewise from w2, w3, and the two
+ * Box-Muller deviates sqrt(-2 ln u1) (cos, sin)(2 pi u2). The reference's generator is a
+ * sequential one; this stream has the same distribution, not the same numbers. */
+void bmo_rad_gauss6(uint64_t seed, uint32_t id, int32_t turn, int32_t ix1, int32_t edge, double *g)
+{
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int b = 0; b < 3; b++) {
+        const uint32_t ctr[4] = {id, (uint32_t)turn, (uint32_t)ix1, (uint32_t)(4 * edge + b)};
+        uint32_t w[4];
+        bmo_philox4x32_10(ctr, key, w);
+        const uint64_t m1 = ((uint64_t)w[0] << 21) | (w[1] >> 11);
+        const uint64_t m2 = ((uint64_t)w[2] << 21) | (w[3] >> 11);
+        const double u1 = (m1 + 0.5) * 0x1p-53;  /* in (0, 1]: never 0, so ln u1 is finite */
+        const double u2 = (m2 + 0.5) * 0x1p-53;
+        const double r = sqrt(-2 * log(u1));
+        g[2 * b] = r * cos(TWOPI * u2);
+        g[2 * b + 1] = r * sin(TWOPI * u2);
+    }
+}
+
+/* Where a radiation kick is made: the element, the particle's index and the turn. */
+typedef struct {
+    const bmx_rad_map *maps;       /* rm0, rm1 of the element */
+    const bmx_rad_config *cfg;
+    uint32_t id;
+    int32_t turn, ix1;
+} rad_at_t;
+
+/* bmad/modules/radiation_mod.f90:61-147 track1_radiation for a charged particle, time_dir = 1,
+ * the element's maps as given (no radiation_map_setup): the damping kick about ref_orb, the
+ * constant part unless zero_average (:109-112), the fluctuation kick stoc_mat g with the lower
+ * triangle of stoc_mat (:115-118), lost_pz below pz = -1 (:122-125). vec only: beta is not
+ * updated (no statement of the routine touches it). */
+static void track1_radiation(coord_t *o, const bmx_ele *e, const rad_at_t *ra, int edge)
+{
+    const bmx_rad_config *c = ra->cfg;
+    if (!c->damping_on && !c->fluctuations_on) return;
+    if (e->l == 0) return;
+    switch (e->key) {
+    case BMX_DRIFT: case BMX_PIPE: case BMX_MARKER: case BMX_MULTIPOLE: case BMX_AB_MULTIPOLE:
+        return;
+    default: break;
+    }
+    const bmx_rad_map *m = &ra->maps[edge];
+    double *v = o->vec;
+    if (c->damping_on) {
+        const double rr = c->synch_rad_scale;
+        double d[6], w[6];
+        for (int j = 0; j < 6; j++) d[j] = v[j] - m->ref_orb[j];
+        for (int i = 0; i < 6; i++) {
+            double s = 0;
+            for (int j = 0; j < 6; j++) s = s + m->damp_dmat[i][j] * d[j];
+            w[i] = v[i] + rr * s;
+        }
+        for (int i = 0; i < 6; i++) v[i] = w[i];
+        if (!c->zero_average)
+            for (int i = 0; i < 6; i++) v[i] = v[i] + rr * m->xfer_damp_vec[i];
+    }
+    if (c->fluctuations_on) {
+        double g[6];
+        bmo_rad_gauss6(c->seed, ra->id, ra->turn, ra->ix1, edge, g);
+        for (int i = 0; i < 6; i++) {
+            double s = 0;
+            for (int j = 0; j <= i; j++) s = s + m->stoc_mat[i][j] * g[j];
+            v[i] = v[i] + c->synch_rad_scale * s;
+        }
+    }
+    if (v[5] < -1.0) o->state = BMX_STATE_LOST_PZ;
+}
+
+/* bmad/code/track1.f90:44-343, bmad_standard, no spin/space charge/hooks; radiation kicks
+ * (:197-202, :319-324) when ra is given and the element is on. */
+static int track1_rad(coord_t *o, const bmx_ele *e, const rad_at_t *ra)
 {
     if (o->state != BMX_STATE_ALIVE) return 0;
     if (e->p0c != 0 && fabs(e->p0c - o->p0c) > 1e-12 * fabs(e->p0c)) return BMX_ERR_P0C;
+    const int rad = ra && e->is_on;
     double u;
     check_aperture_limit(o, e, FIRST_TRACK_EDGE, &u);
+    if (rad) track1_radiation(o, e, ra, 0);
     if (o->state != BMX_STATE_ALIVE) return 0;
     int err = track1_bmad(o, e);
     if (err) return err;
@@ -2096,9 +2195,16 @@ static int track1(coord_t *o, const bmx_ele *e)
     if (no_momentum_change ? BMO_REPEATED_TEST(orbit_too_large(o, NULL)) : orbit_too_large(o, NULL))
         return 0;
     if (o->state != BMX_STATE_ALIVE) return 0;
+    if (rad) {
+        track1_radiation(o, e, ra, 1);
+        /* the kick moved the particle: the orbit test inside check_aperture_limit is a new one */
+        if (orbit_too_large(o, &u)) return 0;
+    }
     check_aperture_limit(o, e, SECOND_TRACK_EDGE, &u);
     return 0;
 }
+
+static int track1(coord_t *o, const bmx_ele *e) { return track1_rad(o, e, NULL); }
 
 int32_t bmo_track1(const bmx_ele *ele, double mc2, int32_t charge, double p0c, double *vec,
                    double *beta, int32_t *state)
@@ -2119,11 +2225,11 @@ int32_t bmo_track1(const bmx_ele *ele, double mc2, int32_t charge, double p0c, d
 
 /* Bunch loop: ltt_run_beam_mode (lt_tracking_mod.f90:1631) -> track_bunch (beam_mod.f90:159-166)
  * -> track1_bunch_hom (beam_utils.f90:81-86). */
-int32_t bmo_track_bunch(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge,
-                        double p0c, int64_t n, double *x, double *px, double *y, double *py,
-                        double *z, double *pz, double *beta, int32_t *state, int32_t *loss_ele,
-                        int32_t *loss_turn, int32_t n_turns, int32_t turn0, int32_t n_threads,
-                        int64_t *n_tracked)
+int32_t bmo_track_bunch_rad(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge,
+                            double p0c, int64_t n, double *x, double *px, double *y, double *py,
+                            double *z, double *pz, double *beta, int32_t *state, int32_t *loss_ele,
+                            int32_t *loss_turn, int32_t n_turns, int32_t turn0, int32_t n_threads,
+                            int64_t *n_tracked, const bmx_rad_map *maps, const bmx_rad_config *cfg)
 {
     int32_t err_all = 0;
     int64_t tracked = 0;
@@ -2148,7 +2254,8 @@ int32_t bmo_track_bunch(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t 
                 o.mc2 = mc2;
                 o.charge = charge;
                 o.state = BMX_STATE_ALIVE;
-                int err = track1(&o, e);
+                rad_at_t ra = {maps ? maps + 2 * ie : NULL, cfg, (uint32_t)ip, turn0 + it, ie + 1};
+                int err = track1_rad(&o, e, maps ? &ra : NULL);
                 err_e |= err;
                 cnt++;
                 x[ip] = o.vec[0]; px[ip] = o.vec[1]; y[ip] = o.vec[2];
@@ -2167,6 +2274,16 @@ int32_t bmo_track_bunch(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t 
     }
     if (n_tracked) *n_tracked = tracked;
     return err_all;
+}
+
+int32_t bmo_track_bunch(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge,
+                        double p0c, int64_t n, double *x, double *px, double *y, double *py,
+                        double *z, double *pz, double *beta, int32_t *state, int32_t *loss_ele,
+                        int32_t *loss_turn, int32_t n_turns, int32_t turn0, int32_t n_threads,
+                        int64_t *n_tracked)
+{
+    return bmo_track_bunch_rad(eles, n_ele, mc2, charge, p0c, n, x, px, y, py, z, pz, beta, state,
+                               loss_ele, loss_turn, n_turns, turn0, n_threads, n_tracked, NULL, NULL);
 }
 
 int32_t bmo_track_trace(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge,

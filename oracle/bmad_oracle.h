@@ -55,6 +55,19 @@ int32_t bmo_track_bunch(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t 
                         int32_t *loss_turn, int32_t n_turns, int32_t turn0, int32_t n_threads,
                         int64_t *n_tracked);
 
+/* The same with radiation kicks (track1_radiation, radiation_mod.f90:61-147) at both ends of
+ * the elements that radiate, with maps[2 i], maps[2 i + 1] = rm0, rm1 of element i and the
+ * settings cfg; the fluctuation stream of particle ip is keyed by (cfg->seed, ip, turn,
+ * element, edge). maps = NULL is bmo_track_bunch. */
+int32_t bmo_track_bunch_rad(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge,
+                            double p0c, int64_t n, double *x, double *px, double *y, double *py,
+                            double *z, double *pz, double *beta, int32_t *state, int32_t *loss_ele,
+                            int32_t *loss_turn, int32_t n_turns, int32_t turn0, int32_t n_threads,
+                            int64_t *n_tracked, const bmx_rad_map *maps, const bmx_rad_config *cfg);
+/* Philox4x32-10 of one counter block under a key, and the six deviates of one kick. */
+void bmo_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+void bmo_rad_gauss6(uint64_t seed, uint32_t id, int32_t turn, int32_t ix1, int32_t edge, double *g);
+
 /* Single-particle trace: coordinates after each element for n_turns (out has
  * 6 * n_ele * n_turns doubles; entries after loss are the frozen lost coordinates). */
 int32_t bmo_track_trace(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge,

@@ -93,6 +93,7 @@ inline cdouble asin(const cdouble &a) { return fn(::asin(a.v), a); }
 inline cdouble sinh(const cdouble &a) { return fn(::sinh(a.v), a); }
 inline cdouble cosh(const cdouble &a) { return fn(::cosh(a.v), a); }
 inline cdouble exp(const cdouble &a) { return fn(::exp(a.v), a); }
+inline cdouble log(const cdouble &a) { return fn(::log(a.v), a); }
 inline cdouble atan(const cdouble &a) { return fn(::atan(a.v), a); }
 inline cdouble tan(const cdouble &a) { return fn(::tan(a.v), a); }
 inline cdouble copysign(double m, const cdouble &s) { return cdouble(::copysign(m, s.v), false); }
@@ -122,6 +123,7 @@ using cnt::cos;
 using cnt::asin;
 using cnt::sinh;
 using cnt::cosh;
+using cnt::log;
 using cnt::fabs;
 using cnt::atan2;
 using cnt::fmod;

@@ -52,6 +52,14 @@ def lib(fma: bool = False, path: str | None = None):
                                       + [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                          ctypes.POINTER(ctypes.c_int64)])
         L.bmo_track_bunch.restype = ctypes.c_int32
+        L.bmo_track_bunch_rad.argtypes = L.bmo_track_bunch.argtypes + [ctypes.c_void_p,
+                                                                       ctypes.c_void_p]
+        L.bmo_track_bunch_rad.restype = ctypes.c_int32
+        L.bmo_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint32)] * 3
+        L.bmo_philox4x32_10.restype = None
+        L.bmo_rad_gauss6.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32,
+                                     ctypes.c_int32, ctypes.c_int32, _D]
+        L.bmo_rad_gauss6.restype = None
         L.bmo_track_trace.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
                                       ctypes.c_int32, ctypes.c_double, _D, _D, _I, ctypes.c_int32,
                                       _D]
@@ -100,11 +108,42 @@ def track1(ele, vec, beta, state=1, mc2=0.51099895069e6, charge=-1, p0c=None):
     return v, b.value, s.value
 
 
+class RadConfig(ctypes.Structure):
+    """bmx_rad_config (include/bmx.h)."""
+    _fields_ = [("damping_on", ctypes.c_int32), ("fluctuations_on", ctypes.c_int32),
+                ("zero_average", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("synch_rad_scale", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+# bmx_rad_map (include/bmx.h), row-major matrices
+RAD_MAP_DTYPE = np.dtype([("ref_orb", "<f8", (6,)), ("damp_dmat", "<f8", (6, 6)),
+                          ("xfer_damp_vec", "<f8", (6,)), ("stoc_mat", "<f8", (6, 6))])
+
+
+def philox4x32_10(ctr, key):
+    """One Philox4x32-10 block: four 32-bit words of the counter, two of the key."""
+    c = (ctypes.c_uint32 * 4)(*[int(v) & 0xFFFFFFFF for v in ctr])
+    k = (ctypes.c_uint32 * 2)(*[int(v) & 0xFFFFFFFF for v in key])
+    o = (ctypes.c_uint32 * 4)()
+    lib().bmo_philox4x32_10(c, k, o)
+    return [o[i] for i in range(4)]
+
+
+def rad_gauss6(seed: int, pid: int, turn: int, ix1: int, edge: int) -> np.ndarray:
+    """The six fluctuation deviates of particle pid at one edge of element ix1 (1-based)."""
+    g = np.zeros(6)
+    lib().bmo_rad_gauss6(int(seed), int(pid), int(turn), int(ix1), int(edge), _ptr(g, _D))
+    return g
+
+
 def track_bunch(ele_arr: np.ndarray, beam: dict, n_turns: int, turn0: int = 0, p0c=None,
                 mc2=0.51099895069e6, charge=-1, n_threads: int = 0, fma: bool = False,
-                lib_path: str | None = None) -> int:
+                lib_path: str | None = None, rad_maps: np.ndarray | None = None,
+                rad: RadConfig | None = None) -> int:
     """Track a SoA beam dict in place; returns the number of particle-element passes.
-    fma=True runs the FMA-contracted build of the same source; lib_path another build."""
+    fma=True runs the FMA-contracted build of the same source; lib_path another build.
+    rad_maps (shape (n_ele, 2) of RAD_MAP_DTYPE: rm0, rm1 per element) and rad: radiation kicks
+    (track1_radiation)."""
     n = beam["x"].shape[0]
     if p0c is None:
         p0c = float(ele_arr[0]["p0c"])
@@ -113,11 +152,21 @@ def track_bunch(ele_arr: np.ndarray, beam: dict, n_turns: int, turn0: int = 0, p
     for k in ("state", "loss_ele", "loss_turn"):
         assert beam[k].dtype == np.int32 and beam[k].flags.c_contiguous
     tracked = ctypes.c_int64(0)
-    err = lib(fma, lib_path).bmo_track_bunch(
+    maps = None
+    if rad_maps is not None:
+        maps = np.ascontiguousarray(rad_maps, dtype=RAD_MAP_DTYPE)
+        assert maps.shape == (len(ele_arr), 2)
+        if rad is None:
+            raise ValueError("rad_maps needs rad settings")
+        rad = RadConfig(rad.damping_on, rad.fluctuations_on, rad.zero_average, 0,
+                        rad.synch_rad_scale, rad.seed)
+    err = lib(fma, lib_path).bmo_track_bunch_rad(
         ele_arr.ctypes.data, len(ele_arr), mc2, charge, p0c, n,
         *[_ptr(beam[k], _D) for k in ("x", "px", "y", "py", "z", "pz", "beta")],
         *[_ptr(beam[k], _I) for k in ("state", "loss_ele", "loss_turn")],
-        n_turns, turn0, n_threads, ctypes.byref(tracked))
+        n_turns, turn0, n_threads, ctypes.byref(tracked),
+        maps.ctypes.data if maps is not None else None,
+        ctypes.byref(rad) if maps is not None else None)
     if err:
         raise OracleError(f"bmo_track_bunch error {err}")
     return tracked.value

@@ -13,6 +13,8 @@ Sources:
   regression_tests/mode3_test/t6.dat and output.correct   -- a one-turn matrix and make_N lines
   regression_tests/hdf5_test/bunch_old.h5 and output.correct -- an openPMD beam file and the
                                                                particle values the test prints
+  regression_tests/radiation_test/output.correct        -- the radiation maps (rm0, rm1) of
+                                                           a bend and a quadrupole
 """
 import os
 import sys
@@ -61,6 +63,10 @@ def main():
         f.write("\n".join(lines) + "\n")
     lines = grab("regression_tests/mode3_test/output.correct", lambda ln: ln.startswith('"make_N'))
     with open(os.path.join(HERE, "mode3", "make_n.correct"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    lines = grab("regression_tests/radiation_test/output.correct",
+                 lambda ln: ln.startswith('"B1-rm') or ln.startswith('"K1-rm'))
+    with open(os.path.join(HERE, "radiation_test_maps.correct"), "w") as f:
         f.write("\n".join(lines) + "\n")
     os.makedirs(os.path.join(HERE, "lattices", "closed_orbit"), exist_ok=True)
     for name in ("ad05f_pos_RF.lat", "ad05f_neg_RF.lat"):

@@ -48,20 +48,20 @@ def rel_dev(got, ref, mask):
     return dev
 
 
-def fma_noise(oracle, arr, beam, n_turns, turn0=0):
+def fma_noise(oracle, arr, beam, n_turns, turn0=0, **kw):
     """Per-particle deviation between the two FP64 builds of the CPU restatement (without and
     with FMA contraction) after n_turns: how much rounding alone moves each orbit. Returns
     (plain-build result, deviation; inf where the builds disagree on survival)."""
     ref = copy_beam(beam)
-    oracle.track_bunch(arr, ref, n_turns, turn0=turn0)
+    oracle.track_bunch(arr, ref, n_turns, turn0=turn0, **kw)
     fma = copy_beam(beam)
-    oracle.track_bunch(arr, fma, n_turns, turn0=turn0, fma=True)
+    oracle.track_bunch(arr, fma, n_turns, turn0=turn0, fma=True, **kw)
     both = (ref["state"] == 1) & (fma["state"] == 1)
     return ref, rel_dev(fma, ref, both)
 
 
 def envelope(oracle, arr, beam, n_turns, turn0=0, delta=1e-14, samples=4, seed=0,
-             with_fma_spread=False):
+             with_fma_spread=False, **kw):
     """Per-particle sensitivity of the restatement's own result: the largest deviation (rel_dev
     measure) from the plain build's result among (a) the FMA-contracted build, which differs
     from it by rounding in every contracted operation along the whole orbit, and (b) `samples`
@@ -70,15 +70,16 @@ def envelope(oracle, arr, beam, n_turns, turn0=0, delta=1e-14, samples=4, seed=0
     rounding noise accumulates; (b) on chaotic ones, where it is amplified exponentially; a
     single ulp-sized rescaling can be absorbed by an addition, hence delta = 1e-14 (about 45
     ulp). Returns (plain-build result, envelope; inf where a run disagrees on survival), and the
-    FMA build's deviation alone as a third item if with_fma_spread."""
-    ref, noise = fma_noise(oracle, arr, beam, n_turns, turn0)
+    FMA build's deviation alone as a third item if with_fma_spread. kw: further arguments of
+    the restatement's track_bunch (radiation maps and settings)."""
+    ref, noise = fma_noise(oracle, arr, beam, n_turns, turn0, **kw)
     env = noise
     rng = np.random.default_rng(seed)
     for _ in range(samples):
         b = copy_beam(beam)
         for k in COORDS:
             b[k] = b[k] * (1 + delta * rng.choice([-1.0, 1.0], size=b[k].shape))
-        oracle.track_bunch(arr, b, n_turns, turn0=turn0)
+        oracle.track_bunch(arr, b, n_turns, turn0=turn0, **kw)
         both = (ref["state"] == 1) & (b["state"] == 1)
         env = np.maximum(env, rel_dev(b, ref, both))
     return (ref, env, noise) if with_fma_spread else (ref, env)
