@@ -86,6 +86,26 @@ type, bind(c) :: bmx_beam_view
   type(c_ptr) :: state = c_null_ptr, loss_ele = c_null_ptr, loss_turn = c_null_ptr
 end type
 
+! bmx_rad_map (include/bmx.h): one half of an element's rad_map_struct (bmad_struct.f90:988-994)
+! without xfer_damp_mat. The C matrices are row-major: fill damp_dmat and stoc_mat with the
+! transposes of the rad_map_struct components.
+type, bind(c), public :: bmx_rad_map_c
+  real(c_double) :: ref_orb(6) = 0
+  real(c_double) :: damp_dmat(6,6) = 0
+  real(c_double) :: xfer_damp_vec(6) = 0
+  real(c_double) :: stoc_mat(6,6) = 0
+end type
+
+! bmx_rad_config: the bmad_com radiation switches, synch_rad_scale and the fluctuation seed.
+type, bind(c), public :: bmx_rad_config_c
+  integer(c_int32_t) :: damping_on = 0
+  integer(c_int32_t) :: fluctuations_on = 0
+  integer(c_int32_t) :: zero_average = 0
+  integer(c_int32_t) :: reserved = 0
+  real(c_double) :: synch_rad_scale = 1
+  integer(c_int64_t) :: seed = 0
+end type
+
 ! The coord_struct components on the path (bmad_struct.f90:551-582).
 type, public :: bmx_coord_struct
   real(bmx_rp) :: vec(6) = 0
@@ -121,6 +141,27 @@ interface
   function c_ctx_synchronize(ctx) bind(c, name = 'bmx_ctx_synchronize') result(st)
     import :: c_int32_t, c_ptr
     type(c_ptr), value :: ctx
+    integer(c_int32_t) :: st
+  end function
+
+  ! bmx_lattice_create with radiation kicks: maps(1, i), maps(2, i) = rm0, rm1 of element i.
+  function bmx_lattice_create_rad(ctx, eles, n_ele, mc2, charge, maps, config, lat) &
+      bind(c, name = 'bmx_lattice_create_rad') result(st)
+    import :: c_int32_t, c_ptr, c_double, bmx_rad_config_c
+    type(c_ptr), value :: ctx, eles
+    integer(c_int32_t), value :: n_ele
+    real(c_double), value :: mc2
+    integer(c_int32_t), value :: charge
+    type(c_ptr), value :: maps
+    type(bmx_rad_config_c), intent(in) :: config
+    type(c_ptr) :: lat
+    integer(c_int32_t) :: st
+  end function
+
+  function bmx_lattice_set_radiation(lat, config) bind(c, name = 'bmx_lattice_set_radiation') result(st)
+    import :: c_int32_t, c_ptr, bmx_rad_config_c
+    type(c_ptr), value :: lat
+    type(bmx_rad_config_c), intent(in) :: config
     integer(c_int32_t) :: st
   end function
 
@@ -253,6 +294,7 @@ end type
 public :: bmx_lat_init, bmx_lat_free, bmx_track_bunch, bmx_track_all, bmx_last_error_message
 public :: bmx_closed_orbit_calc, bmx_write_beam_file
 public :: bmx_track1_bunch_hook, bmx_track_bunch_span, bmx_bunch_free
+public :: bmx_lattice_create_rad, bmx_lattice_set_radiation
 
 contains
 
