@@ -221,7 +221,7 @@ int32_t lattice_upload(bmx_ctx *ctx, HostTable &&table, bmx_lattice **out)
     L->desc.hdr = L->d_hdr;
     L->desc.prm = L->d_prm;
     L->desc.n_ele = (int32_t)nh;
-    L->desc.rare_q = L->host.rare_q ? 1 : 0;
+    L->desc.rare_q = L->host.rare_q ? (L->host.rad ? 2 : 1) : 0;
     L->desc.sp = L->host.sp;
     if (e == hipSuccess) e = hipMemcpy(L->d_desc, &L->desc, sizeof(LatDesc), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -424,7 +424,7 @@ int32_t track_group(bmx_ctx *ctx, const bmx_lattice *L, bmx_beam *B, int32_t fir
     // quadrupoles or general bends that no cell kernel carries on the generic kernels built
     // with them
     cfg.span = first != 0 || count != L->desc.n_ele;
-    const bool ext = L->host.rare_q && (cfg.span || !jit_rare_q(cfg.jit));
+    const int ext = L->desc.rare_q && (cfg.span || jit_level(cfg.jit) < L->desc.rare_q) ? L->desc.rare_q : 0;
     // Run phase (waves chosen automatically, full turns on the cell kernels): while the live
     // particles fill more than kPhaseRounds rounds of the 4-wave kernel's resident waves, the
     // 5-wave build (more resident waves, a few spilled registers) is faster; below that, with

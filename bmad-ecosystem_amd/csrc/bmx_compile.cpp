@@ -582,6 +582,7 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
     out.prm.clear();
     out.kind_mask = 0;
     out.rare_q = false;
+    out.rad = false;
     if (n_ele <= 0 || !eles) { err = "empty lattice"; return BMX_ERR_ARG; }
     if (!(mc2 > 0)) { err = "mass must be positive"; return BMX_ERR_ARG; }
     const double p0c = eles[0].p0c;
@@ -944,6 +945,7 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
         if (rad_here) {
             h.flags |= F_RAD | F_RARE_MAP;
             out.rare_q = true;  // the kernels that carry the kicks
+            out.rad = true;
             double rb[kRadLen];
             rad_edge_block(rad[2 * i], rb);
             rad_edge_block(rad[2 * i + 1], rb + kRadEdge);

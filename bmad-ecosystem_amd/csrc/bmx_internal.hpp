@@ -16,7 +16,7 @@ hipError_t launch_track_tiles(hipStream_t s, int grid, int waves, const Group *d
                               const double *d_pk_in, double *d_pk_out, uint32_t stride,
                               uint32_t *d_cnt_out, uint32_t *d_tile_ctr, int32_t chunk_off,
                               int32_t chunk_turns, unsigned long long *d_work, bool fast, bool span,
-                              bool ext = false);
+                              int ext = 0);
 hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_groups,
                               const uint32_t *d_cnt_in, int32_t next_off, WaveTile *d_tiles,
                               uint32_t *d_n_tiles, uint32_t *d_cnt_zero, uint32_t *d_tile_ctr,
@@ -24,8 +24,10 @@ hipError_t launch_build_tiles(hipStream_t s, const Group *d_groups, int32_t n_gr
 hipError_t launch_pack_alive(hipStream_t s, int grid, const Group *d_groups,
                              const WaveTile *d_tiles, const uint32_t *d_n_tiles, double *d_pk_out,
                              uint32_t stride, uint32_t *d_cnt_out);
+// ext: 0 the plain map sets, 1 with the stepped quadrupole and general bend (LatDesc::rare_q),
+// 2 with radiation as well.
 hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu, bool span = false,
-                                 bool ext = false);
+                                 int ext = 0);
 // Waves per SIMD of the kernel for runs with element spans (Group first/count other than the
 // full turn from element 0).
 constexpr int kSpanWaves = 3;
@@ -51,7 +53,8 @@ struct HostTable {
     std::vector<double> prm;
     Species sp{};
     uint64_t kind_mask = 0;  // bit = bmx_ele key present
-    bool rare_q = false;     // some element compiled to K_QMULT
+    bool rare_q = false;     // some element compiled to K_QMULT or K_BMULT
+    bool rad = false;        // some element carries radiation kicks (rare_q is then set too)
 };
 // Returns BMX_OK or an error code with a message in err.
 // rad: null, or rm0 and rm1 of every element (2 n_ele maps) for radiation kicks; rad_cfg the
@@ -101,6 +104,8 @@ int jit_specialize(int device, const HostTable &t, const JitKernels **out, std::
 hipError_t jit_occupancy(const JitKernels *j, int waves, int *blocks_per_cu);
 // The cell kernels carry the stepped quadrupole (their lattice holds one).
 bool jit_rare_q(const JitKernels *j);
+// The map level the cell kernels carry: 0 plain, 1 stepped elements, 2 with radiation.
+int jit_level(const JitKernels *j);
 hipError_t launch_track_tiles_jit(const JitKernels *j, hipStream_t s, int grid, int waves,
                                   const Group *d_groups, const WaveTile *d_tiles,
                                   const uint32_t *d_n_tiles, const double *d_pk_in,
@@ -141,7 +146,7 @@ inline int32_t default_waves(bool fast, const JitKernels *jit = nullptr)
 // Workgroups of the tracking kernel that fit on the device at once (of the cell-specialised
 // kernels when jit is set).
 int default_grid(int device, bool fast, int waves, const JitKernels *jit = nullptr,
-                 bool span = false, bool ext = false);
+                 bool span = false, int ext = 0);
 
 }  // namespace bmx
 

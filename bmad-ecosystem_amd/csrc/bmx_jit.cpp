@@ -32,6 +32,7 @@ struct JitKernels {
     hipModule_t module = nullptr;
     hipFunction_t fn[3] = {};  // waves per SIMD 3, 4, 5
     bool rare_q = false;       // carries K_QMULT / K_BMULT on its rare path (all built for 2 waves)
+    int level = 0;             // 0 plain, 1 stepped elements, 2 with radiation kicks
 };
 
 namespace {
@@ -91,6 +92,7 @@ std::string generate(const HostTable &t, const Period &pd, std::string &sig)
              << ", " << h.n << ", " << rel << "}, base, L.sp, true, e1 + " << j << ", t_abs);\n";
     }
     if (t.rare_q) key << "/q";
+    if (t.rad) key << "/r";
     sig = key.str();
     std::ostringstream s;
     s << "#include <stdint.h>\n"
@@ -99,7 +101,7 @@ std::string generate(const HostTable &t, const Period &pd, std::string &sig)
       << bmx_jit_src_tiles << '\n'
       << "namespace bmx {\n"
       << "struct CellTurn {\n"
-      << "    static constexpr int kRare = " << (t.rare_q ? "MAPS_RARE_Q" : "MAPS_RARE") << ";\n"
+      << "    static constexpr int kRare = " << (t.rad ? "MAPS_RARE_QR" : t.rare_q ? "MAPS_RARE_Q" : "MAPS_RARE") << ";\n"
       << "    __device__ __forceinline__ static void turn(Particle &p, const LatDesc &L, int32_t t_abs)\n"
       << "    {\n"
       << "        const CP prm = (CP)L.prm;\n";
@@ -171,6 +173,7 @@ int jit_specialize(int device, const HostTable &t, const JitKernels **out, std::
     hiprtcDestroyProgram(&prog);
     auto k = std::make_unique<JitKernels>();
     k->rare_q = t.rare_q;
+    k->level = t.rare_q ? (t.rad ? 2 : 1) : 0;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipModuleLoadData(&k->module, code.data());
     for (int w = 3; w <= 5 && e == hipSuccess; w++) {
@@ -187,6 +190,7 @@ int jit_specialize(int device, const HostTable &t, const JitKernels **out, std::
 }
 
 bool jit_rare_q(const JitKernels *j) { return j && j->rare_q; }
+int jit_level(const JitKernels *j) { return j ? j->level : 0; }
 
 hipError_t jit_occupancy(const JitKernels *j, int waves, int *blocks_per_cu)
 {

@@ -19,7 +19,8 @@ namespace bmx {
 // 5: up to 102, spilling a few values of the rf and bend paths to scratch). SPAN: groups may
 // track element spans (runs with a start element or a partial turn); the full-turn kernels
 // carry no code for them. RARE: the map set of the rare path; MAPS_RARE_Q (with W = 2) for
-// lattices that hold a stepped quadrupole or a general bend.
+// lattices that hold a stepped quadrupole or a general bend, MAPS_RARE_QR for those with
+// radiation.
 template <bool F, int W, bool SPAN = false, int RARE = MAPS_RARE>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) k_track_tiles(
     const Group *__restrict__ groups, const WaveTile *__restrict__ tiles,
@@ -130,7 +131,7 @@ __global__ void __launch_bounds__(kBlock) k_pack_alive(
 
 // One element, whole bunch: track1 (track1.f90:44-343) per live particle. Q: the kernel carries
 // the stepped quadrupole and the general bend (built for lattices that hold one).
-template <bool F, bool Q>
+template <bool F, int Q>
 __device__ __forceinline__ void track_element_pass(const LatDesc &L, int32_t ie, int32_t turn,
                                                    const BeamPtrs &b, int64_t n,
                                                    unsigned long long *work)
@@ -142,7 +143,7 @@ __device__ __forceinline__ void track_element_pass(const LatDesc &L, int32_t ie,
         load_particle(p, b, i);
         refresh_energy(p, L.sp);
         if (!orbit_too_large(p))
-            track_element<F, Q ? MAPS_ALL_Q : MAPS_ALL>(p, load_ele((CHdr)L.hdr + 4 * ie), (CP)L.prm, L.sp,
+            track_element<F, Q == 2 ? MAPS_ALL_QR : Q ? MAPS_ALL_Q : MAPS_ALL>(p, load_ele((CHdr)L.hdr + 4 * ie), (CP)L.prm, L.sp,
                                                       false, ie + 1, turn);
         store_particle(p, b, i);
         if (p.state != ST_ALIVE) {
@@ -166,7 +167,7 @@ __device__ __forceinline__ void track_element_pass(const LatDesc &L, int32_t ie,
     }
 }
 
-template <bool F, bool Q>
+template <bool F, int Q>
 __global__ void __launch_bounds__(kBlock) k_track_element(LatDesc L, int32_t ie, int32_t turn,
                                                           BeamPtrs b, int64_t n,
                                                           unsigned long long *work)
@@ -176,7 +177,7 @@ __global__ void __launch_bounds__(kBlock) k_track_element(LatDesc L, int32_t ie,
 
 // The same with the turn index read from device memory: the launches of one turn captured once
 // in a graph serve every turn, the turn advanced by k_step_turn at the end of each replay.
-template <bool F, bool Q>
+template <bool F, int Q>
 __global__ void __launch_bounds__(kBlock) k_track_element_at(LatDesc L, int32_t ie,
                                                              const int32_t *__restrict__ turn_p,
                                                              BeamPtrs b, int64_t n,
@@ -209,7 +210,7 @@ __device__ __forceinline__ void record_orbit(double *r, const Particle &p, bool 
     r[7] = (double)p.state;
 }
 
-template <bool F, bool Q>
+template <bool F, int Q>
 __global__ void __launch_bounds__(kBlock) k_track_range(LatDesc L, int32_t e0, int32_t e1,
                                                         int32_t turn, BeamPtrs b, int64_t n,
                                                         double *__restrict__ rec,
@@ -230,7 +231,7 @@ __global__ void __launch_bounds__(kBlock) k_track_range(LatDesc L, int32_t e0, i
         } else {
             for (int32_t e = e0; e <= e1; ++e) {
                 const bool at_entrance =
-                    track_element<F, Q ? MAPS_ALL_Q : MAPS_ALL>(p, load_ele((CHdr)L.hdr + 4 * e), (CP)L.prm, L.sp, e > e0, e + 1,
+                    track_element<F, Q == 2 ? MAPS_ALL_QR : Q ? MAPS_ALL_Q : MAPS_ALL>(p, load_ele((CHdr)L.hdr + 4 * e), (CP)L.prm, L.sp, e > e0, e + 1,
                                      turn);
                 passes++;
                 record_orbit(rec + (i * ne + (e - e0)) * 8, p, at_entrance);
@@ -280,13 +281,21 @@ hipError_t launch_track_tiles(hipStream_t s, int grid, int waves, const Group *d
                               const double *d_pk_in, double *d_pk_out, uint32_t stride,
                               uint32_t *d_cnt_out, uint32_t *d_tile_ctr, int32_t chunk_off,
                               int32_t chunk_turns, unsigned long long *d_work, bool fast, bool span,
-                              bool ext)
+                              int ext)
 {
     if (grid <= 0) return hipSuccess;
 #define BMX_TILES(F, W, ...)                                                                      \
     launch_tiles<F, W, ##__VA_ARGS__>(s, grid, d_groups, d_tiles, d_n_tiles, d_pk_in, d_pk_out,   \
                                       stride, d_cnt_out, d_tile_ctr, chunk_off, chunk_turns, d_work)
-    if (ext) {
+    if (ext == 2) {
+        if (span) {
+            if (fast) BMX_TILES(true, kExtWaves, true, MAPS_RARE_QR);
+            else BMX_TILES(false, kExtWaves, true, MAPS_RARE_QR);
+        } else {
+            if (fast) BMX_TILES(true, kExtWaves, false, MAPS_RARE_QR);
+            else BMX_TILES(false, kExtWaves, false, MAPS_RARE_QR);
+        }
+    } else if (ext) {
         if (span) {
             if (fast) BMX_TILES(true, kExtWaves, true, MAPS_RARE_Q);
             else BMX_TILES(false, kExtWaves, true, MAPS_RARE_Q);
@@ -330,10 +339,16 @@ hipError_t launch_pack_alive(hipStream_t s, int grid, const Group *d_groups,
     return hipGetLastError();
 }
 
-hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu, bool span, bool ext)
+hipError_t track_tiles_occupancy(bool fast, int waves, int *blocks_per_cu, bool span, int ext)
 {
     const void *k;
-    if (ext && span)
+    if (ext == 2 && span)
+        k = fast ? (const void *)k_track_tiles<true, kExtWaves, true, MAPS_RARE_QR>
+                 : (const void *)k_track_tiles<false, kExtWaves, true, MAPS_RARE_QR>;
+    else if (ext == 2)
+        k = fast ? (const void *)k_track_tiles<true, kExtWaves, false, MAPS_RARE_QR>
+                 : (const void *)k_track_tiles<false, kExtWaves, false, MAPS_RARE_QR>;
+    else if (ext && span)
         k = fast ? (const void *)k_track_tiles<true, kExtWaves, true, MAPS_RARE_Q>
                  : (const void *)k_track_tiles<false, kExtWaves, true, MAPS_RARE_Q>;
     else if (ext)
@@ -355,12 +370,15 @@ hipError_t launch_track_element(hipStream_t s, const LatDesc &lat, int32_t ie, i
     if (grid <= 0) return hipSuccess;
 #define BMX_ELEMENT(F, Q) \
     hipLaunchKernelGGL((k_track_element<F, Q>), dim3((unsigned)grid), dim3(kBlock), 0, s, lat, ie, turn, b, n, d_work)
-    if (lat.rare_q) {
-        if (fast) BMX_ELEMENT(true, true);
-        else BMX_ELEMENT(false, true);
+    if (lat.rare_q == 2) {
+        if (fast) BMX_ELEMENT(true, 2);
+        else BMX_ELEMENT(false, 2);
+    } else if (lat.rare_q) {
+        if (fast) BMX_ELEMENT(true, 1);
+        else BMX_ELEMENT(false, 1);
     } else {
-        if (fast) BMX_ELEMENT(true, false);
-        else BMX_ELEMENT(false, false);
+        if (fast) BMX_ELEMENT(true, 0);
+        else BMX_ELEMENT(false, 0);
     }
 #undef BMX_ELEMENT
     return hipGetLastError();
@@ -374,12 +392,15 @@ hipError_t launch_track_element_at(hipStream_t s, const LatDesc &lat, int32_t ie
     if (grid <= 0) return hipSuccess;
 #define BMX_ELEMENT(F, Q) \
     hipLaunchKernelGGL((k_track_element_at<F, Q>), dim3((unsigned)grid), dim3(kBlock), 0, s, lat, ie, d_turn, b, n, d_work)
-    if (lat.rare_q) {
-        if (fast) BMX_ELEMENT(true, true);
-        else BMX_ELEMENT(false, true);
+    if (lat.rare_q == 2) {
+        if (fast) BMX_ELEMENT(true, 2);
+        else BMX_ELEMENT(false, 2);
+    } else if (lat.rare_q) {
+        if (fast) BMX_ELEMENT(true, 1);
+        else BMX_ELEMENT(false, 1);
     } else {
-        if (fast) BMX_ELEMENT(true, false);
-        else BMX_ELEMENT(false, false);
+        if (fast) BMX_ELEMENT(true, 0);
+        else BMX_ELEMENT(false, 0);
     }
 #undef BMX_ELEMENT
     return hipGetLastError();
@@ -399,12 +420,15 @@ hipError_t launch_track_range(hipStream_t s, const LatDesc &lat, int32_t e0, int
     if (grid <= 0) return hipSuccess;
 #define BMX_RANGE(F, Q) \
     hipLaunchKernelGGL((k_track_range<F, Q>), dim3((unsigned)grid), dim3(kBlock), 0, s, lat, e0, e1, turn, b, n, d_rec, d_work)
-    if (lat.rare_q) {
-        if (fast) BMX_RANGE(true, true);
-        else BMX_RANGE(false, true);
+    if (lat.rare_q == 2) {
+        if (fast) BMX_RANGE(true, 2);
+        else BMX_RANGE(false, 2);
+    } else if (lat.rare_q) {
+        if (fast) BMX_RANGE(true, 1);
+        else BMX_RANGE(false, 1);
     } else {
-        if (fast) BMX_RANGE(true, false);
-        else BMX_RANGE(false, false);
+        if (fast) BMX_RANGE(true, 0);
+        else BMX_RANGE(false, 0);
     }
 #undef BMX_RANGE
     return hipGetLastError();

@@ -1330,12 +1330,21 @@ __device__ __forceinline__ bool quad_multipole(Particle &p, CP q, int32_t n_step
 // momentum windows; MAPS_RARE has only the kinds that can carry them (no bends) with all three;
 // MAPS_ALL_Q and MAPS_RARE_Q add the stepped quadrupole (K_QMULT) and the general bend
 // (K_BMULT) to MAPS_ALL and MAPS_RARE, for kernels built for lattices that hold one (their
-// code would otherwise shape the register assignment of every kernel).
+// code would otherwise shape the register assignment of every kernel); MAPS_ALL_QR and
+// MAPS_RARE_QR add the radiation kicks to those, for lattices with radiation (the kick's call
+// raises the register count of every kernel it is compiled into).
 // from_prev: the particle has just left the element before this one in the same lattice, so
 // an entrance test that repeats that element's exit test (F_APE_ENT_SAME, F_APE_ENT cleared)
 // is skipped; otherwise it is made.
 // A particle lost here gets p.loss_ele = ix1 (the element's 1-based index), p.loss_turn = turn.
-enum MapSet : int { MAPS_ALL = 0, MAPS_NO_RF = 1, MAPS_RARE = 2, MAPS_RARE_Q = 3, MAPS_ALL_Q = 4 };
+enum MapSet : int {
+    MAPS_ALL = 0, MAPS_NO_RF = 1, MAPS_RARE = 2, MAPS_RARE_Q = 3, MAPS_ALL_Q = 4, MAPS_RARE_QR = 5,
+    MAPS_ALL_QR = 6
+};
+// The map set carries the stepped quadrupole and the general bend.
+constexpr bool maps_q(int m) { return m == MAPS_RARE_Q || m == MAPS_ALL_Q || m == MAPS_RARE_QR || m == MAPS_ALL_QR; }
+// The map set carries no plain bend (the rare paths).
+constexpr bool maps_rare(int m) { return m == MAPS_RARE || m == MAPS_RARE_Q || m == MAPS_RARE_QR; }
 
 // ---- offset_particle, non-bend branch (bmad/code/offset_particle.f90:183-492) -------------
 // Element orientation +1, particle direction +1; ob is the offset block (bmx_table.hpp).
@@ -1886,10 +1895,9 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
     constexpr bool kOffsets = MAPS != MAPS_NO_RF;
     // the radiation block follows the parameters (track1.f90:199-202: the upstream half's kick
     // after the entrance aperture test, in laboratory coordinates; a particle it loses is lost
-    // at the entrance). Only the map sets of the stepped elements carry the kicks (a lattice
-    // with radiation is compiled for them), so the kernels of the other sets keep their
-    // register budget.
-    constexpr bool kRad = MAPS == MAPS_ALL_Q || MAPS == MAPS_RARE_Q;
+    // at the entrance). Only the radiation map sets carry the kicks (a lattice with radiation
+    // runs on them), so the kernels of the other sets keep their register budget.
+    constexpr bool kRad = MAPS == MAPS_ALL_QR || MAPS == MAPS_RARE_QR;
     CP rb = q + param_len(kind);
     if constexpr (kOffsets) {
         if (kRad && (flags & F_RAD)) {
@@ -1909,7 +1917,7 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
     case K_QUAD: quadrupole<F>(p, q, n, sp); break;
     case K_MULT: thick_multipole<F>(p, q, n, flags); break;
     case K_BEND:
-        if constexpr (MAPS != MAPS_RARE && MAPS != MAPS_RARE_Q) {
+        if constexpr (!maps_rare(MAPS)) {
             if ((flags & (F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE | F_BEND_DRIFT)) ==
                 (F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE))
                 sbend<F, true>(p, flags, q, sp);
@@ -1921,10 +1929,10 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
         if constexpr (MAPS != MAPS_NO_RF) rfcavity<F>(p, q, n, sp);
         break;
     case K_QMULT:
-        if constexpr (MAPS == MAPS_ALL_Q || MAPS == MAPS_RARE_Q) unset = quad_multipole<F>(p, q, n, flags, sp);
+        if constexpr (maps_q(MAPS)) unset = quad_multipole<F>(p, q, n, flags, sp);
         break;
     case K_BMULT:
-        if constexpr (MAPS == MAPS_ALL_Q || MAPS == MAPS_RARE_Q) bend_general<F>(p, q, n, flags, sp);
+        if constexpr (maps_q(MAPS)) bend_general<F>(p, q, n, flags, sp);
         break;
     case K_THIN:
         if constexpr (MAPS != MAPS_NO_RF) thin_multipole(p, q, n, flags);

@@ -109,10 +109,10 @@ void KernelTimer::release()
     pool.clear();
 }
 
-int default_grid(int device, bool fast, int waves, const JitKernels *jit, bool span, bool ext)
+int default_grid(int device, bool fast, int waves, const JitKernels *jit, bool span, int ext)
 {
     int blocks = 0, cus = 0;
-    const hipError_t e = ext   ? track_tiles_occupancy(fast, kExtWaves, &blocks, span, true)
+    const hipError_t e = ext   ? track_tiles_occupancy(fast, kExtWaves, &blocks, span, ext)
                          : span ? track_tiles_occupancy(fast, kSpanWaves, &blocks, true)
                          : jit  ? jit_occupancy(jit, waves, &blocks)
                                 : track_tiles_occupancy(fast, waves, &blocks);
@@ -156,11 +156,11 @@ hipError_t enqueue_run(hipStream_t s, RunBuffers &rb, const Group *h_groups, int
     // Lattices with stepped quadrupoles or general bends (K_QMULT, K_BMULT) run on cell kernels
     // generated with them or on the generic kernels built with them (ext); the other kernels
     // do not carry them.
-    bool rare_q = false;
+    int rare_q = 0;  // the highest map level among the groups (LatDesc::rare_q)
     for (int32_t g = 0; g < n_groups; g++)
-        if (h_groups[g].n_turns > 0 && h_groups[g].lat.rare_q) rare_q = true;
-    const bool use_jit = cfg.jit && !cfg.span && (!rare_q || jit_rare_q(cfg.jit));
-    const bool ext = rare_q && !use_jit;
+        if (h_groups[g].n_turns > 0 && h_groups[g].lat.rare_q > rare_q) rare_q = h_groups[g].lat.rare_q;
+    const bool use_jit = cfg.jit && !cfg.span && jit_level(cfg.jit) >= rare_q;
+    const int ext = use_jit ? 0 : rare_q;
     int c = 0;
     for (int32_t off = 0; off < max_turns && e == hipSuccess; off += K, c++) {
         const int cur = c & 1, nxt = cur ^ 1;

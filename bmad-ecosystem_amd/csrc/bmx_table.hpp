@@ -177,7 +177,8 @@ struct LatDesc {
     const EleHdr *hdr;
     const double *prm;
     int32_t n_ele;
-    int32_t rare_q;  // nonzero when the table holds K_QMULT elements (read on the host only)
+    int32_t rare_q;  // 1 when the table holds K_QMULT or K_BMULT elements, 2 when it also has
+                     // radiation kicks, else 0 (read on the host only)
     Species sp;
 };
 
