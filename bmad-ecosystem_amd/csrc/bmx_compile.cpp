@@ -786,8 +786,7 @@ int compile_table(const bmx_ele *eles, int32_t n_ele, double mc2, int32_t charge
             const int ft = e.fringe_type == 0 ? BMX_FRINGE_NONE : e.fringe_type;
             const bool kick_type = ft == BMX_FRINGE_BASIC_BEND || ft == BMX_FRINGE_HARD_EDGE_ONLY;
             if (!kick_type && ft != BMX_FRINGE_NONE && ft != BMX_FRINGE_FULL) { err = where(i, e) + "bend fringe_type"; return BMX_ERR_UNSUPPORTED; }
-            // (a bend that radiates takes the general bend, whose rare path carries the kicks)
-            const bool plain = ix == -1 && ixe == -1 && ft != BMX_FRINGE_FULL && !misaligned(e) && !rad_here;
+            const bool plain = ix == -1 && ixe == -1 && ft != BMX_FRINGE_FULL && !misaligned(e);
             const bool at_ent = e.fringe_at == BMX_ENTRANCE_END || e.fringe_at == BMX_BOTH_ENDS;
             const bool at_exit = e.fringe_at == BMX_EXIT_END || e.fringe_at == BMX_BOTH_ENDS;
             double fg_ent = e.fint * e.hgap, fg_exit = e.fintx * e.hgapx;

@@ -76,6 +76,8 @@ bool find_period(const HostTable &t, Period &out)
 
 std::string generate(const HostTable &t, const Period &pd, std::string &sig)
 {
+    bool q_kinds = false;  // the lattice holds stepped quadrupoles or general bends
+    for (const EleHdr &h : t.hdr) q_kinds |= h.kind == K_QMULT || h.kind == K_BMULT;
     std::ostringstream body, key;
     key << pd.H << ':' << pd.P << ':' << pd.N << ':' << pd.S << ':' << t.hdr.size() << ':'
         << t.hdr[pd.H].off;
@@ -101,7 +103,8 @@ std::string generate(const HostTable &t, const Period &pd, std::string &sig)
       << bmx_jit_src_tiles << '\n'
       << "namespace bmx {\n"
       << "struct CellTurn {\n"
-      << "    static constexpr int kRare = " << (t.rad ? "MAPS_RARE_QR" : t.rare_q ? "MAPS_RARE_Q" : "MAPS_RARE") << ";\n"
+      << "    static constexpr int kRare = "
+      << (t.rad ? (q_kinds ? "MAPS_RARE_QR" : "MAPS_RARE_R") : t.rare_q ? "MAPS_RARE_Q" : "MAPS_RARE") << ";\n"
       << "    __device__ __forceinline__ static void turn(Particle &p, const LatDesc &L, int32_t t_abs)\n"
       << "    {\n"
       << "        const CP prm = (CP)L.prm;\n";
@@ -121,9 +124,11 @@ std::string generate(const HostTable &t, const Period &pd, std::string &sig)
         "const uint32_t *__restrict__ nt, const double *__restrict__ pi, double *__restrict__ po, "
         "uint32_t ps, uint32_t *__restrict__ co, uint32_t *tc, int32_t off, int32_t K, "
         "unsigned long long *w)";
+    // the stepped quadrupole and the general bend need the 2-wave budget; a lattice that is on
+    // the stepped-element map sets only for its radiation kicks gets the variants asked for
     for (int w = 3; w <= 5; w++)
         s << "extern \"C\" __global__ void __launch_bounds__(bmx::kBlock) "
-          << "__attribute__((amdgpu_waves_per_eu(" << (t.rare_q ? 2 : w) << "))) bmx_cell_tiles_" << w << args
+          << "__attribute__((amdgpu_waves_per_eu(" << (q_kinds ? 2 : w) << "))) bmx_cell_tiles_" << w << args
           << "\n{\n    bmx::track_tiles<true, bmx::CellTurn>(g, t, nt, pi, po, ps, co, tc, off, K, w);\n}\n";
     return s.str();
 }

@@ -1331,20 +1331,27 @@ __device__ __forceinline__ bool quad_multipole(Particle &p, CP q, int32_t n_step
 // MAPS_ALL_Q and MAPS_RARE_Q add the stepped quadrupole (K_QMULT) and the general bend
 // (K_BMULT) to MAPS_ALL and MAPS_RARE, for kernels built for lattices that hold one (their
 // code would otherwise shape the register assignment of every kernel); MAPS_ALL_QR and
-// MAPS_RARE_QR add the radiation kicks to those, for lattices with radiation (the kick's call
-// raises the register count of every kernel it is compiled into).
+// MAPS_RARE_QR add the radiation kicks to those, for lattices with radiation (the kicks raise
+// the register count of every kernel they are compiled into); MAPS_RARE_R is MAPS_RARE with
+// the kicks, for the cell kernels of lattices with radiation and no stepped elements.
 // from_prev: the particle has just left the element before this one in the same lattice, so
 // an entrance test that repeats that element's exit test (F_APE_ENT_SAME, F_APE_ENT cleared)
 // is skipped; otherwise it is made.
 // A particle lost here gets p.loss_ele = ix1 (the element's 1-based index), p.loss_turn = turn.
 enum MapSet : int {
     MAPS_ALL = 0, MAPS_NO_RF = 1, MAPS_RARE = 2, MAPS_RARE_Q = 3, MAPS_ALL_Q = 4, MAPS_RARE_QR = 5,
-    MAPS_ALL_QR = 6
+    MAPS_ALL_QR = 6, MAPS_RARE_R = 7
 };
 // The map set carries the stepped quadrupole and the general bend.
 constexpr bool maps_q(int m) { return m == MAPS_RARE_Q || m == MAPS_ALL_Q || m == MAPS_RARE_QR || m == MAPS_ALL_QR; }
-// The map set carries no plain bend (the rare paths).
-constexpr bool maps_rare(int m) { return m == MAPS_RARE || m == MAPS_RARE_Q || m == MAPS_RARE_QR; }
+// The rare paths (MAPS_RARE_QR carries the plain bend all the same: a bend that radiates takes
+// the rare path).
+constexpr bool maps_rare(int m)
+{
+    return m == MAPS_RARE || m == MAPS_RARE_Q || m == MAPS_RARE_QR || m == MAPS_RARE_R;
+}
+// The map set carries the radiation kicks.
+constexpr bool maps_rad(int m) { return m == MAPS_RARE_QR || m == MAPS_ALL_QR || m == MAPS_RARE_R; }
 
 // ---- offset_particle, non-bend branch (bmad/code/offset_particle.f90:183-492) -------------
 // Element orientation +1, particle direction +1; ob is the offset block (bmx_table.hpp).
@@ -1823,17 +1830,29 @@ __device__ __forceinline__ void gauss6(double *g, uint32_t id, int32_t turn, int
     }
 }
 
+// The phase-space coordinates of a particle, passed to and returned from the radiation kick
+// by value.
+struct Vec6 {
+    double x, px, y, py, z, pz;
+};
+
 // track1_radiation (radiation_mod.f90:61-147) at one edge (0 entrance, 1 exit) with the
 // element's map m (radiation block layout in bmx_table.hpp): damping
 //   vec += rr damp_dmat (vec - ref_orb) [+ rr xfer_damp_vec unless zero_average],
-// then fluctuations vec += scale stoc_mat g, with rr = scale = synch_rad_scale (time_dir = 1);
-// lost_pz if pz < -1 afterwards. beta is left as it was, as the reference leaves it (the
-// caller refreshes the functions of the energy). The settings come as values: flags and
-// scale of RadCfg, key the fluctuation key.
-__device__ __noinline__ void rad_kick(Particle &p, CP m, uint32_t flags, double scale, uint32_t key0,
-                                      uint32_t key1, int32_t edge, int32_t ix1, int32_t turn)
+// then fluctuations vec += scale stoc_mat g, with rr = scale = synch_rad_scale (time_dir = 1).
+// The caller tests pz < -1 (lost_pz) and refreshes the functions of the energy; beta is left as
+// it was, as the reference leaves it. The settings come as values: flags and scale of RadCfg,
+// key the fluctuation key; id is the particle's index.
+__device__ __forceinline__ Vec6 rad_kick(Vec6 in, CP mc, uint32_t flags, double scale, uint32_t key0,
+                                      uint32_t key1, uint32_t id, int32_t edge, int32_t ix1,
+                                      int32_t turn)
 {
-    double v[6] = {p.x, p.px, p.y, p.py, p.z, p.pz};
+    // The 69 map values are read with vector loads from one address on every lane (a pointer
+    // held in VGPRs), so they take vector registers as they are used rather than about 140
+    // scalar registers at once, which the element loop cannot spare.
+    const double *m = (const double *)mc;
+    asm volatile("" : "+v"(m));
+    double v[6] = {in.x, in.px, in.y, in.py, in.z, in.pz};
     if (flags & RAD_DAMP) {
         const double rr = scale;
         double d[6], w[6];
@@ -1849,8 +1868,8 @@ __device__ __noinline__ void rad_kick(Particle &p, CP m, uint32_t flags, double 
     }
     if (flags & RAD_FLUCT) {
         double g[6];
-        gauss6(g, p.id, turn, ix1, edge, key0, key1);
-        CP L = m + 48;
+        gauss6(g, id, turn, ix1, edge, key0, key1);
+        const double *L = m + 48;
         int k = 0;
         for (int i = 0; i < 6; i++) {
             double s = 0;
@@ -1858,13 +1877,24 @@ __device__ __noinline__ void rad_kick(Particle &p, CP m, uint32_t flags, double 
             v[i] = v[i] + scale * s;
         }
     }
-    p.x = v[0];
-    p.px = v[1];
-    p.y = v[2];
-    p.py = v[3];
-    p.z = v[4];
-    p.pz = v[5];
+    return Vec6{v[0], v[1], v[2], v[3], v[4], v[5]};
+}
+
+// The kick at one edge applied to p: the lost_pz test (radiation_mod.f90:122-125) and the
+// functions of the energy refreshed.
+__device__ __forceinline__ void rad_apply(Particle &p, CP m, const Species &sp, int32_t edge,
+                                          int32_t ix1, int32_t turn)
+{
+    const Vec6 o = rad_kick(Vec6{p.x, p.px, p.y, p.py, p.z, p.pz}, m, sp.rad.flags, sp.rad.scale,
+                            sp.rad.key0, sp.rad.key1, p.id, edge, ix1, turn);
+    p.x = o.x;
+    p.px = o.px;
+    p.y = o.y;
+    p.py = o.py;
+    p.z = o.z;
+    p.pz = o.pz;
     if (p.pz < -1) p.state = ST_LOST_PZ;
+    refresh_energy(p, sp);
 }
 
 template <bool F, int MAPS = MAPS_ALL>
@@ -1897,12 +1927,11 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
     // after the entrance aperture test, in laboratory coordinates; a particle it loses is lost
     // at the entrance). Only the radiation map sets carry the kicks (a lattice with radiation
     // runs on them), so the kernels of the other sets keep their register budget.
-    constexpr bool kRad = MAPS == MAPS_ALL_QR || MAPS == MAPS_RARE_QR;
+    constexpr bool kRad = maps_rad(MAPS);
     CP rb = q + param_len(kind);
     if constexpr (kOffsets) {
         if (kRad && (flags & F_RAD)) {
-            rad_kick(p, rb, sp.rad.flags, sp.rad.scale, sp.rad.key0, sp.rad.key1, 0, ix1, turn);
-            refresh_energy(p, sp);
+            rad_apply(p, rb, sp, 0, ix1, turn);
             if (p.state != ST_ALIVE) {
                 p.loss_ele = uniform_here(ix1);
                 p.loss_turn = uniform_here(turn);
@@ -1917,7 +1946,7 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
     case K_QUAD: quadrupole<F>(p, q, n, sp); break;
     case K_MULT: thick_multipole<F>(p, q, n, flags); break;
     case K_BEND:
-        if constexpr (!maps_rare(MAPS)) {
+        if constexpr (!maps_rare(MAPS) || maps_rad(MAPS)) {
             if ((flags & (F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE | F_BEND_DRIFT)) ==
                 (F_EDGE_ENT | F_EDGE_EXIT | F_EDGE_SIMPLE))
                 sbend<F, true>(p, flags, q, sp);
@@ -1957,9 +1986,7 @@ __device__ __forceinline__ bool track_element(Particle &p, EleRef h, CP prm, con
                 done = true;
             }
             if (!done) {
-                rad_kick(p, rb + kRadEdge, sp.rad.flags, sp.rad.scale, sp.rad.key0, sp.rad.key1, 1,
-                         ix1, turn);
-                refresh_energy(p, sp);
+                rad_apply(p, rb + kRadEdge, sp, 1, ix1, turn);
                 if (orbit_bad(p)) orbit_too_large(p);
                 else if ((flags & F_APE_EXIT) && aperture_fails(p, aflags, ap)) aperture(p, aflags, ap);
             }

@@ -88,7 +88,8 @@ enum Flags : int32_t {
     // ref_orb_offset, multipole_mod.f90:250-254)
     F_THIN_REF0 = F_QHARD,
     // radiation damping and/or fluctuation kicks at both ends of the element (track1_radiation,
-    // radiation_mod.f90:61-147): the element's radiation block follows its parameters
+    // radiation_mod.f90:61-147): the element's radiation block follows its parameters, and the
+    // element takes the rare path of the radiation map sets (F_RARE_MAP is set with it)
     F_RAD = 1 << 30,
 };
 constexpr int kOffsetLen = 24;
