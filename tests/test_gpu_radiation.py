@@ -10,6 +10,8 @@ restatement with the same maps, settings and fluctuation stream (needs an MI355X
 * the 10-cell ring with maps on every element for 50 turns on the generic kernel, the
   cell-specialised kernel, the element-at-a-time kernel and a batch of three lattices: losses
   identical, survivors within the restatement's own sensitivity envelope;
+* the 6-D closed orbit with damping (fluctuations switched off, as closed_orbit_calc does)
+  against the restatement's Newton solution;
 * a lattice's settings changed without recompiling it (zero_average, fluctuations off, another
   seed) and the kicks switched off entirely, which leaves the tracking of the lattice without
   maps (to the rounding of the general bend's formulas).
@@ -238,3 +240,54 @@ def test_settings_without_recompiling(bmx, oracle):
     out = b.download()
     assert P.loss_mismatch(out, ref).size == 0
     assert P.rel_err(out, ref, ref["state"] == 1) <= 1e-11
+
+
+def _oracle_closed_orbit6(oracle, arr, maps, cfg, steps=(1e-6, 1e-6, 1e-6, 1e-6, 1e-5, 1e-6)):
+    """The restatement's 6-D closed orbit: Newton steps on all six coordinates with the Jacobian
+    from five-point central differences of 25 tracked probes, as bmx_closed_orbit makes them."""
+    p0c, et = float(arr[0]["p0c"]), float(arr[0]["e_tot"])
+    x = np.zeros(6)
+    for _ in range(50):
+        v = np.tile(x, (25, 1))
+        for d in range(6):
+            for j, f in enumerate((1, -1, 2, -2)):
+                v[1 + 4 * d + j, d] += f * steps[d]
+        beam = {k: v[:, i].copy() for i, k in enumerate(P.COORDS)}
+        pc = p0c * (1 + beam["pz"])
+        beam["beta"] = np.where(beam["pz"] != 0, pc / np.sqrt(pc ** 2 + 0.51099895069e6 ** 2), p0c / et)
+        beam["state"] = np.ones(25, dtype=np.int32)
+        beam["loss_ele"] = np.zeros(25, dtype=np.int32)
+        beam["loss_turn"] = np.zeros(25, dtype=np.int32)
+        oracle.track_bunch(arr, beam, 1, rad_maps=maps, rad=cfg)
+        assert np.all(beam["state"] == 1)
+        out = np.stack([beam[k] for k in P.COORDS], axis=1)
+        M = np.stack([(8 * (out[1 + 4 * d] - out[2 + 4 * d]) - (out[3 + 4 * d] - out[4 + 4 * d]))
+                      / (12 * steps[d]) for d in range(6)], axis=1)
+        step = np.linalg.solve(M - np.eye(6), -(out[0] - x))
+        x = x + step
+        if np.all(np.abs(step) < np.abs(x) * 1e-12 + 1e-15):
+            return x
+    raise AssertionError("restatement closed orbit did not converge")
+
+
+def test_closed_orbit_with_damping(bmx, oracle):
+    """The 6-D closed orbit of the 10-cell ring whose bends radiate (B1's maps scaled by 1e3,
+    damping on, fluctuations on in the settings): closed_orbit_calc keeps the damping and switches
+    the fluctuations off (closed_orbit_calc.f90:264), so the orbit is the damped, energy-losing
+    one -- pz and z shifted from the orbit without radiation, where the rf restores the energy
+    lost -- equal to the restatement's with fluctuations off within 1e-11."""
+    T = bmx.tracking
+    arr = _ring(bmx)
+    maps = R.lattice_maps(arr, bmx, damp=1e3, stoc=1e3)
+    maps[arr["key"] != bmx.elements.SBEND] = np.zeros((), dtype=R.RAD_MAP_DTYPE)
+    ctx = T.Context(0)
+    cfg = _cfg(bmx, damping=True, fluctuations=True, seed=11)
+    lat = T.Lattice(ctx, arr, rad_maps=maps, rad=cfg)
+    plain = T.Lattice(ctx, arr)
+    print("closed orbit with damping", flush=True)
+    orbit, mat, st = T.closed_orbit(ctx, [lat, plain], n_dim=6)
+    assert np.all(st == 0), st
+    want = _oracle_closed_orbit6(oracle, arr, maps, _ocfg(oracle, _cfg(bmx, damping=True, fluctuations=False)))
+    print("orbit", orbit[0].tolist(), "restatement", want.tolist(), flush=True)
+    assert np.abs(orbit[0] - orbit[1]).max() > 1e-7  # the damping moves the orbit
+    assert np.allclose(orbit[0], want, rtol=0, atol=1e-11)
