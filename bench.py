@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -49,7 +50,7 @@ def parse():
                     help="turns between alive-list compactions (0 = none)")
     ap.add_argument("--extras", default="batched,da,served,plain",
                     help="comma list of the other workloads to run after the main one "
-                         "(batched, da, served, plain; empty = none)")
+                         "(batched, da, served, plain, radiation; empty = none)")
     ap.add_argument("--main", choices=["on", "off"], default="on",
                     help="off: skip the single-lattice workload and the CPU baseline (runs of "
                          "the other workloads alone)")
@@ -295,6 +296,59 @@ def main():
     print(json.dumps(out))
 
 
+def bend_rad_maps(bmx, arr):
+    """Synthetic radiation maps of the magnitudes Bmad gives the ring's bends at 5 GeV (each half
+    of a bend: the relative energy loss U = 2/3 r_e gamma^3 g^2 L/2 as xfer_damp_vec's pz, damping
+    of px, py by U and of pz by 2U, a quantum-excitation kick in pz of variance
+    55/(24 sqrt 3) r_e lambda_e gamma^5 |g|^3 L/2); the other elements carry none."""
+    E, T = bmx.elements, bmx.tracking
+    r_e, lam_e = 2.8179403262e-15, 3.8615926796e-13  # classical radius, reduced Compton wavelength
+    maps = T.rad_maps(len(arr))
+    for i in np.nonzero(arr["key"] == E.SBEND)[0]:
+        g, half = float(arr[i]["g"]), float(arr[i]["l"]) / 2
+        gamma = float(arr[i]["e_tot"]) / E.M_ELECTRON
+        u = 2.0 / 3.0 * r_e * gamma ** 3 * g * g * half
+        sig = math.sqrt(55.0 / (24.0 * math.sqrt(3.0)) * r_e * lam_e * gamma ** 5 * abs(g) ** 3 * half)
+        for h in range(2):
+            m = maps[i, h]
+            m["damp_dmat"][1, 1] = m["damp_dmat"][3, 3] = -u
+            m["damp_dmat"][5, 5] = -2 * u
+            m["xfer_damp_vec"][5] = -u
+            m["stoc_mat"][5, 5] = sig
+    return maps
+
+
+def run_radiation(bmx, ctx, flops, specialize, n=1_000_000, turns=100):
+    """The main workload's ring with radiating bends (damping and fluctuation kicks at both ends
+    of each, bend_rad_maps), 1e6 particles over 100 turns: particle-element-turns per second by
+    the wall clock around the run, and the same ring without radiation."""
+    L, E, T = bmx.lattices, bmx.elements, bmx.tracking
+    arr = E.to_array(L.ring1())
+    beam = L.gaussian_beam(n, seed=3, sigma=3 * L.SIGMA0)
+    maps = bend_rad_maps(bmx, arr)
+    out = {"particles": n, "turns": turns, "radiating": int((arr["key"] == E.SBEND).sum())}
+    for label, kw in (("none", {}),
+                      ("damping", dict(rad_maps=maps, rad=T.rad_config(damping=True))),
+                      ("damping_fluctuations", dict(rad_maps=maps, rad=T.rad_config(damping=True, fluctuations=True, seed=5)))):
+        lat = T.Lattice(ctx, arr, **kw)
+        if specialize:
+            lat.specialize()
+        b = T.Beam.from_host(ctx, beam)
+        ctx.synchronize()
+        ctx.take_work()
+        t0 = time.perf_counter()
+        T.track_beam(ctx, lat, b, turns)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        passes = ctx.take_work()
+        out[label] = {"value": passes / dt, "seconds": dt, "alive": int(b.count_alive())}
+        print(f"radiation {label}: {passes / dt:.3e} particle-element-turns/s", file=sys.stderr, flush=True)
+        b.close()
+        lat.close()
+    out["unit"] = "particle·element·turns/s"
+    return out
+
+
 def run_extras(bmx, ctx, args, flops, special):
     extras = [e for e in args.extras.split(",") if e and e != "none"]
     out = {}
@@ -310,6 +364,8 @@ def run_extras(bmx, ctx, args, flops, special):
                 baselines = [b for b in bl.split(",") if b and b != "none"]
                 out[key] = run_served(bmx, ctx, args.served_requests, specialize=special,
                                       stream=stream, baselines=baselines)
+    if "radiation" in extras:
+        out["radiation"] = run_radiation(bmx, ctx, flops, specialize=special)
     if "plain" in extras:
         out["plain"] = run_plain(bmx, ctx, graph=args.plain_graph == "on")
     return out
