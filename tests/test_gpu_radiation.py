@@ -10,6 +10,7 @@ restatement with the same maps, settings and fluctuation stream (needs an MI355X
 * the 10-cell ring with maps on every element for 50 turns on the generic kernel, the
   cell-specialised kernel, the element-at-a-time kernel and a batch of three lattices: losses
   identical, survivors within the restatement's own sensitivity envelope;
+* served requests on radiating lattices, bit-identical to tracking each request alone;
 * the 6-D closed orbit with damping (fluctuations switched off, as closed_orbit_calc does)
   against the restatement's Newton solution;
 * a lattice's settings changed without recompiling it (zero_average, fluctuations off, another
@@ -291,3 +292,28 @@ def test_closed_orbit_with_damping(bmx, oracle):
     print("orbit", orbit[0].tolist(), "restatement", want.tolist(), flush=True)
     assert np.abs(orbit[0] - orbit[1]).max() > 1e-7  # the damping moves the orbit
     assert np.allclose(orbit[0], want, rtol=0, atol=1e-11)
+
+
+def test_served_requests_with_radiation(bmx):
+    """Requests on three radiating lattices (their own maps and seeds) served together with
+    37-turn chunks: every particle equals tracking its request alone, bit for bit -- a request's
+    fluctuation kicks follow its own particles' indices, wherever the server places them."""
+    T = bmx.tracking
+    ctx = T.Context(0)
+    arr = _ring(bmx)
+    mapss = [R.lattice_maps(arr, bmx, damp=1e4 * (1 + v), stoc=1e3) for v in range(3)]
+    cfgs = [_cfg(bmx, damping=True, fluctuations=True, seed=500 + v) for v in range(3)]
+    lats = [T.Lattice(ctx, arr, rad_maps=m, rad=c) for m, c in zip(mapss, cfgs)]
+    reqs = [(i % 3, 400 + 300 * i, 10 + 13 * i) for i in range(6)]
+    beams = [P.random_beam(bmx, n, seed=70 + i, scale=12.0, pz_sigma=3e-3)
+             for i, (_, n, _) in enumerate(reqs)]
+    srv = T.Server(ctx, hbm_budget=1 << 30, chunk_turns=37, compact_turns=10)
+    print("served requests with radiation", flush=True)
+    ids = [srv.submit(lats[v], beams[i], t, 3) for i, (v, n, t) in enumerate(reqs)]
+    results = [srv.wait(r) for r in ids]
+    srv.close()
+    for i, (v, n, t) in enumerate(reqs):
+        out, _ = results[i]
+        alone = T.track_beam_host(arr, beams[i], t, turn0=3, rad_maps=mapss[v], rad=cfgs[v])
+        for k in alone:
+            assert np.array_equal(out[k], alone[k]), (i, k)
